@@ -1,0 +1,984 @@
+/* CPU oracle for the channel-flow dual-time hot path -- TEST INFRASTRUCTURE.
+ *
+ * A plain-C restatement of the reference's FP64 arithmetic, written for
+ * checking the HIP path (tests/, smoke(), bench.py cpu_baseline).  Every
+ * expression keeps the reference's operation order (compile with
+ * -ffp-contract=off) so that results agree with the reference bit for bit;
+ * the code structure (recipe tables, one Roe routine for three directions,
+ * per-point primitive records) is this project's own.
+ *
+ * Reference citations are file:line in ricklee209/Channel_Flow.
+ */
+#include "cf_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define KG 1.4            /* prm.h:9   K   */
+#define RG 287.0          /* prm.h:10  R   */
+#define EPS_PRE 0.0001    /* prm.h:7   e   */
+#define CV 717.5          /* Viscous_terms.h:3 */
+#define MU_L 0.0000185    /* Viscous_terms.h:4 */
+#define PR_L 0.72         /* Viscous_terms.h:5 */
+#define T_WALL2 (2 * 298.0592) /* Y_boundary_condition.cpp:87 */
+
+typedef struct {
+  int Ym, Zm, gend, ny, nz;
+} dims_t;
+
+static dims_t dims(const cfo_grid* g) {
+  dims_t d;
+  d.Ym = g->Y_out + 4;
+  d.Zm = g->Z_out + 4;
+  d.gend = g->X_out + 2;
+  d.ny = g->Y_out + 1;
+  d.nz = g->Z_out + 1;
+  return d;
+}
+
+#define IX(d, i, j, k) ((((size_t)(i)) * (size_t)(d).Ym + (size_t)(j)) * (size_t)(d).Zm + (size_t)(k))
+
+static double dmax(double a, double b) { return a > b ? a : b; }
+
+/* ------------------------------------------------------------------------ */
+/* metrics: Jacobian.cpp:60-244, Jacobian_u.cpp:57-97, Jacobian_v.cpp:77-141,
+ * Jacobian_w.cpp:57-98 (tanh stretching in y, Cartesian in x and z)       */
+/* ------------------------------------------------------------------------ */
+
+static void stretch(const cfo_grid* g, double gamma, double jpos, double* etdy, double* J) {
+  /* Jacobian.cpp:74-99 with (j-1.5) replaced by the caller's offset */
+  double half = 0.5 * g->high;
+  double a = gamma * (1 - 2 * jpos * g->deltaET);
+  double temp = half * (1 - 1. / tanh(gamma) * tanh(a));
+  double temp1 = (1 - temp / half) * tanh(gamma);
+  *etdy = tanh(gamma) / (2 * half * gamma * (1 - temp1 * temp1));
+  double temp2 = 1 / cosh(a);
+  double ydet = g->high * gamma * 1. / tanh(gamma) * temp2 * temp2;
+  *J = 1. / (1 * ydet * 1);
+}
+
+static void set_cart(double* const m[9], size_t p, double etdy) {
+  const double c[9] = {1, 0, 0, 0, 0, 0, 0, 0, 1};
+  for (int q = 0; q < 9; q++) m[q][p] = c[q];
+  m[4][p] = etdy;
+}
+
+void cfo_metrics(const cfo_grid* g, double gamma, double* J, double* const m[9], double* Ju,
+                 double* const mu[9], double* Jv, double* const mv[9], double* Jw, double* const mw[9]) {
+  dims_t d = dims(g);
+  int gend = d.gend, ny = d.ny, nz = d.nz, nyy = ny + 1, nzz = nz + 1;
+  /* cell centres, interior (Jacobian.cpp:60-108): eta = (j-1.5) deltaET */
+  for (int i = 3; i <= gend; i++)
+    for (int j = 2; j <= ny; j++)
+      for (int k = 2; k <= nz; k++) {
+        double ey, jj;
+        stretch(g, gamma, j - 1.5, &ey, &jj);
+        set_cart(m, IX(d, i, j, k), ey);
+        J[IX(d, i, j, k)] = jj;
+      }
+  /* x ghosts (Jacobian.cpp:111-160): all but ztdy/ztdz copied (ztdx is
+   * written three times there), before the y/z ghosts exist */
+  const int xg[2][2] = {{2, 3}, {gend + 1, gend}};
+  for (int s = 0; s < 2; s++) {
+    int ig = xg[s][0], is = xg[s][1];
+    for (int j = 1; j <= nyy; j++)
+      for (int k = 1; k <= nzz; k++) {
+        size_t p = IX(d, ig, j, k), q = IX(d, is, j, k);
+        for (int c = 0; c < 7; c++) m[c][p] = m[c][q];
+        J[p] = J[q];
+      }
+  }
+  /* y ghosts j=1, nyy (Jacobian.cpp:162-203): ztdy/ztdz take ztdx */
+  for (int i = 2; i <= gend + 1; i++)
+    for (int k = 2; k <= nz; k++) {
+      const int yg[2][2] = {{1, 2}, {nyy, ny}};
+      for (int s = 0; s < 2; s++) {
+        size_t p = IX(d, i, yg[s][0], k), q = IX(d, i, yg[s][1], k);
+        for (int c = 0; c < 7; c++) m[c][p] = m[c][q];
+        m[7][p] = m[6][q];
+        m[8][p] = m[6][q];
+        J[p] = J[q];
+      }
+    }
+  /* z ghosts k=1, nzz (Jacobian.cpp:205-244) */
+  for (int i = 2; i <= gend + 1; i++)
+    for (int j = 1; j <= nyy; j++) {
+      const int zg[2][2] = {{1, 2}, {nzz, nz}};
+      for (int s = 0; s < 2; s++) {
+        size_t p = IX(d, i, j, zg[s][0]), q = IX(d, i, j, zg[s][1]);
+        for (int c = 0; c < 7; c++) m[c][p] = m[c][q];
+        m[7][p] = m[6][q];
+        m[8][p] = m[6][q];
+        J[p] = J[q];
+      }
+    }
+  /* x faces (Jacobian_u.cpp:57-97): i in [2,gend], j in [1,ny-1], eta = (j-0.5) */
+  for (int i = 2; i <= gend; i++)
+    for (int j = 1; j < ny; j++)
+      for (int k = 1; k < nz; k++) {
+        double ey, jj;
+        stretch(g, gamma, j - 0.5, &ey, &jj);
+        set_cart(mu, IX(d, i, j, k), ey);
+        Ju[IX(d, i, j, k)] = jj;
+      }
+  /* y faces (Jacobian_v.cpp:77-141): eta = (j-1.0); wall faces take the
+   * cell-centre ghost values */
+  for (int i = 2; i <= gend - 1; i++)
+    for (int j = 1; j <= ny; j++)
+      for (int k = 1; k < nz; k++) {
+        double ey, jj;
+        stretch(g, gamma, j - 1.0, &ey, &jj);
+        set_cart(mv, IX(d, i, j, k), ey);
+        Jv[IX(d, i, j, k)] = jj;
+      }
+  for (int i = 2; i <= gend - 1; i++)
+    for (int k = 1; k < nz; k++) {
+      size_t w0 = IX(d, i, 1, k), w1 = IX(d, i, ny, k);
+      size_t c0 = IX(d, i, 1, k), c1 = IX(d, i, nyy, k);
+      for (int c = 3; c < 6; c++) {
+        mv[c][w0] = m[c][c0];
+        mv[c][w1] = m[c][c1];
+      }
+      Jv[w1] = J[c1];
+      Jv[w0] = J[c0];
+    }
+  /* z faces (Jacobian_w.cpp:57-98): j in [1,ny-1], k in [1,nz], eta = (j-0.5) */
+  for (int i = 2; i <= gend - 1; i++)
+    for (int j = 1; j < ny; j++)
+      for (int k = 1; k <= nz; k++) {
+        double ey, jj;
+        stretch(g, gamma, j - 0.5, &ey, &jj);
+        set_cart(mw, IX(d, i, j, k), ey);
+        Jw[IX(d, i, j, k)] = jj;
+      }
+}
+
+int cfo_hyperplane_counts(const cfo_grid* g, int* counts, int ncounts) {
+  dims_t d = dims(g);
+  int mmax = d.gend + d.ny + d.nz;
+  int n = mmax - 7 + 1;
+  if (ncounts < n) return -1;
+  memset(counts, 0, sizeof(int) * (size_t)n);
+  for (int i = 3; i <= d.gend; i++)
+    for (int j = 2; j <= d.ny; j++)
+      for (int k = 2; k <= d.nz; k++) counts[i + j + k - 7]++;
+  return n;
+}
+
+/* ------------------------------------------------------------------------ */
+/* ghosts: main.cpp:203-240, Y_boundary_condition.cpp:48-120,               */
+/* Z_boundary_condition.cpp:48-66                                           */
+/* ------------------------------------------------------------------------ */
+
+void cfo_halo_bc(const cfo_grid* g, double* const U_[5], const double* J) {
+  dims_t d = dims(g);
+  int gend = d.gend, ny = d.ny, nz = d.nz, nyy = ny + 1, nzz = nz + 1, nyyy = nyy + 1;
+  size_t plane = (size_t)d.Ym * d.Zm;
+  /* x: one plane each side; a single rank receives its own plane back */
+  int src2 = g->x_periodic ? gend : 3;
+  int srcE = g->x_periodic ? 3 : gend;
+  for (int v = 0; v < 5; v++) {
+    memmove(U_[v] + IX(d, 2, 0, 0), U_[v] + IX(d, src2, 0, 0), plane * sizeof(double));
+    memmove(U_[v] + IX(d, gend + 1, 0, 0), U_[v] + IX(d, srcE, 0, 0), plane * sizeof(double));
+  }
+  /* isothermal no-slip walls: P extrapolated, T mirrored about T_w, u reflected */
+  for (int i = 2; i <= gend + 1; i++)
+    for (int k = 2; k <= nz; k++) {
+      const int wall[2][2] = {{2, 1}, {ny, nyy}};
+      for (int s = 0; s < 2; s++) {
+        size_t a = IX(d, i, wall[s][0], k), b = IX(d, i, wall[s][1], k);
+        double rho = U_[0][a] * J[a];
+        double U = U_[1][a] / U_[0][a];
+        double V = U_[2][a] / U_[0][a];
+        double W = U_[3][a] / U_[0][a];
+        double VV = U * U + V * V + W * W;
+        double P = (U_[4][a] * J[a] - 0.5 * rho * VV) * (KG - 1);
+        double temp = P / rho / RG;
+        double T = T_WALL2 - temp;
+        rho = P / RG / T;
+        U_[0][b] = P / RG / T / J[b];
+        U_[1][b] = -P / RG / T * U / J[b];
+        U_[2][b] = -P / RG / T * V / J[b];
+        U_[3][b] = -P / RG / T * W / J[b];
+        U_[4][b] = (rho * RG * T / (KG - 1) + 0.5 * P / RG / T * VV) / J[b];
+      }
+    }
+  /* periodic z, all rows including the wall ghosts */
+  for (int i = 2; i <= gend + 1; i++)
+    for (int j = 0; j <= nyyy; j++)
+      for (int v = 0; v < 5; v++) {
+        U_[v][IX(d, i, j, 1)] = U_[v][IX(d, i, j, nz)];
+        U_[v][IX(d, i, j, nzz)] = U_[v][IX(d, i, j, 2)];
+      }
+}
+
+/* ------------------------------------------------------------------------ */
+/* MUSCL reconstruction recipes (Flux_X.cpp:92-215, Flux_Y.cpp:91-240,      */
+/* Flux_Z.cpp:96-300).  A face f separates cells f and f+1 along the         */
+/* direction.  Each state is sum(w*Q/J) / sum(w/J) over consecutive cells    */
+/* starting at `first`.                                                      */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+  int first;    /* first cell of the stencil (offset from face index f) */
+  int n;        /* number of weights */
+  double w[5];
+} recipe_t;
+
+static const recipe_t R_O5L = {-2, 5, {2, -13, 47, 27, -3}};  /* left,  centred on f   */
+static const recipe_t R_O5R = {-1, 5, {-3, 27, 47, -13, 2}};  /* right, centred on f+1 */
+static const recipe_t R_O3L = {-1, 3, {-1, 5, 2}};
+static const recipe_t R_O3R = {0, 3, {2, 5, -1}};
+static const recipe_t R_AVG = {0, 2, {1, 1}};                 /* cells f, f+1 */
+static const recipe_t R_AVGM = {-1, 2, {1, 1}};               /* cells f-1, f */
+static const recipe_t R_AVG1 = {1, 2, {1, 1}};                /* cells f+1, f+2 */
+
+/* face range and recipe choice per direction (single rank) */
+static void face_range(const dims_t* d, int dir, int* f0, int* f1) {
+  if (dir == 0) { *f0 = 2; *f1 = d->gend; }
+  else if (dir == 1) { *f0 = 1; *f1 = d->ny; }
+  else { *f0 = 1; *f1 = d->nz; }
+}
+
+static const recipe_t* recipe_left(const dims_t* d, int dir, int f) {
+  if (dir == 0) {
+    int ge = d->gend;
+    if (f == 2) return &R_AVG;
+    if (f == 3 || f == ge) return &R_O3L;
+    return &R_O5L;
+  } else if (dir == 1) {
+    int ny = d->ny;
+    if (f == 1 || f == 2) return &R_AVG;
+    if (f == 3 || f == ny - 1) return &R_O3L;
+    if (f == ny) return &R_AVGM;
+    return &R_O5L;
+  } else {
+    int nz = d->nz;
+    if (f == 1) return &R_AVG;
+    if (f == 2 || f == nz) return &R_O3L;
+    return &R_O5L;
+  }
+}
+
+static const recipe_t* recipe_right(const dims_t* d, int dir, int f) {
+  if (dir == 0) {
+    int ge = d->gend;
+    if (f == 2 || f == ge - 1) return &R_O3R;
+    if (f == ge) return &R_AVG;
+    return &R_O5R;
+  } else if (dir == 1) {
+    int ny = d->ny;
+    if (f == 1) return &R_AVG1;
+    if (f == 2 || f == ny - 2) return &R_O3R;
+    if (f == ny - 1 || f == ny) return &R_AVG;
+    return &R_O5R;
+  } else {
+    int nz = d->nz;
+    if (f == 1 || f == nz - 1) return &R_O3R;
+    if (f == nz) return &R_AVG;
+    return &R_O5R;
+  }
+}
+
+static void reconstruct(const recipe_t* r, double* const U_[5], const double* J, size_t base, size_t stride,
+                        int f, double q[5]) {
+  size_t p0 = base + (size_t)((long)(f + r->first)) * stride;
+  double s = r->w[0] / J[p0];
+  for (int t = 1; t < r->n; t++) s += r->w[t] / J[p0 + (size_t)t * stride];
+  double temp = 1. / s;
+  for (int v = 0; v < 5; v++) {
+    double a = r->w[0] * U_[v][p0];
+    for (int t = 1; t < r->n; t++) a += r->w[t] * U_[v][p0 + (size_t)t * stride];
+    q[v] = temp * a;
+  }
+}
+
+/* Roe flux, ROE == 1 branch (Flux_X.cpp:221-420): n = contravariant metric
+ * row of the face direction.  The energy dissipation uses deltaP times the
+ * un-normalised contravariant velocity in x and z (Flux_X.cpp:412,
+ * Flux_Z.cpp:499) and the normalised one in y (Flux_Y.cpp:441). */
+static void roe_flux(const double L[5], const double Rr[5], const double n[3], double Jf, int energy_norm,
+                     double F[5], double* Fav3_out, double* Pavg_out) {
+  double insqr = 1.0 / sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  double N0 = n[0] * insqr, N1 = n[1] * insqr, N2 = n[2] * insqr;
+  double _rho = L[0], _u = L[1] / _rho, _v = L[2] / _rho, _w = L[3] / _rho;
+  double _U = n[0] * _u + n[1] * _v + n[2] * _w;
+  double _VV = _u * _u + _v * _v + _w * _w;
+  double _P = (L[4] - 0.5 * _rho * _VV) * (KG - 1);
+  double _C = KG * _P / _rho;
+  double _H = 0.5 * _VV + _C / (KG - 1);
+  double rho_ = Rr[0], u_ = Rr[1] / rho_, v_ = Rr[2] / rho_, w_ = Rr[3] / rho_;
+  double U_ = n[0] * u_ + n[1] * v_ + n[2] * w_;
+  double VV_ = u_ * u_ + v_ * v_ + w_ * w_;
+  double P_ = (Rr[4] - 0.5 * rho_ * VV_) * (KG - 1);
+  double C_ = KG * P_ / rho_;
+  double H_ = 0.5 * VV_ + C_ / (KG - 1);
+  double t5 = sqrt(_rho), t6 = sqrt(rho_), t4 = t5 + t6;
+  double rho = sqrt(_rho * rho_);
+  double u = (t5 * _u + t6 * u_) / t4;
+  double v = (t5 * _v + t6 * v_) / t4;
+  double w = (t5 * _w + t6 * w_) / t4;
+  double U = (t5 * _U + t6 * U_) / t4;
+  double VV = u * u + v * v + w * w;
+  double H = (t5 * _H + t6 * H_) / t4;
+  double C = (H - 0.5 * VV) * (KG - 1);
+  double dU1 = rho_ - _rho;
+  double dU2 = rho_ * u_ - _rho * _u;
+  double dU3 = rho_ * v_ - _rho * _v;
+  double dU4 = rho_ * w_ - _rho * _w;
+  double dU5 = (P_ / (KG - 1) + 0.5 * rho_ * VV_) - (_P / (KG - 1) + 0.5 * _rho * _VV);
+  double S = sqrt(C);
+  double temp1 = (P_ - _P) / rho / C;
+  double temp2 = U * insqr / S * (U_ - _U) * insqr;
+  double deltaU = (S - fabs(U * insqr)) * temp1 + temp2;
+  double deltaP = U * insqr / S * (P_ - _P) + (S - fabs(U * insqr)) * rho * (U_ - _U) * insqr;
+  double a = fabs(U * insqr);
+  double Fav1 = a * dU1 + deltaU * rho;
+  double Fav2 = a * dU2 + deltaU * rho * u + deltaP * N0;
+  double Fav3 = a * dU3 + deltaU * rho * v + deltaP * N1;
+  double Fav4 = a * dU4 + deltaU * rho * w + deltaP * N2;
+  double Fav5 = energy_norm ? a * dU5 + deltaU * rho * H + deltaP * U * insqr
+                            : a * dU5 + deltaU * rho * H + deltaP * U;
+  F[0] = 0.5 * ((_rho * _U + rho_ * U_) - Fav1 / insqr) / Jf;
+  F[1] = 0.5 * ((_rho * _u * _U + rho_ * u_ * U_ + n[0] * (_P + P_)) - Fav2 / insqr) / Jf;
+  F[2] = 0.5 * ((_rho * _v * _U + rho_ * v_ * U_ + n[1] * (_P + P_)) - Fav3 / insqr) / Jf;
+  F[3] = 0.5 * ((_rho * _w * _U + rho_ * w_ * U_ + n[2] * (_P + P_)) - Fav4 / insqr) / Jf;
+  F[4] = 0.5 * ((_U * (3.5 * _P + 0.5 * _rho * _VV) + U_ * (3.5 * P_ + 0.5 * rho_ * VV_)) - Fav5 / insqr) / Jf;
+  *Fav3_out = Fav3 / insqr;
+  *Pavg_out = _P + P_;
+}
+
+void cfo_flux(const cfo_grid* g, int dir, double* const U_[5], const double* J, const double* Jf,
+              double* const mf[9], double* const inF[5]) {
+  dims_t d = dims(g);
+  int f0, f1;
+  face_range(&d, dir, &f0, &f1);
+  size_t sI = (size_t)d.Ym * d.Zm, sJ = (size_t)d.Zm, sK = 1;
+  /* the two transverse loops: cells (a,b) of the face's plane */
+  int a0, a1, b0, b1;
+  size_t stride, sa, sb;
+  if (dir == 0) { a0 = 2; a1 = d.ny; b0 = 2; b1 = d.nz; stride = sI; sa = sJ; sb = sK; }
+  else if (dir == 1) { a0 = 3; a1 = d.gend; b0 = 2; b1 = d.nz; stride = sJ; sa = sI; sb = sK; }
+  else { a0 = 3; a1 = d.gend; b0 = 2; b1 = d.ny; stride = sK; sa = sI; sb = sJ; }
+  const int mrow = 3 * dir; /* xi, eta or zeta row of the metric */
+  for (int f = f0; f <= f1; f++) {
+    const recipe_t* rl = recipe_left(&d, dir, f);
+    const recipe_t* rr = recipe_right(&d, dir, f);
+    for (int a = a0; a <= a1; a++)
+      for (int b = b0; b <= b1; b++) {
+        size_t base = (size_t)a * sa + (size_t)b * sb;
+        double L[5], Rr[5], F[5], fav3, psum;
+        reconstruct(rl, U_, J, base, stride, f, L);
+        reconstruct(rr, U_, J, base, stride, f, Rr);
+        /* face arrays carry the transverse cell indices minus one */
+        size_t o = (size_t)f * stride + (size_t)(a - 1) * sa + (size_t)(b - 1) * sb;
+        double n[3] = {mf[mrow][o], mf[mrow + 1][o], mf[mrow + 2][o]};
+        roe_flux(L, Rr, n, Jf[o], dir == 1, F, &fav3, &psum);
+        if (dir == 1 && (f == 1 || f == d.ny)) {
+          /* impermeable wall faces: only the wall-normal momentum flux
+           * (pressure + dissipation) survives (Flux_Y.cpp:452-460) */
+          F[0] = 0;
+          F[1] = 0;
+          F[2] = 0.5 * (n[1] * psum - fav3) / Jf[o];
+          F[3] = 0;
+          F[4] = 0;
+        }
+        for (int v = 0; v < 5; v++) inF[v][o] = F[v];
+      }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* viscous terms: Viscous_terms.cpp:212-883                                 */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+  double U, V, W, T;
+} prim_t;
+
+static prim_t prim_visc(double* const U_[5], const double* J, size_t p) {
+  /* Viscous_terms.cpp:261-395: rho here is rho/J */
+  prim_t q;
+  double rho = U_[0][p];
+  q.U = U_[1][p] / rho;
+  q.V = U_[2][p] / rho;
+  q.W = U_[3][p] / rho;
+  double VV = q.U * q.U + q.V * q.V + q.W * q.W;
+  double P = (U_[4][p] - 0.5 * rho * VV) * (KG - 1) * J[p];
+  q.T = P / (rho * RG) / J[p];
+  return q;
+}
+
+/* derivative set of one face: d/dxi, d/deta, d/dzeta of u,v,w, of the
+ * products uu,vv,ww,uv,vw,uw and of T */
+typedef struct {
+  double u, v, w, uu, vv, ww, uv, vw, uw, T;
+} dset_t;
+
+static dset_t d_two(prim_t a, prim_t b, double inv) {
+  /* (a - b) * inv, a = face's upper cell */
+  dset_t r;
+  r.u = (a.U - b.U) * inv;
+  r.v = (a.V - b.V) * inv;
+  r.w = (a.W - b.W) * inv;
+  r.uu = (a.U * a.U - b.U * b.U) * inv;
+  r.vv = (a.V * a.V - b.V * b.V) * inv;
+  r.ww = (a.W * a.W - b.W * b.W) * inv;
+  r.uv = (a.U * a.V - b.U * b.V) * inv;
+  r.vw = (a.V * a.W - b.V * b.W) * inv;
+  r.uw = (a.U * a.W - b.U * b.W) * inv;
+  r.T = (a.T - b.T) * inv;
+  return r;
+}
+
+static dset_t d_four(prim_t p1, prim_t p2, prim_t m1, prim_t m2, double inv4) {
+  /* (p1 + p2 - m1 - m2) * inv4: the tangential derivative averaged over the
+   * two cells of the face */
+  dset_t r;
+  r.u = (p1.U + p2.U - m1.U - m2.U) * inv4;
+  r.v = (p1.V + p2.V - m1.V - m2.V) * inv4;
+  r.w = (p1.W + p2.W - m1.W - m2.W) * inv4;
+  r.uu = (p1.U * p1.U + p2.U * p2.U - m1.U * m1.U - m2.U * m2.U) * inv4;
+  r.vv = (p1.V * p1.V + p2.V * p2.V - m1.V * m1.V - m2.V * m2.V) * inv4;
+  r.ww = (p1.W * p1.W + p2.W * p2.W - m1.W * m1.W - m2.W * m2.W) * inv4;
+  r.uv = (p1.U * p1.V + p2.U * p2.V - m1.U * m1.V - m2.U * m2.V) * inv4;
+  r.vw = (p1.V * p1.W + p2.V * p2.W - m1.V * m1.W - m2.V * m2.W) * inv4;
+  r.uw = (p1.U * p1.W + p2.U * p2.W - m1.U * m1.W - m2.U * m2.W) * inv4;
+  r.T = (p1.T + p2.T - m1.T - m2.T) * inv4;
+  return r;
+}
+
+/* metric coefficient groups of one face (Viscous_terms.cpp:471-526) */
+typedef struct {
+  double a[8], b[8], c[8], d[11], e[11], f[11]; /* 1-based like the reference */
+} coef_t;
+
+static void coefs(const double xi[3], const double et[3], const double zt[3], coef_t* q) {
+  double xix = xi[0], xiy = xi[1], xiz = xi[2];
+  double etx = et[0], ety = et[1], etz = et[2];
+  double ztx = zt[0], zty = zt[1], ztz = zt[2];
+  q->a[1] = (4. / 3) * xix * xix + xiy * xiy + xiz * xiz;
+  q->a[2] = xix * xix + (4. / 3) * xiy * xiy + xiz * xiz;
+  q->a[3] = xix * xix + xiy * xiy + (4. / 3) * xiz * xiz;
+  q->a[4] = xix * xix + xiy * xiy + xiz * xiz;
+  q->a[5] = (1. / 3) * xix * xiy;
+  q->a[6] = (1. / 3) * xiy * xiz;
+  q->a[7] = (1. / 3) * xix * xiz;
+  q->b[1] = (4. / 3) * etx * etx + ety * ety + etz * etz;
+  q->b[2] = etx * etx + (4. / 3) * ety * ety + etz * etz;
+  q->b[3] = etx * etx + ety * ety + (4. / 3) * etz * etz;
+  q->b[4] = etx * etx + ety * ety + etz * etz;
+  q->b[5] = (1. / 3) * etx * ety;
+  q->b[6] = (1. / 3) * ety * etz;
+  q->b[7] = (1. / 3) * etx * etz;
+  q->c[1] = (4. / 3) * ztx * ztx + zty * zty + ztz * ztz;
+  q->c[2] = ztx * ztx + (4. / 3) * zty * zty + ztz * ztz;
+  q->c[3] = ztx * ztx + zty * zty + (4. / 3) * ztz * ztz;
+  q->c[4] = ztx * ztx + zty * zty + ztz * ztz;
+  q->c[5] = (1. / 3) * ztx * zty;
+  q->c[6] = (1. / 3) * zty * ztz;
+  q->c[7] = (1. / 3) * ztx * ztz;
+  /* mixed groups share one pattern: g(p,q) for metric rows p and q */
+  const double* P[3] = {xi, xi, et};
+  const double* Q[3] = {et, zt, zt};
+  double* G[3] = {q->d, q->e, q->f};
+  for (int s = 0; s < 3; s++) {
+    double px = P[s][0], py = P[s][1], pz = P[s][2];
+    double qx = Q[s][0], qy = Q[s][1], qz = Q[s][2];
+    double* o = G[s];
+    o[1] = (4. / 3) * px * qx + py * qy + pz * qz;
+    o[2] = px * qx + (4. / 3) * py * qy + pz * qz;
+    o[3] = px * qx + py * qy + (4. / 3) * pz * qz;
+    o[4] = px * qx + py * qy + pz * qz;
+    o[5] = px * qy - (2. / 3) * py * qx;
+    o[6] = px * qz - (2. / 3) * pz * qx;
+    o[7] = py * qx - (2. / 3) * px * qy;
+    o[8] = py * qz - (2. / 3) * pz * qy;
+    o[9] = pz * qx - (2. / 3) * px * qz;
+    o[10] = pz * qy - (2. / 3) * py * qz;
+  }
+}
+
+static void load_face_metric(double* const mf[9], size_t o, double xi[3], double et[3], double zt[3]) {
+  for (int c = 0; c < 3; c++) {
+    xi[c] = mf[c][o];
+    et[c] = mf[3 + c][o];
+    zt[c] = mf[6 + c][o];
+  }
+}
+
+#define CVK_PR(g4, dT) (CV * KG * (g4) * (dT) / (PR_L))
+
+void cfo_viscous(const cfo_grid* g, double* const U_[5], const double* J, const double* Ju,
+                 double* const mu[9], const double* Jv, double* const mv[9], const double* Jw,
+                 double* const mw[9], double* const vF[4]) {
+  dims_t d = dims(g);
+  int gend = d.gend, ny = d.ny, nz = d.nz;
+  double invXI = 1. / (g->deltaXI), invET = 1. / (g->deltaET), invZT = 1. / (g->deltaZT);
+  double inv4XI = 1. / (4 * g->deltaXI), inv4ET = 1. / (4 * g->deltaET), inv4ZT = 1. / (4 * g->deltaZT);
+  size_t N = (size_t)(g->X_out + 6) * d.Ym * d.Zm;
+  /* face fluxes: LL on the x face (i-1,i), ML on the y face (j-1,j), NL on
+   * the z face (k-1,k), all stored at the upper cell (i,j,k) */
+  double* LL[4];
+  double* ML[4];
+  double* NL[4];
+  for (int v = 0; v < 4; v++) {
+    LL[v] = (double*)calloc(N, sizeof(double));
+    ML[v] = (double*)calloc(N, sizeof(double));
+    NL[v] = (double*)calloc(N, sizeof(double));
+  }
+  size_t sI = (size_t)d.Ym * d.Zm, sJ = (size_t)d.Zm;
+  for (int i = 3; i <= gend + 1; i++)
+    for (int j = 2; j <= ny + 1; j++)
+      for (int k = 2; k <= nz + 1; k++) {
+        size_t c = IX(d, i, j, k);
+        size_t o = IX(d, i - 1, j - 1, k - 1); /* face metric index */
+        prim_t P0 = prim_visc(U_, J, c);
+        /* x face between i-1 and i: needed for i in [3,gend+1], j,k interior */
+        if (j <= ny && k <= nz) {
+          prim_t iP = prim_visc(U_, J, c - sI);
+          prim_t Pj = prim_visc(U_, J, c + sJ), iPj = prim_visc(U_, J, c - sI + sJ);
+          prim_t jP = prim_visc(U_, J, c - sJ), ijP = prim_visc(U_, J, c - sI - sJ);
+          prim_t Pk = prim_visc(U_, J, c + 1), iPk = prim_visc(U_, J, c - sI + 1);
+          prim_t kP = prim_visc(U_, J, c - 1), ikP = prim_visc(U_, J, c - sI - 1);
+          dset_t dx = d_two(P0, iP, invXI);
+          dset_t dy = d_four(Pj, iPj, jP, ijP, inv4ET);
+          dset_t dz = d_four(Pk, iPk, kP, ikP, inv4ZT);
+          double xi[3], et[3], zt[3];
+          load_face_metric(mu, o, xi, et, zt);
+          coef_t q;
+          coefs(xi, et, zt, &q);
+          double Ux = 0.5 * (P0.U + iP.U), Vx = 0.5 * (P0.V + iP.V), Wx = 0.5 * (P0.W + iP.W);
+          double Jf = Ju[o];
+          const double *a = q.a, *dd = q.d, *e = q.e;
+          LL[0][c] = MU_L * (a[1] * dx.u + a[5] * dx.v + a[7] * dx.w + dd[1] * dy.u + dd[7] * dy.v + dd[9] * dy.w +
+                             e[1] * dz.u + e[7] * dz.v + e[9] * dz.w) / Jf;
+          LL[1][c] = MU_L * (a[5] * dx.u + a[2] * dx.v + a[6] * dx.w + dd[5] * dy.u + dd[2] * dy.v + dd[10] * dy.w +
+                             e[5] * dz.u + e[2] * dz.v + e[10] * dz.w) / Jf;
+          LL[2][c] = MU_L * (a[7] * dx.u + a[6] * dx.v + a[3] * dx.w + dd[6] * dy.u + dd[8] * dy.v + dd[3] * dy.w +
+                             e[6] * dz.u + e[8] * dz.v + e[3] * dz.w) / Jf;
+          LL[3][c] = MU_L * (0.5 * a[1] * dx.uu + 0.5 * a[2] * dx.vv + 0.5 * a[3] * dx.ww + a[5] * dx.uv +
+                             a[6] * dx.vw + a[7] * dx.uw + CVK_PR(a[4], dx.T) +
+                             0.5 * dd[1] * dy.uu + 0.5 * dd[2] * dy.vv + 0.5 * dd[3] * dy.ww + dd[5] * Vx * dy.u +
+                             dd[6] * Wx * dy.u + dd[7] * Ux * dy.v + dd[8] * Wx * dy.v + dd[9] * Ux * dy.w +
+                             dd[10] * Vx * dy.w + CVK_PR(dd[4], dy.T) +
+                             0.5 * e[1] * dz.uu + 0.5 * e[2] * dz.vv + 0.5 * e[3] * dz.ww + e[5] * Vx * dz.u +
+                             e[6] * Wx * dz.u + e[7] * Ux * dz.v + e[8] * Wx * dz.v + e[9] * Ux * dz.w +
+                             e[10] * Vx * dz.w + CVK_PR(e[4], dz.T)) / Jf;
+        }
+        /* y face between j-1 and j: i interior, j in [2,ny+1], k interior */
+        if (i <= gend && k <= nz) {
+          prim_t Pi = prim_visc(U_, J, c + sI), jPi = prim_visc(U_, J, c + sI - sJ);
+          prim_t iP = prim_visc(U_, J, c - sI), ijP = prim_visc(U_, J, c - sI - sJ);
+          prim_t jP = prim_visc(U_, J, c - sJ);
+          prim_t Pk = prim_visc(U_, J, c + 1), jPk = prim_visc(U_, J, c - sJ + 1);
+          prim_t kP = prim_visc(U_, J, c - 1), jkP = prim_visc(U_, J, c - sJ - 1);
+          dset_t dx = d_four(Pi, jPi, iP, ijP, inv4XI);
+          dset_t dy = d_two(P0, jP, invET);
+          dset_t dz = d_four(Pk, jPk, kP, jkP, inv4ZT);
+          double xi[3], et[3], zt[3];
+          load_face_metric(mv, o, xi, et, zt);
+          coef_t q;
+          coefs(xi, et, zt, &q);
+          double Uy = 0.5 * (P0.U + jP.U), Vy = 0.5 * (P0.V + jP.V), Wy = 0.5 * (P0.W + jP.W);
+          double Jf = Jv[o];
+          const double *b = q.b, *dd = q.d, *f = q.f;
+          ML[0][c] = MU_L * (dd[1] * dx.u + dd[5] * dx.v + dd[6] * dx.w + b[1] * dy.u + b[5] * dy.v + b[7] * dy.w +
+                             f[1] * dz.u + f[7] * dz.v + f[9] * dz.w) / Jf;
+          ML[1][c] = MU_L * (dd[7] * dx.u + dd[2] * dx.v + dd[8] * dx.w + b[5] * dy.u + b[2] * dy.v + b[6] * dy.w +
+                             f[5] * dz.u + f[2] * dz.v + f[10] * dz.w) / Jf;
+          ML[2][c] = MU_L * (dd[9] * dx.u + dd[10] * dx.v + dd[3] * dx.w + b[7] * dy.u + b[6] * dy.v + b[3] * dy.w +
+                             f[6] * dz.u + f[8] * dz.v + f[3] * dz.w) / Jf;
+          ML[3][c] = MU_L * (0.5 * b[1] * dy.uu + 0.5 * b[2] * dy.vv + 0.5 * b[3] * dy.ww + b[5] * dy.uv +
+                             b[6] * dy.vw + b[7] * dy.uw + CVK_PR(b[4], dy.T) +
+                             0.5 * dd[1] * dx.uu + 0.5 * dd[2] * dx.vv + 0.5 * dd[3] * dx.ww + dd[5] * Uy * dx.v +
+                             dd[6] * Uy * dx.w + dd[7] * Vy * dx.u + dd[8] * Vy * dx.w + dd[9] * Wy * dx.u +
+                             dd[10] * Wy * dx.v + CVK_PR(dd[4], dx.T) +
+                             0.5 * f[1] * dz.uu + 0.5 * f[2] * dz.vv + 0.5 * f[3] * dz.ww + f[5] * Vy * dz.u +
+                             f[6] * Wy * dz.u + f[7] * Uy * dz.v + f[8] * Wy * dz.v + f[9] * Uy * dz.w +
+                             f[10] * Vy * dz.w + CVK_PR(f[4], dz.T)) / Jf;
+        }
+        /* z face between k-1 and k: i, j interior, k in [2,nz+1] */
+        if (i <= gend && j <= ny) {
+          prim_t Pi = prim_visc(U_, J, c + sI), kPi = prim_visc(U_, J, c + sI - 1);
+          prim_t iP = prim_visc(U_, J, c - sI), ikP = prim_visc(U_, J, c - sI - 1);
+          prim_t Pj = prim_visc(U_, J, c + sJ), kPj = prim_visc(U_, J, c + sJ - 1);
+          prim_t jP = prim_visc(U_, J, c - sJ), jkP = prim_visc(U_, J, c - sJ - 1);
+          prim_t kP = prim_visc(U_, J, c - 1);
+          prim_t iPj = prim_visc(U_, J, c - sI + sJ);
+          dset_t dx = d_four(Pi, kPi, iP, ikP, inv4XI);
+          dset_t dy = d_four(Pj, kPj, jP, jkP, inv4ET);
+          /* Viscous_terms.cpp:728-730: the second product of duv/dvw/duw
+           * d/deta takes its V or W from cell (i-1,j+1) */
+          dy.uv = (Pj.U * Pj.V + kPj.U * iPj.V - jP.U * jP.V - jkP.U * jkP.V) * inv4ET;
+          dy.vw = (Pj.V * Pj.W + kPj.V * iPj.W - jP.V * jP.W - jkP.V * jkP.W) * inv4ET;
+          dy.uw = (Pj.U * Pj.W + kPj.U * iPj.W - jP.U * jP.W - jkP.U * jkP.W) * inv4ET;
+          dset_t dz = d_two(P0, kP, invZT);
+          double xi[3], et[3], zt[3];
+          load_face_metric(mw, o, xi, et, zt);
+          coef_t q;
+          coefs(xi, et, zt, &q);
+          double Uz = 0.5 * (P0.U + kP.U), Vz = 0.5 * (P0.V + kP.V), Wz = 0.5 * (P0.W + kP.W);
+          double Jf = Jw[o];
+          const double *cc = q.c, *e = q.e, *f = q.f;
+          NL[0][c] = MU_L * (e[1] * dx.u + e[5] * dx.v + e[6] * dx.w + f[1] * dy.u + f[5] * dy.v + f[6] * dy.w +
+                             cc[1] * dz.u + cc[5] * dz.v + cc[7] * dz.w) / Jf;
+          NL[1][c] = MU_L * (e[7] * dx.u + e[2] * dx.v + e[8] * dx.w + f[7] * dy.u + f[2] * dy.v + f[8] * dy.w +
+                             cc[5] * dz.u + cc[2] * dz.v + cc[6] * dz.w) / Jf;
+          NL[2][c] = MU_L * (e[9] * dx.u + e[10] * dx.v + e[3] * dx.w + f[9] * dy.u + f[10] * dy.v + f[3] * dy.w +
+                             cc[7] * dz.u + cc[6] * dz.v + cc[3] * dz.w) / Jf;
+          NL[3][c] = MU_L * (0.5 * cc[1] * dz.uu + 0.5 * cc[2] * dz.vv + 0.5 * cc[3] * dz.ww + cc[5] * dz.uv +
+                             cc[6] * dz.vw + cc[7] * dz.uw + CVK_PR(cc[4], dz.T) +
+                             0.5 * e[1] * dx.uu + 0.5 * e[2] * dx.vv + 0.5 * e[3] * dx.ww + e[5] * Uz * dx.v +
+                             e[6] * Uz * dx.w + e[7] * Vz * dx.u + e[8] * Vz * dx.w + e[9] * Wz * dx.u +
+                             e[10] * Wz * dx.v + CVK_PR(e[4], dx.T) +
+                             0.5 * f[1] * dy.uu + 0.5 * f[2] * dy.vv + 0.5 * f[3] * dy.ww + f[5] * Uz * dy.v +
+                             f[6] * Uz * dy.w + f[7] * Vz * dy.u + f[8] * Vz * dy.w + f[9] * Wz * dy.u +
+                             f[10] * Wz * dy.v + CVK_PR(f[4], dy.T)) / Jf;
+        }
+      }
+  /* divergence (Viscous_terms.cpp:849-883) */
+  for (int i = 3; i <= gend; i++)
+    for (int j = 2; j <= ny; j++)
+      for (int k = 2; k <= nz; k++) {
+        size_t c = IX(d, i, j, k);
+        for (int v = 0; v < 4; v++)
+          vF[v][c] = (LL[v][c + sI] - LL[v][c]) * invXI + (ML[v][c + sJ] - ML[v][c]) * invET +
+                     (NL[v][c + 1] - NL[v][c]) * invZT;
+      }
+  for (int v = 0; v < 4; v++) {
+    free(LL[v]);
+    free(ML[v]);
+    free(NL[v]);
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* dual time stepping: Dual_time_stepping.cpp                               */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+  double rho, u, v, w, VV, P, C, T, H, beta;
+} pstate_t;
+
+static pstate_t prim_dt(double* const U_[5], const double* J, size_t p) {
+  /* Dual_time_stepping.cpp:343-372 */
+  pstate_t s;
+  s.rho = U_[0][p] * J[p];
+  s.u = U_[1][p] / U_[0][p];
+  s.v = U_[2][p] / U_[0][p];
+  s.w = U_[3][p] / U_[0][p];
+  s.VV = s.u * s.u + s.v * s.v + s.w * s.w;
+  s.P = (U_[4][p] * J[p] - 0.5 * s.rho * s.VV) * (KG - 1);
+  s.C = KG * s.P / s.rho;
+  s.T = s.P / s.rho;
+  s.H = 0.5 * s.VV + s.C / (KG - 1);
+  s.beta = dmax(s.VV / s.C, EPS_PRE);
+  return s;
+}
+
+/* spectral radius of the preconditioned flux Jacobian along metric row n */
+static double lambda_dir(const pstate_t* s, const double n[3], double* Uc) {
+  double U = n[0] * s->u + n[1] * s->v + n[2] * s->w;
+  double S = sqrt(U * U * (s->beta - 1) * (s->beta - 1) + 4 * s->beta * s->C * (n[0] * n[0] + n[1] * n[1] + n[2] * n[2]));
+  *Uc = U;
+  return ((s->beta + 1) * fabs(U) + S) / 2;
+}
+
+/* diagonal D^-1 entries d11, d22(=d33=d44=d55), d51 at a cell
+ * (Dual_time_stepping.cpp:464-489) */
+static void dinv(const pstate_t* s, const double m9[9], const cfo_grid* g, double deltaT, double* d11, double* d22,
+                 double* d51) {
+  double U, V, W;
+  double lx = lambda_dir(s, m9 + 0, &U);
+  double ly = lambda_dir(s, m9 + 3, &V);
+  double lz = lambda_dir(s, m9 + 6, &W);
+  double lambda = lx / g->deltaXI + ly / g->deltaET + lz / g->deltaZT;
+  double beta = s->beta;
+  *d11 = 2 * deltaT / (3 * beta + 2 * lambda * deltaT);
+  *d22 = 2 * deltaT / (3 + 2 * lambda * deltaT);
+  double temp1 = 6 * (KG - 1) * (beta - 1) * deltaT;
+  double temp2 = KG * (3 + 2 * lambda * deltaT) * (3 * beta + 2 * lambda * deltaT) * s->rho;
+  *d51 = -temp1 / temp2;
+}
+
+/* split flux Jacobian of one neighbour along metric row n
+ * (Dual_time_stepping.cpp:608-632): entries 11..55 of A+ and the diagonal
+ * of A- (backward sweep, :968-972) */
+typedef struct {
+  double p11, p12, p13, p14, p21, p22, p31, p33, p41, p44, p51, p52, p53, p54, p55;
+  double m11, m22, m33, m44, m55;
+} split_t;
+
+static split_t split_jac(const pstate_t* s, const double n[3]) {
+  split_t a;
+  double U;
+  double lam = lambda_dir(s, n, &U);
+  double dd = fabs(lam);
+  double beta = s->beta, P = s->P, rho = s->rho, T = s->T;
+  a.p11 = 0.5 * (beta * U + dd);
+  a.p12 = 0.5 * P * KG * beta * n[0];
+  a.p13 = 0.5 * P * KG * beta * n[1];
+  a.p14 = 0.5 * P * KG * beta * n[2];
+  a.p21 = 0.5 * n[0] / rho;
+  a.p22 = 0.5 * (U + dd);
+  a.p31 = 0.5 * n[1] / rho;
+  a.p33 = 0.5 * (U + dd);
+  a.p41 = 0.5 * n[2] / rho;
+  a.p44 = 0.5 * (U + dd);
+  a.p51 = 0.5 * U * (KG - 1) * (beta - 1) / rho / KG;
+  a.p55 = 0.5 * (U + dd);
+  a.m11 = 0.5 * (beta * U - dd);
+  a.m22 = 0.5 * (U - dd);
+  a.m33 = 0.5 * (U - dd);
+  a.m44 = 0.5 * (U - dd);
+  a.m55 = 0.5 * (U - dd);
+  return a;
+}
+
+/* A+ entries 52..54 are written xi*beta*T*(K-1) in x (:624-626) and
+ * beta*T*(K-1)*eta in y and z (:680-682, :737-739) */
+static void split_p5(split_t* a, const pstate_t* s, const double n[3], int dir) {
+  double beta = s->beta, T = s->T;
+  if (dir == 0) {
+    a->p52 = 0.5 * n[0] * beta * T * (KG - 1);
+    a->p53 = 0.5 * n[1] * beta * T * (KG - 1);
+    a->p54 = 0.5 * n[2] * beta * T * (KG - 1);
+  } else {
+    a->p52 = 0.5 * beta * T * (KG - 1) * n[0];
+    a->p53 = 0.5 * beta * T * (KG - 1) * n[1];
+    a->p54 = 0.5 * beta * T * (KG - 1) * n[2];
+  }
+}
+
+static void load_m9(double* const m[9], size_t p, double o[9]) {
+  for (int c = 0; c < 9; c++) o[c] = m[c][p];
+}
+
+typedef struct {
+  const cfo_grid* g;
+  dims_t d;
+  double deltaT;
+  double* const* U_;
+  const double* J;
+  double* const* m;
+  double* const* MR;
+  double* const* NL;
+  double* const* NR;
+} sweep_ctx;
+
+static void fwd_cell(const sweep_ctx* x, int i, int j, int k) {
+  const dims_t d = x->d;
+  size_t c = IX(d, i, j, k);
+  size_t nb[3] = {c - (size_t)d.Ym * d.Zm, c - (size_t)d.Zm, c - 1};
+  double sum[3][5];
+  for (int dir = 0; dir < 3; dir++) {
+    pstate_t s = prim_dt(x->U_, x->J, nb[dir]);
+    double n[3] = {x->m[3 * dir][nb[dir]], x->m[3 * dir + 1][nb[dir]], x->m[3 * dir + 2][nb[dir]]};
+    split_t a = split_jac(&s, n);
+    split_p5(&a, &s, n, dir);
+    double N1 = x->NL[0][nb[dir]], N2 = x->NL[1][nb[dir]], N3 = x->NL[2][nb[dir]], N4 = x->NL[3][nb[dir]],
+           N5 = x->NL[4][nb[dir]];
+    sum[dir][0] = -a.p11 * N1 - a.p12 * N2 - a.p13 * N3 - a.p14 * N4;
+    sum[dir][1] = -a.p21 * N1 - a.p22 * N2;
+    sum[dir][2] = -a.p31 * N1 - a.p33 * N3;
+    sum[dir][3] = -a.p41 * N1 - a.p44 * N4;
+    sum[dir][4] = -a.p51 * N1 - a.p52 * N2 - a.p53 * N3 - a.p54 * N4 - a.p55 * N5;
+  }
+  double LLv[5];
+  for (int v = 0; v < 5; v++)
+    LLv[v] = sum[0][v] / x->g->deltaXI + sum[1][v] / x->g->deltaET + sum[2][v] / x->g->deltaZT;
+  pstate_t s0 = prim_dt(x->U_, x->J, c);
+  double m9[9], d11, d22, d51;
+  load_m9(x->m, c, m9);
+  dinv(&s0, m9, x->g, x->deltaT, &d11, &d22, &d51);
+  x->NL[0][c] = d11 * (x->MR[0][c] - LLv[0]);
+  x->NL[1][c] = d22 * (x->MR[1][c] - LLv[1]);
+  x->NL[2][c] = d22 * (x->MR[2][c] - LLv[2]);
+  x->NL[3][c] = d22 * (x->MR[3][c] - LLv[3]);
+  x->NL[4][c] = d51 * (x->MR[0][c] - LLv[0]) + d22 * (x->MR[4][c] - LLv[4]);
+}
+
+static void bwd_cell(const sweep_ctx* x, int i, int j, int k) {
+  const dims_t d = x->d;
+  size_t c = IX(d, i, j, k);
+  size_t nb[3] = {c + (size_t)d.Ym * d.Zm, c + (size_t)d.Zm, c + 1};
+  double sum[3][5];
+  for (int dir = 0; dir < 3; dir++) {
+    pstate_t s = prim_dt(x->U_, x->J, nb[dir]);
+    double n[3] = {x->m[3 * dir][nb[dir]], x->m[3 * dir + 1][nb[dir]], x->m[3 * dir + 2][nb[dir]]};
+    split_t a = split_jac(&s, n);
+    split_p5(&a, &s, n, dir);
+    double N1 = x->NR[0][nb[dir]], N2 = x->NR[1][nb[dir]], N3 = x->NR[2][nb[dir]], N4 = x->NR[3][nb[dir]],
+           N5 = x->NR[4][nb[dir]];
+    sum[dir][0] = +a.m11 * N1 + a.p12 * N2 + a.p13 * N3 + a.p14 * N4;
+    sum[dir][1] = +a.p21 * N1 + a.m22 * N2;
+    sum[dir][2] = +a.p31 * N1 + a.m33 * N3;
+    sum[dir][3] = +a.p41 * N1 + a.m44 * N4;
+    sum[dir][4] = +a.p51 * N1 + a.p52 * N2 + a.p53 * N3 + a.p54 * N4 + a.m55 * N5;
+  }
+  double UUv[5];
+  for (int v = 0; v < 5; v++)
+    UUv[v] = sum[0][v] / x->g->deltaXI + sum[1][v] / x->g->deltaET + sum[2][v] / x->g->deltaZT;
+  pstate_t s0 = prim_dt(x->U_, x->J, c);
+  double m9[9], d11, d22, d51;
+  load_m9(x->m, c, m9);
+  dinv(&s0, m9, x->g, x->deltaT, &d11, &d22, &d51);
+  x->NR[0][c] = x->NL[0][c] - d11 * UUv[0];
+  x->NR[1][c] = x->NL[1][c] - d22 * UUv[1];
+  x->NR[2][c] = x->NL[2][c] - d22 * UUv[2];
+  x->NR[3][c] = x->NL[3][c] - d22 * UUv[3];
+  x->NR[4][c] = x->NL[4][c] - (d51 * UUv[0] + d22 * UUv[4]);
+}
+
+static void dual_time_impl(const cfo_grid* g, double deltaT, double* const U_[5], double* const U[5],
+                           double* const Uq[5], const double* J, double* const m[9], double* const inFx[5],
+                           double* const inFy[5], double* const inFz[5], double* const vF[4],
+                           double* const MR[5], double* const NL[5], double* const NR[5], double er[5],
+                           int hyperplane) {
+  dims_t d = dims(g);
+  int gend = d.gend, ny = d.ny, nz = d.nz, nyy = ny + 1, nzz = nz + 1;
+  size_t sI = (size_t)d.Ym * d.Zm, sJ = (size_t)d.Zm;
+  /* zero implicit coupling through every ghost layer (:195-316) */
+  for (int v = 0; v < 5; v++) {
+    for (int j = 1; j <= nyy; j++)
+      for (int k = 1; k <= nzz; k++) {
+        NL[v][IX(d, 2, j, k)] = NR[v][IX(d, 2, j, k)] = 0;
+        NL[v][IX(d, gend + 1, j, k)] = NR[v][IX(d, gend + 1, j, k)] = 0;
+      }
+    for (int i = 2; i <= gend + 1; i++) {
+      for (int k = 1; k <= nzz; k++) {
+        NL[v][IX(d, i, 1, k)] = NR[v][IX(d, i, 1, k)] = 0;
+        NL[v][IX(d, i, nyy, k)] = NR[v][IX(d, i, nyy, k)] = 0;
+      }
+      for (int j = 1; j <= nyy; j++) {
+        NL[v][IX(d, i, j, 1)] = NR[v][IX(d, i, j, 1)] = 0;
+        NL[v][IX(d, i, j, nzz)] = NR[v][IX(d, i, j, nzz)] = 0;
+      }
+    }
+  }
+  /* unsteady residual, preconditioned primitive transform and D^-1 (:338-507) */
+  for (int i = 3; i <= gend; i++)
+    for (int j = 2; j <= ny; j++)
+      for (int k = 2; k <= nz; k++) {
+        size_t c = IX(d, i, j, k);
+        size_t fx = IX(d, i, j - 1, k - 1), fxm = fx - sI;   /* inFx[i][_j][_k], [i-1] */
+        size_t fy = IX(d, i - 1, j, k - 1), fym = fy - sJ;   /* inFy[i-1][j][_k], [_j] */
+        size_t fz = IX(d, i - 1, j - 1, k), fzm = fz - 1;    /* inFz[i-1][_j][k], [_k] */
+        pstate_t s = prim_dt(U_, J, c);
+        double beta = s.beta, rho = s.rho, u = s.u, v = s.v, w = s.w, VV = s.VV, T = s.T, H = s.H;
+        double temp = rho * KG;
+        double d11 = -(H * (KG - 1) + VV - KG * (T + VV)) * beta;
+        double d12 = -(KG - 1) * u * beta, d13 = -(KG - 1) * v * beta, d14 = -(KG - 1) * w * beta;
+        double d15 = (KG - 1) * beta;
+        double d21 = -u / rho, d22 = 1 / rho, d31 = -v / rho, d41 = -w / rho;
+        double d51 = (KG - 1) * (VV + KG * T * beta + (KG - 1) * VV * beta + H * (-1 + beta - KG * beta)) / temp;
+        double d52 = -(KG - 1) * u * (1 + (KG - 1) * beta) / temp;
+        double d53 = -(KG - 1) * v * (1 + (KG - 1) * beta) / temp;
+        double d54 = -(KG - 1) * w * (1 + (KG - 1) * beta) / temp;
+        double d55 = (KG - 1) * (1 + (KG - 1) * beta) / temp;
+        double Rk[5];
+        for (int q = 0; q < 5; q++) {
+          double Rp = -(3 * U_[q][c] - 4 * U[q][c] + Uq[q][c]) / (2 * deltaT);
+          double Rf = -((inFx[q][fx] - inFx[q][fxm]) / g->deltaXI + (inFy[q][fy] - inFy[q][fym]) / g->deltaET +
+                        (inFz[q][fz] - inFz[q][fzm]) / g->deltaZT);
+          Rk[q] = Rp + Rf;
+        }
+        Rk[1] = Rk[1] + vF[0][c] + g->f_drive / J[c];
+        Rk[2] = Rk[2] + vF[1][c];
+        Rk[3] = Rk[3] + vF[2][c];
+        Rk[4] = Rk[4] + vF[3][c] + g->f_drive * u / J[c];
+        /* the zero entries of Gamma^-1 are kept as explicit products so that
+         * -0/NaN propagate as in the reference */
+        MR[0][c] = (d11 * Rk[0] + d12 * Rk[1] + d13 * Rk[2] + d14 * Rk[3] + d15 * Rk[4]);
+        MR[1][c] = (d21 * Rk[0] + d22 * Rk[1] + 0 * Rk[2] + 0 * Rk[3] + 0 * Rk[4]);
+        MR[2][c] = (d31 * Rk[0] + 0 * Rk[1] + d22 * Rk[2] + 0 * Rk[3] + 0 * Rk[4]);
+        MR[3][c] = (d41 * Rk[0] + 0 * Rk[1] + 0 * Rk[2] + d22 * Rk[3] + 0 * Rk[4]);
+        MR[4][c] = (d51 * Rk[0] + d52 * Rk[1] + d53 * Rk[2] + d54 * Rk[3] + d55 * Rk[4]);
+        double m9[9], e11, e22, e51;
+        load_m9(m, c, m9);
+        dinv(&s, m9, g, deltaT, &e11, &e22, &e51);
+        NL[0][c] = e11 * MR[0][c];
+        NL[1][c] = e22 * MR[1][c];
+        NL[2][c] = e22 * MR[2][c];
+        NL[3][c] = e22 * MR[3][c];
+        NL[4][c] = e51 * MR[0][c] + e22 * MR[4][c];
+        for (int q = 0; q < 5; q++) NR[q][c] = NL[q][c];
+      }
+  /* symmetric Gauss-Seidel sweeps (:514-1277) */
+  sweep_ctx x = {g, d, deltaT, U_, J, m, MR, NL, NR};
+  int mmax = gend + ny + nz;
+  for (int sweep = 1; sweep <= g->n_sweeps; sweep++) {
+    if (!hyperplane) {
+      for (int i = 3; i <= gend; i++)
+        for (int j = 2; j <= ny; j++)
+          for (int k = 2; k <= nz; k++) fwd_cell(&x, i, j, k);
+      for (int i = gend; i >= 3; i--)
+        for (int j = ny; j >= 2; j--)
+          for (int k = nz; k >= 2; k--) bwd_cell(&x, i, j, k);
+    } else {
+      for (int mm = 7; mm <= mmax; mm++)
+        for (int i = 3; i <= gend; i++)
+          for (int j = 2; j <= ny; j++) {
+            int k = mm - i - j;
+            if (k >= 2 && k <= nz) fwd_cell(&x, i, j, k);
+          }
+      for (int mm = mmax; mm >= 7; mm--)
+        for (int i = 3; i <= gend; i++)
+          for (int j = 2; j <= ny; j++) {
+            int k = mm - i - j;
+            if (k >= 2 && k <= nz) bwd_cell(&x, i, j, k);
+          }
+    }
+    for (int q = 0; q < 5; q++)
+      for (int i = 3; i <= gend; i++)
+        for (int j = 2; j <= ny; j++)
+          for (int k = 2; k <= nz; k++) NL[q][IX(d, i, j, k)] = NR[q][IX(d, i, j, k)];
+  }
+  /* primitive update and residual norms (:1290-1352) */
+  double e[5] = {0, 0, 0, 0, 0};
+  double DN = 1. / ((double)g->X_out * g->Y_out * g->Z_out);
+  for (int i = 3; i <= gend; i++)
+    for (int j = 2; j <= ny; j++)
+      for (int k = 2; k <= nz; k++) {
+        size_t c = IX(d, i, j, k);
+        double rhoold = U_[0][c] * J[c];
+        double Uold = U_[1][c] / U_[0][c];
+        double Vold = U_[2][c] / U_[0][c];
+        double Wold = U_[3][c] / U_[0][c];
+        double VVold = Uold * Uold + Vold * Vold + Wold * Wold;
+        double Pold = (U_[4][c] * J[c] - 0.5 * rhoold * VVold) * (KG - 1);
+        double Told = Pold / rhoold;
+        double P = Pold + NR[0][c] * J[c];
+        double Uu = Uold + NR[1][c] * J[c];
+        double V = Vold + NR[2][c] * J[c];
+        double W = Wold + NR[3][c] * J[c];
+        double T = Told + NR[4][c] * J[c];
+        double rho = P / T;
+        U_[0][c] = rho / J[c];
+        U_[1][c] = rho * Uu / J[c];
+        U_[2][c] = rho * V / J[c];
+        U_[3][c] = rho * W / J[c];
+        U_[4][c] = (P / (KG - 1) + 0.5 * rho * (Uu * Uu + V * V + W * W)) / J[c];
+        double r[5] = {(P - Pold) * (P - Pold), (Uu - Uold) * (Uu - Uold), (V - Vold) * (V - Vold),
+                       (W - Wold) * (W - Wold), (T - Told) * (T - Told)};
+        for (int q = 0; q < 5; q++) e[q] = e[q] + r[q] * r[q];
+      }
+  for (int q = 0; q < 5; q++) er[q] = sqrt(e[q]) * DN;
+  (void)sJ;
+}
+
+void cfo_dual_time(const cfo_grid* g, double deltaT, double* const U_[5], double* const U[5],
+                   double* const Uq[5], const double* J, double* const m[9], double* const inFx[5],
+                   double* const inFy[5], double* const inFz[5], double* const vF[4], double* const MR[5],
+                   double* const NL[5], double* const NR[5], double er[5]) {
+  dual_time_impl(g, deltaT, U_, U, Uq, J, m, inFx, inFy, inFz, vF, MR, NL, NR, er, 0);
+}
+
+void cfo_dual_time_hyperplane(const cfo_grid* g, double deltaT, double* const U_[5], double* const U[5],
+                              double* const Uq[5], const double* J, double* const m[9], double* const inFx[5],
+                              double* const inFy[5], double* const inFz[5], double* const vF[4],
+                              double* const MR[5], double* const NL[5], double* const NR[5], double er[5]) {
+  dual_time_impl(g, deltaT, U_, U, Uq, J, m, inFx, inFy, inFz, vF, MR, NL, NR, er, 1);
+}

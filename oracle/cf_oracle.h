@@ -1,0 +1,75 @@
+/* CPU oracle for the channel-flow dual-time hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * solver's arithmetic (ricklee209/Channel_Flow), used by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg as the checker.
+ * The product (channel_flow_amd, libchannel_flow.so) never links or calls it.
+ *
+ * Arrays use the reference layout: one FP64 array per variable,
+ * [X_np][Y_m][Z_m], k fastest (Array.h), single rank: X_np = X_out+6,
+ * Y_m = Y_out+4, Z_m = Z_out+4, interior i in [3,gend], j in [2,ny],
+ * k in [2,nz] with gend = X_out+2, ny = Y_out+1, nz = Z_out+1.
+ *
+ * Parity status per function (what pins it is listed in oracle/README.md):
+ *   cfo_metrics, cfo_halo_bc, cfo_flux, cfo_viscous, cfo_dual_time,
+ *   cfo_hyperplane_counts, cfo_inner_iteration: pinned by golden vectors from
+ *   the reference built in oracle/_ref plus closed-form checks.
+ */
+#ifndef CF_ORACLE_H
+#define CF_ORACLE_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int X_out, Y_out, Z_out;     /* Resolution.h:5-7 */
+  double deltaXI, deltaET, deltaZT; /* Resolution.h:17-19 */
+  double high;                 /* prm.h:18 */
+  double f_drive;              /* prm.h:13 */
+  int x_periodic;              /* 0: single-rank self copy (main.cpp:207-238) */
+  int n_sweeps;                /* Dual_time_stepping.cpp:514 (2) */
+} cfo_grid;
+
+/* Jacobian.cpp, Jacobian_u.cpp, Jacobian_v.cpp, Jacobian_w.cpp.
+ * m[9] = xidx,xidy,xidz,etdx,etdy,etdz,ztdx,ztdy,ztdz (same for faces). */
+void cfo_metrics(const cfo_grid* g, double gamma, double* J, double* const m[9], double* Ju,
+                 double* const mu[9], double* Jv, double* const mv[9], double* Jw, double* const mw[9]);
+
+/* Hyperplane.cpp:50-78: number of interior cells on plane m = i+j+k,
+ * counts[m - 7] for m = 7 .. gend+ny+nz. */
+int cfo_hyperplane_counts(const cfo_grid* g, int* counts, int ncounts);
+
+/* main.cpp:156-240 (single rank) + Y_boundary_condition.cpp:48-120 +
+ * Z_boundary_condition.cpp:48-66 */
+void cfo_halo_bc(const cfo_grid* g, double* const U_[5], const double* J);
+
+/* Flux_X.cpp / Flux_Y.cpp / Flux_Z.cpp (ROE == 1), dir 0,1,2 */
+void cfo_flux(const cfo_grid* g, int dir, double* const U_[5], const double* J, const double* Jf,
+              double* const mf[9], double* const inF[5]);
+
+/* Viscous_terms.cpp:212-883; writes vF[0..3] = vF2..vF5 */
+void cfo_viscous(const cfo_grid* g, double* const U_[5], const double* J, const double* Ju,
+                 double* const mu[9], const double* Jv, double* const mv[9], const double* Jw,
+                 double* const mw[9], double* const vF[4]);
+
+/* Dual_time_stepping.cpp:195-1361: one inner iteration's implicit part.
+ * Updates U_ in place, fills MR/NL/NR like the reference, er[5]. */
+void cfo_dual_time(const cfo_grid* g, double deltaT, double* const U_[5], double* const U[5],
+                   double* const Uq[5], const double* J, double* const m[9], double* const inFx[5],
+                   double* const inFy[5], double* const inFz[5], double* const vF[4], double* const MR[5],
+                   double* const NL[5], double* const NR[5], double er[5]);
+
+/* Same, with the LU-SGS sweeps visited plane by plane (i+j+k = m ascending /
+ * descending) instead of lexicographically: identical arithmetic, used to
+ * show that the visiting order inside the dependency DAG does not matter. */
+void cfo_dual_time_hyperplane(const cfo_grid* g, double deltaT, double* const U_[5], double* const U[5],
+                              double* const Uq[5], const double* J, double* const m[9], double* const inFx[5],
+                              double* const inFy[5], double* const inFz[5], double* const vF[4],
+                              double* const MR[5], double* const NL[5], double* const NR[5], double er[5]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

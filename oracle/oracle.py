@@ -1,0 +1,141 @@
+"""ctypes bindings of the CPU oracle (oracle/cf_oracle.c) -- TEST INFRASTRUCTURE.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, and only as the checker.  Arrays are numpy float64 in the
+reference layout [X_np][Y_m][Z_m] (Array.h), k fastest.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "_build" / "libcforacle.so"
+
+
+class Grid(C.Structure):
+    _fields_ = [
+        ("X_out", C.c_int), ("Y_out", C.c_int), ("Z_out", C.c_int),
+        ("deltaXI", C.c_double), ("deltaET", C.c_double), ("deltaZT", C.c_double),
+        ("high", C.c_double), ("f_drive", C.c_double),
+        ("x_periodic", C.c_int), ("n_sweeps", C.c_int),
+    ]
+
+
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        _lib = C.CDLL(str(LIB))
+        P = C.POINTER(C.c_double)
+        PP = C.POINTER(P)
+        _lib.cfo_metrics.argtypes = [C.POINTER(Grid), C.c_double, P, PP, P, PP, P, PP, P, PP]
+        _lib.cfo_hyperplane_counts.argtypes = [C.POINTER(Grid), C.POINTER(C.c_int), C.c_int]
+        _lib.cfo_hyperplane_counts.restype = C.c_int
+        _lib.cfo_halo_bc.argtypes = [C.POINTER(Grid), PP, P]
+        _lib.cfo_flux.argtypes = [C.POINTER(Grid), C.c_int, PP, P, P, PP, PP]
+        _lib.cfo_viscous.argtypes = [C.POINTER(Grid), PP, P, P, PP, P, PP, P, PP, PP]
+        dt_args = [C.POINTER(Grid), C.c_double, PP, PP, PP, P, PP, PP, PP, PP, PP, PP, PP, PP, P]
+        _lib.cfo_dual_time.argtypes = dt_args
+        _lib.cfo_dual_time_hyperplane.argtypes = dt_args
+    return _lib
+
+
+def grid_of(cfg) -> Grid:
+    return Grid(cfg.X_out, cfg.Y_out, cfg.Z_out, cfg.deltaXI, cfg.deltaET, cfg.deltaZT,
+                cfg.high, cfg.f_drive, int(cfg.x_periodic), int(cfg.n_sweeps))
+
+
+def _p(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _pp(arrs):
+    arr = (C.POINTER(C.c_double) * len(arrs))(*[_p(a) for a in arrs])
+    return arr
+
+
+def zeros(cfg, n=None):
+    if n is None:
+        return np.zeros(cfg.shape)
+    return [np.zeros(cfg.shape) for _ in range(n)]
+
+
+def metrics(cfg):
+    """Return dict with J, m(9), J_u, m_u, J_v, m_v, J_w, m_w."""
+    g = grid_of(cfg)
+    J, Ju, Jv, Jw = zeros(cfg, 4)
+    m, mu, mv, mw = zeros(cfg, 9), zeros(cfg, 9), zeros(cfg, 9), zeros(cfg, 9)
+    lib().cfo_metrics(C.byref(g), cfg.gamma_metric, _p(J), _pp(m), _p(Ju), _pp(mu), _p(Jv), _pp(mv),
+                      _p(Jw), _pp(mw))
+    return dict(J=J, m=m, J_u=Ju, m_u=mu, J_v=Jv, m_v=mv, J_w=Jw, m_w=mw)
+
+
+def hyperplane_counts(cfg):
+    g = grid_of(cfg)
+    n = cfg.gend + cfg.ny + cfg.nz - 6
+    out = (C.c_int * n)()
+    r = lib().cfo_hyperplane_counts(C.byref(g), out, n)
+    assert r == n
+    return np.array(out[:], dtype=np.int64)
+
+
+def halo_bc(cfg, U_, J):
+    lib().cfo_halo_bc(C.byref(grid_of(cfg)), _pp(U_), _p(J))
+
+
+def flux(cfg, direction, U_, J, Jf, mf):
+    out = zeros(cfg, 5)
+    lib().cfo_flux(C.byref(grid_of(cfg)), int(direction), _pp(U_), _p(J), _p(Jf), _pp(mf), _pp(out))
+    return out
+
+
+def viscous(cfg, U_, met):
+    out = zeros(cfg, 4)
+    lib().cfo_viscous(C.byref(grid_of(cfg)), _pp(U_), _p(met["J"]), _p(met["J_u"]), _pp(met["m_u"]),
+                      _p(met["J_v"]), _pp(met["m_v"]), _p(met["J_w"]), _pp(met["m_w"]), _pp(out))
+    return out
+
+
+def dual_time(cfg, U_, U, Uq, met, inFx, inFy, inFz, vF, hyperplane=False, deltaT=None):
+    """Updates U_ in place; returns (er[5], MR, NL, NR)."""
+    MR, NL, NR = zeros(cfg, 5), zeros(cfg, 5), zeros(cfg, 5)
+    er = (C.c_double * 5)()
+    fn = lib().cfo_dual_time_hyperplane if hyperplane else lib().cfo_dual_time
+    fn(C.byref(grid_of(cfg)), cfg.deltaT if deltaT is None else deltaT, _pp(U_), _pp(U), _pp(Uq),
+       _p(met["J"]), _pp(met["m"]), _pp(inFx), _pp(inFy), _pp(inFz), _pp(vF), _pp(MR), _pp(NL), _pp(NR), er)
+    return np.array(er[:]), MR, NL, NR
+
+
+def inner_iteration(cfg, U_, U, Uq, met):
+    """One inner iteration of the dual-time path (main.cpp:156-353 with the DT
+    calls enabled): ghosts, three fluxes, viscous terms, dual time step."""
+    halo_bc(cfg, U_, met["J"])
+    inFx = flux(cfg, 0, U_, met["J"], met["J_u"], met["m_u"])
+    inFy = flux(cfg, 1, U_, met["J"], met["J_v"], met["m_v"])
+    inFz = flux(cfg, 2, U_, met["J"], met["J_w"], met["m_w"])
+    vF = viscous(cfg, U_, met)
+    er, MR, NL, NR = dual_time(cfg, U_, U, Uq, met, inFx, inFy, inFz, vF)
+    return er
+
+
+def state_from_Q(cfg, Q, J):
+    """Interior Q (5,X,Y,Z) -> reference-layout U = Q/J (Initial_condition.cpp:100-110)."""
+    U = zeros(cfg, 5)
+    for v in range(5):
+        U[v][3:cfg.gend + 1, 2:cfg.ny + 1, 2:cfg.nz + 1] = Q[v] / J[3:cfg.gend + 1, 2:cfg.ny + 1, 2:cfg.nz + 1]
+    return U
