@@ -1,0 +1,186 @@
+"""Benchmark of the dual-time inner iteration on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+One step = one inner iteration of the dual-time path (ghosts, Flux_X/Y/Z,
+Viscous_terms, Dual_time_stepping with its LU-SGS sweeps, update and norms)
+on the 256x192x256 channel (FP64, tanh stretching beta=2.0, Re_b=5600,
+Mach 0.2, seeded laminar Poiseuille + 5% noise), state resident in HBM.
+value = Mcell-iter/s = cells * K / wall seconds.  Also reported: per-stage
+HIP-event times with the roofline of the dominant stage, the wall time of
+one physical step run to a 1e-8 residual drop, and the reference CPU code
+timed on this host (bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+GRID = (256, 192, 256)
+HBM_PEAK_GBS = 8000.0
+# algorithmic FP64 fields read + written per interior cell (geometry excluded,
+# each field counted once): fluxes U_ 5 -> F 5; viscous U_ 5 -> vF 4; RHS
+# U_,U,Uq,inF x3,vF -> MR, NL; each sweep direction per sweep 20; update 15
+FIELDS = {"flux_x": 10, "flux_y": 10, "flux_z": 10, "viscous": 9, "rhs": 44, "lusgs_sweeps": 40,
+          "update_norms": 15, "ghosts": 0}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(cfg, seconds_budget=30.0):
+    """The reference solver (oracle/_ref, built from its own sources) timed on
+    this host's cores for a few inner iterations of the same grid."""
+    exe = ROOT / "oracle" / "_ref" / f"g{GRID[0]}x{GRID[1]}x{GRID[2]}" / "ref_driver"
+    if not exe.exists():
+        return None
+    from channel_flow_amd.config import FlowParams, poiseuille_state, write_initial_bin
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    nit = 3
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as td:
+        Q = poiseuille_state(cfg, FlowParams(), seed=1, stream=0)
+        write_initial_bin(Path(td) / "initial.bin", Q, Q)
+        del Q
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
+        log(f"cpu baseline: reference ref_driver, {threads} threads, {nit} inner iterations")
+        r = subprocess.run([str(exe), "time", td, "5e-8", str(nit)], env=env, capture_output=True, text=True,
+                           timeout=600)
+        if r.returncode != 0:
+            log(f"cpu baseline failed: {r.stderr[-500:]}")
+            return None
+        line = [ln for ln in r.stdout.splitlines() if ln.startswith("{")][-1]
+        rec = json.loads(line)
+    mcell = cfg.cells * rec["inner_iterations"] / rec["seconds"] / 1e6
+    cpu_model = ""
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                cpu_model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": round(mcell, 4), "unit": "Mcell-iter/s", "cores": rec["threads"], "kind": "reference",
+            "sample": f"{rec['inner_iterations']} inner iterations of the reference Dual_time_stepping path at "
+                      f"{GRID[0]}x{GRID[1]}x{GRID[2]} (OpenMP, {rec['seconds']:.1f} s, {cpu_model})"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-converge", action="store_true", help="skip the physical step to 1e-8")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from channel_flow_amd.config import ChannelConfig, FlowParams, poiseuille_state
+    from channel_flow_amd.solver import Solver
+
+    torch.cuda.set_device(0)
+    flow = FlowParams(Re_bulk=5600.0, Mach=0.2)
+    cfg = ChannelConfig(*GRID, gamma_metric=2.0, gamma_grid=2.0, deltaT=5e-8,
+                        f_drive=flow.driving_force(0.005), x_periodic=False)
+    Q = poiseuille_state(cfg, flow, seed=1, stream=0)
+    log(f"grid {GRID}, cells {cfg.cells}")
+    s = Solver(cfg)
+    s.Jacobian_all(cfg.gamma_metric, want_arrays=False)
+    s.load_state(Q, Q)
+    del Q
+    for _ in range(args.warmup):
+        s.inner_iteration()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    er = None
+    for _ in range(args.steps):
+        er = s.inner_iteration()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_per_step = dt / args.steps * 1e3
+    value = cfg.cells * args.steps / dt / 1e6
+    log(f"timed {args.steps} inner iterations: {ms_per_step:.3f} ms/iter, {value:.1f} Mcell-iter/s, er={er}")
+
+    # per-stage HIP-event times (events on the context's stream)
+    stages = {}
+    for _ in range(3):
+        _, st = s.profile_inner_iteration()
+        for k, (ms, nl) in st.items():
+            stages.setdefault(k, []).append((ms, nl))
+    stage_ms = {k: float(np.median([m for m, _ in v])) for k, v in stages.items()}
+    launches = {k: v[0][1] for k, v in stages.items()}
+    dom = max(stage_ms, key=stage_ms.get)
+    per_launch_ms = stage_ms[dom] / launches[dom]
+    bytes_total = FIELDS[dom] * 8 * cfg.cells
+    bytes_per_launch = bytes_total / launches[dom]
+    achieved = bytes_per_launch / (per_launch_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "launches_per_iteration": launches[dom], "avg_launch_us": round(per_launch_ms * 1e3, 3),
+                "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+    per_stage = {k: {"ms": round(v, 4), "launches": launches[k],
+                     "GBps_algorithmic": (round(FIELDS[k] * 8 * cfg.cells / (v * 1e-3) / 1e9, 1)
+                                          if FIELDS[k] else None)} for k, v in stage_ms.items()}
+    log(f"stages: {json.dumps(per_stage)}")
+
+    conv = None
+    if not args.no_converge:
+        # one physical step with inner iterations until max_k er_k/er_k(1) <= 1e-8 (k = 1..4)
+        s.advance_levels()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        first = None
+        its = 0
+        ratio = 1.0
+        cap = 400
+        while its < cap:
+            e = s.inner_iteration()
+            its += 1
+            if first is None:
+                first = e.copy()
+            ratio = float(np.max(e[:4] / first[:4]))
+            if ratio <= 1e-8:
+                break
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t1
+        conv = {"wall_s": round(wall, 4), "inner_iterations": its, "residual_drop": ratio,
+                "reached_1e-8": ratio <= 1e-8, "cap": cap}
+        log(f"physical step to 1e-8: {conv}")
+    s.close()
+
+    cpu = None if args.no_cpu else cpu_baseline(cfg)
+    out = {
+        "metric": "Mcell-iter/s at 256x192x256 + wall-time/phys step to 1e-8 residual drop",
+        "value": round(value, 3), "unit": "Mcell-iter/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: seeded laminar Poiseuille + 5% uniform velocity noise",
+        "config": {"workload": "plain FP64 dual-time inner iteration, 256x192x256 channel "
+                               "(2pi x 2 x pi, tanh beta=2.0, Re_b=5600, Mach 0.2, dt=5e-8)",
+                   "grid": "x".join(map(str, GRID)), "cells": cfg.cells, "n_sweeps": cfg.n_sweeps,
+                   "parallelism": "single GPU"},
+        "phys_step_to_1e-8": conv,
+        "stages_ms": per_stage,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

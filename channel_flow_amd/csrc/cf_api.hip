@@ -1,0 +1,658 @@
+// C ABI of the channel-flow dual-time path (include/channel_flow.h).
+//
+// Host side of the MI355X build: owns one GPU's device memory for a runtime
+// grid, moves reference-layout host arrays in and out, and sequences the
+// kernels of one inner iteration on the context's stream.  Replaces the
+// driver in main.cpp:143-498 and the call surface of main.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/channel_flow.h"
+#include "cf_kernels.hpp"
+
+using cf::Geo;
+
+namespace {
+constexpr int NPROF = 40;  // 4 sets x (J + 9 metrics)
+}
+
+struct cf_ctx {
+  cf_params p{};
+  Geo g{};
+  hipStream_t stream = nullptr;
+  int device = 0;
+  // device fields (each nf doubles, padded layout)
+  double* Ui = nullptr;   // U_ : inner iterate (5)
+  double* Un = nullptr;   // U  : level n (5)
+  double* Uq = nullptr;   // Uq : level n-1 (5)
+  double* Fx = nullptr;   // inFx (5)
+  double* Fy = nullptr;   // inFy (5)
+  double* Fz = nullptr;   // inFz (5)
+  double* vF = nullptr;   // vF2..vF5 (4)
+  double* MR = nullptr;   // (5)
+  double* NL = nullptr;   // (5)
+  double* NR = nullptr;   // (5)
+  double* Pr = nullptr;   // viscous primitives U,V,W,T (4)
+  double* VS = nullptr;   // viscous face fluxes LL/ML/NL (12)
+  double* partial = nullptr;
+  double* er_dev = nullptr;
+  double* prof = nullptr;  // metric j-profiles (NPROF * Ym)
+  std::vector<double> hprof;
+  std::vector<double> staging;  // reference-layout staging for 2D copies
+  bool have_metrics = false, have_state = false;
+  double E[5] = {0, 0, 0, 0, 0};  // running maxima of er (main.cpp:63-67)
+  std::string err;
+};
+
+static int fail(cf_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPCHK(c, expr)                                                                    \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess) return fail((c), CF_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+static long host_elems(const cf_ctx* c) { return (long)c->g.Xnp * c->g.Ym * c->g.Zm; }
+
+// reference layout (Z_m pitch) <-> device layout (Zp pitch), one field
+static int h2d(cf_ctx* c, double* dst, const double* src) {
+  HIPCHK(c, hipMemcpy2DAsync(dst, (size_t)c->g.Zp * 8, src, (size_t)c->g.Zm * 8, (size_t)c->g.Zm * 8,
+                             (size_t)c->g.Xnp * c->g.Ym, hipMemcpyHostToDevice, c->stream));
+  return CF_OK;
+}
+static int d2h(cf_ctx* c, double* dst, const double* src) {
+  HIPCHK(c, hipMemcpy2DAsync(dst, (size_t)c->g.Zm * 8, src, (size_t)c->g.Zp * 8, (size_t)c->g.Zm * 8,
+                             (size_t)c->g.Xnp * c->g.Ym, hipMemcpyDeviceToHost, c->stream));
+  return CF_OK;
+}
+static int h2d_n(cf_ctx* c, double* dst, const double* const* src, int n) {
+  for (int v = 0; v < n; v++) {
+    if (!src[v]) return fail(c, CF_ERR_ARG, "null input array %d", v);
+    int r = h2d(c, dst + v * c->g.nf, src[v]);
+    if (r) return r;
+  }
+  return CF_OK;
+}
+static int d2h_n(cf_ctx* c, double* const* dst, const double* src, int n) {
+  for (int v = 0; v < n; v++) {
+    if (!dst[v]) continue;
+    int r = d2h(c, dst[v], src + v * c->g.nf);
+    if (r) return r;
+  }
+  return CF_OK;
+}
+static int sync(cf_ctx* c) {
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipGetLastError());
+  return CF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// metric j-profiles: the reference metrics (Jacobian*.cpp) depend only on j.
+// A 3-D array is reduced to its j-profile after checking that it is constant
+// along i and k over the region where the kernels read it.
+// ---------------------------------------------------------------------------
+struct Region {
+  int i0, i1, j0, j1, k0, k1;
+};
+
+static Region region_of(const cf_ctx* c, int set, int comp) {
+  const Geo& g = c->g;
+  int gend = g.gend, ny = g.ny, nz = g.nz;
+  switch (set) {
+    case cf::SET_C:
+      if (comp == 0) return {2, gend + 1, 1, ny + 1, 1, nz + 1};
+      return {3, gend, 2, ny, 2, nz};
+    case cf::SET_U: return {2, gend, 1, ny - 1, 1, nz - 1};
+    case cf::SET_V: return {2, gend - 1, 1, ny, 1, nz - 1};
+    default: return {2, gend - 1, 1, ny - 1, 1, nz};
+  }
+}
+
+static int ingest(cf_ctx* c, const double* A, int set, int comp, const char* name) {
+  if (!A) return fail(c, CF_ERR_ARG, "null metric array %s", name);
+  const Geo& g = c->g;
+  Region r = region_of(c, set, comp);
+  double* prof = &c->hprof[(size_t)(set * 10 + comp) * g.Ym];
+  for (int j = r.j0; j <= r.j1; j++) {
+    double v0 = A[((size_t)r.i0 * g.Ym + j) * g.Zm + r.k0];
+    for (int i = r.i0; i <= r.i1; i++)
+      for (int k = r.k0; k <= r.k1; k++) {
+        double v = A[((size_t)i * g.Ym + j) * g.Zm + k];
+        if (!(v == v0 || (std::isnan(v) && std::isnan(v0))))
+          return fail(c, CF_ERR_METRICS, "%s varies along i/k at (%d,%d,%d): %.17g vs %.17g", name, i, j, k, v, v0);
+      }
+    prof[j] = v0;
+  }
+  return CF_OK;
+}
+
+static int upload_prof(cf_ctx* c) {
+  HIPCHK(c, hipMemcpyAsync(c->prof, c->hprof.data(), c->hprof.size() * 8, hipMemcpyHostToDevice, c->stream));
+  return CF_OK;
+}
+
+static int ingest_set(cf_ctx* c, int set, const double* J, const double* const m[9]) {
+  static const char* nm[9] = {"xidx", "xidy", "xidz", "etdx", "etdy", "etdz", "ztdx", "ztdy", "ztdz"};
+  int r = ingest(c, J, set, 0, "J");
+  if (r) return r;
+  if (m)
+    for (int q = 0; q < 9; q++) {
+      r = ingest(c, m[q], set, 1 + q, nm[q]);
+      if (r) return r;
+    }
+  return CF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+extern "C" int cf_create(const cf_params* p, cf_ctx** out) {
+  if (!p || !out) return CF_ERR_ARG;
+  *out = nullptr;
+  if (p->X_out < 8 || p->Y_out < 8 || p->Z_out < 8 || p->X_out > 1000 || p->Y_out > 1000 || p->Z_out > 1000)
+    return CF_ERR_ARG;
+  if (!(p->deltaXI > 0) || !(p->deltaET > 0) || !(p->deltaZT > 0) || !(p->deltaT > 0)) return CF_ERR_ARG;
+  if (p->n_sweeps < 1 || p->iteration_end_step < 1) return CF_ERR_ARG;
+  if (p->precision != CF_FP64) return CF_ERR_ARG;
+  cf_ctx* c = new cf_ctx();
+  c->p = *p;
+  Geo& g = c->g;
+  g.Xnp = p->X_out + 6;
+  g.Ym = p->Y_out + 4;
+  g.Zm = p->Z_out + 4;
+  g.Zp = (g.Zm + 7) / 8 * 8;
+  g.gend = p->X_out + 2;
+  g.ny = p->Y_out + 1;
+  g.nz = p->Z_out + 1;
+  g.plane = (long)g.Ym * g.Zp;
+  g.nf = (long)g.Xnp * g.plane;
+  g.dXI = p->deltaXI;
+  g.dET = p->deltaET;
+  g.dZT = p->deltaZT;
+  c->hprof.assign((size_t)NPROF * g.Ym, 0.0);
+  auto bail = [&](int code) {
+    cf_destroy(c);
+    return code;
+  };
+  if (hipGetDevice(&c->device) != hipSuccess) return bail(CF_ERR_HIP);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(CF_ERR_HIP);
+  size_t nf = (size_t)g.nf;
+  struct {
+    double** ptr;
+    int n;
+  } fields[] = {{&c->Ui, 5}, {&c->Un, 5}, {&c->Uq, 5}, {&c->Fx, 5}, {&c->Fy, 5}, {&c->Fz, 5},
+                {&c->vF, 4}, {&c->MR, 5}, {&c->NL, 5}, {&c->NR, 5}, {&c->Pr, 4}, {&c->VS, 12}};
+  for (auto& f : fields) {
+    if (hipMalloc(f.ptr, nf * f.n * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
+    // the reference leaves work arrays uninitialised; here they start at zero
+    if (hipMemsetAsync(*f.ptr, 0, nf * f.n * sizeof(double), c->stream) != hipSuccess) return bail(CF_ERR_HIP);
+  }
+  long nb = cf::update_blocks(g);
+  if (hipMalloc(&c->partial, (size_t)nb * 5 * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
+  if (hipMalloc(&c->er_dev, 8 * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
+  if (hipMalloc(&c->prof, c->hprof.size() * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
+  if (hipMemsetAsync(c->prof, 0, c->hprof.size() * sizeof(double), c->stream) != hipSuccess) return bail(CF_ERR_HIP);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(CF_ERR_HIP);
+  *out = c;
+  return CF_OK;
+}
+
+extern "C" void cf_destroy(cf_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  double* ptrs[] = {c->Ui, c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->MR, c->NL,
+                    c->NR, c->Pr, c->VS, c->partial, c->er_dev, c->prof};
+  for (double* q : ptrs)
+    if (q) (void)hipFree(q);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+extern "C" const char* cf_last_error(const cf_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+extern "C" long long cf_field_elems(const cf_ctx* c) { return c ? host_elems(c) : 0; }
+
+extern "C" int cf_synchronize(cf_ctx* c) {
+  if (!c) return CF_ERR_ARG;
+  return sync(c);
+}
+
+// ---------------------------------------------------------------------------
+// geometry
+// ---------------------------------------------------------------------------
+// Jacobian.cpp:74-99 for one wall-normal position eta = jpos*deltaET
+static void stretch(double high, double gamma, double deltaET, double jpos, double* etdy, double* J) {
+  double half = 0.5 * high;
+  double a = gamma * (1 - 2 * jpos * deltaET);
+  double temp = half * (1 - 1. / tanh(gamma) * tanh(a));
+  double temp1 = (1 - temp / half) * tanh(gamma);
+  *etdy = tanh(gamma) / (2 * half * gamma * (1 - temp1 * temp1));
+  double temp2 = 1 / cosh(a);
+  double ydet = high * gamma * 1. / tanh(gamma) * temp2 * temp2;
+  *J = 1. / (1 * ydet * 1);
+}
+
+extern "C" int cf_jacobian_all(cf_ctx* c, double gamma, double* J, double* const m[9], double* J_u,
+                               double* const m_u[9], double* J_v, double* const m_v[9], double* J_w,
+                               double* const m_w[9]) {
+  if (!c || !(gamma > 0)) return fail(c, CF_ERR_ARG, "bad gamma");
+  const Geo& g = c->g;
+  int gend = g.gend, ny = g.ny, nz = g.nz, Ym = g.Ym;
+  std::fill(c->hprof.begin(), c->hprof.end(), 0.0);
+  auto P = [&](int set, int comp) { return &c->hprof[(size_t)(set * 10 + comp) * Ym]; };
+  const double cart[9] = {1, 0, 0, 0, 0, 0, 0, 0, 1};
+  // cell centres j in [2,ny]: (j-1.5); ghosts j=1, nyy copy the first interior
+  for (int j = 2; j <= ny; j++) {
+    double ey, jj;
+    stretch(c->p.high, gamma, g.dET, j - 1.5, &ey, &jj);
+    P(cf::SET_C, 0)[j] = jj;
+    for (int q = 0; q < 9; q++) P(cf::SET_C, 1 + q)[j] = cart[q];
+    P(cf::SET_C, 5)[j] = ey;
+  }
+  for (int q = 0; q < 10; q++) {
+    P(cf::SET_C, q)[1] = P(cf::SET_C, q)[2];
+    P(cf::SET_C, q)[ny + 1] = P(cf::SET_C, q)[ny];
+  }
+  // faces: x and z at (j-0.5) for j in [1,ny-1]; y at (j-1.0) for j in [1,ny]
+  for (int j = 1; j <= ny; j++) {
+    double ey, jj;
+    if (j <= ny - 1) {
+      stretch(c->p.high, gamma, g.dET, j - 0.5, &ey, &jj);
+      for (int set : {cf::SET_U, cf::SET_W}) {
+        P(set, 0)[j] = jj;
+        for (int q = 0; q < 9; q++) P(set, 1 + q)[j] = cart[q];
+        P(set, 5)[j] = ey;
+      }
+    }
+    stretch(c->p.high, gamma, g.dET, j - 1.0, &ey, &jj);
+    P(cf::SET_V, 0)[j] = jj;
+    for (int q = 0; q < 9; q++) P(cf::SET_V, 1 + q)[j] = cart[q];
+    P(cf::SET_V, 5)[j] = ey;
+  }
+  // wall faces take the cell-centre ghost values (Jacobian_v.cpp:124-141)
+  for (int q = 0; q < 4; q++) {
+    int comp = q == 0 ? 0 : 3 + q;  // J, etdx, etdy, etdz
+    P(cf::SET_V, comp)[1] = P(cf::SET_C, comp)[1];
+    P(cf::SET_V, comp)[ny] = P(cf::SET_C, comp)[ny + 1];
+  }
+  c->have_metrics = true;
+  int r = upload_prof(c);
+  if (r) return r;
+  // optional reference-layout expansion, ghost copies as Jacobian.cpp:111-244
+  auto fill = [&](double* A, int set, int comp, Region rg) {
+    if (!A) return;
+    const double* pr = P(set, comp);
+    for (int i = rg.i0; i <= rg.i1; i++)
+      for (int j = rg.j0; j <= rg.j1; j++)
+        for (int k = rg.k0; k <= rg.k1; k++) A[((size_t)i * Ym + j) * g.Zm + k] = pr[j];
+  };
+  Region cellall{2, gend + 1, 1, ny + 1, 1, nz + 1};
+  fill(J, cf::SET_C, 0, cellall);
+  if (m)
+    for (int q = 0; q < 9; q++) {
+      if (!m[q]) continue;
+      if (q < 7) {
+        fill(m[q], cf::SET_C, 1 + q, cellall);
+      } else {
+        // ztdy/ztdz: interior, y and z ghosts take ztdx (0); x-ghost planes
+        // are left untouched as in the reference
+        fill(m[q], cf::SET_C, 1 + q, Region{3, gend, 2, ny, 2, nz});
+        for (int i = 2; i <= gend + 1; i++)
+          for (int j = 1; j <= ny + 1; j++)
+            for (int k = 1; k <= nz + 1; k++) {
+              bool ghost = j == 1 || j == ny + 1 || k == 1 || k == nz + 1;
+              if (ghost) m[q][((size_t)i * Ym + j) * g.Zm + k] = P(cf::SET_C, 7)[j];
+            }
+      }
+    }
+  fill(J_u, cf::SET_U, 0, region_of(c, cf::SET_U, 0));
+  fill(J_v, cf::SET_V, 0, region_of(c, cf::SET_V, 0));
+  fill(J_w, cf::SET_W, 0, region_of(c, cf::SET_W, 0));
+  for (int q = 0; q < 9; q++) {
+    if (m_u && m_u[q]) fill(m_u[q], cf::SET_U, 1 + q, region_of(c, cf::SET_U, 0));
+    if (m_v && m_v[q]) fill(m_v[q], cf::SET_V, 1 + q, region_of(c, cf::SET_V, 0));
+    if (m_w && m_w[q]) fill(m_w[q], cf::SET_W, 1 + q, region_of(c, cf::SET_W, 0));
+  }
+  return sync(c);
+}
+
+extern "C" int cf_set_metrics(cf_ctx* c, const double* J, const double* const m[9], const double* J_u,
+                              const double* const m_u[9], const double* J_v, const double* const m_v[9],
+                              const double* J_w, const double* const m_w[9]) {
+  if (!c) return CF_ERR_ARG;
+  int r;
+  if ((r = ingest_set(c, cf::SET_C, J, m))) return r;
+  if ((r = ingest_set(c, cf::SET_U, J_u, m_u))) return r;
+  if ((r = ingest_set(c, cf::SET_V, J_v, m_v))) return r;
+  if ((r = ingest_set(c, cf::SET_W, J_w, m_w))) return r;
+  c->have_metrics = true;
+  if ((r = upload_prof(c))) return r;
+  return sync(c);
+}
+
+extern "C" int cf_hyperplane(cf_ctx* c, int* counts, int capacity, int* nplanes) {
+  if (!c || !counts || !nplanes) return fail(c, CF_ERR_ARG, "null argument");
+  const Geo& g = c->g;
+  int mmax = g.gend + g.ny + g.nz, n = mmax - 7 + 1;
+  *nplanes = n;
+  if (capacity < n) return fail(c, CF_ERR_ARG, "capacity %d < %d planes", capacity, n);
+  // plane m holds the (i,j) pairs with 2 <= m-i-j <= nz: O(planes * X) count
+  for (int m = 7; m <= mmax; m++) {
+    long cnt = 0;
+    for (int i = 3; i <= g.gend; i++) {
+      int jlo = std::max(2, m - i - g.nz), jhi = std::min(g.ny, m - i - 2);
+      if (jhi >= jlo) cnt += jhi - jlo + 1;
+    }
+    counts[m - 7] = (int)cnt;
+  }
+  return CF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// per-function entry points
+// ---------------------------------------------------------------------------
+extern "C" int cf_boundary_conditions(cf_ctx* c, double* const U_[5], const double* J) {
+  if (!c || !U_) return CF_ERR_ARG;
+  int r;
+  if ((r = ingest(c, J, cf::SET_C, 0, "J")) || (r = upload_prof(c))) return r;
+  if ((r = h2d_n(c, c->Ui, (const double* const*)U_, 5))) return r;
+  cf::launch_halo_bc(c->stream, c->Ui, c->prof, c->g, c->p.x_periodic);
+  if ((r = d2h_n(c, U_, c->Ui, 5))) return r;
+  return sync(c);
+}
+
+static int flux_entry(cf_ctx* c, int dir, const double* const U_[5], const double* J, const double* Jf,
+                      const double* const mf[9], double* const inF[5]) {
+  if (!c || !U_ || !inF) return CF_ERR_ARG;
+  int r;
+  const int set = dir == 0 ? cf::SET_U : dir == 1 ? cf::SET_V : cf::SET_W;
+  if ((r = ingest(c, J, cf::SET_C, 0, "J")) || (r = ingest_set(c, set, Jf, mf)) || (r = upload_prof(c))) return r;
+  if ((r = h2d_n(c, c->Ui, U_, 5))) return r;
+  double* F = dir == 0 ? c->Fx : dir == 1 ? c->Fy : c->Fz;
+  // outputs outside the face range come back as zeros
+  HIPCHK(c, hipMemsetAsync(F, 0, (size_t)c->g.nf * 5 * 8, c->stream));
+  cf::launch_flux(c->stream, dir, c->Ui, c->prof, F, c->g);
+  if ((r = d2h_n(c, inF, F, 5))) return r;
+  return sync(c);
+}
+
+extern "C" int cf_flux_x(cf_ctx* c, const double* const U_[5], const double* J, const double* J_u,
+                         const double* const m_u[9], double* const inFx[5]) {
+  return flux_entry(c, 0, U_, J, J_u, m_u, inFx);
+}
+extern "C" int cf_flux_y(cf_ctx* c, const double* const U_[5], const double* J, const double* J_v,
+                         const double* const m_v[9], double* const inFy[5]) {
+  return flux_entry(c, 1, U_, J, J_v, m_v, inFy);
+}
+extern "C" int cf_flux_z(cf_ctx* c, const double* const U_[5], const double* J, const double* J_w,
+                         const double* const m_w[9], double* const inFz[5]) {
+  return flux_entry(c, 2, U_, J, J_w, m_w, inFz);
+}
+
+extern "C" int cf_viscous_terms(cf_ctx* c, const double* const U_[5], double* const vF[4], const double* J,
+                                const double* const m[9], const double* J_u, const double* const m_u[9],
+                                const double* J_v, const double* const m_v[9], const double* J_w,
+                                const double* const m_w[9]) {
+  if (!c || !U_ || !vF) return CF_ERR_ARG;
+  int r;
+  if ((r = ingest(c, J, cf::SET_C, 0, "J")) || (r = ingest_set(c, cf::SET_U, J_u, m_u)) ||
+      (r = ingest_set(c, cf::SET_V, J_v, m_v)) || (r = ingest_set(c, cf::SET_W, J_w, m_w)) || (r = upload_prof(c)))
+    return r;
+  (void)m;
+  if ((r = h2d_n(c, c->Ui, U_, 5))) return r;
+  HIPCHK(c, hipMemsetAsync(c->vF, 0, (size_t)c->g.nf * 4 * 8, c->stream));
+  cf::launch_viscous(c->stream, c->Ui, c->prof, c->Pr, c->VS, c->vF, c->g);
+  if ((r = d2h_n(c, vF, c->vF, 4))) return r;
+  return sync(c);
+}
+
+// the implicit part of one inner iteration on device-resident fields
+static int dual_time_device(cf_ctx* c, double deltaT, double er[5], bool want_nl) {
+  const Geo& g = c->g;
+  cf::launch_rhs(c->stream, c->Ui, c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->prof, c->MR,
+                 want_nl ? c->NL : nullptr, g, deltaT, c->p.f_drive);
+  cf::launch_sweeps(c->stream, c->Ui, c->MR, c->NL, c->NR, c->prof, g, deltaT, c->p.n_sweeps);
+  if (want_nl) cf::launch_copy_interior(c->stream, c->NR, c->NL, g);
+  double DN = 1. / ((double)c->p.X_out * c->p.Y_out * c->p.Z_out);
+  cf::launch_update(c->stream, c->Ui, c->NR, c->prof, c->partial, c->er_dev, g, DN);
+  HIPCHK(c, hipMemcpyAsync(er, c->er_dev, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  int r = sync(c);
+  if (r) return r;
+  for (int v = 0; v < 5; v++)
+    if (!std::isfinite(er[v])) return fail(c, CF_ERR_NONFINITE, "residual norm er%d is not finite", v + 1);
+  return CF_OK;
+}
+
+extern "C" int cf_dual_time_stepping(cf_ctx* c, double deltaT, double er[5], double* const U_[5],
+                                     const double* const U[5], const double* const Uq[5], const double* J,
+                                     const double* const m[9], const double* const inFx[5],
+                                     const double* const inFy[5], const double* const inFz[5],
+                                     const double* const vF[4], double* const MR[5], double* const NR[5],
+                                     double* const NL[5]) {
+  if (!c || !er || !U_ || !U || !Uq || !inFx || !inFy || !inFz || !vF) return fail(c, CF_ERR_ARG, "null argument");
+  if (!(deltaT > 0)) return fail(c, CF_ERR_ARG, "deltaT must be positive");
+  int r;
+  if ((r = ingest_set(c, cf::SET_C, J, m)) || (r = upload_prof(c))) return r;
+  if ((r = h2d_n(c, c->Ui, (const double* const*)U_, 5)) || (r = h2d_n(c, c->Un, U, 5)) ||
+      (r = h2d_n(c, c->Uq, Uq, 5)) || (r = h2d_n(c, c->Fx, inFx, 5)) || (r = h2d_n(c, c->Fy, inFy, 5)) ||
+      (r = h2d_n(c, c->Fz, inFz, 5)) || (r = h2d_n(c, c->vF, vF, 4)))
+    return r;
+  // ghost layers of the implicit work arrays carry no coupling (:195-316)
+  HIPCHK(c, hipMemsetAsync(c->NL, 0, (size_t)c->g.nf * 5 * 8, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->NR, 0, (size_t)c->g.nf * 5 * 8, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->MR, 0, (size_t)c->g.nf * 5 * 8, c->stream));
+  if ((r = dual_time_device(c, deltaT, er, true))) {
+    d2h_n(c, U_, c->Ui, 5);
+    sync(c);
+    return r;
+  }
+  if ((r = d2h_n(c, U_, c->Ui, 5))) return r;
+  if (MR && (r = d2h_n(c, MR, c->MR, 5))) return r;
+  if (NR && (r = d2h_n(c, NR, c->NR, 5))) return r;
+  if (NL && (r = d2h_n(c, NL, c->NL, 5))) return r;
+  return sync(c);
+}
+
+// ---------------------------------------------------------------------------
+// device-resident path
+// ---------------------------------------------------------------------------
+// interior Q [X_out][Y_out][Z_out] -> U = Q/J in the device layout
+static int upload_Q(cf_ctx* c, double* dst, const double* const Q[5]) {
+  const Geo& g = c->g;
+  const double* Jp = &c->hprof[0];
+  std::vector<double> buf((size_t)g.nf, 0.0);
+  size_t X = c->p.X_out, Y = c->p.Y_out, Z = c->p.Z_out;
+  for (int v = 0; v < 5; v++) {
+    if (!Q[v]) return fail(c, CF_ERR_ARG, "null state array %d", v);
+    std::fill(buf.begin(), buf.end(), 0.0);
+    for (size_t i = 0; i < X; i++)
+      for (size_t j = 0; j < Y; j++)
+        for (size_t k = 0; k < Z; k++)
+          buf[(i + 3) * g.plane + (j + 2) * g.Zp + (k + 2)] = Q[v][(i * Y + j) * Z + k] / Jp[j + 2];
+    HIPCHK(c, hipMemcpy(dst + v * g.nf, buf.data(), (size_t)g.nf * 8, hipMemcpyHostToDevice));
+  }
+  return CF_OK;
+}
+
+static int download_Q(cf_ctx* c, double* const Q[5], const double* src) {
+  const Geo& g = c->g;
+  const double* Jp = &c->hprof[0];
+  std::vector<double> buf((size_t)g.nf);
+  size_t X = c->p.X_out, Y = c->p.Y_out, Z = c->p.Z_out;
+  for (int v = 0; v < 5; v++) {
+    if (!Q[v]) continue;
+    HIPCHK(c, hipMemcpy(buf.data(), src + v * g.nf, (size_t)g.nf * 8, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < X; i++)
+      for (size_t j = 0; j < Y; j++)
+        for (size_t k = 0; k < Z; k++)
+          Q[v][(i * Y + j) * Z + k] = buf[(i + 3) * g.plane + (j + 2) * g.Zp + (k + 2)] * Jp[j + 2];
+  }
+  return CF_OK;
+}
+
+extern "C" int cf_load_state(cf_ctx* c, const double* const Qn[5], const double* const Qnm1[5]) {
+  if (!c || !Qn || !Qnm1) return CF_ERR_ARG;
+  if (!c->have_metrics) return fail(c, CF_ERR_STATE, "metrics not set (cf_jacobian_all / cf_set_metrics)");
+  int r;
+  if ((r = sync(c))) return r;
+  if ((r = upload_Q(c, c->Un, Qn)) || (r = upload_Q(c, c->Uq, Qnm1)) || (r = upload_Q(c, c->Ui, Qn))) return r;
+  for (int v = 0; v < 5; v++) c->E[v] = 0;
+  c->have_state = true;
+  return CF_OK;
+}
+
+extern "C" int cf_set_iterate(cf_ctx* c, const double* const Q[5]) {
+  if (!c || !Q) return CF_ERR_ARG;
+  if (!c->have_state) return fail(c, CF_ERR_STATE, "no state loaded");
+  int r;
+  if ((r = sync(c))) return r;
+  return upload_Q(c, c->Ui, Q);
+}
+
+extern "C" int cf_load_initial(cf_ctx* c, const char* path) {
+  if (!c || !path) return CF_ERR_ARG;
+  FILE* f = fopen(path, "rb");
+  if (!f) return fail(c, CF_ERR_IO, "cannot open %s", path);
+  size_t n = (size_t)c->p.X_out * c->p.Y_out * c->p.Z_out;
+  std::vector<double> data(10 * n);
+  size_t got = fread(data.data(), sizeof(double), 10 * n, f);
+  fclose(f);
+  if (got != 10 * n) return fail(c, CF_ERR_IO, "%s: expected %zu doubles, read %zu", path, 10 * n, got);
+  const double* Qn[5];
+  const double* Qm[5];
+  for (int v = 0; v < 5; v++) {
+    Qn[v] = data.data() + v * n;
+    Qm[v] = data.data() + (5 + v) * n;
+  }
+  return cf_load_state(c, Qn, Qm);
+}
+
+extern "C" int cf_read_state(cf_ctx* c, double* const Qn[5], double* const Qnm1[5], double* const Qiter[5]) {
+  if (!c) return CF_ERR_ARG;
+  if (!c->have_state) return fail(c, CF_ERR_STATE, "no state loaded");
+  int r;
+  if ((r = sync(c))) return r;
+  if (Qn && (r = download_Q(c, Qn, c->Un))) return r;
+  if (Qnm1 && (r = download_Q(c, Qnm1, c->Uq))) return r;
+  if (Qiter && (r = download_Q(c, Qiter, c->Ui))) return r;
+  return CF_OK;
+}
+
+extern "C" int cf_inner_iteration(cf_ctx* c, double er[5]) {
+  if (!c || !er) return CF_ERR_ARG;
+  if (!c->have_state) return fail(c, CF_ERR_STATE, "no state loaded");
+  const Geo& g = c->g;
+  cf::launch_halo_bc(c->stream, c->Ui, c->prof, g, c->p.x_periodic);
+  cf::launch_flux(c->stream, 0, c->Ui, c->prof, c->Fx, g);
+  cf::launch_flux(c->stream, 1, c->Ui, c->prof, c->Fy, g);
+  cf::launch_flux(c->stream, 2, c->Ui, c->prof, c->Fz, g);
+  cf::launch_viscous(c->stream, c->Ui, c->prof, c->Pr, c->VS, c->vF, g);
+  return dual_time_device(c, c->p.deltaT, er, false);
+}
+
+extern "C" int cf_advance_levels(cf_ctx* c) {
+  if (!c) return CF_ERR_ARG;
+  if (!c->have_state) return fail(c, CF_ERR_STATE, "no state loaded");
+  // U^{n-1} <- U^n, U^n <- U_ on the interior (main.cpp:444-464)
+  cf::launch_copy_interior(c->stream, c->Un, c->Uq, c->g);
+  cf::launch_copy_interior(c->stream, c->Ui, c->Un, c->g);
+  return sync(c);
+}
+
+extern "C" int cf_profile_inner_iteration(cf_ctx* c, double er[5], double stage_ms[8], int launches[8]) {
+  if (!c || !er || !stage_ms || !launches) return CF_ERR_ARG;
+  if (!c->have_state) return fail(c, CF_ERR_STATE, "no state loaded");
+  const Geo& g = c->g;
+  hipEvent_t ev[9];
+  for (int q = 0; q < 9; q++) HIPCHK(c, hipEventCreate(&ev[q]));
+  int nplanes = g.gend + g.ny + g.nz - 6;
+  HIPCHK(c, hipEventRecord(ev[0], c->stream));
+  cf::launch_halo_bc(c->stream, c->Ui, c->prof, g, c->p.x_periodic);
+  HIPCHK(c, hipEventRecord(ev[1], c->stream));
+  cf::launch_flux(c->stream, 0, c->Ui, c->prof, c->Fx, g);
+  HIPCHK(c, hipEventRecord(ev[2], c->stream));
+  cf::launch_flux(c->stream, 1, c->Ui, c->prof, c->Fy, g);
+  HIPCHK(c, hipEventRecord(ev[3], c->stream));
+  cf::launch_flux(c->stream, 2, c->Ui, c->prof, c->Fz, g);
+  HIPCHK(c, hipEventRecord(ev[4], c->stream));
+  cf::launch_viscous(c->stream, c->Ui, c->prof, c->Pr, c->VS, c->vF, g);
+  HIPCHK(c, hipEventRecord(ev[5], c->stream));
+  cf::launch_rhs(c->stream, c->Ui, c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->prof, c->MR, nullptr, g,
+                 c->p.deltaT, c->p.f_drive);
+  HIPCHK(c, hipEventRecord(ev[6], c->stream));
+  cf::launch_sweeps(c->stream, c->Ui, c->MR, c->NL, c->NR, c->prof, g, c->p.deltaT, c->p.n_sweeps);
+  HIPCHK(c, hipEventRecord(ev[7], c->stream));
+  double DN = 1. / ((double)c->p.X_out * c->p.Y_out * c->p.Z_out);
+  cf::launch_update(c->stream, c->Ui, c->NR, c->prof, c->partial, c->er_dev, g, DN);
+  HIPCHK(c, hipEventRecord(ev[8], c->stream));
+  HIPCHK(c, hipMemcpyAsync(er, c->er_dev, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  int r = sync(c);
+  if (r) return r;
+  const int nl[8] = {3, 1, 1, 1, 3, 1, 2 * nplanes * c->p.n_sweeps, 2};
+  for (int q = 0; q < 8; q++) {
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, ev[q], ev[q + 1]));
+    stage_ms[q] = ms;
+    launches[q] = nl[q];
+  }
+  for (int q = 0; q < 9; q++) (void)hipEventDestroy(ev[q]);
+  for (int v = 0; v < 5; v++)
+    if (!std::isfinite(er[v])) return fail(c, CF_ERR_NONFINITE, "residual norm er%d is not finite", v + 1);
+  return CF_OK;
+}
+
+extern "C" int cf_physical_step(cf_ctx* c, int* iterations, double er[5]) {
+  if (!c || !iterations || !er) return CF_ERR_ARG;
+  if (!c->have_state) return fail(c, CF_ERR_STATE, "no state loaded");
+  for (int it = 1;; it++) {
+    int r = cf_inner_iteration(c, er);
+    if (r) return r;
+    for (int v = 0; v < 5; v++) {
+      c->E[v] = std::max(c->E[v], er[v]);
+      er[v] = er[v] / c->E[v];
+    }
+    bool conv = er[0] < c->p.tol && er[1] < c->p.tol && er[2] < c->p.tol && er[3] < c->p.tol;
+    if (conv || it == c->p.iteration_end_step) {
+      *iterations = it;
+      return cf_advance_levels(c);
+    }
+  }
+}
+
+extern "C" int cf_write_output(cf_ctx* c, int step, const char* dir) {
+  if (!c || !dir) return CF_ERR_ARG;
+  if (!c->have_state) return fail(c, CF_ERR_STATE, "no state loaded");
+  size_t n = (size_t)c->p.X_out * c->p.Y_out * c->p.Z_out;
+  std::vector<double> data(10 * n);
+  double* Qn[5];
+  double* Qm[5];
+  for (int v = 0; v < 5; v++) {
+    Qn[v] = data.data() + v * n;
+    Qm[v] = data.data() + (5 + v) * n;
+  }
+  int r = cf_read_state(c, Qn, Qm, nullptr);
+  if (r) return r;
+  char path[4096];
+  snprintf(path, sizeof path, "%s/LES%05d.bin", dir, step);
+  FILE* f = fopen(path, "wb");
+  if (!f) return fail(c, CF_ERR_IO, "cannot open %s", path);
+  size_t put = fwrite(data.data(), sizeof(double), 10 * n, f);
+  fclose(f);
+  if (put != 10 * n) return fail(c, CF_ERR_IO, "short write %s", path);
+  return CF_OK;
+}

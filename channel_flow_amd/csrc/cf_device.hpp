@@ -1,0 +1,426 @@
+// Device-side arithmetic of the channel-flow dual-time path (gfx950).
+//
+// Every expression follows the reference's FP64 operation order
+// (ricklee209/Channel_Flow, cited per function) and the library is compiled
+// with -ffp-contract=off, so each add/mul/div/sqrt is one correctly rounded
+// IEEE operation in the same order as the reference.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cf {
+
+constexpr double KG = 1.4;         // prm.h:9
+constexpr double RG = 287.0;       // prm.h:10
+constexpr double EPS_PRE = 0.0001; // prm.h:7
+constexpr double CV = 717.5;       // Viscous_terms.h:3
+constexpr double MU_L = 0.0000185; // Viscous_terms.h:4
+constexpr double PR_L = 0.72;      // Viscous_terms.h:5
+constexpr double T_WALL2 = 2 * 298.0592;  // Y_boundary_condition.cpp:87
+
+// Device layout of one field: the reference's [X_np][Y_m][Z_m] with the k
+// row padded to Zp doubles (64-byte multiple).  All fields of the context
+// share it; `nf` is the element stride between consecutive fields.
+struct Geo {
+  int Xnp, Ym, Zp, Zm;
+  int gend, ny, nz;
+  long plane;   // Ym * Zp
+  long nf;      // Xnp * plane
+  double dXI, dET, dZT;
+};
+
+__device__ __forceinline__ long at(const Geo& g, int i, int j, int k) {
+  return (long)i * g.plane + (long)j * g.Zp + k;
+}
+
+// j-profiles of the metrics: prof[(set*10 + comp) * Ym + j]; set 0 = cell
+// centres, 1/2/3 = x/y/z faces; comp 0 = J, 1..9 = xidx..ztdz.
+enum { SET_C = 0, SET_U = 1, SET_V = 2, SET_W = 3 };
+__device__ __forceinline__ double pf(const double* __restrict__ prof, int Ym, int set, int comp, int j) {
+  return prof[(set * 10 + comp) * Ym + j];
+}
+
+// ------------------------------------------------------------------------
+// MUSCL recipes (Flux_X.cpp:92-215, Flux_Y.cpp:91-240, Flux_Z.cpp:96-300)
+// ------------------------------------------------------------------------
+enum Recipe : int { O5L = 0, O5R, O3L, O3R, AVG, AVGM, AVG1 };
+
+__device__ __forceinline__ int recipe_left(int dir, int f, int gend, int ny, int nz) {
+  if (dir == 0) {
+    if (f == 2) return AVG;
+    if (f == 3 || f == gend) return O3L;
+    return O5L;
+  } else if (dir == 1) {
+    if (f == 1 || f == 2) return AVG;
+    if (f == 3 || f == ny - 1) return O3L;
+    if (f == ny) return AVGM;
+    return O5L;
+  }
+  if (f == 1) return AVG;
+  if (f == 2 || f == nz) return O3L;
+  return O5L;
+}
+
+__device__ __forceinline__ int recipe_right(int dir, int f, int gend, int ny, int nz) {
+  if (dir == 0) {
+    if (f == 2 || f == gend - 1) return O3R;
+    if (f == gend) return AVG;
+    return O5R;
+  } else if (dir == 1) {
+    if (f == 1) return AVG1;
+    if (f == 2 || f == ny - 2) return O3R;
+    if (f == ny - 1 || f == ny) return AVG;
+    return O5R;
+  }
+  if (f == 1 || f == nz - 1) return O3R;
+  if (f == nz) return AVG;
+  return O5R;
+}
+
+// Weighted state sum(w*U)/sum(w/J) over the recipe's cells; u[v][t] and
+// jj[t] hold U and J of cells f-2 .. f+3 (t = cell - f + 2).  Each recipe is
+// a compile-time instance so every register index is a constant.
+template <int FIRST, int N>
+__device__ __forceinline__ void rec_w(const double (&u)[5][6], const double (&jj)[6], const double (&w)[N],
+                                      double q[5]) {
+  constexpr int t0 = FIRST + 2;
+  double s = w[0] / jj[t0];
+#pragma unroll
+  for (int t = 1; t < N; t++) s += w[t] / jj[t0 + t];
+  double temp = 1. / s;
+#pragma unroll
+  for (int v = 0; v < 5; v++) {
+    double a = w[0] * u[v][t0];
+#pragma unroll
+    for (int t = 1; t < N; t++) a += w[t] * u[v][t0 + t];
+    q[v] = temp * a;
+  }
+}
+
+__device__ __forceinline__ void reconstruct(int r, const double (&u)[5][6], const double (&jj)[6], double q[5]) {
+  switch (r) {
+    case O5L: { const double w[5] = {2, -13, 47, 27, -3}; rec_w<-2, 5>(u, jj, w, q); break; }
+    case O5R: { const double w[5] = {-3, 27, 47, -13, 2}; rec_w<-1, 5>(u, jj, w, q); break; }
+    case O3L: { const double w[3] = {-1, 5, 2}; rec_w<-1, 3>(u, jj, w, q); break; }
+    case O3R: { const double w[3] = {2, 5, -1}; rec_w<0, 3>(u, jj, w, q); break; }
+    case AVG: { const double w[2] = {1, 1}; rec_w<0, 2>(u, jj, w, q); break; }
+    case AVGM: { const double w[2] = {1, 1}; rec_w<-1, 2>(u, jj, w, q); break; }
+    default: { const double w[2] = {1, 1}; rec_w<1, 2>(u, jj, w, q); break; }  // AVG1
+  }
+}
+
+// Roe flux, ROE == 1 (Flux_X.cpp:221-420, Flux_Y.cpp:247-460,
+// Flux_Z.cpp:307-507).  energy_norm selects deltaP*V*insqr (Flux_Y.cpp:441)
+// over deltaP*U (Flux_X.cpp:412, Flux_Z.cpp:499); wall turns on the
+// impermeable-wall face of Flux_Y.cpp:452-460.
+__device__ __forceinline__ void roe_flux(const double L[5], const double Rr[5], const double n[3], double Jf,
+                                         bool energy_norm, bool wall, double F[5]) {
+  double insqr = 1.0 / sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  double N0 = n[0] * insqr, N1 = n[1] * insqr, N2 = n[2] * insqr;
+  double _rho = L[0], _u = L[1] / _rho, _v = L[2] / _rho, _w = L[3] / _rho;
+  double _U = n[0] * _u + n[1] * _v + n[2] * _w;
+  double _VV = _u * _u + _v * _v + _w * _w;
+  double _P = (L[4] - 0.5 * _rho * _VV) * (KG - 1);
+  double _C = KG * _P / _rho;
+  double _H = 0.5 * _VV + _C / (KG - 1);
+  double rho_ = Rr[0], u_ = Rr[1] / rho_, v_ = Rr[2] / rho_, w_ = Rr[3] / rho_;
+  double U_ = n[0] * u_ + n[1] * v_ + n[2] * w_;
+  double VV_ = u_ * u_ + v_ * v_ + w_ * w_;
+  double P_ = (Rr[4] - 0.5 * rho_ * VV_) * (KG - 1);
+  double C_ = KG * P_ / rho_;
+  double H_ = 0.5 * VV_ + C_ / (KG - 1);
+  double t5 = sqrt(_rho), t6 = sqrt(rho_), t4 = t5 + t6;
+  double rho = sqrt(_rho * rho_);
+  double u = (t5 * _u + t6 * u_) / t4;
+  double v = (t5 * _v + t6 * v_) / t4;
+  double w = (t5 * _w + t6 * w_) / t4;
+  double U = (t5 * _U + t6 * U_) / t4;
+  double VV = u * u + v * v + w * w;
+  double H = (t5 * _H + t6 * H_) / t4;
+  double C = (H - 0.5 * VV) * (KG - 1);
+  double dU1 = rho_ - _rho;
+  double dU2 = rho_ * u_ - _rho * _u;
+  double dU3 = rho_ * v_ - _rho * _v;
+  double dU4 = rho_ * w_ - _rho * _w;
+  double dU5 = (P_ / (KG - 1) + 0.5 * rho_ * VV_) - (_P / (KG - 1) + 0.5 * _rho * _VV);
+  double S = sqrt(C);
+  double temp1 = (P_ - _P) / rho / C;
+  double temp2 = U * insqr / S * (U_ - _U) * insqr;
+  double deltaU = (S - fabs(U * insqr)) * temp1 + temp2;
+  double deltaP = U * insqr / S * (P_ - _P) + (S - fabs(U * insqr)) * rho * (U_ - _U) * insqr;
+  double a = fabs(U * insqr);
+  double Fav1 = a * dU1 + deltaU * rho;
+  double Fav2 = a * dU2 + deltaU * rho * u + deltaP * N0;
+  double Fav3 = a * dU3 + deltaU * rho * v + deltaP * N1;
+  double Fav4 = a * dU4 + deltaU * rho * w + deltaP * N2;
+  double Fav5 = energy_norm ? a * dU5 + deltaU * rho * H + deltaP * U * insqr : a * dU5 + deltaU * rho * H + deltaP * U;
+  if (wall) {
+    F[0] = 0;
+    F[1] = 0;
+    F[2] = 0.5 * (n[1] * (_P + P_) - Fav3 / insqr) / Jf;
+    F[3] = 0;
+    F[4] = 0;
+    return;
+  }
+  F[0] = 0.5 * ((_rho * _U + rho_ * U_) - Fav1 / insqr) / Jf;
+  F[1] = 0.5 * ((_rho * _u * _U + rho_ * u_ * U_ + n[0] * (_P + P_)) - Fav2 / insqr) / Jf;
+  F[2] = 0.5 * ((_rho * _v * _U + rho_ * v_ * U_ + n[1] * (_P + P_)) - Fav3 / insqr) / Jf;
+  F[3] = 0.5 * ((_rho * _w * _U + rho_ * w_ * U_ + n[2] * (_P + P_)) - Fav4 / insqr) / Jf;
+  F[4] = 0.5 * ((_U * (3.5 * _P + 0.5 * _rho * _VV) + U_ * (3.5 * P_ + 0.5 * rho_ * VV_)) - Fav5 / insqr) / Jf;
+}
+
+// ------------------------------------------------------------------------
+// dual time stepping (Dual_time_stepping.cpp)
+// ------------------------------------------------------------------------
+struct PState {
+  double rho, u, v, w, VV, P, C, T, H, beta;
+};
+
+// Dual_time_stepping.cpp:343-372
+__device__ __forceinline__ PState prim_dt(double U1, double U2, double U3, double U4, double U5, double J) {
+  PState s;
+  s.rho = U1 * J;
+  s.u = U2 / U1;
+  s.v = U3 / U1;
+  s.w = U4 / U1;
+  s.VV = s.u * s.u + s.v * s.v + s.w * s.w;
+  s.P = (U5 * J - 0.5 * s.rho * s.VV) * (KG - 1);
+  s.C = KG * s.P / s.rho;
+  s.T = s.P / s.rho;
+  s.H = 0.5 * s.VV + s.C / (KG - 1);
+  double r = s.VV / s.C;
+  s.beta = r > EPS_PRE ? r : EPS_PRE;  // max(VV/C, e)
+  return s;
+}
+
+__device__ __forceinline__ double lambda_dir(const PState& s, double n0, double n1, double n2, double* Uc) {
+  double U = n0 * s.u + n1 * s.v + n2 * s.w;
+  double S = sqrt(U * U * (s.beta - 1) * (s.beta - 1) + 4 * s.beta * s.C * (n0 * n0 + n1 * n1 + n2 * n2));
+  *Uc = U;
+  return ((s.beta + 1) * fabs(U) + S) / 2;
+}
+
+// D^-1 = {d11, d22 (= d33 = d44 = d55), d51}, Dual_time_stepping.cpp:464-489
+__device__ __forceinline__ void dinv(const PState& s, const double m9[9], double dXI, double dET, double dZT,
+                                     double deltaT, double* d11, double* d22, double* d51) {
+  double U, V, W;
+  double lx = lambda_dir(s, m9[0], m9[1], m9[2], &U);
+  double ly = lambda_dir(s, m9[3], m9[4], m9[5], &V);
+  double lz = lambda_dir(s, m9[6], m9[7], m9[8], &W);
+  double lambda = lx / dXI + ly / dET + lz / dZT;
+  double beta = s.beta;
+  *d11 = 2 * deltaT / (3 * beta + 2 * lambda * deltaT);
+  *d22 = 2 * deltaT / (3 + 2 * lambda * deltaT);
+  double temp1 = 6 * (KG - 1) * (beta - 1) * deltaT;
+  double temp2 = KG * (3 + 2 * lambda * deltaT) * (3 * beta + 2 * lambda * deltaT) * s.rho;
+  *d51 = -temp1 / temp2;
+}
+
+// Split flux Jacobian of one cell along metric row n (Dual_time_stepping.cpp
+// :608-632 forward, :968-972 backward diagonals).  Entries 52..54 are
+// written n*beta*T*(K-1) in x and beta*T*(K-1)*n in y and z.
+struct Split {
+  double p11, p12, p13, p14, p21, p22, p31, p41, p51, p52, p53, p54;
+  double m11, m22;
+};
+
+__device__ __forceinline__ Split split_jac(const PState& s, double n0, double n1, double n2, int dir) {
+  Split a;
+  double U;
+  double lam = lambda_dir(s, n0, n1, n2, &U);
+  double dd = fabs(lam);
+  double beta = s.beta, P = s.P, rho = s.rho, T = s.T;
+  a.p11 = 0.5 * (beta * U + dd);
+  a.p12 = 0.5 * P * KG * beta * n0;
+  a.p13 = 0.5 * P * KG * beta * n1;
+  a.p14 = 0.5 * P * KG * beta * n2;
+  a.p21 = 0.5 * n0 / rho;
+  a.p22 = 0.5 * (U + dd);
+  a.p31 = 0.5 * n1 / rho;
+  a.p41 = 0.5 * n2 / rho;
+  a.p51 = 0.5 * U * (KG - 1) * (beta - 1) / rho / KG;
+  if (dir == 0) {
+    a.p52 = 0.5 * n0 * beta * T * (KG - 1);
+    a.p53 = 0.5 * n1 * beta * T * (KG - 1);
+    a.p54 = 0.5 * n2 * beta * T * (KG - 1);
+  } else {
+    a.p52 = 0.5 * beta * T * (KG - 1) * n0;
+    a.p53 = 0.5 * beta * T * (KG - 1) * n1;
+    a.p54 = 0.5 * beta * T * (KG - 1) * n2;
+  }
+  a.m11 = 0.5 * (beta * U - dd);
+  a.m22 = 0.5 * (U - dd);
+  return a;
+}
+
+// lower (forward) contribution -A+ N of one neighbour, Dual_time_stepping.cpp:751-811
+__device__ __forceinline__ void lower_sum(const Split& a, const double N[5], double o[5]) {
+  o[0] = -a.p11 * N[0] - a.p12 * N[1] - a.p13 * N[2] - a.p14 * N[3];
+  o[1] = -a.p21 * N[0] - a.p22 * N[1];
+  o[2] = -a.p31 * N[0] - a.p22 * N[2];
+  o[3] = -a.p41 * N[0] - a.p22 * N[3];
+  o[4] = -a.p51 * N[0] - a.p52 * N[1] - a.p53 * N[2] - a.p54 * N[3] - a.p22 * N[4];
+}
+
+// upper (backward) contribution of one neighbour, Dual_time_stepping.cpp:1116-1177
+__device__ __forceinline__ void upper_sum(const Split& a, const double N[5], double o[5]) {
+  o[0] = +a.m11 * N[0] + a.p12 * N[1] + a.p13 * N[2] + a.p14 * N[3];
+  o[1] = +a.p21 * N[0] + a.m22 * N[1];
+  o[2] = +a.p31 * N[0] + a.m22 * N[2];
+  o[3] = +a.p41 * N[0] + a.m22 * N[3];
+  o[4] = +a.p51 * N[0] + a.p52 * N[1] + a.p53 * N[2] + a.p54 * N[3] + a.m22 * N[4];
+}
+
+// ------------------------------------------------------------------------
+// viscous terms (Viscous_terms.cpp)
+// ------------------------------------------------------------------------
+struct Prim {
+  double U, V, W, T;
+};
+
+// Viscous_terms.cpp:261-395 (rho here is rho/J)
+__device__ __forceinline__ Prim prim_visc(double U1, double U2, double U3, double U4, double U5, double J) {
+  Prim q;
+  double rho = U1;
+  q.U = U2 / rho;
+  q.V = U3 / rho;
+  q.W = U4 / rho;
+  double VV = q.U * q.U + q.V * q.V + q.W * q.W;
+  double P = (U5 - 0.5 * rho * VV) * (KG - 1) * J;
+  q.T = P / (rho * RG) / J;
+  return q;
+}
+
+struct DSet {
+  double u, v, w, uu, vv, ww, uv, vw, uw, T;
+};
+
+__device__ __forceinline__ DSet d_two(const Prim& a, const Prim& b, double inv) {
+  DSet r;
+  r.u = (a.U - b.U) * inv;
+  r.v = (a.V - b.V) * inv;
+  r.w = (a.W - b.W) * inv;
+  r.uu = (a.U * a.U - b.U * b.U) * inv;
+  r.vv = (a.V * a.V - b.V * b.V) * inv;
+  r.ww = (a.W * a.W - b.W * b.W) * inv;
+  r.uv = (a.U * a.V - b.U * b.V) * inv;
+  r.vw = (a.V * a.W - b.V * b.W) * inv;
+  r.uw = (a.U * a.W - b.U * b.W) * inv;
+  r.T = (a.T - b.T) * inv;
+  return r;
+}
+
+__device__ __forceinline__ DSet d_four(const Prim& p1, const Prim& p2, const Prim& m1, const Prim& m2, double inv4) {
+  DSet r;
+  r.u = (p1.U + p2.U - m1.U - m2.U) * inv4;
+  r.v = (p1.V + p2.V - m1.V - m2.V) * inv4;
+  r.w = (p1.W + p2.W - m1.W - m2.W) * inv4;
+  r.uu = (p1.U * p1.U + p2.U * p2.U - m1.U * m1.U - m2.U * m2.U) * inv4;
+  r.vv = (p1.V * p1.V + p2.V * p2.V - m1.V * m1.V - m2.V * m2.V) * inv4;
+  r.ww = (p1.W * p1.W + p2.W * p2.W - m1.W * m1.W - m2.W * m2.W) * inv4;
+  r.uv = (p1.U * p1.V + p2.U * p2.V - m1.U * m1.V - m2.U * m2.V) * inv4;
+  r.vw = (p1.V * p1.W + p2.V * p2.W - m1.V * m1.W - m2.V * m2.W) * inv4;
+  r.uw = (p1.U * p1.W + p2.U * p2.W - m1.U * m1.W - m2.U * m2.W) * inv4;
+  r.T = (p1.T + p2.T - m1.T - m2.T) * inv4;
+  return r;
+}
+
+// coefficient groups a..f of one face (Viscous_terms.cpp:471-526), 1-based
+struct Coef {
+  double a[8], b[8], c[8], d[11], e[11], f[11];
+};
+
+__device__ __forceinline__ void diag_group(const double* p, double* o) {
+  double x = p[0], y = p[1], z = p[2];
+  o[1] = (4. / 3) * x * x + y * y + z * z;
+  o[2] = x * x + (4. / 3) * y * y + z * z;
+  o[3] = x * x + y * y + (4. / 3) * z * z;
+  o[4] = x * x + y * y + z * z;
+  o[5] = (1. / 3) * x * y;
+  o[6] = (1. / 3) * y * z;
+  o[7] = (1. / 3) * x * z;
+}
+
+__device__ __forceinline__ void mixed_group(const double* p, const double* q, double* o) {
+  double px = p[0], py = p[1], pz = p[2], qx = q[0], qy = q[1], qz = q[2];
+  o[1] = (4. / 3) * px * qx + py * qy + pz * qz;
+  o[2] = px * qx + (4. / 3) * py * qy + pz * qz;
+  o[3] = px * qx + py * qy + (4. / 3) * pz * qz;
+  o[4] = px * qx + py * qy + pz * qz;
+  o[5] = px * qy - (2. / 3) * py * qx;
+  o[6] = px * qz - (2. / 3) * pz * qx;
+  o[7] = py * qx - (2. / 3) * px * qy;
+  o[8] = py * qz - (2. / 3) * pz * qy;
+  o[9] = pz * qx - (2. / 3) * px * qz;
+  o[10] = pz * qy - (2. / 3) * py * qz;
+}
+
+__device__ __forceinline__ void coefs(const double m[9], Coef& q) {
+  diag_group(m + 0, q.a);
+  diag_group(m + 3, q.b);
+  diag_group(m + 6, q.c);
+  mixed_group(m + 0, m + 3, q.d);
+  mixed_group(m + 0, m + 6, q.e);
+  mixed_group(m + 3, m + 6, q.f);
+}
+
+#define CF_CVK_PR(g4, dT) (CV * KG * (g4) * (dT) / (PR_L))
+
+// x-face flux LL2..LL5 (Viscous_terms.cpp:533-547)
+__device__ __forceinline__ void visc_flux_x(const Coef& q, const DSet& dx, const DSet& dy, const DSet& dz, double Ux,
+                                            double Vx, double Wx, double Jf, double o[4]) {
+  const double *a = q.a, *d = q.d, *e = q.e;
+  o[0] = MU_L * (a[1] * dx.u + a[5] * dx.v + a[7] * dx.w + d[1] * dy.u + d[7] * dy.v + d[9] * dy.w + e[1] * dz.u +
+                 e[7] * dz.v + e[9] * dz.w) / Jf;
+  o[1] = MU_L * (a[5] * dx.u + a[2] * dx.v + a[6] * dx.w + d[5] * dy.u + d[2] * dy.v + d[10] * dy.w + e[5] * dz.u +
+                 e[2] * dz.v + e[10] * dz.w) / Jf;
+  o[2] = MU_L * (a[7] * dx.u + a[6] * dx.v + a[3] * dx.w + d[6] * dy.u + d[8] * dy.v + d[3] * dy.w + e[6] * dz.u +
+                 e[8] * dz.v + e[3] * dz.w) / Jf;
+  o[3] = MU_L * (0.5 * a[1] * dx.uu + 0.5 * a[2] * dx.vv + 0.5 * a[3] * dx.ww + a[5] * dx.uv + a[6] * dx.vw +
+                 a[7] * dx.uw + CF_CVK_PR(a[4], dx.T) + 0.5 * d[1] * dy.uu + 0.5 * d[2] * dy.vv + 0.5 * d[3] * dy.ww +
+                 d[5] * Vx * dy.u + d[6] * Wx * dy.u + d[7] * Ux * dy.v + d[8] * Wx * dy.v + d[9] * Ux * dy.w +
+                 d[10] * Vx * dy.w + CF_CVK_PR(d[4], dy.T) + 0.5 * e[1] * dz.uu + 0.5 * e[2] * dz.vv +
+                 0.5 * e[3] * dz.ww + e[5] * Vx * dz.u + e[6] * Wx * dz.u + e[7] * Ux * dz.v + e[8] * Wx * dz.v +
+                 e[9] * Ux * dz.w + e[10] * Vx * dz.w + CF_CVK_PR(e[4], dz.T)) /
+         Jf;
+}
+
+// y-face flux ML2..ML5 (Viscous_terms.cpp:681-695)
+__device__ __forceinline__ void visc_flux_y(const Coef& q, const DSet& dx, const DSet& dy, const DSet& dz, double Uy,
+                                            double Vy, double Wy, double Jf, double o[4]) {
+  const double *b = q.b, *d = q.d, *f = q.f;
+  o[0] = MU_L * (d[1] * dx.u + d[5] * dx.v + d[6] * dx.w + b[1] * dy.u + b[5] * dy.v + b[7] * dy.w + f[1] * dz.u +
+                 f[7] * dz.v + f[9] * dz.w) / Jf;
+  o[1] = MU_L * (d[7] * dx.u + d[2] * dx.v + d[8] * dx.w + b[5] * dy.u + b[2] * dy.v + b[6] * dy.w + f[5] * dz.u +
+                 f[2] * dz.v + f[10] * dz.w) / Jf;
+  o[2] = MU_L * (d[9] * dx.u + d[10] * dx.v + d[3] * dx.w + b[7] * dy.u + b[6] * dy.v + b[3] * dy.w + f[6] * dz.u +
+                 f[8] * dz.v + f[3] * dz.w) / Jf;
+  o[3] = MU_L * (0.5 * b[1] * dy.uu + 0.5 * b[2] * dy.vv + 0.5 * b[3] * dy.ww + b[5] * dy.uv + b[6] * dy.vw +
+                 b[7] * dy.uw + CF_CVK_PR(b[4], dy.T) + 0.5 * d[1] * dx.uu + 0.5 * d[2] * dx.vv + 0.5 * d[3] * dx.ww +
+                 d[5] * Uy * dx.v + d[6] * Uy * dx.w + d[7] * Vy * dx.u + d[8] * Vy * dx.w + d[9] * Wy * dx.u +
+                 d[10] * Wy * dx.v + CF_CVK_PR(d[4], dx.T) + 0.5 * f[1] * dz.uu + 0.5 * f[2] * dz.vv +
+                 0.5 * f[3] * dz.ww + f[5] * Vy * dz.u + f[6] * Wy * dz.u + f[7] * Uy * dz.v + f[8] * Wy * dz.v +
+                 f[9] * Uy * dz.w + f[10] * Vy * dz.w + CF_CVK_PR(f[4], dz.T)) /
+         Jf;
+}
+
+// z-face flux NL2..NL5 (Viscous_terms.cpp:827-841)
+__device__ __forceinline__ void visc_flux_z(const Coef& q, const DSet& dx, const DSet& dy, const DSet& dz, double Uz,
+                                            double Vz, double Wz, double Jf, double o[4]) {
+  const double *c = q.c, *e = q.e, *f = q.f;
+  o[0] = MU_L * (e[1] * dx.u + e[5] * dx.v + e[6] * dx.w + f[1] * dy.u + f[5] * dy.v + f[6] * dy.w + c[1] * dz.u +
+                 c[5] * dz.v + c[7] * dz.w) / Jf;
+  o[1] = MU_L * (e[7] * dx.u + e[2] * dx.v + e[8] * dx.w + f[7] * dy.u + f[2] * dy.v + f[8] * dy.w + c[5] * dz.u +
+                 c[2] * dz.v + c[6] * dz.w) / Jf;
+  o[2] = MU_L * (e[9] * dx.u + e[10] * dx.v + e[3] * dx.w + f[9] * dy.u + f[10] * dy.v + f[3] * dy.w + c[7] * dz.u +
+                 c[6] * dz.v + c[3] * dz.w) / Jf;
+  o[3] = MU_L * (0.5 * c[1] * dz.uu + 0.5 * c[2] * dz.vv + 0.5 * c[3] * dz.ww + c[5] * dz.uv + c[6] * dz.vw +
+                 c[7] * dz.uw + CF_CVK_PR(c[4], dz.T) + 0.5 * e[1] * dx.uu + 0.5 * e[2] * dx.vv + 0.5 * e[3] * dx.ww +
+                 e[5] * Uz * dx.v + e[6] * Uz * dx.w + e[7] * Vz * dx.u + e[8] * Vz * dx.w + e[9] * Wz * dx.u +
+                 e[10] * Wz * dx.v + CF_CVK_PR(e[4], dx.T) + 0.5 * f[1] * dy.uu + 0.5 * f[2] * dy.vv +
+                 0.5 * f[3] * dy.ww + f[5] * Uz * dy.v + f[6] * Uz * dy.w + f[7] * Vz * dy.u + f[8] * Vz * dy.w +
+                 f[9] * Wz * dy.u + f[10] * Wz * dy.v + CF_CVK_PR(f[4], dy.T)) /
+         Jf;
+}
+
+}  // namespace cf

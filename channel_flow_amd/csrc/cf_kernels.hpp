@@ -1,0 +1,21 @@
+// Launchers of the channel-flow kernels (host-callable, stream-ordered).
+#pragma once
+#include "cf_device.hpp"
+
+namespace cf {
+
+void launch_halo_bc(hipStream_t st, double* U, const double* prof, const Geo& g, int x_periodic);
+void launch_flux(hipStream_t st, int dir, const double* U, const double* prof, double* F, const Geo& g);
+void launch_viscous(hipStream_t st, const double* U, const double* prof, double* Pr, double* VS, double* vF,
+                    const Geo& g);
+void launch_rhs(hipStream_t st, const double* Ui, const double* Un, const double* Uq, const double* Fx,
+                const double* Fy, const double* Fz, const double* vF, const double* prof, double* MR, double* NL,
+                const Geo& g, double deltaT, double f_drive);
+void launch_sweeps(hipStream_t st, const double* Ui, const double* MR, double* NL, double* NR, const double* prof,
+                   const Geo& g, double deltaT, int n_sweeps);
+void launch_copy_interior(hipStream_t st, const double* src, double* dst, const Geo& g);
+long update_blocks(const Geo& g);
+void launch_update(hipStream_t st, double* Ui, const double* NR, const double* prof, double* partial, double* er_dev,
+                   const Geo& g, double DN);
+
+}  // namespace cf

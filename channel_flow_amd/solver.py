@@ -1,0 +1,260 @@
+"""Python mirror of the reference call surface over the C ABI.
+
+`Solver` owns one `cf_ctx` (include/channel_flow.h) and exposes the
+reference's functions under their own names (main.h:10-1069): `Jacobian_all`,
+`Hyperplane`, `boundary_conditions`, `Flux_X/Y/Z`, `Viscous_terms`,
+`Dual_time_stepping`, plus the device-resident driver (`inner_iteration`,
+`physical_step`, `Output`).  Arrays are numpy float64 in the reference layout
+[X_np][Y_m][Z_m].  All compute runs in libchannel_flow.so (HIP, gfx950); if
+the library is missing, or no GPU is visible, construction raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from .config import ChannelConfig
+
+LIB_PATH = Path(__file__).resolve().parent / "libchannel_flow.so"
+
+CF_OK = 0
+ERRORS = {-1: "CF_ERR_ARG", -2: "CF_ERR_HIP", -3: "CF_ERR_NONFINITE", -4: "CF_ERR_METRICS",
+          -5: "CF_ERR_IO", -6: "CF_ERR_STATE", -7: "CF_ERR_TIMEOUT"}
+
+EXPORTS = [
+    "cf_create", "cf_destroy", "cf_last_error", "cf_field_elems", "cf_jacobian_all", "cf_set_metrics",
+    "cf_hyperplane", "cf_boundary_conditions", "cf_flux_x", "cf_flux_y", "cf_flux_z", "cf_viscous_terms",
+    "cf_dual_time_stepping", "cf_load_state", "cf_load_initial", "cf_set_iterate", "cf_read_state",
+    "cf_inner_iteration", "cf_physical_step", "cf_advance_levels", "cf_write_output", "cf_synchronize",
+    "cf_profile_inner_iteration",
+]
+
+
+class CfParams(C.Structure):
+    _fields_ = [
+        ("X_out", C.c_int), ("Y_out", C.c_int), ("Z_out", C.c_int),
+        ("deltaXI", C.c_double), ("deltaET", C.c_double), ("deltaZT", C.c_double),
+        ("high", C.c_double), ("f_drive", C.c_double), ("deltaT", C.c_double),
+        ("x_periodic", C.c_int), ("n_sweeps", C.c_int), ("iteration_end_step", C.c_int),
+        ("tol", C.c_double), ("precision", C.c_int),
+    ]
+
+
+class CfError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load_library() -> C.CDLL:
+    """Loads libchannel_flow.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise CfError(f"{LIB_PATH} is missing: build it with `python -m channel_flow_amd.build`")
+    lib = C.CDLL(str(LIB_PATH))
+    P = C.POINTER(C.c_double)
+    PP = C.POINTER(P)
+    vp = C.c_void_p
+    lib.cf_create.argtypes = [C.POINTER(CfParams), C.POINTER(vp)]
+    lib.cf_destroy.argtypes = [vp]
+    lib.cf_destroy.restype = None
+    lib.cf_last_error.argtypes = [vp]
+    lib.cf_last_error.restype = C.c_char_p
+    lib.cf_field_elems.argtypes = [vp]
+    lib.cf_field_elems.restype = C.c_longlong
+    lib.cf_jacobian_all.argtypes = [vp, C.c_double, P, PP, P, PP, P, PP, P, PP]
+    lib.cf_set_metrics.argtypes = [vp, P, PP, P, PP, P, PP, P, PP]
+    lib.cf_hyperplane.argtypes = [vp, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]
+    lib.cf_boundary_conditions.argtypes = [vp, PP, P]
+    for n in ("cf_flux_x", "cf_flux_y", "cf_flux_z"):
+        getattr(lib, n).argtypes = [vp, PP, P, P, PP, PP]
+    lib.cf_viscous_terms.argtypes = [vp, PP, PP, P, PP, P, PP, P, PP, P, PP]
+    lib.cf_dual_time_stepping.argtypes = [vp, C.c_double, P, PP, PP, PP, P, PP, PP, PP, PP, PP, PP, PP, PP]
+    lib.cf_load_state.argtypes = [vp, PP, PP]
+    lib.cf_load_initial.argtypes = [vp, C.c_char_p]
+    lib.cf_set_iterate.argtypes = [vp, PP]
+    lib.cf_read_state.argtypes = [vp, PP, PP, PP]
+    lib.cf_inner_iteration.argtypes = [vp, P]
+    lib.cf_physical_step.argtypes = [vp, C.POINTER(C.c_int), P]
+    lib.cf_advance_levels.argtypes = [vp]
+    lib.cf_profile_inner_iteration.argtypes = [vp, P, P, C.POINTER(C.c_int)]
+    lib.cf_write_output.argtypes = [vp, C.c_int, C.c_char_p]
+    lib.cf_synchronize.argtypes = [vp]
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous):
+        raise CfError("arrays must be C-contiguous float64")
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _pp(arrs):
+    if arrs is None:
+        return None
+    return (C.POINTER(C.c_double) * len(arrs))(*[_p(a) for a in arrs])
+
+
+class Solver:
+    """One GPU context for one grid (replaces main.cpp's driver state)."""
+
+    def __init__(self, cfg: ChannelConfig, precision: int = 0):
+        self.cfg = cfg
+        self.lib = load_library()
+        p = CfParams(cfg.X_out, cfg.Y_out, cfg.Z_out, cfg.deltaXI, cfg.deltaET, cfg.deltaZT, cfg.high,
+                     cfg.f_drive, cfg.deltaT, int(cfg.x_periodic), int(cfg.n_sweeps),
+                     int(cfg.iteration_end_step), cfg.tol, int(precision))
+        h = C.c_void_p()
+        r = self.lib.cf_create(C.byref(p), C.byref(h))
+        if r != CF_OK:
+            raise CfError(f"cf_create failed: {ERRORS.get(r, r)}")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.cf_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, r):
+        if r != CF_OK:
+            msg = self.lib.cf_last_error(self._h).decode()
+            raise CfError(f"{ERRORS.get(r, r)}: {msg}")
+
+    def zeros(self, n=None):
+        if n is None:
+            return np.zeros(self.cfg.shape)
+        return [np.zeros(self.cfg.shape) for _ in range(n)]
+
+    # --- geometry -----------------------------------------------------------
+    def Jacobian_all(self, gamma=None, want_arrays=True):
+        """Jacobian + Jacobian_u/_v/_w (Jacobian*.cpp); returns the metric dict."""
+        gamma = self.cfg.gamma_metric if gamma is None else gamma
+        if not want_arrays:
+            self._chk(self.lib.cf_jacobian_all(self._h, gamma, None, None, None, None, None, None, None, None))
+            return None
+        J, Ju, Jv, Jw = self.zeros(4)
+        m, mu, mv, mw = self.zeros(9), self.zeros(9), self.zeros(9), self.zeros(9)
+        self._chk(self.lib.cf_jacobian_all(self._h, gamma, _p(J), _pp(m), _p(Ju), _pp(mu), _p(Jv), _pp(mv),
+                                           _p(Jw), _pp(mw)))
+        return dict(J=J, m=m, J_u=Ju, m_u=mu, J_v=Jv, m_v=mv, J_w=Jw, m_w=mw)
+
+    def set_metrics(self, met):
+        self._chk(self.lib.cf_set_metrics(self._h, _p(met["J"]), _pp(met["m"]), _p(met["J_u"]), _pp(met["m_u"]),
+                                          _p(met["J_v"]), _pp(met["m_v"]), _p(met["J_w"]), _pp(met["m_w"])))
+
+    def Hyperplane(self):
+        cap = self.cfg.gend + self.cfg.ny + self.cfg.nz
+        out = (C.c_int * cap)()
+        n = C.c_int()
+        self._chk(self.lib.cf_hyperplane(self._h, out, cap, C.byref(n)))
+        return np.array(out[: n.value], dtype=np.int64)
+
+    # --- per-function entry points -------------------------------------------
+    def boundary_conditions(self, U_, J):
+        """Single-rank x halo + Y_boundary_condition + Z_boundary_condition, in place."""
+        self._chk(self.lib.cf_boundary_conditions(self._h, _pp(U_), _p(J)))
+
+    def Flux_X(self, U_, J, J_u, m_u):
+        out = self.zeros(5)
+        self._chk(self.lib.cf_flux_x(self._h, _pp(U_), _p(J), _p(J_u), _pp(m_u), _pp(out)))
+        return out
+
+    def Flux_Y(self, U_, J, J_v, m_v):
+        out = self.zeros(5)
+        self._chk(self.lib.cf_flux_y(self._h, _pp(U_), _p(J), _p(J_v), _pp(m_v), _pp(out)))
+        return out
+
+    def Flux_Z(self, U_, J, J_w, m_w):
+        out = self.zeros(5)
+        self._chk(self.lib.cf_flux_z(self._h, _pp(U_), _p(J), _p(J_w), _pp(m_w), _pp(out)))
+        return out
+
+    def Viscous_terms(self, U_, met):
+        out = self.zeros(4)
+        self._chk(self.lib.cf_viscous_terms(self._h, _pp(U_), _pp(out), _p(met["J"]), _pp(met["m"]),
+                                            _p(met["J_u"]), _pp(met["m_u"]), _p(met["J_v"]), _pp(met["m_v"]),
+                                            _p(met["J_w"]), _pp(met["m_w"])))
+        return out
+
+    def Dual_time_stepping(self, U_, U, Uq, met, inFx, inFy, inFz, vF, deltaT=None):
+        """Updates U_ in place; returns (er[5], MR, NR, NL)."""
+        er = (C.c_double * 5)()
+        MR, NR, NL = self.zeros(5), self.zeros(5), self.zeros(5)
+        dt = self.cfg.deltaT if deltaT is None else deltaT
+        self._chk(self.lib.cf_dual_time_stepping(self._h, dt, er, _pp(U_), _pp(U), _pp(Uq), _p(met["J"]),
+                                                 _pp(met["m"]), _pp(inFx), _pp(inFy), _pp(inFz), _pp(vF), _pp(MR),
+                                                 _pp(NR), _pp(NL)))
+        return np.array(er[:]), MR, NR, NL
+
+    # --- device-resident path --------------------------------------------------
+    def load_state(self, Qn, Qnm1):
+        Qn = [np.ascontiguousarray(q, dtype=np.float64) for q in Qn]
+        Qnm1 = [np.ascontiguousarray(q, dtype=np.float64) for q in Qnm1]
+        self._chk(self.lib.cf_load_state(self._h, _pp(Qn), _pp(Qnm1)))
+
+    def load_initial(self, path):
+        self._chk(self.lib.cf_load_initial(self._h, str(path).encode()))
+
+    def set_iterate(self, Q):
+        Q = [np.ascontiguousarray(q, dtype=np.float64) for q in Q]
+        self._chk(self.lib.cf_set_iterate(self._h, _pp(Q)))
+
+    def read_state(self):
+        c = self.cfg
+        shp = (c.X_out, c.Y_out, c.Z_out)
+        Qn = [np.zeros(shp) for _ in range(5)]
+        Qm = [np.zeros(shp) for _ in range(5)]
+        Qi = [np.zeros(shp) for _ in range(5)]
+        self._chk(self.lib.cf_read_state(self._h, _pp(Qn), _pp(Qm), _pp(Qi)))
+        return np.stack(Qn), np.stack(Qm), np.stack(Qi)
+
+    def inner_iteration(self):
+        er = (C.c_double * 5)()
+        self._chk(self.lib.cf_inner_iteration(self._h, er))
+        return np.array(er[:])
+
+    STAGES = ["ghosts", "flux_x", "flux_y", "flux_z", "viscous", "rhs", "lusgs_sweeps", "update_norms"]
+
+    def profile_inner_iteration(self):
+        """One inner iteration with per-stage HIP-event times: (er, {stage: (ms, launches)})."""
+        er = (C.c_double * 5)()
+        ms = (C.c_double * 8)()
+        nl = (C.c_int * 8)()
+        self._chk(self.lib.cf_profile_inner_iteration(self._h, er, ms, nl))
+        return np.array(er[:]), {n: (ms[q], nl[q]) for q, n in enumerate(self.STAGES)}
+
+    def physical_step(self):
+        er = (C.c_double * 5)()
+        it = C.c_int()
+        self._chk(self.lib.cf_physical_step(self._h, C.byref(it), er))
+        return it.value, np.array(er[:])
+
+    def advance_levels(self):
+        """U^{n-1} <- U^n <- U_ (main.cpp:444-464)."""
+        self._chk(self.lib.cf_advance_levels(self._h))
+
+    def Output(self, step, directory):
+        self._chk(self.lib.cf_write_output(self._h, int(step), str(directory).encode()))
+
+    def synchronize(self):
+        self._chk(self.lib.cf_synchronize(self._h))

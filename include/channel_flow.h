@@ -1,0 +1,156 @@
+/* channel_flow.h -- C ABI of the MI355X-native channel-flow dual-time path.
+ *
+ * Drop-in boundary for the time-step hot path of ricklee209/Channel_Flow.
+ * The reference exposes free C++ functions (main.h:10-1069) that take raw
+ * `double (*)[Y_m][Z_m]` arrays sized by compile-time macros and are driven by
+ * main.cpp:143-498.  This header replaces them with:
+ *
+ *   - a context that owns one GPU's device memory for a runtime grid;
+ *   - per-function entry points that take reference-layout host arrays
+ *     ([X_np][Y_m][Z_m] FP64, k fastest; single rank X_np = X_out+6,
+ *     Y_m = Y_out+4, Z_m = Z_out+4) with the same meaning and argument order
+ *     as the reference function they replace (the int myid and the MUSCL
+ *     staging arrays ML/MR are dropped: one context holds the whole domain,
+ *     and the staging lives on the device);
+ *   - a device-resident path (state stays in HBM between calls).
+ *
+ * Conventions: every function returns CF_OK (0) or a negative cf_status;
+ * cf_last_error() gives the message.  The caller owns host buffers, the
+ * context owns device buffers.  A context is not thread-safe; all work runs on
+ * the context's one HIP stream and every host-array entry point is
+ * synchronous.  No C++ exception crosses this boundary.
+ *
+ * Geometry: the reference metrics (Jacobian*.cpp) vary only with j.  Entry
+ * points that receive 3-D metric arrays reduce them to j-profiles and return
+ * CF_ERR_METRICS if a metric varies along i or k inside the region it is used.
+ */
+#ifndef CHANNEL_FLOW_H
+#define CHANNEL_FLOW_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cf_ctx cf_ctx;
+
+enum cf_status {
+  CF_OK = 0,
+  CF_ERR_ARG = -1,        /* bad argument (null pointer, bad size) */
+  CF_ERR_HIP = -2,        /* a HIP runtime call failed */
+  CF_ERR_NONFINITE = -3,  /* residual norm is NaN/Inf */
+  CF_ERR_METRICS = -4,    /* metric arrays are not functions of j only */
+  CF_ERR_IO = -5,         /* file missing / short */
+  CF_ERR_STATE = -6,      /* call order (e.g. no metrics or state loaded) */
+  CF_ERR_TIMEOUT = -7     /* a device-side dependency wait exceeded its bound */
+};
+
+enum cf_precision {
+  CF_FP64 = 0,            /* everything FP64, as the reference */
+  CF_FP32_SWEEPS = 1      /* LU-SGS sweeps and split Jacobians in FP32 */
+};
+
+/* Runtime replacement of Resolution.h / prm.h / main.cpp:47-57 constants. */
+typedef struct {
+  int X_out, Y_out, Z_out;          /* Resolution.h:5-7 */
+  double deltaXI, deltaET, deltaZT; /* Resolution.h:17-19 */
+  double high;                      /* prm.h:18 (channel height) */
+  double f_drive;                   /* prm.h:13 (constant driving force) */
+  double deltaT;                    /* main.cpp:53 */
+  int x_periodic;                   /* 0: single-rank zero-gradient x ghosts
+                                       (main.cpp:207-238 with np=1);
+                                       1: true periodic x */
+  int n_sweeps;                     /* Dual_time_stepping.cpp:514 (2) */
+  int iteration_end_step;           /* main.cpp:48 (200) */
+  double tol;                       /* main.cpp:406 (0.001) */
+  int precision;                    /* enum cf_precision */
+} cf_params;
+
+/* --- context ------------------------------------------------------------ */
+int cf_create(const cf_params* p, cf_ctx** out);
+void cf_destroy(cf_ctx* ctx);
+const char* cf_last_error(const cf_ctx* ctx);
+/* number of FP64 elements of one reference-layout array (X_np*Y_m*Z_m) */
+long long cf_field_elems(const cf_ctx* ctx);
+
+/* --- geometry ------------------------------------------------------------ */
+/* Jacobian/Jacobian_u/_v/_w (main.h:51-160, Jacobian*.cpp): computes the
+ * cell and face metrics for stretching factor gamma on the host, keeps the
+ * j-profiles on the device, and optionally returns the reference-layout
+ * arrays (any output pointer may be NULL).  m[9] order: xidx,xidy,xidz,
+ * etdx,etdy,etdz,ztdx,ztdy,ztdz. */
+int cf_jacobian_all(cf_ctx* ctx, double gamma, double* J, double* const m[9], double* J_u,
+                    double* const m_u[9], double* J_v, double* const m_v[9], double* J_w,
+                    double* const m_w[9]);
+/* Installs metrics given in reference layout (reduces them to j-profiles). */
+int cf_set_metrics(cf_ctx* ctx, const double* J, const double* const m[9], const double* J_u,
+                   const double* const m_u[9], const double* J_v, const double* const m_v[9],
+                   const double* J_w, const double* const m_w[9]);
+/* Hyperplane (main.h:974-983, Hyperplane.cpp:50-78): per-plane cell counts
+ * for m = 7 .. gend+ny+nz; counts[m-7].  Returns the number of planes in
+ * *nplanes. */
+int cf_hyperplane(cf_ctx* ctx, int* counts, int capacity, int* nplanes);
+
+/* --- per-function entry points (reference-layout host arrays) ------------ */
+/* main.cpp:156-250: single-rank x halo, then Y_boundary_condition
+ * (main.h:192-207) and Z_boundary_condition (main.h:209-224), in place. */
+int cf_boundary_conditions(cf_ctx* ctx, double* const U_[5], const double* J);
+/* Flux_X / Flux_Y / Flux_Z (main.h:226-374): 5th-order MUSCL + Roe. */
+int cf_flux_x(cf_ctx* ctx, const double* const U_[5], const double* J, const double* J_u,
+              const double* const m_u[9], double* const inFx[5]);
+int cf_flux_y(cf_ctx* ctx, const double* const U_[5], const double* J, const double* J_v,
+              const double* const m_v[9], double* const inFy[5]);
+int cf_flux_z(cf_ctx* ctx, const double* const U_[5], const double* J, const double* J_w,
+              const double* const m_w[9], double* const inFz[5]);
+/* Viscous_terms (main.h:525-651): writes vF[0..3] = vF2..vF5. */
+int cf_viscous_terms(cf_ctx* ctx, const double* const U_[5], double* const vF[4], const double* J,
+                     const double* const m[9], const double* J_u, const double* const m_u[9],
+                     const double* J_v, const double* const m_v[9], const double* J_w,
+                     const double* const m_w[9]);
+/* Dual_time_stepping (main.h:653-764): one inner iteration's implicit part;
+ * U_ updated in place, er[5] = er1..er5, MR/NR/NL as the reference leaves
+ * them (NL = NR on exit).  Any of MR/NR/NL may be NULL. */
+int cf_dual_time_stepping(cf_ctx* ctx, double deltaT, double er[5], double* const U_[5],
+                          const double* const U[5], const double* const Uq[5], const double* J,
+                          const double* const m[9], const double* const inFx[5],
+                          const double* const inFy[5], const double* const inFz[5],
+                          const double* const vF[4], double* const MR[5], double* const NR[5],
+                          double* const NL[5]);
+
+/* --- device-resident path ------------------------------------------------ */
+/* Loads level n and n-1 from interior Q arrays ([X_out][Y_out][Z_out] per
+ * variable, rho, rho u, rho v, rho w, rho E; Initial_condition.cpp:87-116)
+ * and sets the inner iterate to level n (main.cpp:112-127). */
+int cf_load_state(cf_ctx* ctx, const double* const Qn[5], const double* const Qnm1[5]);
+/* Initial_condition (main.h:22-49): reads an initial.bin / LES%05d.bin file. */
+int cf_load_initial(cf_ctx* ctx, const char* path);
+/* Optional: overrides the inner iterate U_ with interior Q arrays. */
+int cf_set_iterate(cf_ctx* ctx, const double* const Q[5]);
+/* Interior Q of level n, n-1 and of the inner iterate (NULL to skip). */
+int cf_read_state(cf_ctx* ctx, double* const Qn[5], double* const Qnm1[5], double* const Qiter[5]);
+/* One inner iteration: halo + BC, Flux_X/Y/Z, Viscous_terms,
+ * Dual_time_stepping (main.cpp:156-353 with the dual-time calls enabled).
+ * er[5] are the raw norms e_k = sqrt(sum dq^4)/N. */
+int cf_inner_iteration(cf_ctx* ctx, double er[5]);
+/* One physical step: inner iterations until er1..er4 (normalised by their
+ * running maxima, main.cpp:387-397) fall below tol or iteration_end_step is
+ * reached, then U^{n-1} <- U^n <- U_ (main.cpp:444-464).  Returns the
+ * iteration count and the last normalised er[5]. */
+int cf_physical_step(cf_ctx* ctx, int* iterations, double er[5]);
+/* The time-level shift alone: U^{n-1} <- U^n, U^n <- U_ on the interior
+ * (main.cpp:444-464); cf_physical_step calls it on convergence. */
+int cf_advance_levels(cf_ctx* ctx);
+/* Output (main.h:872-896, Output.cpp:56-190): writes <dir>/LES%05d.bin. */
+int cf_write_output(cf_ctx* ctx, int step, const char* dir);
+/* Runs one inner iteration like cf_inner_iteration and times its stages
+ * with HIP events recorded on the context's stream.  stage_ms[8]: ghosts,
+ * Flux_X, Flux_Y, Flux_Z, Viscous_terms, RHS, LU-SGS sweeps, update+norms;
+ * launches[8]: kernel launches per stage.  No counterpart in the reference
+ * (it has no timers, MPI_prm.h:14). */
+int cf_profile_inner_iteration(cf_ctx* ctx, double er[5], double stage_ms[8], int launches[8]);
+/* Synchronises the context's stream. */
+int cf_synchronize(cf_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,69 @@
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+sys.path.insert(0, str(ROOT))
+
+GOLDEN = ROOT / "tests" / "golden"
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
+    config.addinivalue_line("markers", "slow: long CPU test")
+
+
+def gpu_available() -> bool:
+    try:
+        import torch
+        return torch.cuda.is_available()
+    except Exception:
+        return False
+
+
+@pytest.fixture(scope="session")
+def golden():
+    def load(name):
+        return np.load(GOLDEN / f"{name}.npz")
+    return load
+
+
+def unit_cfg(grid):
+    from channel_flow_amd.config import ChannelConfig
+    return ChannelConfig(X_out=grid[0], Y_out=grid[1], Z_out=grid[2], gamma_metric=1.5)
+
+
+UNIT_GRIDS = [(12, 10, 16), (8, 12, 9)]
+METRIC_NAMES = ["xidx", "xidy", "xidz", "etdx", "etdy", "etdz", "ztdx", "ztdy", "ztdz"]
+
+
+def fixture_metrics(z):
+    return dict(J=z["J"], J_u=z["J_u"], J_v=z["J_v"], J_w=z["J_w"],
+                m=[z[n] for n in METRIC_NAMES], m_u=[z[n + "_u"] for n in METRIC_NAMES],
+                m_v=[z[n + "_v"] for n in METRIC_NAMES], m_w=[z[n + "_w"] for n in METRIC_NAMES])
+
+
+def fields(z, base, n=5):
+    return [np.ascontiguousarray(z[f"{base}{v + 1}"]) for v in range(n)]
+
+
+def masks(cfg):
+    """Regions where each reference output is defined (single rank)."""
+    g, ny, nz = cfg.gend, cfg.ny, cfg.nz
+    sh = cfg.shape
+    out = {}
+    m = np.zeros(sh, bool); m[2:g + 1, 1:ny, 1:nz] = True; out["x"] = m
+    m = np.zeros(sh, bool); m[2:g, 1:ny + 1, 1:nz] = True; out["y"] = m
+    m = np.zeros(sh, bool); m[2:g, 1:ny, 1:nz + 1] = True; out["z"] = m
+    m = np.zeros(sh, bool); m[3:g + 1, 2:ny + 1, 2:nz + 1] = True; out["int"] = m
+    m = np.zeros(sh, bool); m[2:g + 2, 1:ny + 2, 1:nz + 2] = True; out["bc"] = m
+    return out
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64).ravel()
+    b = np.asarray(b, dtype=np.float64).ravel()
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)

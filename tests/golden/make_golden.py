@@ -1,0 +1,127 @@
+"""Writes the golden fixtures of tests/golden/ from the reference solver.
+
+Runs only in a container that has the reference sources (/root/reference)
+and MPICH: oracle/ref_build.sh compiles the reference where it lies for each
+grid, oracle/_ref/<grid>/ref_driver runs it, and this script keeps the arrays
+the tests compare against.  Inputs are this project's seeded initial
+condition (channel_flow_amd.config); outputs are whatever the reference
+computes from them.
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import json
+import subprocess
+import sys
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+from channel_flow_amd.config import ChannelConfig, FlowParams, poiseuille_state, write_initial_bin  # noqa: E402
+
+OUT = Path(__file__).resolve().parent
+DT_UNIT = 5e-8          # main.cpp:53
+UNIT_GRIDS = [(12, 10, 16), (8, 12, 9)]
+TRAJ = dict(grid=(16, 12, 16), seed=7, deltaT=5e-8, steps=3, iteration_end_step=6)
+TRAJ64 = dict(grid=(64, 64, 64), seed=1, deltaT=5e-8, steps=10, iteration_end_step=20)
+
+
+def ref_driver(grid) -> Path:
+    X, Y, Z = grid
+    exe = ROOT / "oracle" / "_ref" / f"g{X}x{Y}x{Z}" / "ref_driver"
+    subprocess.run([str(ROOT / "oracle" / "ref_build.sh"), str(X), str(Y), str(Z)], check=True,
+                   stdout=subprocess.DEVNULL)
+    return exe
+
+
+def write_inputs(d: Path, cfg: ChannelConfig, seed: int, iterate: bool):
+    fl = FlowParams()
+    Qn = poiseuille_state(cfg, fl, seed=seed, stream=0)
+    Qm = poiseuille_state(cfg, fl, seed=seed, stream=1) if iterate else Qn
+    write_initial_bin(d / "initial.bin", Qn, Qm)
+    if iterate:
+        np.ascontiguousarray(poiseuille_state(cfg, fl, seed=seed, stream=2)).tofile(d / "iterate.bin")
+
+
+def load(d: Path, name: str, cfg: ChannelConfig) -> np.ndarray:
+    return np.fromfile(d / f"{name}.bin").reshape(cfg.shape)
+
+
+def unit_fixture(grid, seed):
+    cfg = ChannelConfig(X_out=grid[0], Y_out=grid[1], Z_out=grid[2])
+    exe = ref_driver(grid)
+    with tempfile.TemporaryDirectory() as td:
+        d = Path(td)
+        write_inputs(d, cfg, seed, iterate=True)
+        subprocess.run([str(exe), "unit", str(d), repr(DT_UNIT)], check=True, stdout=subprocess.DEVNULL)
+        arrs = {}
+        names = ["J", "J_u", "J_v", "J_w", "Y_point", "EpX", "EpY", "EpZ", "EpX_s", "EpY_s", "EpZ_s"]
+        for m in ["xidx", "xidy", "xidz", "etdx", "etdy", "etdz", "ztdx", "ztdy", "ztdz"]:
+            names += [m, m + "_u", m + "_v", m + "_w"]
+        for base, n in [("U_pre", 5), ("U", 5), ("Uq", 5), ("U_bc", 5), ("inFx", 5), ("inFy", 5), ("inFz", 5),
+                        ("vF", 4), ("U_dt", 5), ("MR_dt", 5), ("NL_dt", 5), ("NR_dt", 5), ("inFxL", 5),
+                        ("inFyL", 5), ("inFzL", 5), ("inFxS", 5), ("inFyS", 5), ("inFzS", 5), ("U_rk1_", 5),
+                        ("U_rk2_", 5), ("U_rk3_", 5)]:
+            names += [f"{base}{v + 1}" for v in range(n)]
+        for n in names:
+            arrs[n] = load(d, n, cfg)
+        er = [float(x) for x in (d / "scalars.txt").read_text().split()[1:6]]
+        hp = np.loadtxt(d / "hyperplane.txt").astype(np.int64)
+        arrs["hyperplane"] = hp
+        arrs["er_dt"] = np.array(er)
+    meta = dict(grid=list(grid), seed=seed, deltaT=DT_UNIT, gamma_metric=1.5, f_drive=599.29876710,
+                reference_calls=["halo (main.cpp:156-240)", "Y_boundary_condition", "Z_boundary_condition",
+                                 "Flux_X", "Flux_Y", "Flux_Z", "Viscous_terms", "Dual_time_stepping",
+                                 "Filter", "Flux_X/Y/Z_limiter", "RK3NODT"])
+    name = f"unit_g{grid[0]}x{grid[1]}x{grid[2]}"
+    np.savez_compressed(OUT / f"{name}.npz", **arrs)
+    return name, meta
+
+
+def traj_fixture(spec, keep_state: bool):
+    grid = spec["grid"]
+    cfg = ChannelConfig(X_out=grid[0], Y_out=grid[1], Z_out=grid[2])
+    exe = ref_driver(grid)
+    with tempfile.TemporaryDirectory() as td:
+        d = Path(td)
+        write_inputs(d, cfg, spec["seed"], iterate=False)
+        subprocess.run([str(exe), "traj", str(d), repr(spec["deltaT"]), str(spec["steps"]),
+                        str(spec["iteration_end_step"])], check=True, stdout=subprocess.DEVNULL)
+        er = np.loadtxt(d / "er.txt")
+        les = np.fromfile(d / f"LES{spec['steps']:05d}.bin").reshape(10, *grid)
+    arrs = dict(er=er)
+    if keep_state:
+        arrs["les"] = les
+    else:
+        # per-variable sums and one x-plane of both levels
+        arrs["les_sum"] = les.reshape(10, -1).sum(axis=1)
+        arrs["les_plane"] = les[:, grid[0] // 2].copy()
+    name = f"traj_g{grid[0]}x{grid[1]}x{grid[2]}"
+    np.savez_compressed(OUT / f"{name}.npz", **arrs)
+    meta = dict(spec, grid=list(grid), gamma_metric=1.5, f_drive=599.29876710,
+                driver="main.cpp:143-498 with Viscous_terms + Dual_time_stepping enabled")
+    return name, meta
+
+
+def main():
+    manifest = {"generator": "tests/golden/make_golden.py",
+                "reference": "ricklee209/Channel_Flow built by oracle/ref_build.sh (np=1, MPICH 3.3.2)",
+                "fixtures": {}}
+    for i, g in enumerate(UNIT_GRIDS):
+        n, m = unit_fixture(g, seed=11 + i)
+        manifest["fixtures"][n] = m
+    n, m = traj_fixture(TRAJ, keep_state=True)
+    manifest["fixtures"][n] = m
+    n, m = traj_fixture(TRAJ64, keep_state=False)
+    manifest["fixtures"][n] = m
+    (OUT / "manifest.json").write_text(json.dumps(manifest, indent=1))
+    for f in sorted(OUT.glob("*.npz")):
+        print(f"{f.name}: {f.stat().st_size / 1e6:.2f} MB")
+
+
+if __name__ == "__main__":
+    main()

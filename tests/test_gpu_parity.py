@@ -1,0 +1,190 @@
+"""HIP path (libchannel_flow.so through the C ABI) against the oracle and the
+reference's golden vectors.
+
+The library is compiled with -ffp-contract=off, so every FP64 operation is
+rounded once in the reference's order; the per-function outputs are expected
+to match the reference bit for bit.  The residual norms are reduced in a
+different (tree) order on the GPU, so er and the states downstream of a
+physical step are compared with a relative tolerance of 1e-12 (north-star
+bar: relative L2 <= 1e-12 per kernel, state after 10 steps <= 1e-10).
+"""
+import numpy as np
+import pytest
+
+from conftest import UNIT_GRIDS, fields, fixture_metrics, gpu_available, masks, rel_l2, unit_cfg
+
+pytestmark = pytest.mark.gpu
+
+TOL_KERNEL = 1e-12
+TOL_STATE = 1e-10
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if not gpu_available():
+        pytest.fail("no GPU visible: the gpu tests must run on an MI355X")
+
+
+def solver(cfg):
+    from channel_flow_amd.solver import Solver
+    return Solver(cfg)
+
+
+@pytest.mark.parametrize("grid", UNIT_GRIDS)
+def test_gpu_metrics_and_hyperplane(grid, golden):
+    z = golden(f"unit_g{grid[0]}x{grid[1]}x{grid[2]}")
+    cfg = unit_cfg(grid)
+    with solver(cfg) as s:
+        met = s.Jacobian_all(1.5)
+        ref = fixture_metrics(z)
+        for key in ("J", "J_u", "J_v", "J_w"):
+            mk = {"J": "bc", "J_u": "x", "J_v": "y", "J_w": "z"}[key]
+            m = masks(cfg)[mk]
+            np.testing.assert_array_equal(met[key][m], ref[key][m], err_msg=key)
+        counts = s.Hyperplane()
+        np.testing.assert_array_equal(counts, z["hyperplane"][:, 2])
+
+
+@pytest.mark.parametrize("grid", UNIT_GRIDS)
+def test_gpu_boundary_conditions(grid, golden):
+    z = golden(f"unit_g{grid[0]}x{grid[1]}x{grid[2]}")
+    cfg = unit_cfg(grid)
+    U_ = [a.copy() for a in fields(z, "U_pre")]
+    with solver(cfg) as s:
+        s.boundary_conditions(U_, np.ascontiguousarray(z["J"]))
+    m = masks(cfg)["bc"]
+    for v, ref in enumerate(fields(z, "U_bc")):
+        np.testing.assert_array_equal(U_[v][m], ref[m])
+
+
+@pytest.mark.parametrize("grid", UNIT_GRIDS)
+def test_gpu_fluxes(grid, golden):
+    z = golden(f"unit_g{grid[0]}x{grid[1]}x{grid[2]}")
+    cfg = unit_cfg(grid)
+    met = fixture_metrics(z)
+    U_ = fields(z, "U_bc")
+    with solver(cfg) as s:
+        outs = [s.Flux_X(U_, met["J"], met["J_u"], met["m_u"]), s.Flux_Y(U_, met["J"], met["J_v"], met["m_v"]),
+                s.Flux_Z(U_, met["J"], met["J_w"], met["m_w"])]
+    for d in range(3):
+        m = masks(cfg)["xyz"[d]]
+        for v, ref in enumerate(fields(z, "inF" + "xyz"[d])):
+            assert rel_l2(outs[d][v][m], ref[m]) <= TOL_KERNEL
+            np.testing.assert_array_equal(outs[d][v][m], ref[m], err_msg=f"inF{'xyz'[d]}{v + 1}")
+
+
+@pytest.mark.parametrize("grid", UNIT_GRIDS)
+def test_gpu_viscous(grid, golden):
+    z = golden(f"unit_g{grid[0]}x{grid[1]}x{grid[2]}")
+    cfg = unit_cfg(grid)
+    with solver(cfg) as s:
+        out = s.Viscous_terms(fields(z, "U_bc"), fixture_metrics(z))
+    m = masks(cfg)["int"]
+    for v, ref in enumerate(fields(z, "vF", 4)):
+        assert rel_l2(out[v][m], ref[m]) <= TOL_KERNEL
+        np.testing.assert_array_equal(out[v][m], ref[m], err_msg=f"vF{v + 2}")
+
+
+@pytest.mark.parametrize("grid", UNIT_GRIDS)
+def test_gpu_dual_time(grid, golden):
+    z = golden(f"unit_g{grid[0]}x{grid[1]}x{grid[2]}")
+    cfg = unit_cfg(grid)
+    U_ = [a.copy() for a in fields(z, "U_bc")]
+    with solver(cfg) as s:
+        er, MR, NR, NL = s.Dual_time_stepping(U_, fields(z, "U"), fields(z, "Uq"), fixture_metrics(z),
+                                              fields(z, "inFx"), fields(z, "inFy"), fields(z, "inFz"),
+                                              fields(z, "vF", 4), deltaT=5e-8)
+    m = masks(cfg)["int"]
+    for v in range(5):
+        np.testing.assert_array_equal(MR[v][m], z[f"MR_dt{v + 1}"][m])
+        np.testing.assert_array_equal(NR[v][m], z[f"NR_dt{v + 1}"][m])
+        np.testing.assert_array_equal(NL[v][m], z[f"NL_dt{v + 1}"][m])
+        np.testing.assert_array_equal(U_[v][m], z[f"U_dt{v + 1}"][m])
+    np.testing.assert_allclose(er, z["er_dt"], rtol=TOL_KERNEL, atol=0)
+
+
+def gpu_trajectory(cfg, Qn, Qm, steps):
+    rows = []
+    with solver(cfg) as s:
+        s.Jacobian_all(cfg.gamma_metric, want_arrays=False)
+        s.load_state(Qn, Qm)
+        E = np.zeros(5)
+        for step in range(1, steps + 1):
+            for it in range(1, 50000):
+                er = s.inner_iteration()
+                E = np.maximum(E, er)
+                ern = er / E
+                rows.append([step, it, *er, *ern])
+                if (ern[:4] < cfg.tol).all() or it == cfg.iteration_end_step:
+                    break
+            s.advance_levels()
+        qn, qm, _ = s.read_state()
+    return np.array(rows), np.concatenate([qn, qm])
+
+
+def test_gpu_trajectory_16x12x16(golden):
+    """3 physical steps of the dual-time loop against the reference run."""
+    from channel_flow_amd.config import ChannelConfig, FlowParams, poiseuille_state
+    z = golden("traj_g16x12x16")
+    cfg = ChannelConfig(16, 12, 16, gamma_metric=1.5, deltaT=5e-8, iteration_end_step=6)
+    Qn = poiseuille_state(cfg, FlowParams(), seed=7, stream=0)
+    rows, les = gpu_trajectory(cfg, Qn, Qn, steps=3)
+    ref = z["er"]
+    assert rows.shape == ref.shape
+    np.testing.assert_array_equal(rows[:, :2], ref[:, :2])
+    np.testing.assert_allclose(rows[:, 2:], ref[:, 2:], rtol=1e-10, atol=0)
+    assert rel_l2(les, z["les"]) <= TOL_STATE
+
+
+def test_gpu_physical_step_api_16x12x16(golden):
+    """cf_physical_step (device-side shift, running maxima) against the reference."""
+    from channel_flow_amd.config import ChannelConfig, FlowParams, poiseuille_state
+    from channel_flow_amd.solver import Solver
+    z = golden("traj_g16x12x16")
+    cfg = ChannelConfig(16, 12, 16, gamma_metric=1.5, deltaT=5e-8, iteration_end_step=6)
+    Qn = poiseuille_state(cfg, FlowParams(), seed=7, stream=0)
+    ref = z["er"]
+    with Solver(cfg) as s:
+        s.Jacobian_all(1.5, want_arrays=False)
+        s.load_state(Qn, Qn)
+        for step in range(1, 4):
+            it, ern = s.physical_step()
+            last = ref[ref[:, 0] == step][-1]
+            assert it == int(last[1])
+            np.testing.assert_allclose(ern, last[7:], rtol=1e-10)
+        qn, qm, _ = s.read_state()
+    assert rel_l2(np.concatenate([qn, qm]), z["les"]) <= TOL_STATE
+
+
+def test_gpu_trajectory_64cubed_config1(golden):
+    """Config 1: 64^3, 10 physical steps x 20 inner iterations, residual
+    history and state after 10 steps against the reference (<= 1e-10)."""
+    from channel_flow_amd.config import ChannelConfig, FlowParams, poiseuille_state
+    z = golden("traj_g64x64x64")
+    cfg = ChannelConfig(64, 64, 64, gamma_metric=1.5, deltaT=5e-8, iteration_end_step=20)
+    Qn = poiseuille_state(cfg, FlowParams(), seed=1, stream=0)
+    rows, les = gpu_trajectory(cfg, Qn, Qn, steps=10)
+    ref = z["er"]
+    assert rows.shape == ref.shape
+    np.testing.assert_array_equal(rows[:, :2], ref[:, :2])
+    np.testing.assert_allclose(rows[:, 2:], ref[:, 2:], rtol=1e-10, atol=0)
+    np.testing.assert_allclose(les.reshape(10, -1).sum(axis=1), z["les_sum"], rtol=1e-12)
+    assert rel_l2(les[:, 32], z["les_plane"]) <= TOL_STATE
+
+
+def test_gpu_repeatable_bits():
+    """Two runs of the same inner iterations give identical bits (the
+    residual reduction has a fixed order)."""
+    from channel_flow_amd.config import ChannelConfig, FlowParams, poiseuille_state
+    from channel_flow_amd.solver import Solver
+    cfg = ChannelConfig(32, 24, 40, gamma_metric=2.0, deltaT=5e-8)
+    Qn = poiseuille_state(cfg, FlowParams(), seed=3, stream=0)
+    outs = []
+    for _ in range(2):
+        with Solver(cfg) as s:
+            s.Jacobian_all(2.0, want_arrays=False)
+            s.load_state(Qn, Qn)
+            ers = [s.inner_iteration() for _ in range(3)]
+            outs.append((np.array(ers), s.read_state()[2]))
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
