@@ -55,7 +55,7 @@ def cpu_baseline(cfg, seconds_budget=30.0):
         del Q
         env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
         log(f"cpu baseline: reference ref_driver, {threads} threads, {nit} inner iterations")
-        r = subprocess.run([str(exe), "time", td, "5e-8", str(nit)], env=env, capture_output=True, text=True,
+        r = subprocess.run([str(exe), "time", td, "1e-8", str(nit)], env=env, capture_output=True, text=True,
                            timeout=600)
         if r.returncode != 0:
             log(f"cpu baseline failed: {r.stderr[-500:]}")
@@ -95,7 +95,7 @@ def main():
 
     torch.cuda.set_device(0)
     flow = FlowParams(Re_bulk=5600.0, Mach=0.2)
-    cfg = ChannelConfig(*GRID, gamma_metric=2.0, gamma_grid=2.0, deltaT=5e-8,
+    cfg = ChannelConfig(*GRID, gamma_metric=2.0, gamma_grid=2.0, deltaT=1e-8,
                         f_drive=flow.driving_force(0.005), x_periodic=False)
     Q = poiseuille_state(cfg, flow, seed=1, stream=0)
     log(f"grid {GRID}, cells {cfg.cells}")
@@ -171,7 +171,7 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic: seeded laminar Poiseuille + 5% uniform velocity noise",
         "config": {"workload": "plain FP64 dual-time inner iteration, 256x192x256 channel "
-                               "(2pi x 2 x pi, tanh beta=2.0, Re_b=5600, Mach 0.2, dt=5e-8)",
+                               "(2pi x 2 x pi, tanh beta=2.0, Re_b=5600, Mach 0.2, dt=1e-8)",
                    "grid": "x".join(map(str, GRID)), "cells": cfg.cells, "n_sweeps": cfg.n_sweeps,
                    "parallelism": "single GPU"},
         "phys_step_to_1e-8": conv,

@@ -40,6 +40,13 @@ struct cf_ctx {
   double* NR = nullptr;   // (5)
   double* Pr = nullptr;   // viscous primitives U,V,W,T (4)
   double* VS = nullptr;   // viscous face fluxes LL/ML/NL (12)
+  double* Sm = nullptr;   // per-cell sweep summary (S_N)
+  int* order = nullptr;   // sweep tiles in dependency order
+  int* sync = nullptr;    // ticket, error flag, per-tile progress (x2 directions)
+  int ntiles = 0;
+  int schedule = 0;       // 0: persistent tiles, 1: one hyperplane per launch
+  int pub = 1;            // steps between progress publications
+  long spin_limit = 1L << 22;
   double* partial = nullptr;
   double* er_dev = nullptr;
   double* prof = nullptr;  // metric j-profiles (NPROF * Ym)
@@ -202,6 +209,24 @@ extern "C" int cf_create(const cf_params* p, cf_ctx** out) {
     // the reference leaves work arrays uninitialised; here they start at zero
     if (hipMemsetAsync(*f.ptr, 0, nf * f.n * sizeof(double), c->stream) != hipSuccess) return bail(CF_ERR_HIP);
   }
+  if (hipMalloc(&c->Sm, nf * cf::S_N * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
+  {
+    int nTI, nTJ;
+    c->ntiles = cf::sweep_tiles(g, &nTI, &nTJ);
+    std::vector<int> ord;
+    ord.reserve(c->ntiles);
+    for (int d = 0; d <= nTI + nTJ - 2; d++)
+      for (int tx = 0; tx < nTI; tx++) {
+        int ty = d - tx;
+        if (ty >= 0 && ty < nTJ) ord.push_back(tx + ty * nTI);
+      }
+    if (hipMalloc(&c->order, ord.size() * sizeof(int)) != hipSuccess) return bail(CF_ERR_HIP);
+    if (hipMemcpy(c->order, ord.data(), ord.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+      return bail(CF_ERR_HIP);
+    if (hipMalloc(&c->sync, 2 * (c->ntiles + 2) * sizeof(int)) != hipSuccess) return bail(CF_ERR_HIP);
+    if (hipMemsetAsync(c->sync, 0, 2 * (c->ntiles + 2) * sizeof(int), c->stream) != hipSuccess)
+      return bail(CF_ERR_HIP);
+  }
   long nb = cf::update_blocks(g);
   if (hipMalloc(&c->partial, (size_t)nb * 5 * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
   if (hipMalloc(&c->er_dev, 8 * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
@@ -216,9 +241,11 @@ extern "C" void cf_destroy(cf_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   double* ptrs[] = {c->Ui, c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->MR, c->NL,
-                    c->NR, c->Pr, c->VS, c->partial, c->er_dev, c->prof};
+                    c->NR, c->Pr, c->VS, c->Sm, c->partial, c->er_dev, c->prof};
   for (double* q : ptrs)
     if (q) (void)hipFree(q);
+  if (c->order) (void)hipFree(c->order);
+  if (c->sync) (void)hipFree(c->sync);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -421,21 +448,43 @@ extern "C" int cf_viscous_terms(cf_ctx* c, const double* const U_[5], double* co
   return sync(c);
 }
 
+// LU-SGS forward/backward sweeps (Dual_time_stepping.cpp:514-1277)
+static void sweeps(cf_ctx* c, double deltaT) {
+  const Geo& g = c->g;
+  if (c->schedule == 1) {
+    cf::launch_sweeps_planes(c->stream, c->Ui, c->MR, c->NL, c->NR, c->prof, g, deltaT, c->p.n_sweeps);
+    return;
+  }
+  cf::launch_sweep_prep(c->stream, c->Ui, c->prof, c->Sm, g, deltaT);
+  cf::launch_sweeps(c->stream, c->Sm, c->MR, c->NL, c->NR, c->prof, g, c->p.n_sweeps, c->order, c->sync, c->pub,
+                    c->spin_limit);
+}
+
+// er to the host and the sweep error flags checked
+static int finish_iteration(cf_ctx* c, double er[5]) {
+  int flags[2] = {0, 0};
+  HIPCHK(c, hipMemcpyAsync(er, c->er_dev, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&flags[0], c->sync + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&flags[1], c->sync + c->ntiles + 3, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  int r = sync(c);
+  if (r) return r;
+  if (c->schedule == 0 && (flags[0] || flags[1]))
+    return fail(c, CF_ERR_TIMEOUT, "LU-SGS sweep: a tile waited past its bound for an upstream tile");
+  for (int v = 0; v < 5; v++)
+    if (!std::isfinite(er[v])) return fail(c, CF_ERR_NONFINITE, "residual norm er%d is not finite", v + 1);
+  return CF_OK;
+}
+
 // the implicit part of one inner iteration on device-resident fields
 static int dual_time_device(cf_ctx* c, double deltaT, double er[5], bool want_nl) {
   const Geo& g = c->g;
   cf::launch_rhs(c->stream, c->Ui, c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->prof, c->MR,
                  want_nl ? c->NL : nullptr, g, deltaT, c->p.f_drive);
-  cf::launch_sweeps(c->stream, c->Ui, c->MR, c->NL, c->NR, c->prof, g, deltaT, c->p.n_sweeps);
+  sweeps(c, deltaT);
   if (want_nl) cf::launch_copy_interior(c->stream, c->NR, c->NL, g);
   double DN = 1. / ((double)c->p.X_out * c->p.Y_out * c->p.Z_out);
   cf::launch_update(c->stream, c->Ui, c->NR, c->prof, c->partial, c->er_dev, g, DN);
-  HIPCHK(c, hipMemcpyAsync(er, c->er_dev, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  int r = sync(c);
-  if (r) return r;
-  for (int v = 0; v < 5; v++)
-    if (!std::isfinite(er[v])) return fail(c, CF_ERR_NONFINITE, "residual norm er%d is not finite", v + 1);
-  return CF_OK;
+  return finish_iteration(c, er);
 }
 
 extern "C" int cf_dual_time_stepping(cf_ctx* c, double deltaT, double er[5], double* const U_[5],
@@ -595,15 +644,15 @@ extern "C" int cf_profile_inner_iteration(cf_ctx* c, double er[5], double stage_
   cf::launch_rhs(c->stream, c->Ui, c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->prof, c->MR, nullptr, g,
                  c->p.deltaT, c->p.f_drive);
   HIPCHK(c, hipEventRecord(ev[6], c->stream));
-  cf::launch_sweeps(c->stream, c->Ui, c->MR, c->NL, c->NR, c->prof, g, c->p.deltaT, c->p.n_sweeps);
+  sweeps(c, c->p.deltaT);
   HIPCHK(c, hipEventRecord(ev[7], c->stream));
   double DN = 1. / ((double)c->p.X_out * c->p.Y_out * c->p.Z_out);
   cf::launch_update(c->stream, c->Ui, c->NR, c->prof, c->partial, c->er_dev, g, DN);
   HIPCHK(c, hipEventRecord(ev[8], c->stream));
-  HIPCHK(c, hipMemcpyAsync(er, c->er_dev, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  int r = sync(c);
+  int r = finish_iteration(c, er);
   if (r) return r;
-  const int nl[8] = {3, 1, 1, 1, 3, 1, 2 * nplanes * c->p.n_sweeps, 2};
+  const int nsw = c->schedule == 1 ? 2 * nplanes * c->p.n_sweeps : 1 + 4 * c->p.n_sweeps;
+  const int nl[8] = {3, 1, 1, 1, 3, 1, nsw, 2};
   for (int q = 0; q < 8; q++) {
     float ms = 0;
     HIPCHK(c, hipEventElapsedTime(&ms, ev[q], ev[q + 1]));
@@ -611,8 +660,13 @@ extern "C" int cf_profile_inner_iteration(cf_ctx* c, double er[5], double stage_
     launches[q] = nl[q];
   }
   for (int q = 0; q < 9; q++) (void)hipEventDestroy(ev[q]);
-  for (int v = 0; v < 5; v++)
-    if (!std::isfinite(er[v])) return fail(c, CF_ERR_NONFINITE, "residual norm er%d is not finite", v + 1);
+  return CF_OK;
+}
+
+extern "C" int cf_set_sweep_schedule(cf_ctx* c, int schedule, int publish_every) {
+  if (!c || schedule < 0 || schedule > 1 || publish_every < 1) return fail(c, CF_ERR_ARG, "bad schedule");
+  c->schedule = schedule;
+  c->pub = publish_every;
   return CF_OK;
 }
 

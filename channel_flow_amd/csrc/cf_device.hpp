@@ -200,13 +200,10 @@ __device__ __forceinline__ double lambda_dir(const PState& s, double n0, double 
   return ((s.beta + 1) * fabs(U) + S) / 2;
 }
 
-// D^-1 = {d11, d22 (= d33 = d44 = d55), d51}, Dual_time_stepping.cpp:464-489
-__device__ __forceinline__ void dinv(const PState& s, const double m9[9], double dXI, double dET, double dZT,
-                                     double deltaT, double* d11, double* d22, double* d51) {
-  double U, V, W;
-  double lx = lambda_dir(s, m9[0], m9[1], m9[2], &U);
-  double ly = lambda_dir(s, m9[3], m9[4], m9[5], &V);
-  double lz = lambda_dir(s, m9[6], m9[7], m9[8], &W);
+// D^-1 = {d11, d22 (= d33 = d44 = d55), d51}, Dual_time_stepping.cpp:464-489,
+// from the three spectral radii of the cell
+__device__ __forceinline__ void dinv_lam(const PState& s, double lx, double ly, double lz, double dXI, double dET,
+                                         double dZT, double deltaT, double* d11, double* d22, double* d51) {
   double lambda = lx / dXI + ly / dET + lz / dZT;
   double beta = s.beta;
   *d11 = 2 * deltaT / (3 * beta + 2 * lambda * deltaT);
@@ -215,6 +212,16 @@ __device__ __forceinline__ void dinv(const PState& s, const double m9[9], double
   double temp2 = KG * (3 + 2 * lambda * deltaT) * (3 * beta + 2 * lambda * deltaT) * s.rho;
   *d51 = -temp1 / temp2;
 }
+
+__device__ __forceinline__ void dinv(const PState& s, const double m9[9], double dXI, double dET, double dZT,
+                                     double deltaT, double* d11, double* d22, double* d51) {
+  double U, V, W;
+  double lx = lambda_dir(s, m9[0], m9[1], m9[2], &U);
+  double ly = lambda_dir(s, m9[3], m9[4], m9[5], &V);
+  double lz = lambda_dir(s, m9[6], m9[7], m9[8], &W);
+  dinv_lam(s, lx, ly, lz, dXI, dET, dZT, deltaT, d11, d22, d51);
+}
+
 
 // Split flux Jacobian of one cell along metric row n (Dual_time_stepping.cpp
 // :608-632 forward, :968-972 backward diagonals).  Entries 52..54 are
@@ -230,6 +237,45 @@ __device__ __forceinline__ Split split_jac(const PState& s, double n0, double n1
   double lam = lambda_dir(s, n0, n1, n2, &U);
   double dd = fabs(lam);
   double beta = s.beta, P = s.P, rho = s.rho, T = s.T;
+  a.p11 = 0.5 * (beta * U + dd);
+  a.p12 = 0.5 * P * KG * beta * n0;
+  a.p13 = 0.5 * P * KG * beta * n1;
+  a.p14 = 0.5 * P * KG * beta * n2;
+  a.p21 = 0.5 * n0 / rho;
+  a.p22 = 0.5 * (U + dd);
+  a.p31 = 0.5 * n1 / rho;
+  a.p41 = 0.5 * n2 / rho;
+  a.p51 = 0.5 * U * (KG - 1) * (beta - 1) / rho / KG;
+  if (dir == 0) {
+    a.p52 = 0.5 * n0 * beta * T * (KG - 1);
+    a.p53 = 0.5 * n1 * beta * T * (KG - 1);
+    a.p54 = 0.5 * n2 * beta * T * (KG - 1);
+  } else {
+    a.p52 = 0.5 * beta * T * (KG - 1) * n0;
+    a.p53 = 0.5 * beta * T * (KG - 1) * n1;
+    a.p54 = 0.5 * beta * T * (KG - 1) * n2;
+  }
+  a.m11 = 0.5 * (beta * U - dd);
+  a.m22 = 0.5 * (U - dd);
+  return a;
+}
+
+// Per-cell summary used by the sweeps: the cell quantities the split
+// Jacobians need (rho, P, T, beta and, per direction, the contravariant
+// velocity and spectral radius) plus D^-1.  Each is the value the reference
+// recomputes at every use, computed once.
+enum SumField : int { S_RHO = 0, S_P, S_T, S_BETA, S_UX, S_LX, S_UY, S_LY, S_UZ, S_LZ, S_D11, S_D22, S_D51, S_N };
+
+struct Nb {
+  double rho, P, T, beta, U, lam;
+};
+
+// split_jac with the cell quantities taken from the summary
+__device__ __forceinline__ Split split_nb(const Nb& c, double n0, double n1, double n2, int dir) {
+  Split a;
+  double U = c.U;
+  double dd = fabs(c.lam);
+  double beta = c.beta, P = c.P, rho = c.rho, T = c.T;
   a.p11 = 0.5 * (beta * U + dd);
   a.p12 = 0.5 * P * KG * beta * n0;
   a.p13 = 0.5 * P * KG * beta * n1;

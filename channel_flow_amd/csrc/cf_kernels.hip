@@ -393,6 +393,240 @@ __global__ void __launch_bounds__(256) k_sweep_bwd(const double* __restrict__ Ui
   NR[4 * nf + c] = N0[4] - (d51 * UU[0] + d22 * UU[4]);
 }
 
+// ---------------------------------------------------------------------------
+// LU-SGS sweeps as one persistent launch per direction.
+//
+// The sweep result depends only on the dependency DAG (each cell needs its
+// three upstream neighbours), so any topological visiting order reproduces
+// the hyperplane order of Dual_time_stepping.cpp:563-1250 bit for bit.  Here
+// one 64-lane wave owns an 8x8 tile of (i,j) columns and walks k: lane (a,b)
+// handles k = kmin + tau - a - b at step tau (a hyperplane inside the tile).
+// Neighbours inside the tile come through double-buffered LDS, the previous
+// cell of the column stays in registers, and values from the two upstream
+// tiles are read from global memory once their progress counters (published
+// with agent-scope release every `pub` steps) cover the step.  Tiles take
+// tickets in dependency order, so a wave only ever waits on tiles that are
+// already running; every wait is bounded and reports CF_ERR_TIMEOUT.
+// ---------------------------------------------------------------------------
+constexpr int TI = 8, TJ = 8;
+
+__global__ void __launch_bounds__(256) k_sweep_prep(const double* __restrict__ Ui, const double* __restrict__ prof,
+                                                    double* __restrict__ S, Geo g, double deltaT) {
+  int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  int i = 3 + blockIdx.z;
+  if (k > g.nz || j > g.ny) return;
+  const long nf = g.nf;
+  long c = at(g, i, j, k);
+  PState s = prim_dt(Ui[c], Ui[nf + c], Ui[2 * nf + c], Ui[3 * nf + c], Ui[4 * nf + c], pf(prof, g.Ym, SET_C, 0, j));
+  double m9[9];
+#pragma unroll
+  for (int q = 0; q < 9; q++) m9[q] = pf(prof, g.Ym, SET_C, 1 + q, j);
+  double Ux, Uy, Uz;
+  double lx = lambda_dir(s, m9[0], m9[1], m9[2], &Ux);
+  double ly = lambda_dir(s, m9[3], m9[4], m9[5], &Uy);
+  double lz = lambda_dir(s, m9[6], m9[7], m9[8], &Uz);
+  double d11, d22, d51;
+  dinv_lam(s, lx, ly, lz, g.dXI, g.dET, g.dZT, deltaT, &d11, &d22, &d51);
+  const double v[S_N] = {s.rho, s.P, s.T, s.beta, Ux, lx, Uy, ly, Uz, lz, d11, d22, d51};
+#pragma unroll
+  for (int q = 0; q < S_N; q++) S[q * nf + c] = v[q];
+}
+
+__global__ void k_sweep_reset(int* __restrict__ sync, int n) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) sync[t] = 0;
+}
+
+__device__ __forceinline__ int wait_progress(int* p, int need, int* err, long limit) {
+  int v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  long n = 0;
+  while (v < need) {
+    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return need;
+    if (++n > limit) {
+      __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return need;
+    }
+    __builtin_amdgcn_s_sleep(2);
+    v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return v;
+}
+
+__device__ __forceinline__ Nb nb_from_summary(const double* __restrict__ S, long nf, long p, int dir) {
+  Nb c;
+  c.rho = S[S_RHO * nf + p];
+  c.P = S[S_P * nf + p];
+  c.T = S[S_T * nf + p];
+  c.beta = S[S_BETA * nf + p];
+  c.U = S[(S_UX + 2 * dir) * nf + p];
+  c.lam = S[(S_LX + 2 * dir) * nf + p];
+  return c;
+}
+
+// LDS slot of one lane: N(5), rho, P, T, beta, Ux, lx, Uy, ly
+constexpr int LDS_F = 13;
+
+template <bool FWD>
+__global__ void __launch_bounds__(64) k_sweep_tiles(const double* __restrict__ S, const double* __restrict__ SRC,
+                                                    double* __restrict__ DST, const double* __restrict__ prof,
+                                                    Geo g, int nTI, int nTJ, const int* __restrict__ order,
+                                                    int* __restrict__ sync, int pub, long spin_limit) {
+  // FWD: SRC = MR, DST = NL   (NL = D^-1 (MR - L))
+  // BWD: SRC = NL, DST = NR   (NR = NL - D^-1 U)
+  __shared__ double lds[2][LDS_F][64];
+  __shared__ int s_ticket;
+  const int ntiles = nTI * nTJ;
+  int* ticket = sync;
+  int* err = sync + 1;
+  int* progress = sync + 2;
+  const int lane = threadIdx.x;
+  const int a = lane & 7, b = lane >> 3;
+  if (lane == 0) s_ticket = atomicAdd(ticket, 1);
+  __syncthreads();
+  const int t = s_ticket;
+  if (t >= ntiles) return;
+  const int tile = order[FWD ? t : ntiles - 1 - t];
+  const int TX = tile % nTI, TY = tile / nTI;
+  const int i = 3 + TX * TI + a, j = 2 + TY * TJ + b;
+  const bool col = i <= g.gend && j <= g.ny;
+  const int Nz = g.nz - 1;
+  const int nsteps = Nz + (TI - 1) + (TJ - 1);
+  const int upx = FWD ? (TX > 0 ? tile - 1 : -1) : (TX < nTI - 1 ? tile + 1 : -1);
+  const int upy = FWD ? (TY > 0 ? tile - nTI : -1) : (TY < nTJ - 1 ? tile + nTI : -1);
+  const int off = FWD ? a + b : (TI - 1 - a) + (TJ - 1 - b);
+  const int ax_lane = FWD ? lane - 1 : lane + 1;   // (a-1,b) or (a+1,b)
+  const int ay_lane = FWD ? lane - 8 : lane + 8;   // (a,b-1) or (a,b+1)
+  const bool x_in_tile = FWD ? a > 0 : a < TI - 1;
+  const bool y_in_tile = FWD ? b > 0 : b < TJ - 1;
+  const bool x_dom = FWD ? i - 1 >= 3 : i + 1 <= g.gend;
+  const bool y_dom = FWD ? j - 1 >= 2 : j + 1 <= g.ny;
+  const int jy = FWD ? j - 1 : j + 1;
+  const long nf = g.nf;
+  const long sx = FWD ? -g.plane : g.plane;
+  const long sy = FWD ? -(long)g.Zp : (long)g.Zp;
+  // metric rows of the neighbours (they vary with j only)
+  const double mx0 = pf(prof, g.Ym, SET_C, 1, j), mx1 = pf(prof, g.Ym, SET_C, 2, j), mx2 = pf(prof, g.Ym, SET_C, 3, j);
+  const double my0 = pf(prof, g.Ym, SET_C, 4, jy), my1 = pf(prof, g.Ym, SET_C, 5, jy),
+               my2 = pf(prof, g.Ym, SET_C, 6, jy);
+  const double mz0 = pf(prof, g.Ym, SET_C, 7, j), mz1 = pf(prof, g.Ym, SET_C, 8, j), mz2 = pf(prof, g.Ym, SET_C, 9, j);
+  int seen_x = 0, seen_y = 0;
+  double zN[5] = {0, 0, 0, 0, 0};
+  Nb zc = {1, 0, 0, 0, 0, 0};
+  for (int tau = 0; tau < nsteps; tau++) {
+    const int kk = tau - off;
+    const bool act = col && kk >= 0 && kk < Nz;
+    const int k = FWD ? 2 + kk : g.nz - kk;
+    if (upx >= 0 && seen_x < min(tau + TI, nsteps))
+      seen_x = wait_progress(progress + upx, min(tau + TI, nsteps), err, spin_limit);
+    if (upy >= 0 && seen_y < min(tau + TJ, nsteps))
+      seen_y = wait_progress(progress + upy, min(tau + TJ, nsteps), err, spin_limit);
+    const int pb = (tau + 1) & 1, cb = tau & 1;
+    if (act) {
+      const long c = at(g, i, j, k);
+      double acc[3][5];
+      // x neighbour
+      if (x_dom) {
+        Nb nc;
+        double N[5];
+        if (x_in_tile) {
+#pragma unroll
+          for (int v = 0; v < 5; v++) N[v] = lds[pb][v][ax_lane];
+          nc.rho = lds[pb][5][ax_lane]; nc.P = lds[pb][6][ax_lane]; nc.T = lds[pb][7][ax_lane];
+          nc.beta = lds[pb][8][ax_lane]; nc.U = lds[pb][9][ax_lane]; nc.lam = lds[pb][10][ax_lane];
+        } else {
+          const long p = c + sx;
+#pragma unroll
+          for (int v = 0; v < 5; v++) N[v] = DST[v * nf + p];
+          nc = nb_from_summary(S, nf, p, 0);
+        }
+        Split sp = split_nb(nc, mx0, mx1, mx2, 0);
+        if (FWD) lower_sum(sp, N, acc[0]); else upper_sum(sp, N, acc[0]);
+      } else {
+#pragma unroll
+        for (int v = 0; v < 5; v++) acc[0][v] = 0.0;
+      }
+      // y neighbour
+      if (y_dom) {
+        Nb nc;
+        double N[5];
+        if (y_in_tile) {
+#pragma unroll
+          for (int v = 0; v < 5; v++) N[v] = lds[pb][v][ay_lane];
+          nc.rho = lds[pb][5][ay_lane]; nc.P = lds[pb][6][ay_lane]; nc.T = lds[pb][7][ay_lane];
+          nc.beta = lds[pb][8][ay_lane]; nc.U = lds[pb][11][ay_lane]; nc.lam = lds[pb][12][ay_lane];
+        } else {
+          const long p = c + sy;
+#pragma unroll
+          for (int v = 0; v < 5; v++) N[v] = DST[v * nf + p];
+          nc = nb_from_summary(S, nf, p, 1);
+        }
+        Split sp = split_nb(nc, my0, my1, my2, 1);
+        if (FWD) lower_sum(sp, N, acc[1]); else upper_sum(sp, N, acc[1]);
+      } else {
+#pragma unroll
+        for (int v = 0; v < 5; v++) acc[1][v] = 0.0;
+      }
+      // z neighbour: the previous cell of this column
+      if (kk > 0) {
+        Split sp = split_nb(zc, mz0, mz1, mz2, 2);
+        if (FWD) lower_sum(sp, zN, acc[2]); else upper_sum(sp, zN, acc[2]);
+      } else {
+#pragma unroll
+        for (int v = 0; v < 5; v++) acc[2][v] = 0.0;
+      }
+      double L[5];
+#pragma unroll
+      for (int v = 0; v < 5; v++) L[v] = acc[0][v] / g.dXI + acc[1][v] / g.dET + acc[2][v] / g.dZT;
+      const double d11 = S[S_D11 * nf + c], d22 = S[S_D22 * nf + c], d51 = S[S_D51 * nf + c];
+      double M[5], out[5];
+#pragma unroll
+      for (int v = 0; v < 5; v++) M[v] = SRC[v * nf + c];
+      if (FWD) {
+        out[0] = d11 * (M[0] - L[0]);
+        out[1] = d22 * (M[1] - L[1]);
+        out[2] = d22 * (M[2] - L[2]);
+        out[3] = d22 * (M[3] - L[3]);
+        out[4] = d51 * (M[0] - L[0]) + d22 * (M[4] - L[4]);
+      } else {
+        out[0] = M[0] - d11 * L[0];
+        out[1] = M[1] - d22 * L[1];
+        out[2] = M[2] - d22 * L[2];
+        out[3] = M[3] - d22 * L[3];
+        out[4] = M[4] - (d51 * L[0] + d22 * L[4]);
+      }
+#pragma unroll
+      for (int v = 0; v < 5; v++) DST[v * nf + c] = out[v];
+      // publish to the in-tile neighbours and keep the column's own cell
+      Nb me;
+      me.rho = S[S_RHO * nf + c];
+      me.P = S[S_P * nf + c];
+      me.T = S[S_T * nf + c];
+      me.beta = S[S_BETA * nf + c];
+#pragma unroll
+      for (int v = 0; v < 5; v++) {
+        lds[cb][v][lane] = out[v];
+        zN[v] = out[v];
+      }
+      lds[cb][5][lane] = me.rho;
+      lds[cb][6][lane] = me.P;
+      lds[cb][7][lane] = me.T;
+      lds[cb][8][lane] = me.beta;
+      lds[cb][9][lane] = S[S_UX * nf + c];
+      lds[cb][10][lane] = S[S_LX * nf + c];
+      lds[cb][11][lane] = S[S_UY * nf + c];
+      lds[cb][12][lane] = S[S_LY * nf + c];
+      zc = me;
+      zc.U = S[S_UZ * nf + c];
+      zc.lam = S[S_LZ * nf + c];
+    }
+    __syncthreads();
+    if (lane == 0 && ((tau + 1) % pub == 0 || tau + 1 == nsteps))
+      __hip_atomic_store(progress + tile, tau + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 __global__ void k_copy_interior(const double* __restrict__ src, double* __restrict__ dst, Geo g) {
   int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
   int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
@@ -458,18 +692,18 @@ __global__ void __launch_bounds__(256) k_update(double* __restrict__ Ui, const d
   }
 }
 
-__global__ void __launch_bounds__(256) k_reduce_partials(const double* __restrict__ partial, long nblocks,
-                                                         double* __restrict__ out, double DN) {
-  __shared__ double red[5][256];
+__global__ void __launch_bounds__(1024) k_reduce_partials(const double* __restrict__ partial, long nblocks,
+                                                          double* __restrict__ out, double DN) {
+  __shared__ double red[5][1024];
   int tid = threadIdx.x;
   double s[5] = {0, 0, 0, 0, 0};
-  for (long b = tid; b < nblocks; b += 256)
+  for (long b = tid; b < nblocks; b += 1024)
 #pragma unroll
     for (int v = 0; v < 5; v++) s[v] += partial[b * 5 + v];
 #pragma unroll
   for (int v = 0; v < 5; v++) red[v][tid] = s[v];
   __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
+  for (int st = 512; st > 0; st >>= 1) {
     if (tid < st)
 #pragma unroll
       for (int v = 0; v < 5; v++) red[v][tid] += red[v][tid + st];
@@ -522,8 +756,34 @@ void launch_rhs(hipStream_t st, const double* Ui, const double* Un, const double
                      MR, NL, g, deltaT, f_drive);
 }
 
-void launch_sweeps(hipStream_t st, const double* Ui, const double* MR, double* NL, double* NR, const double* prof,
-                   const Geo& g, double deltaT, int n_sweeps) {
+void launch_sweep_prep(hipStream_t st, const double* Ui, const double* prof, double* S, const Geo& g,
+                       double deltaT) {
+  dim3 blk(64, 4);
+  hipLaunchKernelGGL(k_sweep_prep, box_grid(g.nz - 1, g.ny - 1, g.gend - 2, blk), blk, 0, st, Ui, prof, S, g, deltaT);
+}
+
+int sweep_tiles(const Geo& g, int* nTI, int* nTJ) {
+  *nTI = (g.gend - 2 + TI - 1) / TI;
+  *nTJ = (g.ny - 1 + TJ - 1) / TJ;
+  return *nTI * *nTJ;
+}
+
+void launch_sweeps(hipStream_t st, const double* S, const double* MR, double* NL, double* NR, const double* prof,
+                   const Geo& g, int n_sweeps, const int* order, int* sync, int pub, long spin_limit) {
+  int nTI, nTJ;
+  int nt = sweep_tiles(g, &nTI, &nTJ);
+  for (int sweep = 0; sweep < n_sweeps; sweep++) {
+    hipLaunchKernelGGL(k_sweep_reset, dim3((nt + 2 + 255) / 256), dim3(256), 0, st, sync, nt + 2);
+    hipLaunchKernelGGL(k_sweep_tiles<true>, dim3(nt), dim3(64), 0, st, S, MR, NL, prof, g, nTI, nTJ, order, sync, pub,
+                       spin_limit);
+    hipLaunchKernelGGL(k_sweep_reset, dim3((nt + 2 + 255) / 256), dim3(256), 0, st, sync + nt + 2, nt + 2);
+    hipLaunchKernelGGL(k_sweep_tiles<false>, dim3(nt), dim3(64), 0, st, S, NL, NR, prof, g, nTI, nTJ, order,
+                       sync + nt + 2, pub, spin_limit);
+  }
+}
+
+void launch_sweeps_planes(hipStream_t st, const double* Ui, const double* MR, double* NL, double* NR,
+                          const double* prof, const Geo& g, double deltaT, int n_sweeps) {
   int mmin = 7, mmax = g.gend + g.ny + g.nz;
   int nj = g.ny - 1;
   dim3 blk(64);
@@ -557,7 +817,7 @@ void launch_update(hipStream_t st, double* Ui, const double* NR, const double* p
   dim3 blk(64, 4);
   dim3 gr = box_grid(g.nz - 1, g.ny - 1, g.gend - 2, blk);
   hipLaunchKernelGGL(k_update, gr, blk, 0, st, Ui, NR, prof, partial, g);
-  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, st, partial, update_blocks(g), er_dev, DN);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, st, partial, update_blocks(g), er_dev, DN);
 }
 
 }  // namespace cf

@@ -11,8 +11,13 @@ void launch_viscous(hipStream_t st, const double* U, const double* prof, double*
 void launch_rhs(hipStream_t st, const double* Ui, const double* Un, const double* Uq, const double* Fx,
                 const double* Fy, const double* Fz, const double* vF, const double* prof, double* MR, double* NL,
                 const Geo& g, double deltaT, double f_drive);
-void launch_sweeps(hipStream_t st, const double* Ui, const double* MR, double* NL, double* NR, const double* prof,
-                   const Geo& g, double deltaT, int n_sweeps);
+void launch_sweep_prep(hipStream_t st, const double* Ui, const double* prof, double* S, const Geo& g,
+                       double deltaT);
+int sweep_tiles(const Geo& g, int* nTI, int* nTJ);
+void launch_sweeps(hipStream_t st, const double* S, const double* MR, double* NL, double* NR, const double* prof,
+                   const Geo& g, int n_sweeps, const int* order, int* sync, int pub, long spin_limit);
+void launch_sweeps_planes(hipStream_t st, const double* Ui, const double* MR, double* NL, double* NR,
+                          const double* prof, const Geo& g, double deltaT, int n_sweeps);
 void launch_copy_interior(hipStream_t st, const double* src, double* dst, const Geo& g);
 long update_blocks(const Geo& g);
 void launch_update(hipStream_t st, double* Ui, const double* NR, const double* prof, double* partial, double* er_dev,

@@ -147,6 +147,12 @@ int cf_write_output(cf_ctx* ctx, int step, const char* dir);
  * launches[8]: kernel launches per stage.  No counterpart in the reference
  * (it has no timers, MPI_prm.h:14). */
 int cf_profile_inner_iteration(cf_ctx* ctx, double er[5], double stage_ms[8], int launches[8]);
+/* Visiting order of the LU-SGS sweeps; both give identical bits because each
+ * cell depends only on its three upstream neighbours.  schedule 0 (default):
+ * one persistent launch per sweep direction, 8x8-column tiles in dependency
+ * order, progress published every publish_every steps; schedule 1: one
+ * launch per hyperplane i+j+k = m (the reference order, Hyperplane.cpp). */
+int cf_set_sweep_schedule(cf_ctx* ctx, int schedule, int publish_every);
 /* Synchronises the context's stream. */
 int cf_synchronize(cf_ctx* ctx);
 
