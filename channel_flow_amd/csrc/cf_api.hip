@@ -45,7 +45,6 @@ struct cf_ctx {
   int* sync = nullptr;    // ticket, error flag, per-tile progress (x2 directions)
   int ntiles = 0;
   int schedule = 0;       // 0: persistent tiles, 1: one hyperplane per launch
-  int pub = 1;            // steps between progress publications
   long spin_limit = 1L << 22;
   double* partial = nullptr;
   double* er_dev = nullptr;
@@ -209,7 +208,8 @@ extern "C" int cf_create(const cf_params* p, cf_ctx** out) {
     // the reference leaves work arrays uninitialised; here they start at zero
     if (hipMemsetAsync(*f.ptr, 0, nf * f.n * sizeof(double), c->stream) != hipSuccess) return bail(CF_ERR_HIP);
   }
-  if (hipMalloc(&c->Sm, nf * cf::S_N * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
+  if (hipMalloc(&c->Sm, (size_t)cf::sweep_field_elems(g) * (cf::S_N + 15) * sizeof(double)) != hipSuccess)
+    return bail(CF_ERR_HIP);
   {
     int nTI, nTJ;
     c->ntiles = cf::sweep_tiles(g, &nTI, &nTJ);
@@ -227,7 +227,7 @@ extern "C" int cf_create(const cf_params* p, cf_ctx** out) {
     if (hipMemsetAsync(c->sync, 0, 2 * (c->ntiles + 2) * sizeof(int), c->stream) != hipSuccess)
       return bail(CF_ERR_HIP);
   }
-  long nb = cf::update_blocks(g);
+  long nb = std::max(cf::update_blocks(g), cf::sweep_update_blocks(g));
   if (hipMalloc(&c->partial, (size_t)nb * 5 * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
   if (hipMalloc(&c->er_dev, 8 * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
   if (hipMalloc(&c->prof, c->hprof.size() * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
@@ -455,9 +455,17 @@ static void sweeps(cf_ctx* c, double deltaT) {
     cf::launch_sweeps_planes(c->stream, c->Ui, c->MR, c->NL, c->NR, c->prof, g, deltaT, c->p.n_sweeps);
     return;
   }
-  cf::launch_sweep_prep(c->stream, c->Ui, c->prof, c->Sm, g, deltaT);
-  cf::launch_sweeps(c->stream, c->Sm, c->MR, c->NL, c->NR, c->prof, g, c->p.n_sweeps, c->order, c->sync, c->pub,
-                    c->spin_limit);
+  cf::launch_sweeps(c->stream, c->Ui, c->MR, c->Sm, c->NL, c->NR, c->prof, g, deltaT, c->p.n_sweeps, c->order,
+                    c->sync, c->spin_limit);
+}
+
+// primitive update + norms after the sweeps (reads NR where the schedule left it)
+static void update(cf_ctx* c) {
+  double DN = 1. / ((double)c->p.X_out * c->p.Y_out * c->p.Z_out);
+  if (c->schedule == 1)
+    cf::launch_update(c->stream, c->Ui, c->NR, c->prof, c->partial, c->er_dev, c->g, DN);
+  else
+    cf::launch_update_sk(c->stream, c->Ui, c->Sm, c->prof, c->partial, c->er_dev, c->g, DN);
 }
 
 // er to the host and the sweep error flags checked
@@ -481,9 +489,11 @@ static int dual_time_device(cf_ctx* c, double deltaT, double er[5], bool want_nl
   cf::launch_rhs(c->stream, c->Ui, c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->prof, c->MR,
                  want_nl ? c->NL : nullptr, g, deltaT, c->p.f_drive);
   sweeps(c, deltaT);
-  if (want_nl) cf::launch_copy_interior(c->stream, c->NR, c->NL, g);
-  double DN = 1. / ((double)c->p.X_out * c->p.Y_out * c->p.Z_out);
-  cf::launch_update(c->stream, c->Ui, c->NR, c->prof, c->partial, c->er_dev, g, DN);
+  if (want_nl) {
+    if (c->schedule == 0) cf::launch_sweep_outputs(c->stream, c->Sm, c->NL, c->NR, g);
+    cf::launch_copy_interior(c->stream, c->NR, c->NL, g);
+  }
+  update(c);
   return finish_iteration(c, er);
 }
 
@@ -646,12 +656,11 @@ extern "C" int cf_profile_inner_iteration(cf_ctx* c, double er[5], double stage_
   HIPCHK(c, hipEventRecord(ev[6], c->stream));
   sweeps(c, c->p.deltaT);
   HIPCHK(c, hipEventRecord(ev[7], c->stream));
-  double DN = 1. / ((double)c->p.X_out * c->p.Y_out * c->p.Z_out);
-  cf::launch_update(c->stream, c->Ui, c->NR, c->prof, c->partial, c->er_dev, g, DN);
+  update(c);
   HIPCHK(c, hipEventRecord(ev[8], c->stream));
   int r = finish_iteration(c, er);
   if (r) return r;
-  const int nsw = c->schedule == 1 ? 2 * nplanes * c->p.n_sweeps : 1 + 4 * c->p.n_sweeps;
+  const int nsw = c->schedule == 1 ? 2 * nplanes * c->p.n_sweeps : 1 + 6 * c->p.n_sweeps;
   const int nl[8] = {3, 1, 1, 1, 3, 1, nsw, 2};
   for (int q = 0; q < 8; q++) {
     float ms = 0;
@@ -663,10 +672,9 @@ extern "C" int cf_profile_inner_iteration(cf_ctx* c, double er[5], double stage_
   return CF_OK;
 }
 
-extern "C" int cf_set_sweep_schedule(cf_ctx* c, int schedule, int publish_every) {
-  if (!c || schedule < 0 || schedule > 1 || publish_every < 1) return fail(c, CF_ERR_ARG, "bad schedule");
+extern "C" int cf_set_sweep_schedule(cf_ctx* c, int schedule) {
+  if (!c || schedule < 0 || schedule > 1) return fail(c, CF_ERR_ARG, "bad schedule");
   c->schedule = schedule;
-  c->pub = publish_every;
   return CF_OK;
 }
 

@@ -402,35 +402,108 @@ __global__ void __launch_bounds__(256) k_sweep_bwd(const double* __restrict__ Ui
 // one 64-lane wave owns an 8x8 tile of (i,j) columns and walks k: lane (a,b)
 // handles k = kmin + tau - a - b at step tau (a hyperplane inside the tile).
 // Neighbours inside the tile come through double-buffered LDS, the previous
-// cell of the column stays in registers, and values from the two upstream
-// tiles are read from global memory once their progress counters (published
-// with agent-scope release every `pub` steps) cover the step.  Tiles take
-// tickets in dependency order, so a wave only ever waits on tiles that are
-// already running; every wait is bounded and reports CF_ERR_TIMEOUT.
+// cell of the column stays in registers, and results of the two upstream
+// tiles are read from global memory.
+//
+// Cross-tile hand-off: before a sweep direction starts, the cells whose
+// result another tile reads are filled with a NaN-payload sentinel that no
+// arithmetic produces.  The producer writes its results with agent-scope
+// relaxed atomic stores (coherent at device scope, no L2 write-back needed)
+// and the consumer reads them with agent-scope relaxed atomic loads, re-
+// reading while it still sees the sentinel.  The data carries its own
+// readiness, so there is no flag, no fence and no store-completion wait;
+// each step's loads are issued one step ahead to overlap their latency.
+// Tiles take tickets in dependency order, so a wave only ever waits on tiles
+// that are already running; every wait is bounded and reports
+// CF_ERR_TIMEOUT.
 // ---------------------------------------------------------------------------
 constexpr int TI = 8, TJ = 8;
+constexpr unsigned long long SENTINEL = 0x7FF4C0FFEE5EED01ULL;
 
-__global__ void __launch_bounds__(256) k_sweep_prep(const double* __restrict__ Ui, const double* __restrict__ prof,
-                                                    double* __restrict__ S, Geo g, double deltaT) {
-  int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-  int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
-  int i = 3 + blockIdx.z;
-  if (k > g.nz || j > g.ny) return;
+__device__ __forceinline__ bool is_sentinel(double v) {
+  return (unsigned long long)__double_as_longlong(v) == SENTINEL;
+}
+
+// Sweep-order layout: element (tile, s, lane) with lane = a + 8b and
+// s = (k-2) + a + b, i.e. the step at which the forward walk visits the cell;
+// the backward walk visits it at step nsteps-1-s.  At every step the 64 lanes
+// of a tile touch one contiguous 512-byte row of each field.
+struct Sk {
+  int nTI, nTJ, ntiles, Nz, nsteps;
+  long nsk;  // elements per field
+};
+
+__device__ __forceinline__ long sk_at(const Sk& q, int tile, int s, int lane) {
+  return ((long)tile * q.nsteps + s) * 64 + lane;
+}
+
+// thread (s, lane) of a block of 16 steps x 64 lanes -> cell; false if the
+// slot holds no interior cell
+__device__ __forceinline__ bool sk_cell(const Sk& q, const Geo& g, int tile, int s, int lane, int* i, int* j,
+                                        int* k) {
+  const int a = lane & 7, b = lane >> 3;
+  const int kk = s - a - b;
+  *i = 3 + (tile % q.nTI) * TI + a;
+  *j = 2 + (tile / q.nTI) * TJ + b;
+  *k = 2 + kk;
+  return s < q.nsteps && kk >= 0 && kk < q.Nz && *i <= g.gend && *j <= g.ny;
+}
+
+// summary S and MR in sweep order (gathered from the natural layout: a
+// lane's 16 consecutive steps read 16 consecutive k of its column)
+__global__ void __launch_bounds__(1024) k_sweep_prep(const double* __restrict__ Ui, const double* __restrict__ MR,
+                                                     const double* __restrict__ prof, double* __restrict__ S,
+                                                     double* __restrict__ MRs, Geo g, Sk q, double deltaT) {
+  const int lane = threadIdx.x & 63;
+  const int s = blockIdx.x * 16 + (threadIdx.x >> 6);
+  const int tile = blockIdx.y;
+  int i, j, k;
+  if (!sk_cell(q, g, tile, s, lane, &i, &j, &k)) return;
   const long nf = g.nf;
-  long c = at(g, i, j, k);
-  PState s = prim_dt(Ui[c], Ui[nf + c], Ui[2 * nf + c], Ui[3 * nf + c], Ui[4 * nf + c], pf(prof, g.Ym, SET_C, 0, j));
+  const long c = at(g, i, j, k);
+  const long o = sk_at(q, tile, s, lane);
+  PState st = prim_dt(Ui[c], Ui[nf + c], Ui[2 * nf + c], Ui[3 * nf + c], Ui[4 * nf + c], pf(prof, g.Ym, SET_C, 0, j));
   double m9[9];
 #pragma unroll
-  for (int q = 0; q < 9; q++) m9[q] = pf(prof, g.Ym, SET_C, 1 + q, j);
+  for (int r = 0; r < 9; r++) m9[r] = pf(prof, g.Ym, SET_C, 1 + r, j);
   double Ux, Uy, Uz;
-  double lx = lambda_dir(s, m9[0], m9[1], m9[2], &Ux);
-  double ly = lambda_dir(s, m9[3], m9[4], m9[5], &Uy);
-  double lz = lambda_dir(s, m9[6], m9[7], m9[8], &Uz);
+  double lx = lambda_dir(st, m9[0], m9[1], m9[2], &Ux);
+  double ly = lambda_dir(st, m9[3], m9[4], m9[5], &Uy);
+  double lz = lambda_dir(st, m9[6], m9[7], m9[8], &Uz);
   double d11, d22, d51;
-  dinv_lam(s, lx, ly, lz, g.dXI, g.dET, g.dZT, deltaT, &d11, &d22, &d51);
-  const double v[S_N] = {s.rho, s.P, s.T, s.beta, Ux, lx, Uy, ly, Uz, lz, d11, d22, d51};
+  dinv_lam(st, lx, ly, lz, g.dXI, g.dET, g.dZT, deltaT, &d11, &d22, &d51);
+  const double v[S_N] = {st.rho, st.P, st.T, st.beta, Ux, lx, Uy, ly, Uz, lz, d11, d22, d51};
 #pragma unroll
-  for (int q = 0; q < S_N; q++) S[q * nf + c] = v[q];
+  for (int r = 0; r < S_N; r++) S[r * q.nsk + o] = v[r];
+#pragma unroll
+  for (int r = 0; r < 5; r++) MRs[r * q.nsk + o] = MR[r * nf + c];
+}
+
+// sentinel in the slots that a neighbouring tile reads during the coming
+// sweep direction: lanes of the last tile column/row (forward) or the first
+// (backward)
+__global__ void __launch_bounds__(256) k_sweep_sentinel(double* __restrict__ DST, Sk q, int fwd) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)q.ntiles * q.nsteps * 64) return;
+  const int lane = (int)(idx & 63);
+  const int a = lane & 7, b = lane >> 3;
+  const bool edge = fwd ? (a == TI - 1 || b == TJ - 1) : (a == 0 || b == 0);
+  if (!edge) return;
+  const double sv = __longlong_as_double((long long)SENTINEL);
+#pragma unroll
+  for (int v = 0; v < 5; v++) DST[v * q.nsk + idx] = sv;
+}
+
+// sweep-order field(s) back to the natural layout
+__global__ void __launch_bounds__(1024) k_sk_to_natural(const double* __restrict__ src, double* __restrict__ dst,
+                                                        Geo g, Sk q, int nfields) {
+  const int lane = threadIdx.x & 63;
+  const int s = blockIdx.x * 16 + (threadIdx.x >> 6);
+  const int tile = blockIdx.y;
+  int i, j, k;
+  if (!sk_cell(q, g, tile, s, lane, &i, &j, &k)) return;
+  const long c = at(g, i, j, k), o = sk_at(q, tile, s, lane);
+  for (int v = 0; v < nfields; v++) dst[v * g.nf + c] = src[v * q.nsk + o];
 }
 
 __global__ void k_sweep_reset(int* __restrict__ sync, int n) {
@@ -438,32 +511,21 @@ __global__ void k_sweep_reset(int* __restrict__ sync, int n) {
   if (t < n) sync[t] = 0;
 }
 
-__device__ __forceinline__ int wait_progress(int* p, int need, int* err, long limit) {
-  int v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  long n = 0;
-  while (v < need) {
-    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return need;
-    if (++n > limit) {
-      __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      return need;
-    }
-    __builtin_amdgcn_s_sleep(2);
-    v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  return v;
+__device__ __forceinline__ double ld_coherent(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ Nb nb_from_summary(const double* __restrict__ S, long nf, long p, int dir) {
-  Nb c;
-  c.rho = S[S_RHO * nf + p];
-  c.P = S[S_P * nf + p];
-  c.T = S[S_T * nf + p];
-  c.beta = S[S_BETA * nf + p];
-  c.U = S[(S_UX + 2 * dir) * nf + p];
-  c.lam = S[(S_LX + 2 * dir) * nf + p];
-  return c;
+__device__ __forceinline__ void st_coherent(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+
+// one step's global inputs of a lane
+struct StepIn {
+  double own[S_N];   // summary of the lane's cell
+  double src[5];     // MR (forward) or NL (backward) of the lane's cell
+  double xN[5], yN[5];
+  double xS[6], yS[6];  // remote neighbours: rho, P, T, beta, U, lam
+};
 
 // LDS slot of one lane: N(5), rho, P, T, beta, Ux, lx, Uy, ly
 constexpr int LDS_F = 13;
@@ -471,16 +533,17 @@ constexpr int LDS_F = 13;
 template <bool FWD>
 __global__ void __launch_bounds__(64) k_sweep_tiles(const double* __restrict__ S, const double* __restrict__ SRC,
                                                     double* __restrict__ DST, const double* __restrict__ prof,
-                                                    Geo g, int nTI, int nTJ, const int* __restrict__ order,
-                                                    int* __restrict__ sync, int pub, long spin_limit) {
+                                                    Geo g, Sk q, const int* __restrict__ order,
+                                                    int* __restrict__ sync, long spin_limit) {
+  // all arrays in sweep order (Sk)
   // FWD: SRC = MR, DST = NL   (NL = D^-1 (MR - L))
   // BWD: SRC = NL, DST = NR   (NR = NL - D^-1 U)
+  const int nTI = q.nTI;
   __shared__ double lds[2][LDS_F][64];
   __shared__ int s_ticket;
-  const int ntiles = nTI * nTJ;
+  const int ntiles = q.ntiles;
   int* ticket = sync;
   int* err = sync + 1;
-  int* progress = sync + 2;
   const int lane = threadIdx.x;
   const int a = lane & 7, b = lane >> 3;
   if (lane == 0) s_ticket = atomicAdd(ticket, 1);
@@ -492,9 +555,7 @@ __global__ void __launch_bounds__(64) k_sweep_tiles(const double* __restrict__ S
   const int i = 3 + TX * TI + a, j = 2 + TY * TJ + b;
   const bool col = i <= g.gend && j <= g.ny;
   const int Nz = g.nz - 1;
-  const int nsteps = Nz + (TI - 1) + (TJ - 1);
-  const int upx = FWD ? (TX > 0 ? tile - 1 : -1) : (TX < nTI - 1 ? tile + 1 : -1);
-  const int upy = FWD ? (TY > 0 ? tile - nTI : -1) : (TY < nTJ - 1 ? tile + nTI : -1);
+  const int nsteps = q.nsteps;
   const int off = FWD ? a + b : (TI - 1 - a) + (TJ - 1 - b);
   const int ax_lane = FWD ? lane - 1 : lane + 1;   // (a-1,b) or (a+1,b)
   const int ay_lane = FWD ? lane - 8 : lane + 8;   // (a,b-1) or (a,b+1)
@@ -502,31 +563,80 @@ __global__ void __launch_bounds__(64) k_sweep_tiles(const double* __restrict__ S
   const bool y_in_tile = FWD ? b > 0 : b < TJ - 1;
   const bool x_dom = FWD ? i - 1 >= 3 : i + 1 <= g.gend;
   const bool y_dom = FWD ? j - 1 >= 2 : j + 1 <= g.ny;
+  const bool x_remote = col && x_dom && !x_in_tile;
+  const bool y_remote = col && y_dom && !y_in_tile;
   const int jy = FWD ? j - 1 : j + 1;
-  const long nf = g.nf;
-  const long sx = FWD ? -g.plane : g.plane;
-  const long sy = FWD ? -(long)g.Zp : (long)g.Zp;
-  // metric rows of the neighbours (they vary with j only)
+  const long nf = q.nsk;
+  // remote neighbours: same k, lane (7,b)/(a,7) of the upstream tile (forward)
+  // or (0,b)/(a,0) of the downstream tile (backward), 7 steps apart
+  const long sx = FWD ? (-(long)q.nsteps * 64 + 7 * 64 + 7) : ((long)q.nsteps * 64 - 7 * 64 - 7);
+  const long sy = FWD ? (-(long)nTI * q.nsteps * 64 + 7 * 64 + 56) : ((long)nTI * q.nsteps * 64 - 7 * 64 - 56);
   const double mx0 = pf(prof, g.Ym, SET_C, 1, j), mx1 = pf(prof, g.Ym, SET_C, 2, j), mx2 = pf(prof, g.Ym, SET_C, 3, j);
   const double my0 = pf(prof, g.Ym, SET_C, 4, jy), my1 = pf(prof, g.Ym, SET_C, 5, jy),
                my2 = pf(prof, g.Ym, SET_C, 6, jy);
   const double mz0 = pf(prof, g.Ym, SET_C, 7, j), mz1 = pf(prof, g.Ym, SET_C, 8, j), mz2 = pf(prof, g.Ym, SET_C, 9, j);
-  int seen_x = 0, seen_y = 0;
+
+  auto cell_of = [&](int tau) -> long { return sk_at(q, tile, FWD ? tau : q.nsteps - 1 - tau, lane); };
+  auto active = [&](int tau) -> bool {
+    const int kk = tau - off;
+    return col && kk >= 0 && kk < Nz;
+  };
+  auto load_step = [&](int tau, StepIn& in) {
+    if (!active(tau)) return;
+    const long c = cell_of(tau);
+#pragma unroll
+    for (int q = 0; q < S_N; q++) in.own[q] = S[q * nf + c];
+#pragma unroll
+    for (int v = 0; v < 5; v++) in.src[v] = SRC[v * nf + c];
+    if (x_remote) {
+      const long p = c + sx;
+#pragma unroll
+      for (int v = 0; v < 5; v++) in.xN[v] = ld_coherent(DST + v * nf + p);
+      in.xS[0] = S[S_RHO * nf + p]; in.xS[1] = S[S_P * nf + p]; in.xS[2] = S[S_T * nf + p];
+      in.xS[3] = S[S_BETA * nf + p]; in.xS[4] = S[S_UX * nf + p]; in.xS[5] = S[S_LX * nf + p];
+    }
+    if (y_remote) {
+      const long p = c + sy;
+#pragma unroll
+      for (int v = 0; v < 5; v++) in.yN[v] = ld_coherent(DST + v * nf + p);
+      in.yS[0] = S[S_RHO * nf + p]; in.yS[1] = S[S_P * nf + p]; in.yS[2] = S[S_T * nf + p];
+      in.yS[3] = S[S_BETA * nf + p]; in.yS[4] = S[S_UY * nf + p]; in.yS[5] = S[S_LY * nf + p];
+    }
+  };
+  // re-reads remote values that were not ready when first loaded
+  auto settle = [&](double (&N)[5], long p) {
+    long n = 0;
+    bool pending = true;
+    while (pending) {
+      pending = false;
+#pragma unroll
+      for (int v = 0; v < 5; v++) pending |= is_sentinel(N[v]);
+      if (!pending) break;
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || ++n > spin_limit) {
+        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+      for (int v = 0; v < 5; v++) N[v] = ld_coherent(DST + v * nf + p);
+    }
+  };
+
   double zN[5] = {0, 0, 0, 0, 0};
   Nb zc = {1, 0, 0, 0, 0, 0};
+  StepIn nxt;
+  load_step(0, nxt);
   for (int tau = 0; tau < nsteps; tau++) {
+    StepIn cur = nxt;
+    if (tau + 1 < nsteps) load_step(tau + 1, nxt);
+    const bool act = active(tau);
     const int kk = tau - off;
-    const bool act = col && kk >= 0 && kk < Nz;
-    const int k = FWD ? 2 + kk : g.nz - kk;
-    if (upx >= 0 && seen_x < min(tau + TI, nsteps))
-      seen_x = wait_progress(progress + upx, min(tau + TI, nsteps), err, spin_limit);
-    if (upy >= 0 && seen_y < min(tau + TJ, nsteps))
-      seen_y = wait_progress(progress + upy, min(tau + TJ, nsteps), err, spin_limit);
     const int pb = (tau + 1) & 1, cb = tau & 1;
     if (act) {
-      const long c = at(g, i, j, k);
+      const long c = cell_of(tau);
+      if (x_remote) settle(cur.xN, c + sx);
+      if (y_remote) settle(cur.yN, c + sy);
       double acc[3][5];
-      // x neighbour
       if (x_dom) {
         Nb nc;
         double N[5];
@@ -536,10 +646,10 @@ __global__ void __launch_bounds__(64) k_sweep_tiles(const double* __restrict__ S
           nc.rho = lds[pb][5][ax_lane]; nc.P = lds[pb][6][ax_lane]; nc.T = lds[pb][7][ax_lane];
           nc.beta = lds[pb][8][ax_lane]; nc.U = lds[pb][9][ax_lane]; nc.lam = lds[pb][10][ax_lane];
         } else {
-          const long p = c + sx;
 #pragma unroll
-          for (int v = 0; v < 5; v++) N[v] = DST[v * nf + p];
-          nc = nb_from_summary(S, nf, p, 0);
+          for (int v = 0; v < 5; v++) N[v] = cur.xN[v];
+          nc.rho = cur.xS[0]; nc.P = cur.xS[1]; nc.T = cur.xS[2]; nc.beta = cur.xS[3]; nc.U = cur.xS[4];
+          nc.lam = cur.xS[5];
         }
         Split sp = split_nb(nc, mx0, mx1, mx2, 0);
         if (FWD) lower_sum(sp, N, acc[0]); else upper_sum(sp, N, acc[0]);
@@ -547,7 +657,6 @@ __global__ void __launch_bounds__(64) k_sweep_tiles(const double* __restrict__ S
 #pragma unroll
         for (int v = 0; v < 5; v++) acc[0][v] = 0.0;
       }
-      // y neighbour
       if (y_dom) {
         Nb nc;
         double N[5];
@@ -557,10 +666,10 @@ __global__ void __launch_bounds__(64) k_sweep_tiles(const double* __restrict__ S
           nc.rho = lds[pb][5][ay_lane]; nc.P = lds[pb][6][ay_lane]; nc.T = lds[pb][7][ay_lane];
           nc.beta = lds[pb][8][ay_lane]; nc.U = lds[pb][11][ay_lane]; nc.lam = lds[pb][12][ay_lane];
         } else {
-          const long p = c + sy;
 #pragma unroll
-          for (int v = 0; v < 5; v++) N[v] = DST[v * nf + p];
-          nc = nb_from_summary(S, nf, p, 1);
+          for (int v = 0; v < 5; v++) N[v] = cur.yN[v];
+          nc.rho = cur.yS[0]; nc.P = cur.yS[1]; nc.T = cur.yS[2]; nc.beta = cur.yS[3]; nc.U = cur.yS[4];
+          nc.lam = cur.yS[5];
         }
         Split sp = split_nb(nc, my0, my1, my2, 1);
         if (FWD) lower_sum(sp, N, acc[1]); else upper_sum(sp, N, acc[1]);
@@ -568,7 +677,6 @@ __global__ void __launch_bounds__(64) k_sweep_tiles(const double* __restrict__ S
 #pragma unroll
         for (int v = 0; v < 5; v++) acc[1][v] = 0.0;
       }
-      // z neighbour: the previous cell of this column
       if (kk > 0) {
         Split sp = split_nb(zc, mz0, mz1, mz2, 2);
         if (FWD) lower_sum(sp, zN, acc[2]); else upper_sum(sp, zN, acc[2]);
@@ -579,10 +687,9 @@ __global__ void __launch_bounds__(64) k_sweep_tiles(const double* __restrict__ S
       double L[5];
 #pragma unroll
       for (int v = 0; v < 5; v++) L[v] = acc[0][v] / g.dXI + acc[1][v] / g.dET + acc[2][v] / g.dZT;
-      const double d11 = S[S_D11 * nf + c], d22 = S[S_D22 * nf + c], d51 = S[S_D51 * nf + c];
-      double M[5], out[5];
-#pragma unroll
-      for (int v = 0; v < 5; v++) M[v] = SRC[v * nf + c];
+      const double d11 = cur.own[S_D11], d22 = cur.own[S_D22], d51 = cur.own[S_D51];
+      const double* M = cur.src;
+      double out[5];
       if (FWD) {
         out[0] = d11 * (M[0] - L[0]);
         out[1] = d22 * (M[1] - L[1]);
@@ -597,33 +704,32 @@ __global__ void __launch_bounds__(64) k_sweep_tiles(const double* __restrict__ S
         out[4] = M[4] - (d51 * L[0] + d22 * L[4]);
       }
 #pragma unroll
-      for (int v = 0; v < 5; v++) DST[v * nf + c] = out[v];
-      // publish to the in-tile neighbours and keep the column's own cell
-      Nb me;
-      me.rho = S[S_RHO * nf + c];
-      me.P = S[S_P * nf + c];
-      me.T = S[S_T * nf + c];
-      me.beta = S[S_BETA * nf + c];
+      for (int v = 0; v < 5; v++) st_coherent(DST + v * nf + c, out[v]);
 #pragma unroll
       for (int v = 0; v < 5; v++) {
         lds[cb][v][lane] = out[v];
         zN[v] = out[v];
       }
-      lds[cb][5][lane] = me.rho;
-      lds[cb][6][lane] = me.P;
-      lds[cb][7][lane] = me.T;
-      lds[cb][8][lane] = me.beta;
-      lds[cb][9][lane] = S[S_UX * nf + c];
-      lds[cb][10][lane] = S[S_LX * nf + c];
-      lds[cb][11][lane] = S[S_UY * nf + c];
-      lds[cb][12][lane] = S[S_LY * nf + c];
-      zc = me;
-      zc.U = S[S_UZ * nf + c];
-      zc.lam = S[S_LZ * nf + c];
+      lds[cb][5][lane] = cur.own[S_RHO];
+      lds[cb][6][lane] = cur.own[S_P];
+      lds[cb][7][lane] = cur.own[S_T];
+      lds[cb][8][lane] = cur.own[S_BETA];
+      lds[cb][9][lane] = cur.own[S_UX];
+      lds[cb][10][lane] = cur.own[S_LX];
+      lds[cb][11][lane] = cur.own[S_UY];
+      lds[cb][12][lane] = cur.own[S_LY];
+      zc.rho = cur.own[S_RHO];
+      zc.P = cur.own[S_P];
+      zc.T = cur.own[S_T];
+      zc.beta = cur.own[S_BETA];
+      zc.U = cur.own[S_UZ];
+      zc.lam = cur.own[S_LZ];
     }
-    __syncthreads();
-    if (lane == 0 && ((tau + 1) % pub == 0 || tau + 1 == nsteps))
-      __hip_atomic_store(progress + tile, tau + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    // one wave per block: LDS writes of this step are complete and ordered
+    // before the next step's reads
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
@@ -692,6 +798,59 @@ __global__ void __launch_bounds__(256) k_update(double* __restrict__ Ui, const d
   }
 }
 
+// the same update reading NR in sweep order: block = 4 steps x 64 lanes
+__global__ void __launch_bounds__(256) k_update_sk(double* __restrict__ Ui, const double* __restrict__ NRs,
+                                                   const double* __restrict__ prof, double* __restrict__ partial,
+                                                   Geo g, Sk q) {
+  __shared__ double red[5][256];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int s = blockIdx.x * 4 + (tid >> 6);
+  const int tile = blockIdx.y;
+  int i, j, k;
+  double r[5] = {0, 0, 0, 0, 0};
+  if (sk_cell(q, g, tile, s, lane, &i, &j, &k)) {
+    const long nf = g.nf;
+    const long c = at(g, i, j, k), o = sk_at(q, tile, s, lane);
+    double J = pf(prof, g.Ym, SET_C, 0, j);
+    double U1 = Ui[c], U2 = Ui[nf + c], U3 = Ui[2 * nf + c], U4 = Ui[3 * nf + c], U5 = Ui[4 * nf + c];
+    double rhoold = U1 * J;
+    double Uold = U2 / U1, Vold = U3 / U1, Wold = U4 / U1;
+    double VVold = Uold * Uold + Vold * Vold + Wold * Wold;
+    double Pold = (U5 * J - 0.5 * rhoold * VVold) * (KG - 1);
+    double Told = Pold / rhoold;
+    double P = Pold + NRs[o] * J;
+    double Uu = Uold + NRs[q.nsk + o] * J;
+    double V = Vold + NRs[2 * q.nsk + o] * J;
+    double W = Wold + NRs[3 * q.nsk + o] * J;
+    double T = Told + NRs[4 * q.nsk + o] * J;
+    double rho = P / T;
+    Ui[c] = rho / J;
+    Ui[nf + c] = rho * Uu / J;
+    Ui[2 * nf + c] = rho * V / J;
+    Ui[3 * nf + c] = rho * W / J;
+    Ui[4 * nf + c] = (P / (KG - 1) + 0.5 * rho * (Uu * Uu + V * V + W * W)) / J;
+    double dq[5] = {(P - Pold) * (P - Pold), (Uu - Uold) * (Uu - Uold), (V - Vold) * (V - Vold),
+                    (W - Wold) * (W - Wold), (T - Told) * (T - Told)};
+#pragma unroll
+    for (int v = 0; v < 5; v++) r[v] = dq[v] * dq[v];
+  }
+#pragma unroll
+  for (int v = 0; v < 5; v++) red[v][tid] = r[v];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (tid < st)
+#pragma unroll
+      for (int v = 0; v < 5; v++) red[v][tid] += red[v][tid + st];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    long blk = (long)blockIdx.y * gridDim.x + blockIdx.x;
+#pragma unroll
+    for (int v = 0; v < 5; v++) partial[blk * 5 + v] = red[v][0];
+  }
+}
+
 __global__ void __launch_bounds__(1024) k_reduce_partials(const double* __restrict__ partial, long nblocks,
                                                           double* __restrict__ out, double DN) {
   __shared__ double red[5][1024];
@@ -756,30 +915,74 @@ void launch_rhs(hipStream_t st, const double* Ui, const double* Un, const double
                      MR, NL, g, deltaT, f_drive);
 }
 
-void launch_sweep_prep(hipStream_t st, const double* Ui, const double* prof, double* S, const Geo& g,
-                       double deltaT) {
-  dim3 blk(64, 4);
-  hipLaunchKernelGGL(k_sweep_prep, box_grid(g.nz - 1, g.ny - 1, g.gend - 2, blk), blk, 0, st, Ui, prof, S, g, deltaT);
+Sk sweep_layout(const Geo& g) {
+  Sk q;
+  q.nTI = (g.gend - 2 + TI - 1) / TI;
+  q.nTJ = (g.ny - 1 + TJ - 1) / TJ;
+  q.ntiles = q.nTI * q.nTJ;
+  q.Nz = g.nz - 1;
+  q.nsteps = q.Nz + (TI - 1) + (TJ - 1);
+  q.nsk = (long)q.ntiles * q.nsteps * 64;
+  return q;
 }
 
 int sweep_tiles(const Geo& g, int* nTI, int* nTJ) {
-  *nTI = (g.gend - 2 + TI - 1) / TI;
-  *nTJ = (g.ny - 1 + TJ - 1) / TJ;
-  return *nTI * *nTJ;
+  Sk q = sweep_layout(g);
+  *nTI = q.nTI;
+  *nTJ = q.nTJ;
+  return q.ntiles;
 }
 
-void launch_sweeps(hipStream_t st, const double* S, const double* MR, double* NL, double* NR, const double* prof,
-                   const Geo& g, int n_sweeps, const int* order, int* sync, int pub, long spin_limit) {
-  int nTI, nTJ;
-  int nt = sweep_tiles(g, &nTI, &nTJ);
+long sweep_field_elems(const Geo& g) { return sweep_layout(g).nsk; }
+
+long sweep_update_blocks(const Geo& g) {
+  Sk q = sweep_layout(g);
+  return (long)((q.nsteps + 3) / 4) * q.ntiles;
+}
+
+void launch_sweeps(hipStream_t st, const double* Ui, const double* MR, double* Sbuf, double* NL, double* NR,
+                   const double* prof, const Geo& g, double deltaT, int n_sweeps, const int* order, int* sync,
+                   long spin_limit) {
+  // Sbuf holds S_N summary fields, then MR, NL, NR (5 each) in sweep order
+  Sk q = sweep_layout(g);
+  double* S = Sbuf;
+  double* MRs = Sbuf + (long)S_N * q.nsk;
+  double* NLs = MRs + 5 * q.nsk;
+  double* NRs = NLs + 5 * q.nsk;
+  hipLaunchKernelGGL(k_sweep_prep, dim3((q.nsteps + 15) / 16, q.ntiles), dim3(1024), 0, st, Ui, MR, prof, S, MRs, g,
+                     q, deltaT);
+  const long tot = (long)q.ntiles * q.nsteps * 64;
+  const unsigned fill_blocks = (unsigned)((tot + 255) / 256);
+  const int nt = q.ntiles;
   for (int sweep = 0; sweep < n_sweeps; sweep++) {
     hipLaunchKernelGGL(k_sweep_reset, dim3((nt + 2 + 255) / 256), dim3(256), 0, st, sync, nt + 2);
-    hipLaunchKernelGGL(k_sweep_tiles<true>, dim3(nt), dim3(64), 0, st, S, MR, NL, prof, g, nTI, nTJ, order, sync, pub,
+    hipLaunchKernelGGL(k_sweep_sentinel, dim3(fill_blocks), dim3(256), 0, st, NLs, q, 1);
+    hipLaunchKernelGGL(k_sweep_tiles<true>, dim3(nt), dim3(64), 0, st, S, MRs, NLs, prof, g, q, order, sync,
                        spin_limit);
     hipLaunchKernelGGL(k_sweep_reset, dim3((nt + 2 + 255) / 256), dim3(256), 0, st, sync + nt + 2, nt + 2);
-    hipLaunchKernelGGL(k_sweep_tiles<false>, dim3(nt), dim3(64), 0, st, S, NL, NR, prof, g, nTI, nTJ, order,
-                       sync + nt + 2, pub, spin_limit);
+    hipLaunchKernelGGL(k_sweep_sentinel, dim3(fill_blocks), dim3(256), 0, st, NRs, q, 0);
+    hipLaunchKernelGGL(k_sweep_tiles<false>, dim3(nt), dim3(64), 0, st, S, NLs, NRs, prof, g, q, order, sync + nt + 2,
+                       spin_limit);
   }
+  (void)NL;
+  (void)NR;
+}
+
+void launch_sweep_outputs(hipStream_t st, const double* Sbuf, double* NL, double* NR, const Geo& g) {
+  Sk q = sweep_layout(g);
+  const double* NLs = Sbuf + (long)(S_N + 5) * q.nsk;
+  const double* NRs = NLs + 5 * q.nsk;
+  hipLaunchKernelGGL(k_sk_to_natural, dim3((q.nsteps + 15) / 16, q.ntiles), dim3(1024), 0, st, NLs, NL, g, q, 5);
+  hipLaunchKernelGGL(k_sk_to_natural, dim3((q.nsteps + 15) / 16, q.ntiles), dim3(1024), 0, st, NRs, NR, g, q, 5);
+}
+
+void launch_update_sk(hipStream_t st, double* Ui, const double* Sbuf, const double* prof, double* partial,
+                      double* er_dev, const Geo& g, double DN) {
+  Sk q = sweep_layout(g);
+  const double* NRs = Sbuf + (long)(S_N + 10) * q.nsk;
+  dim3 gr((q.nsteps + 3) / 4, q.ntiles);
+  hipLaunchKernelGGL(k_update_sk, gr, dim3(256), 0, st, Ui, NRs, prof, partial, g, q);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, st, partial, (long)gr.x * gr.y, er_dev, DN);
 }
 
 void launch_sweeps_planes(hipStream_t st, const double* Ui, const double* MR, double* NL, double* NR,

@@ -11,11 +11,17 @@ void launch_viscous(hipStream_t st, const double* U, const double* prof, double*
 void launch_rhs(hipStream_t st, const double* Ui, const double* Un, const double* Uq, const double* Fx,
                 const double* Fy, const double* Fz, const double* vF, const double* prof, double* MR, double* NL,
                 const Geo& g, double deltaT, double f_drive);
-void launch_sweep_prep(hipStream_t st, const double* Ui, const double* prof, double* S, const Geo& g,
-                       double deltaT);
 int sweep_tiles(const Geo& g, int* nTI, int* nTJ);
-void launch_sweeps(hipStream_t st, const double* S, const double* MR, double* NL, double* NR, const double* prof,
-                   const Geo& g, int n_sweeps, const int* order, int* sync, int pub, long spin_limit);
+long sweep_field_elems(const Geo& g);
+long sweep_update_blocks(const Geo& g);
+// persistent tiled sweeps; Sbuf: S_N + 15 fields of sweep_field_elems doubles
+void launch_sweeps(hipStream_t st, const double* Ui, const double* MR, double* Sbuf, double* NL, double* NR,
+                   const double* prof, const Geo& g, double deltaT, int n_sweeps, const int* order, int* sync,
+                   long spin_limit);
+// NL and NR of the last sweep back in the natural layout
+void launch_sweep_outputs(hipStream_t st, const double* Sbuf, double* NL, double* NR, const Geo& g);
+void launch_update_sk(hipStream_t st, double* Ui, const double* Sbuf, const double* prof, double* partial,
+                      double* er_dev, const Geo& g, double DN);
 void launch_sweeps_planes(hipStream_t st, const double* Ui, const double* MR, double* NL, double* NR,
                           const double* prof, const Geo& g, double deltaT, int n_sweeps);
 void launch_copy_interior(hipStream_t st, const double* src, double* dst, const Geo& g);

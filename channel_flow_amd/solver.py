@@ -83,7 +83,7 @@ def load_library() -> C.CDLL:
     lib.cf_physical_step.argtypes = [vp, C.POINTER(C.c_int), P]
     lib.cf_advance_levels.argtypes = [vp]
     lib.cf_profile_inner_iteration.argtypes = [vp, P, P, C.POINTER(C.c_int)]
-    lib.cf_set_sweep_schedule.argtypes = [vp, C.c_int, C.c_int]
+    lib.cf_set_sweep_schedule.argtypes = [vp, C.c_int]
     lib.cf_write_output.argtypes = [vp, C.c_int, C.c_char_p]
     lib.cf_synchronize.argtypes = [vp]
     _lib = lib
@@ -257,9 +257,9 @@ class Solver:
     def Output(self, step, directory):
         self._chk(self.lib.cf_write_output(self._h, int(step), str(directory).encode()))
 
-    def set_sweep_schedule(self, schedule: int = 0, publish_every: int = 1):
+    def set_sweep_schedule(self, schedule: int = 0):
         """0: persistent tiled sweep (default); 1: one launch per hyperplane."""
-        self._chk(self.lib.cf_set_sweep_schedule(self._h, int(schedule), int(publish_every)))
+        self._chk(self.lib.cf_set_sweep_schedule(self._h, int(schedule)))
 
     def synchronize(self):
         self._chk(self.lib.cf_synchronize(self._h))

@@ -149,10 +149,10 @@ int cf_write_output(cf_ctx* ctx, int step, const char* dir);
 int cf_profile_inner_iteration(cf_ctx* ctx, double er[5], double stage_ms[8], int launches[8]);
 /* Visiting order of the LU-SGS sweeps; both give identical bits because each
  * cell depends only on its three upstream neighbours.  schedule 0 (default):
- * one persistent launch per sweep direction, 8x8-column tiles in dependency
- * order, progress published every publish_every steps; schedule 1: one
- * launch per hyperplane i+j+k = m (the reference order, Hyperplane.cpp). */
-int cf_set_sweep_schedule(cf_ctx* ctx, int schedule, int publish_every);
+ * one persistent launch per sweep direction, 8x8-column tiles walking k in
+ * dependency order; schedule 1: one launch per hyperplane i+j+k = m (the
+ * reference order, Hyperplane.cpp:50-78). */
+int cf_set_sweep_schedule(cf_ctx* ctx, int schedule);
 /* Synchronises the context's stream. */
 int cf_synchronize(cf_ctx* ctx);
 
