@@ -190,6 +190,9 @@ extern "C" int cf_create(const cf_params* p, cf_ctx** out) {
   g.dXI = p->deltaXI;
   g.dET = p->deltaET;
   g.dZT = p->deltaZT;
+  g.rXI = 1.0 / p->deltaXI;
+  g.rET = 1.0 / p->deltaET;
+  g.rZT = 1.0 / p->deltaZT;
   c->hprof.assign((size_t)NPROF * g.Ym, 0.0);
   auto bail = [&](int code) {
     cf_destroy(c);
@@ -208,7 +211,7 @@ extern "C" int cf_create(const cf_params* p, cf_ctx** out) {
     // the reference leaves work arrays uninitialised; here they start at zero
     if (hipMemsetAsync(*f.ptr, 0, nf * f.n * sizeof(double), c->stream) != hipSuccess) return bail(CF_ERR_HIP);
   }
-  if (hipMalloc(&c->Sm, (size_t)cf::sweep_field_elems(g) * (cf::S_N + 15) * sizeof(double)) != hipSuccess)
+  if (hipMalloc(&c->Sm, (size_t)cf::sweep_field_elems(g) * (cf::G_N + 15) * sizeof(double)) != hipSuccess)
     return bail(CF_ERR_HIP);
   {
     int nTI, nTJ;
@@ -223,8 +226,8 @@ extern "C" int cf_create(const cf_params* p, cf_ctx** out) {
     if (hipMalloc(&c->order, ord.size() * sizeof(int)) != hipSuccess) return bail(CF_ERR_HIP);
     if (hipMemcpy(c->order, ord.data(), ord.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
       return bail(CF_ERR_HIP);
-    if (hipMalloc(&c->sync, 2 * (c->ntiles + 2) * sizeof(int)) != hipSuccess) return bail(CF_ERR_HIP);
-    if (hipMemsetAsync(c->sync, 0, 2 * (c->ntiles + 2) * sizeof(int), c->stream) != hipSuccess)
+    if (hipMalloc(&c->sync, (2 * c->ntiles + 4) * sizeof(int)) != hipSuccess) return bail(CF_ERR_HIP);
+    if (hipMemsetAsync(c->sync, 0, (2 * c->ntiles + 4) * sizeof(int), c->stream) != hipSuccess)
       return bail(CF_ERR_HIP);
   }
   long nb = std::max(cf::update_blocks(g), cf::sweep_update_blocks(g));
@@ -448,24 +451,36 @@ extern "C" int cf_viscous_terms(cf_ctx* c, const double* const U_[5], double* co
   return sync(c);
 }
 
+// the tiled sweep takes the split Jacobians of diagonal metric rows; other
+// metrics use the one-hyperplane-per-launch schedule
+static bool diagonal_metrics(const cf_ctx* c) {
+  const double* P = c->hprof.data();
+  const int off[6] = {2, 3, 4, 6, 7, 8};  // xidy, xidz, etdx, etdz, ztdx, ztdy
+  for (int q = 0; q < 6; q++)
+    for (int j = 1; j <= c->g.ny + 1; j++)
+      if (P[(size_t)(cf::SET_C * 10 + off[q]) * c->g.Ym + j] != 0.0) return false;
+  return true;
+}
+
+static int effective_schedule(const cf_ctx* c) { return (c->schedule == 0 && diagonal_metrics(c)) ? 0 : 1; }
+
 // LU-SGS forward/backward sweeps (Dual_time_stepping.cpp:514-1277)
 static void sweeps(cf_ctx* c, double deltaT) {
   const Geo& g = c->g;
-  if (c->schedule == 1) {
+  if (effective_schedule(c) == 1) {
     cf::launch_sweeps_planes(c->stream, c->Ui, c->MR, c->NL, c->NR, c->prof, g, deltaT, c->p.n_sweeps);
     return;
   }
+  double DN = 1. / ((double)c->p.X_out * c->p.Y_out * c->p.Z_out);
   cf::launch_sweeps(c->stream, c->Ui, c->MR, c->Sm, c->NL, c->NR, c->prof, g, deltaT, c->p.n_sweeps, c->order,
-                    c->sync, c->spin_limit);
+                    c->sync, c->partial, c->er_dev, DN, c->spin_limit);
 }
 
-// primitive update + norms after the sweeps (reads NR where the schedule left it)
+// primitive update + norms after the sweeps; the tiled schedule applies them
+// inside its last backward sweep
 static void update(cf_ctx* c) {
   double DN = 1. / ((double)c->p.X_out * c->p.Y_out * c->p.Z_out);
-  if (c->schedule == 1)
-    cf::launch_update(c->stream, c->Ui, c->NR, c->prof, c->partial, c->er_dev, c->g, DN);
-  else
-    cf::launch_update_sk(c->stream, c->Ui, c->Sm, c->prof, c->partial, c->er_dev, c->g, DN);
+  if (effective_schedule(c) == 1) cf::launch_update(c->stream, c->Ui, c->NR, c->prof, c->partial, c->er_dev, c->g, DN);
 }
 
 // er to the host and the sweep error flags checked
@@ -476,7 +491,7 @@ static int finish_iteration(cf_ctx* c, double er[5]) {
   HIPCHK(c, hipMemcpyAsync(&flags[1], c->sync + c->ntiles + 3, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   int r = sync(c);
   if (r) return r;
-  if (c->schedule == 0 && (flags[0] || flags[1]))
+  if (effective_schedule(c) == 0 && (flags[0] || flags[1]))
     return fail(c, CF_ERR_TIMEOUT, "LU-SGS sweep: a tile waited past its bound for an upstream tile");
   for (int v = 0; v < 5; v++)
     if (!std::isfinite(er[v])) return fail(c, CF_ERR_NONFINITE, "residual norm er%d is not finite", v + 1);
@@ -490,7 +505,7 @@ static int dual_time_device(cf_ctx* c, double deltaT, double er[5], bool want_nl
                  want_nl ? c->NL : nullptr, g, deltaT, c->p.f_drive);
   sweeps(c, deltaT);
   if (want_nl) {
-    if (c->schedule == 0) cf::launch_sweep_outputs(c->stream, c->Sm, c->NL, c->NR, g);
+    if (effective_schedule(c) == 0) cf::launch_sweep_outputs(c->stream, c->Sm, c->NL, c->NR, g);
     cf::launch_copy_interior(c->stream, c->NR, c->NL, g);
   }
   update(c);
@@ -660,7 +675,7 @@ extern "C" int cf_profile_inner_iteration(cf_ctx* c, double er[5], double stage_
   HIPCHK(c, hipEventRecord(ev[8], c->stream));
   int r = finish_iteration(c, er);
   if (r) return r;
-  const int nsw = c->schedule == 1 ? 2 * nplanes * c->p.n_sweeps : 1 + 6 * c->p.n_sweeps;
+  const int nsw = effective_schedule(c) == 1 ? 2 * nplanes * c->p.n_sweeps : 1 + 6 * c->p.n_sweeps;
   const int nl[8] = {3, 1, 1, 1, 3, 1, nsw, 2};
   for (int q = 0; q < 8; q++) {
     float ms = 0;

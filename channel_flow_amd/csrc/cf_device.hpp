@@ -27,6 +27,7 @@ struct Geo {
   long plane;   // Ym * Zp
   long nf;      // Xnp * plane
   double dXI, dET, dZT;
+  double rXI, rET, rZT;  // RN(1/d) of the spacings, for div_const
 };
 
 __device__ __forceinline__ long at(const Geo& g, int i, int j, int k) {
@@ -297,6 +298,63 @@ __device__ __forceinline__ Split split_nb(const Nb& c, double n0, double n1, dou
   a.m11 = 0.5 * (beta * U - dd);
   a.m22 = 0.5 * (U - dd);
   return a;
+}
+
+// Sweep summary for metrics whose rows are diagonal (xi = (xix,0,0),
+// eta = (0,ety,0), zeta = (0,0,ztz): every grid Jacobian*.cpp produces).
+// The off-diagonal split-Jacobian entries are then exact zeros and the
+// divisions are taken once per cell: PD = 0.5*n_dd/rho (A+21, B+31, C+41) and
+// P51 = 0.5*U*(K-1)*(beta-1)/rho/K for each direction.
+enum DiagField : int {
+  G_P = 0, G_T, G_BETA, G_D11, G_D22, G_D51,
+  G_UX, G_LX, G_PDX, G_P51X,
+  G_UY, G_LY, G_PDY, G_P51Y,
+  G_UZ, G_LZ, G_PDZ, G_P51Z,
+  G_N
+};
+
+struct DNb {  // one neighbour as seen along one direction
+  double P, T, beta, U, lam, pd, p51;
+};
+
+// lower (forward) sum -A+ N of a neighbour along direction D with diagonal
+// metric n (Dual_time_stepping.cpp:751-811 without the exact-zero terms)
+template <int D>
+__device__ __forceinline__ void lower_diag(const DNb& c, double n, const double N[5], double o[5]) {
+  const double dd = fabs(c.lam);
+  const double p11 = 0.5 * (c.beta * c.U + dd);
+  const double p1q = 0.5 * c.P * KG * c.beta * n;
+  const double p22 = 0.5 * (c.U + dd);
+  const double p5q = D == 0 ? 0.5 * n * c.beta * c.T * (KG - 1) : 0.5 * c.beta * c.T * (KG - 1) * n;
+  constexpr int q = D + 1;
+  o[0] = -p11 * N[0] - p1q * N[q];
+#pragma unroll
+  for (int m = 1; m <= 3; m++) o[m] = (m == q) ? -c.pd * N[0] - p22 * N[m] : -(p22 * N[m]);
+  o[4] = -c.p51 * N[0] - p5q * N[q] - p22 * N[4];
+}
+
+// upper (backward) sum of a neighbour (Dual_time_stepping.cpp:1116-1177)
+template <int D>
+__device__ __forceinline__ void upper_diag(const DNb& c, double n, const double N[5], double o[5]) {
+  const double dd = fabs(c.lam);
+  const double m11 = 0.5 * (c.beta * c.U - dd);
+  const double p1q = 0.5 * c.P * KG * c.beta * n;
+  const double m22 = 0.5 * (c.U - dd);
+  const double p5q = D == 0 ? 0.5 * n * c.beta * c.T * (KG - 1) : 0.5 * c.beta * c.T * (KG - 1) * n;
+  constexpr int q = D + 1;
+  o[0] = +m11 * N[0] + p1q * N[q];
+#pragma unroll
+  for (int m = 1; m <= 3; m++) o[m] = (m == q) ? +c.pd * N[0] + m22 * N[m] : m22 * N[m];
+  o[4] = +c.p51 * N[0] + p5q * N[q] + m22 * N[4];
+}
+
+// x/d for a fixed divisor d with r = RN(1/d): q0 = x*r is within one ulp of
+// x/d, and one residual step q0 + fma(-q0,d,x)*r rounds correctly
+// (P. Markstein, IBM J. Res. Dev. 34(1), 1990).  Same bits as x/d.
+__device__ __forceinline__ double div_const(double x, double d, double r) {
+  const double q0 = x * r;
+  const double e = fma(-q0, d, x);
+  return fma(e, r, q0);
 }
 
 // lower (forward) contribution -A+ N of one neighbour, Dual_time_stepping.cpp:751-811

@@ -449,14 +449,22 @@ __device__ __forceinline__ bool sk_cell(const Sk& q, const Geo& g, int tile, int
   return s < q.nsteps && kk >= 0 && kk < q.Nz && *i <= g.gend && *j <= g.ny;
 }
 
-// summary S and MR in sweep order (gathered from the natural layout: a
-// lane's 16 consecutive steps read 16 consecutive k of its column)
+// diagonal-metric summary (DiagField) and MR in sweep order, gathered from
+// the natural layout: a block is 16 steps x 64 lanes of one tile, so a lane's
+// 16 consecutive steps read 16 consecutive k of its column
 __global__ void __launch_bounds__(1024) k_sweep_prep(const double* __restrict__ Ui, const double* __restrict__ MR,
                                                      const double* __restrict__ prof, double* __restrict__ S,
-                                                     double* __restrict__ MRs, Geo g, Sk q, double deltaT) {
+                                                     double* __restrict__ MRs, double* __restrict__ NLs, Geo g, Sk q,
+                                                     double deltaT) {
   const int lane = threadIdx.x & 63;
   const int s = blockIdx.x * 16 + (threadIdx.x >> 6);
   const int tile = blockIdx.y;
+  if (s < q.nsteps && ((lane & 7) == TI - 1 || (lane >> 3) == TJ - 1)) {
+    // slots the first forward sweep hands across tiles start as the sentinel
+    const double sv = __longlong_as_double((long long)SENTINEL);
+#pragma unroll
+    for (int v = 0; v < 5; v++) NLs[v * q.nsk + sk_at(q, tile, s, lane)] = sv;
+  }
   int i, j, k;
   if (!sk_cell(q, g, tile, s, lane, &i, &j, &k)) return;
   const long nf = g.nf;
@@ -466,32 +474,31 @@ __global__ void __launch_bounds__(1024) k_sweep_prep(const double* __restrict__ 
   double m9[9];
 #pragma unroll
   for (int r = 0; r < 9; r++) m9[r] = pf(prof, g.Ym, SET_C, 1 + r, j);
-  double Ux, Uy, Uz;
-  double lx = lambda_dir(st, m9[0], m9[1], m9[2], &Ux);
-  double ly = lambda_dir(st, m9[3], m9[4], m9[5], &Uy);
-  double lz = lambda_dir(st, m9[6], m9[7], m9[8], &Uz);
+  double U3[3], L3[3];
+  L3[0] = lambda_dir(st, m9[0], m9[1], m9[2], &U3[0]);
+  L3[1] = lambda_dir(st, m9[3], m9[4], m9[5], &U3[1]);
+  L3[2] = lambda_dir(st, m9[6], m9[7], m9[8], &U3[2]);
   double d11, d22, d51;
-  dinv_lam(st, lx, ly, lz, g.dXI, g.dET, g.dZT, deltaT, &d11, &d22, &d51);
-  const double v[S_N] = {st.rho, st.P, st.T, st.beta, Ux, lx, Uy, ly, Uz, lz, d11, d22, d51};
+  dinv_lam(st, L3[0], L3[1], L3[2], g.dXI, g.dET, g.dZT, deltaT, &d11, &d22, &d51);
+  double v[G_N];
+  v[G_P] = st.P;
+  v[G_T] = st.T;
+  v[G_BETA] = st.beta;
+  v[G_D11] = d11;
+  v[G_D22] = d22;
+  v[G_D51] = d51;
 #pragma unroll
-  for (int r = 0; r < S_N; r++) S[r * q.nsk + o] = v[r];
+  for (int d = 0; d < 3; d++) {
+    const double n = m9[4 * d];  // xix, ety, ztz
+    v[G_UX + 4 * d] = U3[d];
+    v[G_LX + 4 * d] = L3[d];
+    v[G_PDX + 4 * d] = 0.5 * n / st.rho;
+    v[G_P51X + 4 * d] = 0.5 * U3[d] * (KG - 1) * (st.beta - 1) / st.rho / KG;
+  }
+#pragma unroll
+  for (int r = 0; r < G_N; r++) S[r * q.nsk + o] = v[r];
 #pragma unroll
   for (int r = 0; r < 5; r++) MRs[r * q.nsk + o] = MR[r * nf + c];
-}
-
-// sentinel in the slots that a neighbouring tile reads during the coming
-// sweep direction: lanes of the last tile column/row (forward) or the first
-// (backward)
-__global__ void __launch_bounds__(256) k_sweep_sentinel(double* __restrict__ DST, Sk q, int fwd) {
-  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (long)q.ntiles * q.nsteps * 64) return;
-  const int lane = (int)(idx & 63);
-  const int a = lane & 7, b = lane >> 3;
-  const bool edge = fwd ? (a == TI - 1 || b == TJ - 1) : (a == 0 || b == 0);
-  if (!edge) return;
-  const double sv = __longlong_as_double((long long)SENTINEL);
-#pragma unroll
-  for (int v = 0; v < 5; v++) DST[v * q.nsk + idx] = sv;
 }
 
 // sweep-order field(s) back to the natural layout
@@ -521,23 +528,36 @@ __device__ __forceinline__ void st_coherent(double* p, double v) {
 
 // one step's global inputs of a lane
 struct StepIn {
-  double own[S_N];   // summary of the lane's cell
+  double own[G_N];   // summary of the lane's cell
   double src[5];     // MR (forward) or NL (backward) of the lane's cell
   double xN[5], yN[5];
-  double xS[6], yS[6];  // remote neighbours: rho, P, T, beta, U, lam
+  DNb xc, yc;        // remote neighbours along x and y
 };
 
-// LDS slot of one lane: N(5), rho, P, T, beta, Ux, lx, Uy, ly
-constexpr int LDS_F = 13;
+// LDS slot of one lane: N(5), P, T, beta, x: U, lam, pd, p51, y: U, lam, pd, p51
+constexpr int LDS_F = 16;
+
+__device__ __forceinline__ DNb dnb_at(const double* __restrict__ S, long nsk, long p, int d) {
+  DNb c;
+  c.P = S[G_P * nsk + p];
+  c.T = S[G_T * nsk + p];
+  c.beta = S[G_BETA * nsk + p];
+  c.U = S[(G_UX + 4 * d) * nsk + p];
+  c.lam = S[(G_LX + 4 * d) * nsk + p];
+  c.pd = S[(G_PDX + 4 * d) * nsk + p];
+  c.p51 = S[(G_P51X + 4 * d) * nsk + p];
+  return c;
+}
 
 template <bool FWD>
-__global__ void __launch_bounds__(64) k_sweep_tiles(const double* __restrict__ S, const double* __restrict__ SRC,
-                                                    double* __restrict__ DST, const double* __restrict__ prof,
-                                                    Geo g, Sk q, const int* __restrict__ order,
-                                                    int* __restrict__ sync, long spin_limit) {
-  // all arrays in sweep order (Sk)
-  // FWD: SRC = MR, DST = NL   (NL = D^-1 (MR - L))
-  // BWD: SRC = NL, DST = NR   (NR = NL - D^-1 U)
+__global__ void __launch_bounds__(64) k_sweep_tiles(const double* __restrict__ S, double* __restrict__ SRC,
+                                                    double* __restrict__ DST, double* __restrict__ AUX,
+                                                    const double* __restrict__ prof, Geo g, Sk q,
+                                                    const int* __restrict__ order, int* __restrict__ sync,
+                                                    long spin_limit) {
+  // all arrays in sweep order (Sk); S holds the DiagField summary
+  // FWD: SRC = MR, DST = NL   (NL = D^-1 (MR - L)); seeds the NR (AUX) sentinels
+  // BWD: SRC = NL, DST = NR   (NR = NL - D^-1 U); re-seeds the NL sentinels
   const int nTI = q.nTI;
   __shared__ double lds[2][LDS_F][64];
   __shared__ int s_ticket;
@@ -554,7 +574,7 @@ __global__ void __launch_bounds__(64) k_sweep_tiles(const double* __restrict__ S
   const int TX = tile % nTI, TY = tile / nTI;
   const int i = 3 + TX * TI + a, j = 2 + TY * TJ + b;
   const bool col = i <= g.gend && j <= g.ny;
-  const int Nz = g.nz - 1;
+  const int Nz = q.Nz;
   const int nsteps = q.nsteps;
   const int off = FWD ? a + b : (TI - 1 - a) + (TJ - 1 - b);
   const int ax_lane = FWD ? lane - 1 : lane + 1;   // (a-1,b) or (a+1,b)
@@ -571,10 +591,8 @@ __global__ void __launch_bounds__(64) k_sweep_tiles(const double* __restrict__ S
   // or (0,b)/(a,0) of the downstream tile (backward), 7 steps apart
   const long sx = FWD ? (-(long)q.nsteps * 64 + 7 * 64 + 7) : ((long)q.nsteps * 64 - 7 * 64 - 7);
   const long sy = FWD ? (-(long)nTI * q.nsteps * 64 + 7 * 64 + 56) : ((long)nTI * q.nsteps * 64 - 7 * 64 - 56);
-  const double mx0 = pf(prof, g.Ym, SET_C, 1, j), mx1 = pf(prof, g.Ym, SET_C, 2, j), mx2 = pf(prof, g.Ym, SET_C, 3, j);
-  const double my0 = pf(prof, g.Ym, SET_C, 4, jy), my1 = pf(prof, g.Ym, SET_C, 5, jy),
-               my2 = pf(prof, g.Ym, SET_C, 6, jy);
-  const double mz0 = pf(prof, g.Ym, SET_C, 7, j), mz1 = pf(prof, g.Ym, SET_C, 8, j), mz2 = pf(prof, g.Ym, SET_C, 9, j);
+  // diagonal metric entries of the neighbours (functions of j)
+  const double nx = pf(prof, g.Ym, SET_C, 1, j), ny_ = pf(prof, g.Ym, SET_C, 5, jy), nz_ = pf(prof, g.Ym, SET_C, 9, j);
 
   auto cell_of = [&](int tau) -> long { return sk_at(q, tile, FWD ? tau : q.nsteps - 1 - tau, lane); };
   auto active = [&](int tau) -> bool {
@@ -585,22 +603,20 @@ __global__ void __launch_bounds__(64) k_sweep_tiles(const double* __restrict__ S
     if (!active(tau)) return;
     const long c = cell_of(tau);
 #pragma unroll
-    for (int q = 0; q < S_N; q++) in.own[q] = S[q * nf + c];
+    for (int r = 0; r < G_N; r++) in.own[r] = S[r * nf + c];
 #pragma unroll
     for (int v = 0; v < 5; v++) in.src[v] = SRC[v * nf + c];
     if (x_remote) {
       const long p = c + sx;
 #pragma unroll
       for (int v = 0; v < 5; v++) in.xN[v] = ld_coherent(DST + v * nf + p);
-      in.xS[0] = S[S_RHO * nf + p]; in.xS[1] = S[S_P * nf + p]; in.xS[2] = S[S_T * nf + p];
-      in.xS[3] = S[S_BETA * nf + p]; in.xS[4] = S[S_UX * nf + p]; in.xS[5] = S[S_LX * nf + p];
+      in.xc = dnb_at(S, nf, p, 0);
     }
     if (y_remote) {
       const long p = c + sy;
 #pragma unroll
       for (int v = 0; v < 5; v++) in.yN[v] = ld_coherent(DST + v * nf + p);
-      in.yS[0] = S[S_RHO * nf + p]; in.yS[1] = S[S_P * nf + p]; in.yS[2] = S[S_T * nf + p];
-      in.yS[3] = S[S_BETA * nf + p]; in.yS[4] = S[S_UY * nf + p]; in.yS[5] = S[S_LY * nf + p];
+      in.yc = dnb_at(S, nf, p, 1);
     }
   };
   // re-reads remote values that were not ready when first loaded
@@ -621,9 +637,23 @@ __global__ void __launch_bounds__(64) k_sweep_tiles(const double* __restrict__ S
       for (int v = 0; v < 5; v++) N[v] = ld_coherent(DST + v * nf + p);
     }
   };
+  const bool edge_fwd = a == TI - 1 || b == TJ - 1;   // slots read across tiles by a forward sweep
+  const bool edge_bwd = a == 0 || b == 0;             // ... by a backward sweep
+  const double sv = __longlong_as_double((long long)SENTINEL);
+  auto lds_nb = [&](int pb, int src_lane, int d, DNb& c, double (&N)[5]) {
+#pragma unroll
+    for (int v = 0; v < 5; v++) N[v] = lds[pb][v][src_lane];
+    c.P = lds[pb][5][src_lane];
+    c.T = lds[pb][6][src_lane];
+    c.beta = lds[pb][7][src_lane];
+    c.U = lds[pb][8 + 4 * d][src_lane];
+    c.lam = lds[pb][9 + 4 * d][src_lane];
+    c.pd = lds[pb][10 + 4 * d][src_lane];
+    c.p51 = lds[pb][11 + 4 * d][src_lane];
+  };
 
   double zN[5] = {0, 0, 0, 0, 0};
-  Nb zc = {1, 0, 0, 0, 0, 0};
+  DNb zc = {0, 0, 0, 0, 0, 0, 0};
   StepIn nxt;
   load_step(0, nxt);
   for (int tau = 0; tau < nsteps; tau++) {
@@ -638,56 +668,47 @@ __global__ void __launch_bounds__(64) k_sweep_tiles(const double* __restrict__ S
       if (y_remote) settle(cur.yN, c + sy);
       double acc[3][5];
       if (x_dom) {
-        Nb nc;
+        DNb nc;
         double N[5];
         if (x_in_tile) {
-#pragma unroll
-          for (int v = 0; v < 5; v++) N[v] = lds[pb][v][ax_lane];
-          nc.rho = lds[pb][5][ax_lane]; nc.P = lds[pb][6][ax_lane]; nc.T = lds[pb][7][ax_lane];
-          nc.beta = lds[pb][8][ax_lane]; nc.U = lds[pb][9][ax_lane]; nc.lam = lds[pb][10][ax_lane];
+          lds_nb(pb, ax_lane, 0, nc, N);
         } else {
+          nc = cur.xc;
 #pragma unroll
           for (int v = 0; v < 5; v++) N[v] = cur.xN[v];
-          nc.rho = cur.xS[0]; nc.P = cur.xS[1]; nc.T = cur.xS[2]; nc.beta = cur.xS[3]; nc.U = cur.xS[4];
-          nc.lam = cur.xS[5];
         }
-        Split sp = split_nb(nc, mx0, mx1, mx2, 0);
-        if (FWD) lower_sum(sp, N, acc[0]); else upper_sum(sp, N, acc[0]);
+        if (FWD) lower_diag<0>(nc, nx, N, acc[0]); else upper_diag<0>(nc, nx, N, acc[0]);
       } else {
 #pragma unroll
         for (int v = 0; v < 5; v++) acc[0][v] = 0.0;
       }
       if (y_dom) {
-        Nb nc;
+        DNb nc;
         double N[5];
         if (y_in_tile) {
-#pragma unroll
-          for (int v = 0; v < 5; v++) N[v] = lds[pb][v][ay_lane];
-          nc.rho = lds[pb][5][ay_lane]; nc.P = lds[pb][6][ay_lane]; nc.T = lds[pb][7][ay_lane];
-          nc.beta = lds[pb][8][ay_lane]; nc.U = lds[pb][11][ay_lane]; nc.lam = lds[pb][12][ay_lane];
+          lds_nb(pb, ay_lane, 1, nc, N);
         } else {
+          nc = cur.yc;
 #pragma unroll
           for (int v = 0; v < 5; v++) N[v] = cur.yN[v];
-          nc.rho = cur.yS[0]; nc.P = cur.yS[1]; nc.T = cur.yS[2]; nc.beta = cur.yS[3]; nc.U = cur.yS[4];
-          nc.lam = cur.yS[5];
         }
-        Split sp = split_nb(nc, my0, my1, my2, 1);
-        if (FWD) lower_sum(sp, N, acc[1]); else upper_sum(sp, N, acc[1]);
+        if (FWD) lower_diag<1>(nc, ny_, N, acc[1]); else upper_diag<1>(nc, ny_, N, acc[1]);
       } else {
 #pragma unroll
         for (int v = 0; v < 5; v++) acc[1][v] = 0.0;
       }
       if (kk > 0) {
-        Split sp = split_nb(zc, mz0, mz1, mz2, 2);
-        if (FWD) lower_sum(sp, zN, acc[2]); else upper_sum(sp, zN, acc[2]);
+        if (FWD) lower_diag<2>(zc, nz_, zN, acc[2]); else upper_diag<2>(zc, nz_, zN, acc[2]);
       } else {
 #pragma unroll
         for (int v = 0; v < 5; v++) acc[2][v] = 0.0;
       }
       double L[5];
 #pragma unroll
-      for (int v = 0; v < 5; v++) L[v] = acc[0][v] / g.dXI + acc[1][v] / g.dET + acc[2][v] / g.dZT;
-      const double d11 = cur.own[S_D11], d22 = cur.own[S_D22], d51 = cur.own[S_D51];
+      for (int v = 0; v < 5; v++)
+        L[v] = div_const(acc[0][v], g.dXI, g.rXI) + div_const(acc[1][v], g.dET, g.rET) +
+               div_const(acc[2][v], g.dZT, g.rZT);
+      const double d11 = cur.own[G_D11], d22 = cur.own[G_D22], d51 = cur.own[G_D51];
       const double* M = cur.src;
       double out[5];
       if (FWD) {
@@ -705,25 +726,36 @@ __global__ void __launch_bounds__(64) k_sweep_tiles(const double* __restrict__ S
       }
 #pragma unroll
       for (int v = 0; v < 5; v++) st_coherent(DST + v * nf + c, out[v]);
+      if (FWD && edge_bwd) {
+        // the backward sweep that follows hands these NR slots across tiles
+#pragma unroll
+        for (int v = 0; v < 5; v++) AUX[v * nf + c] = sv;
+      }
+      if (!FWD && edge_fwd) {
+        // this NL slot has been consumed; the next forward sweep hands it on
+#pragma unroll
+        for (int v = 0; v < 5; v++) SRC[v * nf + c] = sv;
+      }
 #pragma unroll
       for (int v = 0; v < 5; v++) {
         lds[cb][v][lane] = out[v];
         zN[v] = out[v];
       }
-      lds[cb][5][lane] = cur.own[S_RHO];
-      lds[cb][6][lane] = cur.own[S_P];
-      lds[cb][7][lane] = cur.own[S_T];
-      lds[cb][8][lane] = cur.own[S_BETA];
-      lds[cb][9][lane] = cur.own[S_UX];
-      lds[cb][10][lane] = cur.own[S_LX];
-      lds[cb][11][lane] = cur.own[S_UY];
-      lds[cb][12][lane] = cur.own[S_LY];
-      zc.rho = cur.own[S_RHO];
-      zc.P = cur.own[S_P];
-      zc.T = cur.own[S_T];
-      zc.beta = cur.own[S_BETA];
-      zc.U = cur.own[S_UZ];
-      zc.lam = cur.own[S_LZ];
+      lds[cb][5][lane] = cur.own[G_P];
+      lds[cb][6][lane] = cur.own[G_T];
+      lds[cb][7][lane] = cur.own[G_BETA];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        lds[cb][8 + r][lane] = cur.own[G_UX + r];
+        lds[cb][12 + r][lane] = cur.own[G_UY + r];
+      }
+      zc.P = cur.own[G_P];
+      zc.T = cur.own[G_T];
+      zc.beta = cur.own[G_BETA];
+      zc.U = cur.own[G_UZ];
+      zc.lam = cur.own[G_LZ];
+      zc.pd = cur.own[G_PDZ];
+      zc.p51 = cur.own[G_P51Z];
     }
     // one wave per block: LDS writes of this step are complete and ordered
     // before the next step's reads
@@ -798,20 +830,19 @@ __global__ void __launch_bounds__(256) k_update(double* __restrict__ Ui, const d
   }
 }
 
-// the same update reading NR in sweep order: block = 4 steps x 64 lanes
-__global__ void __launch_bounds__(256) k_update_sk(double* __restrict__ Ui, const double* __restrict__ NRs,
-                                                   const double* __restrict__ prof, double* __restrict__ partial,
-                                                   Geo g, Sk q) {
+// the same update with NR read from the sweep order (Sk)
+__global__ void __launch_bounds__(256) k_update_from_sweep(double* __restrict__ Ui, const double* __restrict__ NR,
+                                                const double* __restrict__ prof, double* __restrict__ partial, Geo g,
+                                                        Sk q) {
   __shared__ double red[5][256];
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int s = blockIdx.x * 4 + (tid >> 6);
-  const int tile = blockIdx.y;
-  int i, j, k;
+  int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  int i = 3 + blockIdx.z;
   double r[5] = {0, 0, 0, 0, 0};
-  if (sk_cell(q, g, tile, s, lane, &i, &j, &k)) {
+  if (k <= g.nz && j <= g.ny) {
     const long nf = g.nf;
-    const long c = at(g, i, j, k), o = sk_at(q, tile, s, lane);
+    long c = at(g, i, j, k);
     double J = pf(prof, g.Ym, SET_C, 0, j);
     double U1 = Ui[c], U2 = Ui[nf + c], U3 = Ui[2 * nf + c], U4 = Ui[3 * nf + c], U5 = Ui[4 * nf + c];
     double rhoold = U1 * J;
@@ -819,35 +850,39 @@ __global__ void __launch_bounds__(256) k_update_sk(double* __restrict__ Ui, cons
     double VVold = Uold * Uold + Vold * Vold + Wold * Wold;
     double Pold = (U5 * J - 0.5 * rhoold * VVold) * (KG - 1);
     double Told = Pold / rhoold;
-    double P = Pold + NRs[o] * J;
-    double Uu = Uold + NRs[q.nsk + o] * J;
-    double V = Vold + NRs[2 * q.nsk + o] * J;
-    double W = Wold + NRs[3 * q.nsk + o] * J;
-    double T = Told + NRs[4 * q.nsk + o] * J;
+    // NR of this cell in the sweep order
+    const int a = (i - 3) % TI, b = (j - 2) % TJ;
+    const int tile = (i - 3) / TI + ((j - 2) / TJ) * q.nTI;
+    const long o = sk_at(q, tile, (k - 2) + a + b, a + 8 * b);
+    double P = Pold + NR[o] * J;
+    double Uu = Uold + NR[q.nsk + o] * J;
+    double V = Vold + NR[2 * q.nsk + o] * J;
+    double W = Wold + NR[3 * q.nsk + o] * J;
+    double T = Told + NR[4 * q.nsk + o] * J;
     double rho = P / T;
     Ui[c] = rho / J;
     Ui[nf + c] = rho * Uu / J;
     Ui[2 * nf + c] = rho * V / J;
     Ui[3 * nf + c] = rho * W / J;
     Ui[4 * nf + c] = (P / (KG - 1) + 0.5 * rho * (Uu * Uu + V * V + W * W)) / J;
-    double dq[5] = {(P - Pold) * (P - Pold), (Uu - Uold) * (Uu - Uold), (V - Vold) * (V - Vold),
-                    (W - Wold) * (W - Wold), (T - Told) * (T - Told)};
+    double q[5] = {(P - Pold) * (P - Pold), (Uu - Uold) * (Uu - Uold), (V - Vold) * (V - Vold),
+                   (W - Wold) * (W - Wold), (T - Told) * (T - Told)};
 #pragma unroll
-    for (int v = 0; v < 5; v++) r[v] = dq[v] * dq[v];
+    for (int v = 0; v < 5; v++) r[v] = q[v] * q[v];
   }
 #pragma unroll
   for (int v = 0; v < 5; v++) red[v][tid] = r[v];
   __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if (tid < st)
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s)
 #pragma unroll
-      for (int v = 0; v < 5; v++) red[v][tid] += red[v][tid + st];
+      for (int v = 0; v < 5; v++) red[v][tid] += red[v][tid + s];
     __syncthreads();
   }
   if (tid == 0) {
-    long blk = (long)blockIdx.y * gridDim.x + blockIdx.x;
+    long b = ((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
 #pragma unroll
-    for (int v = 0; v < 5; v++) partial[blk * 5 + v] = red[v][0];
+    for (int v = 0; v < 5; v++) partial[b * 5 + v] = red[v][0];
   }
 }
 
@@ -940,50 +975,42 @@ long sweep_update_blocks(const Geo& g) {
   return (long)((q.nsteps + 3) / 4) * q.ntiles;
 }
 
-void launch_sweeps(hipStream_t st, const double* Ui, const double* MR, double* Sbuf, double* NL, double* NR,
+void launch_sweeps(hipStream_t st, double* Ui, const double* MR, double* Sbuf, double* NL, double* NR,
                    const double* prof, const Geo& g, double deltaT, int n_sweeps, const int* order, int* sync,
-                   long spin_limit) {
-  // Sbuf holds S_N summary fields, then MR, NL, NR (5 each) in sweep order
+                   double* partial, double* er_dev, double DN, long spin_limit) {
+  // Sbuf holds G_N summary fields, then MR, NL, NR (5 each) in sweep order
   Sk q = sweep_layout(g);
   double* S = Sbuf;
-  double* MRs = Sbuf + (long)S_N * q.nsk;
+  double* MRs = Sbuf + (long)G_N * q.nsk;
   double* NLs = MRs + 5 * q.nsk;
   double* NRs = NLs + 5 * q.nsk;
-  hipLaunchKernelGGL(k_sweep_prep, dim3((q.nsteps + 15) / 16, q.ntiles), dim3(1024), 0, st, Ui, MR, prof, S, MRs, g,
-                     q, deltaT);
-  const long tot = (long)q.ntiles * q.nsteps * 64;
-  const unsigned fill_blocks = (unsigned)((tot + 255) / 256);
+  hipLaunchKernelGGL(k_sweep_prep, dim3((q.nsteps + 15) / 16, q.ntiles), dim3(1024), 0, st, Ui, MR, prof, S, MRs, NLs,
+                     g, q, deltaT);
   const int nt = q.ntiles;
   for (int sweep = 0; sweep < n_sweeps; sweep++) {
-    hipLaunchKernelGGL(k_sweep_reset, dim3((nt + 2 + 255) / 256), dim3(256), 0, st, sync, nt + 2);
-    hipLaunchKernelGGL(k_sweep_sentinel, dim3(fill_blocks), dim3(256), 0, st, NLs, q, 1);
-    hipLaunchKernelGGL(k_sweep_tiles<true>, dim3(nt), dim3(64), 0, st, S, MRs, NLs, prof, g, q, order, sync,
+    hipLaunchKernelGGL(k_sweep_reset, dim3((2 * nt + 4 + 255) / 256), dim3(256), 0, st, sync, 2 * nt + 4);
+    hipLaunchKernelGGL(k_sweep_tiles<true>, dim3(nt), dim3(64), 0, st, S, MRs, NLs, NRs, prof, g, q, order, sync,
                        spin_limit);
-    hipLaunchKernelGGL(k_sweep_reset, dim3((nt + 2 + 255) / 256), dim3(256), 0, st, sync + nt + 2, nt + 2);
-    hipLaunchKernelGGL(k_sweep_sentinel, dim3(fill_blocks), dim3(256), 0, st, NRs, q, 0);
-    hipLaunchKernelGGL(k_sweep_tiles<false>, dim3(nt), dim3(64), 0, st, S, NLs, NRs, prof, g, q, order, sync + nt + 2,
-                       spin_limit);
+    hipLaunchKernelGGL(k_sweep_tiles<false>, dim3(nt), dim3(64), 0, st, S, NLs, NRs, NRs, prof, g, q, order,
+                       sync + nt + 2, spin_limit);
   }
+  // primitive update reading NR from the sweep order
+  dim3 blk(64, 4);
+  dim3 gr = box_grid(g.nz - 1, g.ny - 1, g.gend - 2, blk);
+  hipLaunchKernelGGL(k_update_from_sweep, gr, blk, 0, st, Ui, NRs, prof, partial, g, q);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, st, partial, (long)gr.x * gr.y * gr.z, er_dev, DN);
   (void)NL;
   (void)NR;
 }
 
 void launch_sweep_outputs(hipStream_t st, const double* Sbuf, double* NL, double* NR, const Geo& g) {
   Sk q = sweep_layout(g);
-  const double* NLs = Sbuf + (long)(S_N + 5) * q.nsk;
+  const double* NLs = Sbuf + (long)(G_N + 5) * q.nsk;
   const double* NRs = NLs + 5 * q.nsk;
   hipLaunchKernelGGL(k_sk_to_natural, dim3((q.nsteps + 15) / 16, q.ntiles), dim3(1024), 0, st, NLs, NL, g, q, 5);
   hipLaunchKernelGGL(k_sk_to_natural, dim3((q.nsteps + 15) / 16, q.ntiles), dim3(1024), 0, st, NRs, NR, g, q, 5);
 }
 
-void launch_update_sk(hipStream_t st, double* Ui, const double* Sbuf, const double* prof, double* partial,
-                      double* er_dev, const Geo& g, double DN) {
-  Sk q = sweep_layout(g);
-  const double* NRs = Sbuf + (long)(S_N + 10) * q.nsk;
-  dim3 gr((q.nsteps + 3) / 4, q.ntiles);
-  hipLaunchKernelGGL(k_update_sk, gr, dim3(256), 0, st, Ui, NRs, prof, partial, g, q);
-  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, st, partial, (long)gr.x * gr.y, er_dev, DN);
-}
 
 void launch_sweeps_planes(hipStream_t st, const double* Ui, const double* MR, double* NL, double* NR,
                           const double* prof, const Geo& g, double deltaT, int n_sweeps) {

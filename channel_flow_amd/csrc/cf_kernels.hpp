@@ -14,14 +14,14 @@ void launch_rhs(hipStream_t st, const double* Ui, const double* Un, const double
 int sweep_tiles(const Geo& g, int* nTI, int* nTJ);
 long sweep_field_elems(const Geo& g);
 long sweep_update_blocks(const Geo& g);
-// persistent tiled sweeps; Sbuf: S_N + 15 fields of sweep_field_elems doubles
-void launch_sweeps(hipStream_t st, const double* Ui, const double* MR, double* Sbuf, double* NL, double* NR,
+// persistent tiled sweeps (diagonal metrics); Sbuf: G_N + 15 fields of sweep_field_elems doubles
+// ... including the primitive update of Ui and er (last backward sweep)
+void launch_sweeps(hipStream_t st, double* Ui, const double* MR, double* Sbuf, double* NL, double* NR,
                    const double* prof, const Geo& g, double deltaT, int n_sweeps, const int* order, int* sync,
-                   long spin_limit);
+                   double* partial, double* er_dev, double DN, long spin_limit);
 // NL and NR of the last sweep back in the natural layout
 void launch_sweep_outputs(hipStream_t st, const double* Sbuf, double* NL, double* NR, const Geo& g);
-void launch_update_sk(hipStream_t st, double* Ui, const double* Sbuf, const double* prof, double* partial,
-                      double* er_dev, const Geo& g, double DN);
+
 void launch_sweeps_planes(hipStream_t st, const double* Ui, const double* MR, double* NL, double* NR,
                           const double* prof, const Geo& g, double deltaT, int n_sweeps);
 void launch_copy_interior(hipStream_t st, const double* src, double* dst, const Geo& g);
