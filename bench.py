@@ -29,11 +29,35 @@ sys.path.insert(0, str(ROOT))
 
 GRID = (256, 192, 256)
 HBM_PEAK_GBS = 8000.0
-# algorithmic FP64 fields read + written per interior cell (geometry excluded,
-# each field counted once): fluxes U_ 5 -> F 5; viscous U_ 5 -> vF 4; RHS
-# U_,U,Uq,inF x3,vF -> MR, NL; each sweep direction per sweep 20; update 15
-FIELDS = {"flux_x": 10, "flux_y": 10, "flux_z": 10, "viscous": 9, "rhs": 44, "lusgs_sweeps": 40,
-          "update_norms": 15, "ghosts": 0}
+# algorithmic FP64 fields read + written per interior cell and inner
+# iteration (geometry excluded, each field counted once): fluxes U_ 5 -> F 5;
+# viscous U_ 5 -> vF 4; RHS U_,U,Uq,inF x3,vF -> MR, NL; forward sweep U_,MR,NL
+# -> NL and backward sweep U_,NL,NR -> NR, 20 each per sweep; update U_,NR ->
+# U_ 15.  The summary pre-pass and the sync reset are implementation work with
+# no algorithmic bytes of their own.
+def stage_fields(n_sweeps):
+    return {"ghosts": 0, "flux_x": 10, "flux_y": 10, "flux_z": 10, "viscous": 9, "rhs": 44, "sweep_prep": 0,
+            "sweep_sync": 0, "lusgs_fwd": 20 * n_sweeps, "lusgs_bwd": 20 * n_sweeps, "update_norms": 15}
+
+
+# stages that are one kernel per launch, and that kernel's name in rocprofv3
+STAGE_KERNEL = {"flux_x": "cf::k_flux<0>", "flux_y": "cf::k_flux<1>", "flux_z": "cf::k_flux<2>", "rhs": "cf::k_rhs",
+                "lusgs_fwd": "cf::k_sweep_tiles<true>", "lusgs_bwd": "cf::k_sweep_tiles<false>",
+                "sweep_prep": "cf::k_sweep_prep"}
+TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_256x192x256.md"
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (copy-calibrated), or None."""
+    try:
+        for ln in TRAFFIC_FILE.read_text().splitlines():
+            cols = [c.strip().strip("`") for c in ln.strip().strip("|").split("|")]
+            if len(cols) >= 2 and cols[0] == kernel:
+                return int(float(cols[-1]))
+    except (OSError, ValueError):
+        return None
+    return None
 
 
 def log(msg):
@@ -83,6 +107,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-converge", action="store_true", help="skip the physical step to 1e-8")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one plain 1 GiB device copy (the reference kernel for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -94,6 +120,15 @@ def main():
     from channel_flow_amd.solver import Solver
 
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy calibration: 1 GiB read + 1 GiB written")
+        del a, b
     flow = FlowParams(Re_bulk=5600.0, Mach=0.2)
     cfg = ChannelConfig(*GRID, gamma_metric=2.0, gamma_grid=2.0, deltaT=1e-8,
                         f_drive=flow.driving_force(0.005), x_periodic=False)
@@ -117,6 +152,7 @@ def main():
     log(f"timed {args.steps} inner iterations: {ms_per_step:.3f} ms/iter, {value:.1f} Mcell-iter/s, er={er}")
 
     # per-stage HIP-event times (events on the context's stream)
+    fields = stage_fields(cfg.n_sweeps)
     stages = {}
     for _ in range(3):
         _, st = s.profile_inner_iteration()
@@ -124,18 +160,20 @@ def main():
             stages.setdefault(k, []).append((ms, nl))
     stage_ms = {k: float(np.median([m for m, _ in v])) for k, v in stages.items()}
     launches = {k: v[0][1] for k, v in stages.items()}
-    dom = max(stage_ms, key=stage_ms.get)
+    # dominant kernel: the single-kernel stage with the largest time per iteration
+    dom = max((k for k in STAGE_KERNEL if launches.get(k)), key=stage_ms.get)
     per_launch_ms = stage_ms[dom] / launches[dom]
-    bytes_total = FIELDS[dom] * 8 * cfg.cells
-    bytes_per_launch = bytes_total / launches[dom]
+    bytes_per_launch = fields[dom] * 8 * cfg.cells / launches[dom]
     achieved = bytes_per_launch / (per_launch_ms * 1e-3) / 1e9
+    traffic = measured_traffic(STAGE_KERNEL[dom])
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
-                "launches_per_iteration": launches[dom], "avg_launch_us": round(per_launch_ms * 1e3, 3),
-                "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": STAGE_KERNEL[dom],
+                "stage": dom, "launches_per_iteration": launches[dom], "avg_launch_us": round(per_launch_ms * 1e3, 3),
+                "algorithmic_bytes_per_launch": int(bytes_per_launch),
+                "traffic_source": str(TRAFFIC_FILE.relative_to(ROOT)) if traffic else None}
     per_stage = {k: {"ms": round(v, 4), "launches": launches[k],
-                     "GBps_algorithmic": (round(FIELDS[k] * 8 * cfg.cells / (v * 1e-3) / 1e9, 1)
-                                          if FIELDS[k] else None)} for k, v in stage_ms.items()}
+                     "GBps_algorithmic": (round(fields[k] * 8 * cfg.cells / (v * 1e-3) / 1e9, 1)
+                                          if fields[k] and v > 0 else None)} for k, v in stage_ms.items()}
     log(f"stages: {json.dumps(per_stage)}")
 
     conv = None

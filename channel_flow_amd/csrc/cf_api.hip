@@ -465,15 +465,15 @@ static bool diagonal_metrics(const cf_ctx* c) {
 static int effective_schedule(const cf_ctx* c) { return (c->schedule == 0 && diagonal_metrics(c)) ? 0 : 1; }
 
 // LU-SGS forward/backward sweeps (Dual_time_stepping.cpp:514-1277)
-static void sweeps(cf_ctx* c, double deltaT) {
+static void sweeps(cf_ctx* c, double deltaT, const hipEvent_t* marks = nullptr) {
   const Geo& g = c->g;
   if (effective_schedule(c) == 1) {
-    cf::launch_sweeps_planes(c->stream, c->Ui, c->MR, c->NL, c->NR, c->prof, g, deltaT, c->p.n_sweeps);
+    cf::launch_sweeps_planes(c->stream, c->Ui, c->MR, c->NL, c->NR, c->prof, g, deltaT, c->p.n_sweeps, marks);
     return;
   }
   double DN = 1. / ((double)c->p.X_out * c->p.Y_out * c->p.Z_out);
   cf::launch_sweeps(c->stream, c->Ui, c->MR, c->Sm, c->NL, c->NR, c->prof, g, deltaT, c->p.n_sweeps, c->order,
-                    c->sync, c->partial, c->er_dev, DN, c->spin_limit);
+                    c->sync, c->partial, c->er_dev, DN, c->spin_limit, marks);
 }
 
 // primitive update + norms after the sweeps; the tiled schedule applies them
@@ -648,13 +648,17 @@ extern "C" int cf_advance_levels(cf_ctx* c) {
   return sync(c);
 }
 
-extern "C" int cf_profile_inner_iteration(cf_ctx* c, double er[5], double stage_ms[8], int launches[8]) {
+extern "C" int cf_profile_inner_iteration(cf_ctx* c, double er[5], double stage_ms[CF_PROFILE_STAGES],
+                                          int launches[CF_PROFILE_STAGES]) {
   if (!c || !er || !stage_ms || !launches) return CF_ERR_ARG;
   if (!c->have_state) return fail(c, CF_ERR_STATE, "no state loaded");
   const Geo& g = c->g;
-  hipEvent_t ev[9];
-  for (int q = 0; q < 9; q++) HIPCHK(c, hipEventCreate(&ev[q]));
-  int nplanes = g.gend + g.ny + g.nz - 6;
+  const int ns = c->p.n_sweeps;
+  // ev[0..6]: stage ends up to the RHS; then the sweep marks; then the end
+  const int nm = 1 + 3 * ns;
+  std::vector<hipEvent_t> ev(7 + nm + 1);
+  for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
+  hipEvent_t* mk = ev.data() + 7;
   HIPCHK(c, hipEventRecord(ev[0], c->stream));
   cf::launch_halo_bc(c->stream, c->Ui, c->prof, g, c->p.x_periodic);
   HIPCHK(c, hipEventRecord(ev[1], c->stream));
@@ -669,21 +673,34 @@ extern "C" int cf_profile_inner_iteration(cf_ctx* c, double er[5], double stage_
   cf::launch_rhs(c->stream, c->Ui, c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->prof, c->MR, nullptr, g,
                  c->p.deltaT, c->p.f_drive);
   HIPCHK(c, hipEventRecord(ev[6], c->stream));
-  sweeps(c, c->p.deltaT);
-  HIPCHK(c, hipEventRecord(ev[7], c->stream));
+  sweeps(c, c->p.deltaT, mk);
   update(c);
-  HIPCHK(c, hipEventRecord(ev[8], c->stream));
+  HIPCHK(c, hipEventRecord(ev[7 + nm], c->stream));
   int r = finish_iteration(c, er);
   if (r) return r;
-  const int nsw = effective_schedule(c) == 1 ? 2 * nplanes * c->p.n_sweeps : 1 + 6 * c->p.n_sweeps;
-  const int nl[8] = {3, 1, 1, 1, 3, 1, nsw, 2};
-  for (int q = 0; q < 8; q++) {
+  auto span = [&](hipEvent_t a, hipEvent_t b, double* acc) -> int {
     float ms = 0;
-    HIPCHK(c, hipEventElapsedTime(&ms, ev[q], ev[q + 1]));
-    stage_ms[q] = ms;
-    launches[q] = nl[q];
+    hipError_t e = hipEventElapsedTime(&ms, a, b);
+    if (e != hipSuccess) return fail(c, CF_ERR_HIP, "hipEventElapsedTime: %s", hipGetErrorString(e));
+    *acc += ms;
+    return CF_OK;
+  };
+  for (int q = 0; q < CF_PROFILE_STAGES; q++) stage_ms[q] = 0;
+  for (int q = 0; q < 6; q++)
+    if ((r = span(ev[q], ev[q + 1], &stage_ms[q]))) return r;
+  if ((r = span(ev[6], mk[0], &stage_ms[6]))) return r;
+  for (int s = 0; s < ns; s++) {
+    if ((r = span(mk[3 * s], mk[1 + 3 * s], &stage_ms[7]))) return r;
+    if ((r = span(mk[1 + 3 * s], mk[2 + 3 * s], &stage_ms[8]))) return r;
+    if ((r = span(mk[2 + 3 * s], mk[3 + 3 * s], &stage_ms[9]))) return r;
   }
-  for (int q = 0; q < 9; q++) (void)hipEventDestroy(ev[q]);
+  if ((r = span(mk[nm - 1], ev[7 + nm], &stage_ms[10]))) return r;
+  const bool tiled = effective_schedule(c) == 0;
+  const int nplanes = g.gend + g.ny + g.nz - 6;
+  const int nl[CF_PROFILE_STAGES] = {3, 1, 1, 1, 3, 1, tiled ? 1 : 0, tiled ? ns : 0, tiled ? ns : nplanes * ns,
+                                     tiled ? ns : nplanes * ns, 2};
+  for (int q = 0; q < CF_PROFILE_STAGES; q++) launches[q] = nl[q];
+  for (auto& e : ev) (void)hipEventDestroy(e);
   return CF_OK;
 }
 

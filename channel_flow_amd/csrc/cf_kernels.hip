@@ -977,7 +977,7 @@ long sweep_update_blocks(const Geo& g) {
 
 void launch_sweeps(hipStream_t st, double* Ui, const double* MR, double* Sbuf, double* NL, double* NR,
                    const double* prof, const Geo& g, double deltaT, int n_sweeps, const int* order, int* sync,
-                   double* partial, double* er_dev, double DN, long spin_limit) {
+                   double* partial, double* er_dev, double DN, long spin_limit, const hipEvent_t* marks) {
   // Sbuf holds G_N summary fields, then MR, NL, NR (5 each) in sweep order
   Sk q = sweep_layout(g);
   double* S = Sbuf;
@@ -986,13 +986,17 @@ void launch_sweeps(hipStream_t st, double* Ui, const double* MR, double* Sbuf, d
   double* NRs = NLs + 5 * q.nsk;
   hipLaunchKernelGGL(k_sweep_prep, dim3((q.nsteps + 15) / 16, q.ntiles), dim3(1024), 0, st, Ui, MR, prof, S, MRs, NLs,
                      g, q, deltaT);
+  if (marks) (void)hipEventRecord(marks[0], st);
   const int nt = q.ntiles;
   for (int sweep = 0; sweep < n_sweeps; sweep++) {
     hipLaunchKernelGGL(k_sweep_reset, dim3((2 * nt + 4 + 255) / 256), dim3(256), 0, st, sync, 2 * nt + 4);
+    if (marks) (void)hipEventRecord(marks[1 + 3 * sweep], st);
     hipLaunchKernelGGL(k_sweep_tiles<true>, dim3(nt), dim3(64), 0, st, S, MRs, NLs, NRs, prof, g, q, order, sync,
                        spin_limit);
+    if (marks) (void)hipEventRecord(marks[2 + 3 * sweep], st);
     hipLaunchKernelGGL(k_sweep_tiles<false>, dim3(nt), dim3(64), 0, st, S, NLs, NRs, NRs, prof, g, q, order,
                        sync + nt + 2, spin_limit);
+    if (marks) (void)hipEventRecord(marks[3 + 3 * sweep], st);
   }
   // primitive update reading NR from the sweep order
   dim3 blk(64, 4);
@@ -1013,11 +1017,13 @@ void launch_sweep_outputs(hipStream_t st, const double* Sbuf, double* NL, double
 
 
 void launch_sweeps_planes(hipStream_t st, const double* Ui, const double* MR, double* NL, double* NR,
-                          const double* prof, const Geo& g, double deltaT, int n_sweeps) {
+                          const double* prof, const Geo& g, double deltaT, int n_sweeps, const hipEvent_t* marks) {
   int mmin = 7, mmax = g.gend + g.ny + g.nz;
   int nj = g.ny - 1;
   dim3 blk(64);
+  if (marks) (void)hipEventRecord(marks[0], st);
   for (int sweep = 0; sweep < n_sweeps; sweep++) {
+    if (marks) (void)hipEventRecord(marks[1 + 3 * sweep], st);
     for (int m = mmin; m <= mmax; m++) {
       int ilo = max(3, m - g.ny - g.nz), ihi = min(g.gend, m - 4);
       hipLaunchKernelGGL(k_sweep_fwd, dim3((nj + 63) / 64, ihi - ilo + 1), blk, 0, st, Ui, MR, NL, prof, g, deltaT, m,
@@ -1025,9 +1031,11 @@ void launch_sweeps_planes(hipStream_t st, const double* Ui, const double* MR, do
     }
     for (int m = mmax; m >= mmin; m--) {
       int ilo = max(3, m - g.ny - g.nz), ihi = min(g.gend, m - 4);
+      if (m == mmax && marks) (void)hipEventRecord(marks[2 + 3 * sweep], st);
       hipLaunchKernelGGL(k_sweep_bwd, dim3((nj + 63) / 64, ihi - ilo + 1), blk, 0, st, Ui, NL, NR, prof, g, deltaT, m,
                          ilo);
     }
+    if (marks) (void)hipEventRecord(marks[3 + 3 * sweep], st);
   }
 }
 

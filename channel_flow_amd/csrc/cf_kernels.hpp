@@ -14,16 +14,20 @@ void launch_rhs(hipStream_t st, const double* Ui, const double* Un, const double
 int sweep_tiles(const Geo& g, int* nTI, int* nTJ);
 long sweep_field_elems(const Geo& g);
 long sweep_update_blocks(const Geo& g);
-// persistent tiled sweeps (diagonal metrics); Sbuf: G_N + 15 fields of sweep_field_elems doubles
-// ... including the primitive update of Ui and er (last backward sweep)
+// marks (optional, 1 + 3 * n_sweeps events) are recorded after the summary
+// pre-pass, then per sweep after the sync reset, the forward and the backward
+// sweep.
+// persistent tiled sweeps (diagonal metrics); Sbuf: G_N + 15 fields of
+// sweep_field_elems doubles; ends with the primitive update of Ui and er
 void launch_sweeps(hipStream_t st, double* Ui, const double* MR, double* Sbuf, double* NL, double* NR,
                    const double* prof, const Geo& g, double deltaT, int n_sweeps, const int* order, int* sync,
-                   double* partial, double* er_dev, double DN, long spin_limit);
+                   double* partial, double* er_dev, double DN, long spin_limit, const hipEvent_t* marks = nullptr);
 // NL and NR of the last sweep back in the natural layout
 void launch_sweep_outputs(hipStream_t st, const double* Sbuf, double* NL, double* NR, const Geo& g);
 
 void launch_sweeps_planes(hipStream_t st, const double* Ui, const double* MR, double* NL, double* NR,
-                          const double* prof, const Geo& g, double deltaT, int n_sweeps);
+                          const double* prof, const Geo& g, double deltaT, int n_sweeps,
+                          const hipEvent_t* marks = nullptr);
 void launch_copy_interior(hipStream_t st, const double* src, double* dst, const Geo& g);
 long update_blocks(const Geo& g);
 void launch_update(hipStream_t st, double* Ui, const double* NR, const double* prof, double* partial, double* er_dev,

@@ -234,13 +234,14 @@ class Solver:
         self._chk(self.lib.cf_inner_iteration(self._h, er))
         return np.array(er[:])
 
-    STAGES = ["ghosts", "flux_x", "flux_y", "flux_z", "viscous", "rhs", "lusgs_sweeps", "update_norms"]
+    STAGES = ["ghosts", "flux_x", "flux_y", "flux_z", "viscous", "rhs", "sweep_prep", "sweep_sync", "lusgs_fwd",
+              "lusgs_bwd", "update_norms"]
 
     def profile_inner_iteration(self):
         """One inner iteration with per-stage HIP-event times: (er, {stage: (ms, launches)})."""
         er = (C.c_double * 5)()
-        ms = (C.c_double * 8)()
-        nl = (C.c_int * 8)()
+        ms = (C.c_double * len(self.STAGES))()
+        nl = (C.c_int * len(self.STAGES))()
         self._chk(self.lib.cf_profile_inner_iteration(self._h, er, ms, nl))
         return np.array(er[:]), {n: (ms[q], nl[q]) for q, n in enumerate(self.STAGES)}
 

@@ -142,11 +142,14 @@ int cf_advance_levels(cf_ctx* ctx);
 /* Output (main.h:872-896, Output.cpp:56-190): writes <dir>/LES%05d.bin. */
 int cf_write_output(cf_ctx* ctx, int step, const char* dir);
 /* Runs one inner iteration like cf_inner_iteration and times its stages
- * with HIP events recorded on the context's stream.  stage_ms[8]: ghosts,
- * Flux_X, Flux_Y, Flux_Z, Viscous_terms, RHS, LU-SGS sweeps, update+norms;
- * launches[8]: kernel launches per stage.  No counterpart in the reference
- * (it has no timers, MPI_prm.h:14). */
-int cf_profile_inner_iteration(cf_ctx* ctx, double er[5], double stage_ms[8], int launches[8]);
+ * with HIP events recorded on the context's stream.  stage_ms: ghosts,
+ * Flux_X, Flux_Y, Flux_Z, Viscous_terms, RHS, sweep summary pre-pass, sweep
+ * sync reset, forward sweeps, backward sweeps, update+norms (summed over the
+ * n_sweeps sweeps); launches: kernel launches per stage.  No counterpart in
+ * the reference (it has no timers, MPI_prm.h:14). */
+#define CF_PROFILE_STAGES 11
+int cf_profile_inner_iteration(cf_ctx* ctx, double er[5], double stage_ms[CF_PROFILE_STAGES],
+                               int launches[CF_PROFILE_STAGES]);
 /* Visiting order of the LU-SGS sweeps; both give identical bits because each
  * cell depends only on its three upstream neighbours.  schedule 0 (default):
  * one persistent launch per sweep direction, 8x8-column tiles walking k in
