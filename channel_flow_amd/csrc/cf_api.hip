@@ -462,7 +462,8 @@ static bool diagonal_metrics(const cf_ctx* c) {
   return true;
 }
 
-static int effective_schedule(const cf_ctx* c) { return (c->schedule == 0 && diagonal_metrics(c)) ? 0 : 1; }
+// 0: tiled (pipelined or, for schedule 2, single-wave tiles), 1: per hyperplane
+static int effective_schedule(const cf_ctx* c) { return (c->schedule != 1 && diagonal_metrics(c)) ? 0 : 1; }
 
 // LU-SGS forward/backward sweeps (Dual_time_stepping.cpp:514-1277)
 static void sweeps(cf_ctx* c, double deltaT, const hipEvent_t* marks = nullptr) {
@@ -473,7 +474,7 @@ static void sweeps(cf_ctx* c, double deltaT, const hipEvent_t* marks = nullptr) 
   }
   double DN = 1. / ((double)c->p.X_out * c->p.Y_out * c->p.Z_out);
   cf::launch_sweeps(c->stream, c->Ui, c->MR, c->Sm, c->NL, c->NR, c->prof, g, deltaT, c->p.n_sweeps, c->order,
-                    c->sync, c->partial, c->er_dev, DN, c->spin_limit, marks);
+                    c->sync, c->partial, c->er_dev, DN, c->spin_limit, marks, c->schedule != 2);
 }
 
 // primitive update + norms after the sweeps; the tiled schedule applies them
@@ -705,7 +706,7 @@ extern "C" int cf_profile_inner_iteration(cf_ctx* c, double er[5], double stage_
 }
 
 extern "C" int cf_set_sweep_schedule(cf_ctx* c, int schedule) {
-  if (!c || schedule < 0 || schedule > 1) return fail(c, CF_ERR_ARG, "bad schedule");
+  if (!c || schedule < 0 || schedule > 2) return fail(c, CF_ERR_ARG, "bad schedule");
   c->schedule = schedule;
   return CF_OK;
 }

@@ -317,35 +317,54 @@ struct DNb {  // one neighbour as seen along one direction
   double P, T, beta, U, lam, pd, p51;
 };
 
-// lower (forward) sum -A+ N of a neighbour along direction D with diagonal
-// metric n (Dual_time_stepping.cpp:751-811 without the exact-zero terms)
-template <int D>
-__device__ __forceinline__ void lower_diag(const DNb& c, double n, const double N[5], double o[5]) {
-  const double dd = fabs(c.lam);
-  const double p11 = 0.5 * (c.beta * c.U + dd);
-  const double p1q = 0.5 * c.P * KG * c.beta * n;
-  const double p22 = 0.5 * (c.U + dd);
-  const double p5q = D == 0 ? 0.5 * n * c.beta * c.T * (KG - 1) : 0.5 * c.beta * c.T * (KG - 1) * n;
-  constexpr int q = D + 1;
-  o[0] = -p11 * N[0] - p1q * N[q];
-#pragma unroll
-  for (int m = 1; m <= 3; m++) o[m] = (m == q) ? -c.pd * N[0] - p22 * N[m] : -(p22 * N[m]);
-  o[4] = -c.p51 * N[0] - p5q * N[q] - p22 * N[4];
+// Split-Jacobian entries of one neighbour along direction D with diagonal
+// metric n: forward A+ (c11 = p11, c22 = p22) or backward A- (m11, m22);
+// c1q, c5q, pd, p51 are shared.  They depend on the neighbour's state and
+// the metric at its j only.
+struct SplitCoef {
+  double c11, c1q, c22, c5q, pd, p51;
+};
+
+template <int D, bool FWD>
+__device__ __forceinline__ SplitCoef diag_coef(double P, double T, double beta, double U, double lam, double pd,
+                                          double p51, double n) {
+  const double dd = fabs(lam);
+  SplitCoef c;
+  c.c11 = FWD ? 0.5 * (beta * U + dd) : 0.5 * (beta * U - dd);
+  c.c1q = 0.5 * P * KG * beta * n;
+  c.c22 = FWD ? 0.5 * (U + dd) : 0.5 * (U - dd);
+  c.c5q = D == 0 ? 0.5 * n * beta * T * (KG - 1) : 0.5 * beta * T * (KG - 1) * n;
+  c.pd = pd;
+  c.p51 = p51;
+  return c;
 }
 
-// upper (backward) sum of a neighbour (Dual_time_stepping.cpp:1116-1177)
+// the neighbour's contribution: forward -A+ N (Dual_time_stepping.cpp:751-811),
+// backward A- N (Dual_time_stepping.cpp:1116-1177), without the exact-zero terms
+template <int D, bool FWD>
+__device__ __forceinline__ void diag_apply(const SplitCoef& c, const double N[5], double o[5]) {
+  constexpr int q = D + 1;
+  if (FWD) {
+    o[0] = -c.c11 * N[0] - c.c1q * N[q];
+#pragma unroll
+    for (int m = 1; m <= 3; m++) o[m] = (m == q) ? -c.pd * N[0] - c.c22 * N[m] : -(c.c22 * N[m]);
+    o[4] = -c.p51 * N[0] - c.c5q * N[q] - c.c22 * N[4];
+  } else {
+    o[0] = +c.c11 * N[0] + c.c1q * N[q];
+#pragma unroll
+    for (int m = 1; m <= 3; m++) o[m] = (m == q) ? +c.pd * N[0] + c.c22 * N[m] : c.c22 * N[m];
+    o[4] = +c.p51 * N[0] + c.c5q * N[q] + c.c22 * N[4];
+  }
+}
+
+template <int D>
+__device__ __forceinline__ void lower_diag(const DNb& c, double n, const double N[5], double o[5]) {
+  diag_apply<D, true>(diag_coef<D, true>(c.P, c.T, c.beta, c.U, c.lam, c.pd, c.p51, n), N, o);
+}
+
 template <int D>
 __device__ __forceinline__ void upper_diag(const DNb& c, double n, const double N[5], double o[5]) {
-  const double dd = fabs(c.lam);
-  const double m11 = 0.5 * (c.beta * c.U - dd);
-  const double p1q = 0.5 * c.P * KG * c.beta * n;
-  const double m22 = 0.5 * (c.U - dd);
-  const double p5q = D == 0 ? 0.5 * n * c.beta * c.T * (KG - 1) : 0.5 * c.beta * c.T * (KG - 1) * n;
-  constexpr int q = D + 1;
-  o[0] = +m11 * N[0] + p1q * N[q];
-#pragma unroll
-  for (int m = 1; m <= 3; m++) o[m] = (m == q) ? +c.pd * N[0] + m22 * N[m] : m22 * N[m];
-  o[4] = +c.p51 * N[0] + p5q * N[q] + m22 * N[4];
+  diag_apply<D, false>(diag_coef<D, false>(c.P, c.T, c.beta, c.U, c.lam, c.pd, c.p51, n), N, o);
 }
 
 // x/d for a fixed divisor d with r = RN(1/d): q0 = x*r is within one ulp of

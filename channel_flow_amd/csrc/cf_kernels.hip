@@ -765,6 +765,395 @@ __global__ void __launch_bounds__(64) k_sweep_tiles(const double* __restrict__ S
   }
 }
 
+// ---------------------------------------------------------------------------
+// Pipelined tiled sweep: the same tiles, steps and arithmetic as
+// k_sweep_tiles, with the loads and the neighbour coefficients taken off the
+// compute wave.  A block is four waves meeting at one LDS-only barrier per
+// step (loads stay in flight across steps):
+//   wave 0  computes one cell per lane and step from LDS only;
+//   wave 1  loads the lane cells' x and y summaries QS steps ahead and turns
+//           them into the split-Jacobian entries their downstream neighbours
+//           use (SplitCoef, diag_coef);
+//   wave 2  loads D^-1, MR (forward) or NL (backward) and the z summary;
+//   wave 3  fetches the cross-tile halo: the neighbour tile's summary (as
+//           SplitCoef) and its NL/NR, polled until the sentinel is overwritten,
+//           QD steps ahead.
+// LDS holds three step slots of per-lane data; slot s-1 carries the
+// neighbour entries of step s, with the halo in columns 64..71 (x: by b,
+// y: by a) so that every lane reads its neighbours at fixed offsets.
+// ---------------------------------------------------------------------------
+#ifndef CF_PIPE_QS
+#define CF_PIPE_QS 3
+#endif
+#ifndef CF_PIPE_QD
+#define CF_PIPE_QD 4
+#endif
+constexpr int PIPE_QS = CF_PIPE_QS;  // own-field lookahead (steps)
+constexpr int PIPE_QD = CF_PIPE_QD;  // halo lookahead (steps)
+constexpr int PO_D11 = 0, PO_D22 = 1, PO_D51 = 2, PO_M = 3, PO_Z = 8, PO_N = 14;  // rows of PipeLds::own
+
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+struct PipeLds {
+  double own[3][PO_N][64];  // D11 D22 D51, MR or NL (5), z SplitCoef (6) of the lane cells
+  double xc[3][6][72];      // x SplitCoef: lanes, then the x halo by b
+  double yc[3][6][72];      // y SplitCoef: lanes, then the y halo by a
+  double N[2][5][80];       // outputs of the previous step: lanes, x halo by b (64+), y halo by a (72+)
+  int ticket;
+};
+
+__device__ __forceinline__ void coef_store(double (&row)[6][72], int col, const SplitCoef& c) {
+  row[0][col] = c.c11;
+  row[1][col] = c.c1q;
+  row[2][col] = c.c22;
+  row[3][col] = c.c5q;
+  row[4][col] = c.pd;
+  row[5][col] = c.p51;
+}
+
+__device__ __forceinline__ SplitCoef coef_load(const double (&row)[6][72], int col) {
+  return SplitCoef{row[0][col], row[1][col], row[2][col], row[3][col], row[4][col], row[5][col]};
+}
+
+// wave 0: one cell per lane and step, inputs from LDS only
+template <bool FWD>
+__device__ void pipe_compute(PipeLds& L, double* __restrict__ SRC, double* __restrict__ DST,
+                             double* __restrict__ AUX, const Geo& g, const Sk& q, int tile, int TX, int TY) {
+  const int lane = threadIdx.x & 63;
+  const int a = lane & 7, b = lane >> 3;
+  const int i = 3 + TX * TI + a, j = 2 + TY * TJ + b;
+  const bool col = i <= g.gend && j <= g.ny;
+  const int Nz = q.Nz, nsteps = q.nsteps;
+  const int off = FWD ? a + b : (TI - 1 - a) + (TJ - 1 - b);
+  const bool x_dom = FWD ? i - 1 >= 3 : i + 1 <= g.gend;
+  const bool y_dom = FWD ? j - 1 >= 2 : j + 1 <= g.ny;
+  const bool x_in = FWD ? a > 0 : a < TI - 1;
+  const bool y_in = FWD ? b > 0 : b < TJ - 1;
+  // neighbour columns: in-tile lanes or the halo
+  const int xi = x_in ? (FWD ? lane - 1 : lane + 1) : 64 + b;
+  const int yi = y_in ? (FWD ? lane - 8 : lane + 8) : 64 + a;
+  const int yiN = y_in ? yi : 72 + a;
+  const long nf = q.nsk;
+  const bool edge_fwd = a == TI - 1 || b == TJ - 1;
+  const bool edge_bwd = a == 0 || b == 0;
+  const double sv = __longlong_as_double((long long)SENTINEL);
+  double zN[5] = {0, 0, 0, 0, 0};
+  SplitCoef zc = {0, 0, 0, 0, 0, 0};
+  int cs = 0, ps = 2;
+  lds_barrier();  // step 0 inputs are in LDS
+  for (int tau = 0; tau < nsteps; tau++) {
+    const int kk = tau - off;
+    const int nr = (tau & 1) ^ 1, nw = tau & 1;
+    if (col && kk >= 0 && kk < Nz) {
+      const double d11 = L.own[cs][PO_D11][lane], d22 = L.own[cs][PO_D22][lane], d51 = L.own[cs][PO_D51][lane];
+      double M[5];
+#pragma unroll
+      for (int v = 0; v < 5; v++) M[v] = L.own[cs][PO_M + v][lane];
+      double acc[3][5];
+      if (x_dom) {
+        double Nb[5];
+#pragma unroll
+        for (int v = 0; v < 5; v++) Nb[v] = L.N[nr][v][xi];
+        diag_apply<0, FWD>(coef_load(L.xc[ps], xi), Nb, acc[0]);
+      } else {
+#pragma unroll
+        for (int v = 0; v < 5; v++) acc[0][v] = 0.0;
+      }
+      if (y_dom) {
+        double Nb[5];
+#pragma unroll
+        for (int v = 0; v < 5; v++) Nb[v] = L.N[nr][v][yiN];
+        diag_apply<1, FWD>(coef_load(L.yc[ps], yi), Nb, acc[1]);
+      } else {
+#pragma unroll
+        for (int v = 0; v < 5; v++) acc[1][v] = 0.0;
+      }
+      if (kk > 0) {
+        diag_apply<2, FWD>(zc, zN, acc[2]);
+      } else {
+#pragma unroll
+        for (int v = 0; v < 5; v++) acc[2][v] = 0.0;
+      }
+      double Lv[5];
+#pragma unroll
+      for (int v = 0; v < 5; v++)
+        Lv[v] = div_const(acc[0][v], g.dXI, g.rXI) + div_const(acc[1][v], g.dET, g.rET) +
+                div_const(acc[2][v], g.dZT, g.rZT);
+      double out[5];
+      if (FWD) {
+        out[0] = d11 * (M[0] - Lv[0]);
+        out[1] = d22 * (M[1] - Lv[1]);
+        out[2] = d22 * (M[2] - Lv[2]);
+        out[3] = d22 * (M[3] - Lv[3]);
+        out[4] = d51 * (M[0] - Lv[0]) + d22 * (M[4] - Lv[4]);
+      } else {
+        out[0] = M[0] - d11 * Lv[0];
+        out[1] = M[1] - d22 * Lv[1];
+        out[2] = M[2] - d22 * Lv[2];
+        out[3] = M[3] - d22 * Lv[3];
+        out[4] = M[4] - (d51 * Lv[0] + d22 * Lv[4]);
+      }
+      const long c = sk_at(q, tile, FWD ? tau : nsteps - 1 - tau, lane);
+#pragma unroll
+      for (int v = 0; v < 5; v++) st_coherent(DST + v * nf + c, out[v]);
+      if (FWD && edge_bwd) {
+        // the backward sweep that follows hands these NR slots across tiles
+#pragma unroll
+        for (int v = 0; v < 5; v++) AUX[v * nf + c] = sv;
+      }
+      if (!FWD && edge_fwd) {
+        // this NL slot has been consumed; the next forward sweep hands it on
+#pragma unroll
+        for (int v = 0; v < 5; v++) SRC[v * nf + c] = sv;
+      }
+#pragma unroll
+      for (int v = 0; v < 5; v++) {
+        L.N[nw][v][lane] = out[v];
+        zN[v] = out[v];
+      }
+      zc = SplitCoef{L.own[cs][PO_Z + 0][lane], L.own[cs][PO_Z + 1][lane], L.own[cs][PO_Z + 2][lane],
+                L.own[cs][PO_Z + 3][lane], L.own[cs][PO_Z + 4][lane], L.own[cs][PO_Z + 5][lane]};
+    }
+    ps = cs;
+    cs = cs == 2 ? 0 : cs + 1;
+    lds_barrier();
+  }
+}
+
+// Loader step loop shared by the loader waves: before barrier tau+1 the data
+// of step tau+1 goes from register slot (tau+1) % Q to LDS and the slot is
+// refilled with step tau+1+Q.  Every iteration of the main loop issues the
+// same loads (the step index is clamped, so addresses stay inside the
+// arrays), which lets the compiler count the loads in flight exactly and wait
+// only for the oldest slot; the guarded remainder runs after it.
+template <int Q, class Consume, class Issue>
+__device__ __forceinline__ void pipe_loader_loop(int nsteps, Consume consume, Issue issue) {
+  static_assert(Q >= 1 && Q <= 6, "lookahead");
+  int t0 = 0;
+  for (; t0 + Q <= nsteps; t0 += Q) {
+#pragma unroll
+    for (int k = 0; k < Q; k++) {
+      consume(t0 + k + 1, (k + 1) % Q);
+      issue(t0 + k + 1 + Q, (k + 1) % Q);
+      lds_barrier();
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < Q; k++) {
+    if (t0 + k < nsteps) {
+      consume(t0 + k + 1, (k + 1) % Q);
+      lds_barrier();
+    }
+  }
+}
+
+template <int Q, int NF>
+struct Slots {
+  double v[Q][NF];
+  __device__ __forceinline__ void put(int sl, const double (&d)[NF]) {
+#pragma unroll
+    for (int k = 0; k < Q; k++)
+      if (k == sl) {
+#pragma unroll
+        for (int f = 0; f < NF; f++) v[k][f] = d[f];
+      }
+  }
+  __device__ __forceinline__ void get(int sl, double (&d)[NF]) const {
+#pragma unroll
+    for (int f = 0; f < NF; f++) d[f] = v[0][f];
+#pragma unroll
+    for (int k = 1; k < Q; k++)
+      if (k == sl) {
+#pragma unroll
+        for (int f = 0; f < NF; f++) d[f] = v[k][f];
+      }
+  }
+};
+
+// waves 1 and 2: the lane cells' fields QS steps ahead.  XY (wave 1): P, T,
+// beta and the x, y summaries -> x, y SplitCoef.  Otherwise (wave 2): D^-1, SRC and
+// the z summary -> own rows.
+template <bool FWD, bool XY, int QS>
+__device__ void pipe_own_loader(PipeLds& L, const double* __restrict__ S, const double* __restrict__ SRC,
+                                const double* __restrict__ prof, const Geo& g, const Sk& q, int tile, int TY) {
+  constexpr int NF = XY ? 11 : 15;
+  const int lane = threadIdx.x & 63;
+  const int nsteps = q.nsteps;
+  const long nf = q.nsk;
+  const long row0 = (long)tile * nsteps * 64 + lane;
+  // metric of the lane's j (clamped for columns outside the domain)
+  const int j = min(2 + TY * TJ + (lane >> 3), g.ny);
+  const double nx = pf(prof, g.Ym, SET_C, 1, j), ny_ = pf(prof, g.Ym, SET_C, 5, j), nz_ = pf(prof, g.Ym, SET_C, 9, j);
+  // fields: XY: P T beta UX LX PDX P51X UY LY PDY P51Y;
+  //         z:  D11 D22 D51 SRC0..4 P T beta UZ LZ PDZ P51Z
+  auto field = [&](int f) -> const double* {
+    if (XY) return S + (long)(f < 3 ? G_P + f : G_UX + (f - 3)) * nf;
+    if (f < 3) return S + (long)(G_D11 + f) * nf;
+    if (f < 8) return SRC + (long)(f - 3) * nf;
+    return S + (long)(f < 11 ? G_P + (f - 8) : G_UZ + (f - 11)) * nf;
+  };
+  Slots<QS, NF> buf;
+  auto issue = [&](int sig, int sl) {
+    const int sc = min(sig, nsteps - 1);
+    const long c = row0 + (long)(FWD ? sc : nsteps - 1 - sc) * 64;
+    double d[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++) d[f] = field(f)[c];
+    buf.put(sl, d);
+  };
+  auto consume = [&](int sig, int sl) {
+    if (sig >= nsteps) return;
+    double d[NF];
+    buf.get(sl, d);
+    const int s3 = sig % 3;
+    if (XY) {
+      coef_store(L.xc[s3], lane, diag_coef<0, FWD>(d[0], d[1], d[2], d[3], d[4], d[5], d[6], nx));
+      coef_store(L.yc[s3], lane, diag_coef<1, FWD>(d[0], d[1], d[2], d[7], d[8], d[9], d[10], ny_));
+    } else {
+#pragma unroll
+      for (int f = 0; f < 8; f++) L.own[s3][f][lane] = d[f];
+      const SplitCoef z = diag_coef<2, FWD>(d[8], d[9], d[10], d[11], d[12], d[13], d[14], nz_);
+      L.own[s3][PO_Z + 0][lane] = z.c11;
+      L.own[s3][PO_Z + 1][lane] = z.c1q;
+      L.own[s3][PO_Z + 2][lane] = z.c22;
+      L.own[s3][PO_Z + 3][lane] = z.c5q;
+      L.own[s3][PO_Z + 4][lane] = z.pd;
+      L.own[s3][PO_Z + 5][lane] = z.p51;
+    }
+  };
+#pragma unroll
+  for (int k = 0; k < QS; k++) issue(k, k);
+  consume(0, 0);
+  issue(QS, 0);
+  lds_barrier();
+  pipe_loader_loop<QS>(nsteps, consume, issue);
+}
+
+// wave 3: the halo of the next step QD steps ahead.  Lanes 0..7 serve the
+// x-edge consumers (b = lane), lanes 8..15 the y-edge consumers (a = lane-8);
+// each loads its remote cell's summary along the direction (7 values) and
+// NL/NR (5 values).  The other lanes load their own tile's cells (discarded).
+template <bool FWD, int QD>
+__device__ void pipe_halo_loader(PipeLds& L, const double* __restrict__ S, const double* __restrict__ DST,
+                                 const double* __restrict__ prof, const Geo& g, const Sk& q, int tile, int TX,
+                                 int TY, int* err, long spin_limit) {
+  const int lane = threadIdx.x & 63;
+  const int nsteps = q.nsteps, Nz = q.Nz;
+  const long nf = q.nsk;
+  const int dir = lane < 8 ? 0 : 1;
+  const int idx = lane & 7;
+  const int a = dir == 0 ? (FWD ? 0 : TI - 1) : idx;  // consumer lane (a, b)
+  const int b = dir == 0 ? idx : (FWD ? 0 : TJ - 1);
+  const int i = 3 + TX * TI + a, j = 2 + TY * TJ + b;
+  const bool dom = dir == 0 ? (FWD ? i - 1 >= 3 : i + 1 <= g.gend) : (FWD ? j - 1 >= 2 : j + 1 <= g.ny);
+  const bool ok = lane < 16 && i <= g.gend && j <= g.ny && dom;
+  const int off = FWD ? a + b : (TI - 1 - a) + (TJ - 1 - b);
+  const long sx = FWD ? (-(long)nsteps * 64 + 7 * 64 + 7) : ((long)nsteps * 64 - 7 * 64 - 7);
+  const long sy = FWD ? (-(long)q.nTI * nsteps * 64 + 7 * 64 + 56) : ((long)q.nTI * nsteps * 64 - 7 * 64 - 56);
+  const long own0 = (long)tile * nsteps * 64 + (a + 8 * b);
+  const long rem0 = own0 + (dir == 0 ? sx : sy);
+  // metric at the remote cell's j
+  const int jr = dir == 0 ? j : (FWD ? j - 1 : j + 1);
+  const double n = pf(prof, g.Ym, SET_C, dir == 0 ? 1 : 5, min(max(jr, 0), g.Ym - 1));
+  const int dU = G_UX + 4 * dir;
+  auto live = [=](int sig) -> bool {
+    const int kk = sig - off;
+    return ok && kk >= 0 && kk < Nz && sig < nsteps;
+  };
+  const long rdiff = rem0 - own0;
+  auto cell = [=](int sig) -> long {
+    const int sc = min(sig, nsteps - 1);
+    return own0 + (live(sig) ? rdiff : 0L) + (long)(FWD ? sc : nsteps - 1 - sc) * 64;
+  };
+  // values: P T beta U lam pd p51 (summary), N0..N4 (dynamic)
+  auto field = [&](int f) -> const double* {
+    if (f < 3) return S + (long)(G_P + f) * nf;
+    if (f < 7) return S + (long)(dU + f - 3) * nf;
+    return DST + (long)(f - 7) * nf;
+  };
+  Slots<QD, 12> buf;
+  auto issue = [&](int sig, int sl) {
+    const long c = cell(sig);
+    double d[12];
+#pragma unroll
+    for (int f = 0; f < 12; f++) d[f] = ld_coherent(field(f) + c);
+    buf.put(sl, d);
+  };
+  auto consume = [&](int sig, int sl) {
+    if (sig >= nsteps) return;
+    double d[12];
+    buf.get(sl, d);
+    if (live(sig)) {
+      long cnt = 0;
+      const long c = cell(sig);
+      bool pend = false;
+#pragma unroll
+      for (int v = 0; v < 5; v++) pend |= is_sentinel(d[7 + v]);
+      while (pend) {
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || ++cnt > spin_limit) {
+          __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        pend = false;
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+          d[7 + v] = ld_coherent(field(7 + v) + c);
+          pend |= is_sentinel(d[7 + v]);
+        }
+      }
+      const int sp = (sig + 2) % 3, np = (sig & 1) ^ 1;
+      if (dir == 0) {
+        coef_store(L.xc[sp], 64 + idx, diag_coef<0, FWD>(d[0], d[1], d[2], d[3], d[4], d[5], d[6], n));
+#pragma unroll
+        for (int v = 0; v < 5; v++) L.N[np][v][64 + idx] = d[7 + v];
+      } else {
+        coef_store(L.yc[sp], 64 + idx, diag_coef<1, FWD>(d[0], d[1], d[2], d[3], d[4], d[5], d[6], n));
+#pragma unroll
+        for (int v = 0; v < 5; v++) L.N[np][v][72 + idx] = d[7 + v];
+      }
+    }
+  };
+#pragma unroll
+  for (int k = 0; k < QD; k++) issue(k, k);
+  consume(0, 0);
+  issue(QD, 0);
+  lds_barrier();
+  pipe_loader_loop<QD>(nsteps, consume, issue);
+}
+
+template <bool FWD, int QS, int QD>
+__global__ void __launch_bounds__(256) k_sweep_pipe(const double* __restrict__ S, double* __restrict__ SRC,
+                                                    double* __restrict__ DST, double* __restrict__ AUX,
+                                                    const double* __restrict__ prof, Geo g, Sk q,
+                                                    const int* __restrict__ order, int* __restrict__ sync,
+                                                    long spin_limit) {
+  // arrays in sweep order (Sk); S holds the DiagField summary
+  // FWD: SRC = MR, DST = NL   (NL = D^-1 (MR - L)); seeds the NR (AUX) sentinels
+  // BWD: SRC = NL, DST = NR   (NR = NL - D^-1 U); re-seeds the NL sentinels
+  __shared__ PipeLds L;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (threadIdx.x == 0) L.ticket = atomicAdd(sync, 1);
+  __syncthreads();
+  const int t = L.ticket;
+  if (t >= q.ntiles) return;
+  const int tile = order[FWD ? t : q.ntiles - 1 - t];
+  const int TX = tile % q.nTI, TY = tile / q.nTI;
+  if (wave == 0) {
+    pipe_compute<FWD>(L, SRC, DST, AUX, g, q, tile, TX, TY);
+  } else if (wave == 1) {
+    pipe_own_loader<FWD, true, QS>(L, S, SRC, prof, g, q, tile, TY);
+  } else if (wave == 2) {
+    pipe_own_loader<FWD, false, QS>(L, S, SRC, prof, g, q, tile, TY);
+  } else {
+    pipe_halo_loader<FWD, QD>(L, S, DST, prof, g, q, tile, TX, TY, sync + 1, spin_limit);
+  }
+}
+
 __global__ void k_copy_interior(const double* __restrict__ src, double* __restrict__ dst, Geo g) {
   int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
   int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
@@ -977,7 +1366,8 @@ long sweep_update_blocks(const Geo& g) {
 
 void launch_sweeps(hipStream_t st, double* Ui, const double* MR, double* Sbuf, double* NL, double* NR,
                    const double* prof, const Geo& g, double deltaT, int n_sweeps, const int* order, int* sync,
-                   double* partial, double* er_dev, double DN, long spin_limit, const hipEvent_t* marks) {
+                   double* partial, double* er_dev, double DN, long spin_limit, const hipEvent_t* marks,
+                   bool pipelined) {
   // Sbuf holds G_N summary fields, then MR, NL, NR (5 each) in sweep order
   Sk q = sweep_layout(g);
   double* S = Sbuf;
@@ -991,11 +1381,19 @@ void launch_sweeps(hipStream_t st, double* Ui, const double* MR, double* Sbuf, d
   for (int sweep = 0; sweep < n_sweeps; sweep++) {
     hipLaunchKernelGGL(k_sweep_reset, dim3((2 * nt + 4 + 255) / 256), dim3(256), 0, st, sync, 2 * nt + 4);
     if (marks) (void)hipEventRecord(marks[1 + 3 * sweep], st);
-    hipLaunchKernelGGL(k_sweep_tiles<true>, dim3(nt), dim3(64), 0, st, S, MRs, NLs, NRs, prof, g, q, order, sync,
-                       spin_limit);
+    if (pipelined)
+      hipLaunchKernelGGL((k_sweep_pipe<true, PIPE_QS, PIPE_QD>), dim3(nt), dim3(256), 0, st, S, MRs, NLs, NRs, prof,
+                         g, q, order, sync, spin_limit);
+    else
+      hipLaunchKernelGGL(k_sweep_tiles<true>, dim3(nt), dim3(64), 0, st, S, MRs, NLs, NRs, prof, g, q, order, sync,
+                         spin_limit);
     if (marks) (void)hipEventRecord(marks[2 + 3 * sweep], st);
-    hipLaunchKernelGGL(k_sweep_tiles<false>, dim3(nt), dim3(64), 0, st, S, NLs, NRs, NRs, prof, g, q, order,
-                       sync + nt + 2, spin_limit);
+    if (pipelined)
+      hipLaunchKernelGGL((k_sweep_pipe<false, PIPE_QS, PIPE_QD>), dim3(nt), dim3(256), 0, st, S, NLs, NRs, NRs, prof,
+                         g, q, order, sync + nt + 2, spin_limit);
+    else
+      hipLaunchKernelGGL(k_sweep_tiles<false>, dim3(nt), dim3(64), 0, st, S, NLs, NRs, NRs, prof, g, q, order,
+                         sync + nt + 2, spin_limit);
     if (marks) (void)hipEventRecord(marks[3 + 3 * sweep], st);
   }
   // primitive update reading NR from the sweep order

@@ -21,7 +21,8 @@ long sweep_update_blocks(const Geo& g);
 // sweep_field_elems doubles; ends with the primitive update of Ui and er
 void launch_sweeps(hipStream_t st, double* Ui, const double* MR, double* Sbuf, double* NL, double* NR,
                    const double* prof, const Geo& g, double deltaT, int n_sweeps, const int* order, int* sync,
-                   double* partial, double* er_dev, double DN, long spin_limit, const hipEvent_t* marks = nullptr);
+                   double* partial, double* er_dev, double DN, long spin_limit, const hipEvent_t* marks = nullptr,
+                   bool pipelined = true);
 // NL and NR of the last sweep back in the natural layout
 void launch_sweep_outputs(hipStream_t st, const double* Sbuf, double* NL, double* NR, const Geo& g);
 

@@ -150,11 +150,13 @@ int cf_write_output(cf_ctx* ctx, int step, const char* dir);
 #define CF_PROFILE_STAGES 11
 int cf_profile_inner_iteration(cf_ctx* ctx, double er[5], double stage_ms[CF_PROFILE_STAGES],
                                int launches[CF_PROFILE_STAGES]);
-/* Visiting order of the LU-SGS sweeps; both give identical bits because each
+/* Visiting order of the LU-SGS sweeps; all give identical bits because each
  * cell depends only on its three upstream neighbours.  schedule 0 (default):
  * one persistent launch per sweep direction, 8x8-column tiles walking k in
- * dependency order; schedule 1: one launch per hyperplane i+j+k = m (the
- * reference order, Hyperplane.cpp:50-78). */
+ * dependency order, each tile one compute wave fed by three loader waves;
+ * schedule 1: one launch per hyperplane i+j+k = m (the reference order,
+ * Hyperplane.cpp:50-78); schedule 2: the tiles of schedule 0 as single waves
+ * that load their own inputs. */
 int cf_set_sweep_schedule(cf_ctx* ctx, int schedule);
 /* Synchronises the context's stream. */
 int cf_synchronize(cf_ctx* ctx);

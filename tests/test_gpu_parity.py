@@ -190,24 +190,27 @@ def test_gpu_repeatable_bits():
     np.testing.assert_array_equal(outs[0][1], outs[1][1])
 
 
-@pytest.mark.parametrize("grid", [(21, 13, 18), (8, 8, 8), (40, 33, 9)])
+@pytest.mark.parametrize("grid", [(21, 13, 18), (8, 8, 8), (40, 33, 9), (70, 45, 37)])
 def test_gpu_sweep_schedules_identical(grid):
-    """Persistent tiled sweeps and one-launch-per-hyperplane sweeps give the
-    same bits (the DAG fixes the arithmetic), on a grid whose sizes are not
-    multiples of the 8x8 tile."""
+    """Pipelined tiles (0), one launch per hyperplane (1) and single-wave tiles
+    (2) give the same bits (the DAG fixes the arithmetic), on grids whose
+    sizes are not multiples of the 8x8 tile."""
     from channel_flow_amd.config import ChannelConfig, FlowParams, poiseuille_state
     from channel_flow_amd.solver import Solver
     cfg = ChannelConfig(*grid, gamma_metric=2.0, deltaT=1e-8)
     Qn = poiseuille_state(cfg, FlowParams(), seed=9, stream=0)
     Qm = poiseuille_state(cfg, FlowParams(), seed=9, stream=1)
     outs = []
-    for schedule in (0, 1):
+    for schedule in (0, 1, 2):
         with Solver(cfg) as s:
             s.set_sweep_schedule(schedule)
             s.Jacobian_all(2.0, want_arrays=False)
             s.load_state(Qn, Qm)
             ers = [s.inner_iteration() for _ in range(3)]
             outs.append((np.array(ers), s.read_state()[2]))
-    # the state is identical; the norms are summed in a different block order
+    # the state is identical; the per-hyperplane norms are summed in a
+    # different block order
     np.testing.assert_array_equal(outs[0][1], outs[1][1])
+    np.testing.assert_array_equal(outs[0][1], outs[2][1])
+    np.testing.assert_array_equal(outs[0][0], outs[2][0])
     np.testing.assert_allclose(outs[0][0], outs[1][0], rtol=1e-14, atol=0)
