@@ -259,7 +259,8 @@ class Solver:
         self._chk(self.lib.cf_write_output(self._h, int(step), str(directory).encode()))
 
     def set_sweep_schedule(self, schedule: int = 0):
-        """0: persistent tiled sweep (default); 1: one launch per hyperplane."""
+        """0: pipelined tiled sweep (default); 1: one launch per hyperplane;
+        2: single-wave tiles.  All three give the same bits."""
         self._chk(self.lib.cf_set_sweep_schedule(self._h, int(schedule)))
 
     def synchronize(self):

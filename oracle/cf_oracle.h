@@ -10,10 +10,13 @@
  * Y_m = Y_out+4, Z_m = Z_out+4, interior i in [3,gend], j in [2,ny],
  * k in [2,nz] with gend = X_out+2, ny = Y_out+1, nz = Z_out+1.
  *
- * Parity status per function (what pins it is listed in oracle/README.md):
- *   cfo_metrics, cfo_halo_bc, cfo_flux, cfo_viscous, cfo_dual_time,
- *   cfo_hyperplane_counts, cfo_inner_iteration: pinned by golden vectors from
- *   the reference built in oracle/_ref plus closed-form checks.
+ * Parity status per function: cfo_metrics, cfo_hyperplane_counts,
+ * cfo_halo_bc, cfo_flux, cfo_viscous, cfo_dual_time (both sweep orders) and
+ * cfo_inner_iteration are pinned bit for bit by golden vectors that the
+ * reference itself wrote (built from its own sources by oracle/ref_build.sh,
+ * driven by oracle/ref_driver.cpp; fixtures in tests/golden/): per-function
+ * outputs at two non-cubic grids, an 18-step trajectory at 16x12x16 and the
+ * first inner iteration at 64^3.
  */
 #ifndef CF_ORACLE_H
 #define CF_ORACLE_H
