@@ -982,3 +982,275 @@ void cfo_dual_time_hyperplane(const cfo_grid* g, double deltaT, double* const U_
                               double* const MR[5], double* const NL[5], double* const NR[5], double er[5]) {
   dual_time_impl(g, deltaT, U_, U, Uq, J, m, inFx, inFy, inFz, vF, MR, NL, NR, er, 1);
 }
+
+/* ------------------------------------------------------------------------ */
+/* limiter fluxes: Flux_X_limiter.cpp, Flux_Y_limiter.cpp, Flux_Z_limiter.cpp */
+/* ------------------------------------------------------------------------ */
+
+/* the reference's min/max macros (Flux_X_limiter.cpp:13-14): (a<b)?a:b and
+ * (a>b)?a:b, so a NaN argument in the second place is returned as is */
+static double mac_min(double a, double b) { return a < b ? a : b; }
+static double mac_max(double a, double b) { return a > b ? a : b; }
+
+/* limited 3-point slopes of one cell (Flux_X_limiter.cpp:149-160):
+ * lim3_up = value at the cell's upper face before the face-J scaling,
+ * lim3_dn = value at its lower face.  sgn = +1 (x, y) or -1 (z, B9). */
+static double lim3_up(double um, double u, double up, double sgn) {
+  double rL = (up - u) / (u - um);
+  double bL = 1. / 3 * (1 + 2 * rL);
+  double m = mac_min(2, mac_min(2 * rL, bL));
+  return sgn > 0 ? u + 0.5 * mac_max(0, m) * (u - um) : u - 0.5 * mac_max(0, m) * (u - um);
+}
+static double lim3_dn(double um, double u, double up) {
+  double rR = (u - um) / (up - u);
+  double bR = 1. / 3 * (1 + 2 * rR);
+  double m = mac_min(2, mac_min(2 * rR, bR));
+  return u - 0.5 * mac_max(0, m) * (up - u);
+}
+/* limited 5-point slopes (Flux_X_limiter.cpp:250-265) */
+static double lim5_up(double umm, double um, double u, double up, double upp, double sgn) {
+  double irL = (u - um) / (um - umm);
+  double rL = (up - u) / (u - um);
+  double rLi = (upp - up) / (up - u);
+  double bL = 1. / 30 * (-2. / irL + 11 + 24 * rL - 3 * rL * rLi);
+  double m = mac_min(2, mac_min(2 * rL, bL));
+  return sgn > 0 ? u + 0.5 * mac_max(0, m) * (u - um) : u - 0.5 * mac_max(0, m) * (u - um);
+}
+static double lim5_dn(double umm, double um, double u, double up, double upp) {
+  double rRi = (up - u) / (upp - up);
+  double rR = (u - um) / (up - u);
+  double irR = (um - umm) / (u - um);
+  double bR = 1. / 30 * (-2. / rRi + 11 + 24 * rR - 3 * rR * irR);
+  double m = mac_min(2, mac_min(2 * rR, bR));
+  return u - 0.5 * mac_max(0, m) * (up - u);
+}
+/* y, near-wall density (Flux_Y_limiter.cpp:146-168): 1e-7 added to the
+ * denominators and the cell value scaled by J while the slope is not (B8) */
+static double lim3y_up(double um, double u, double up, double Jc) {
+  double rL = (up - u) / ((u - um) + 0.0000001);
+  double bL = 1. / 3 * (1 + 2 * rL);
+  double m = mac_min(2, mac_min(2 * rL, bL));
+  return u * Jc + 0.5 * mac_max(0, m) * (u - um);
+}
+static double lim3y_dn(double um, double u, double up, double Jc) {
+  double rR = (u - um) / ((up - u) + 0.0000001);
+  double bR = 1. / 3 * (1 + 2 * rR);
+  double m = mac_min(2, mac_min(2 * rR, bR));
+  return u * Jc - 0.5 * mac_max(0, m) * (up - u);
+}
+
+/* preconditioned mean-state dissipation and flux of one face
+ * (Flux_X_limiter.cpp:361-531): n = the face's metric row along the flux
+ * direction, Ep = sensor, Jf = face Jacobian. */
+static void limiter_face_flux(const double L[5], const double Rr[5], const double n[3], double Jf, double Ep,
+                              double F[5], double* Fav3_out, double* Psum_out) {
+  const double K = KG;
+  double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  double Nx = n[0] / nn, Ny = n[1] / nn, Nz = n[2] / nn;
+  double _rho = L[0], _u = L[1] / _rho, _v = L[2] / _rho, _w = L[3] / _rho;
+  double _U = n[0] * _u + n[1] * _v + n[2] * _w;
+  double __U = Nx * _u + Ny * _v + Nz * _w;
+  double _VV = _u * _u + _v * _v + _w * _w;
+  double _P = (L[4] - 0.5 * _rho * _VV) * (K - 1);
+  double _T = _P / _rho;
+  double _C = K * _P / _rho;
+  double _H = 0.5 * _VV + _C / (K - 1);
+  double rho_ = Rr[0], u_ = Rr[1] / rho_, v_ = Rr[2] / rho_, w_ = Rr[3] / rho_;
+  double U_ = n[0] * u_ + n[1] * v_ + n[2] * w_;
+  double U__ = Nx * u_ + Ny * v_ + Nz * w_;
+  double VV_ = u_ * u_ + v_ * v_ + w_ * w_;
+  double P_ = (Rr[4] - 0.5 * rho_ * VV_) * (K - 1);
+  double T_ = P_ / rho_;
+  double C_ = K * P_ / rho_;
+  double H_ = 0.5 * VV_ + C_ / (K - 1);
+  double rho = 0.5 * (_rho + rho_);
+  double u = 0.5 * (_u + u_), v = 0.5 * (_v + v_), w = 0.5 * (_w + w_);
+  double U = 0.5 * (_U + U_);
+  double _U_ = 0.5 * (__U + U__);
+  double VV = u * u + v * v + w * w;
+  double H = 0.5 * (_H + H_);
+  double C = (H - 0.5 * VV) * (K - 1);
+  double P = rho * C / K;
+  double T = P / rho;
+  double dU1 = P_ - _P, dU2 = u_ - _u, dU3 = v_ - _v, dU4 = w_ - _w, dU5 = T_ - _T;
+  double beta = mac_max(VV / C, EPS_PRE);
+  double S = sqrt(U * U * (beta - 1) * (beta - 1) + 4 * beta * C * (n[0] * n[0] + n[1] * n[1] + n[2] * n[2]));
+  double _S_ = sqrt(_U_ * _U_ * (beta - 1) * (beta - 1) + 4 * beta * C * (Nx * Nx + Ny * Ny + Nz * Nz));
+  double temp = 4 * K * _S_ * T * beta;
+  double temp1 = S + U + U * beta;
+  double temp2 = -S + U + U * beta;
+  double temp3 = _S_ - _U_ + _U_ * beta;
+  double temp4 = _S_ + _U_ - _U_ * beta;
+  double temp5 = _S_ * _S_ - (beta - 1) * (beta - 1) * _U_ * _U_;
+  double tu1 = 2 * T * Nx * K * beta + u * _S_ + u * (beta - 1) * _U_;
+  double tu2 = 2 * T * Nx * K * beta - u * _S_ + u * (beta - 1) * _U_;
+  double tv1 = 2 * T * Ny * K * beta + v * _S_ + v * (beta - 1) * _U_;
+  double tv2 = 2 * T * Ny * K * beta - v * _S_ + v * (beta - 1) * _U_;
+  double tw1 = 2 * T * Nz * K * beta + w * _S_ + w * (beta - 1) * _U_;
+  double tw2 = 2 * T * Nz * K * beta - w * _S_ + w * (beta - 1) * _U_;
+  double tuvw = 2 * T * K * beta * (u * Nx + v * Ny + w * Nz);
+  double th1 = tuvw + H * _S_ + H * _U_ * beta - H * _U_;
+  double th2 = tuvw - H * _S_ + H * _U_ * beta - H * _U_;
+  double aU = fabs(U), a1 = fabs(temp1), a2 = fabs(temp2);
+  double d11 = 1 / temp * (4 * (K - 1) * _S_ * beta * aU + temp3 * a1 + temp4 * a2);
+  double d12 = -1 / (2 * temp) * (Nx * rho * (a2 - a1) * temp5);
+  double d13 = -1 / (2 * temp) * (Ny * rho * (a2 - a1) * temp5);
+  double d14 = -1 / (2 * temp) * (Nz * rho * (a2 - a1) * temp5);
+  double d15 = -rho * aU / T;
+  double d21 = 1 / temp * (u * _S_ * (4 * (K - 1) * beta * aU + a2 + a1) - (a2 - a1) * (2 * T * Nx * K * beta + u * _U_ * (beta - 1)));
+  double d22 = 1 / (2 * temp) * (rho * (8 * T * K * _S_ * beta * (Ny * Ny + Nz * Nz) * aU + Nx * (temp3 * tu2 * a2 + temp4 * tu1 * a1)));
+  double d23 = 1 / (2 * temp) * (rho * Ny * (-8 * T * K * _S_ * beta * Nx * aU + temp3 * tu2 * a2 + temp4 * tu1 * a1));
+  double d24 = 1 / (2 * temp) * (rho * Nz * (-8 * T * K * _S_ * beta * Nx * aU + temp3 * tu2 * a2 + temp4 * tu1 * a1));
+  double d25 = -u * rho * aU / T;
+  double d31 = 1 / temp * (v * _S_ * (4 * (K - 1) * beta * aU + a2 + a1) - (a2 - a1) * (2 * T * Ny * K * beta + v * _U_ * (beta - 1)));
+  double d32 = 1 / (2 * temp) * (rho * Nx * (-8 * T * K * _S_ * beta * Ny * aU + temp3 * tv2 * a2 + temp4 * tv1 * a1));
+  double d33 = 1 / (2 * temp) * (rho * (8 * T * K * _S_ * beta * (Nx * Nx + Nz * Nz) * aU + Ny * (temp3 * tv2 * a2 + temp4 * tv1 * a1)));
+  double d34 = 1 / (2 * temp) * (rho * Nz * (-8 * T * K * _S_ * beta * Ny * aU + temp3 * tv2 * a2 + temp4 * tv1 * a1));
+  double d35 = -v * rho * aU / T;
+  double d41 = 1 / temp * (w * _S_ * (4 * (K - 1) * beta * aU + a2 + a1) - (a2 - a1) * (2 * T * Nz * K * beta + w * _U_ * (beta - 1)));
+  double d42 = 1 / (2 * temp) * (rho * Nx * (-8 * T * K * _S_ * beta * Nz * aU + temp3 * tw2 * a2 + temp4 * tw1 * a1));
+  double d43 = 1 / (2 * temp) * (rho * Ny * (-8 * T * K * _S_ * beta * Nz * aU + temp3 * tw2 * a2 + temp4 * tw1 * a1));
+  double d44 = 1 / (2 * temp) * (rho * (8 * T * K * _S_ * beta * (Nx * Nx + Ny * Ny) * aU + Nz * (temp3 * tw2 * a2 + temp4 * tw1 * a1)));
+  double d45 = -w * rho * aU / T;
+  double d51 = 1 / temp * (_S_ * (4 * beta * (H * K - H - T * K) * aU + H * (a2 + a1)) - (a2 - a1) * (tuvw + H * _U_ * beta - H * _U_));
+  double d52 = 1 / (2 * temp) * (rho * (8 * T * K * _S_ * beta * (-v * Nx * Ny - w * Nx * Nz + u * (Ny * Ny + Nz * Nz) * aU) + Nx * (th2 * temp3 * a2 + th1 * temp4 * a1)));
+  double d53 = 1 / (2 * temp) * (rho * (8 * T * K * _S_ * beta * (-u * Nx * Ny - w * Ny * Nz + v * (Nx * Nx + Nz * Nz) * aU) + Ny * (th2 * temp3 * a2 + th1 * temp4 * a1)));
+  double d54 = 1 / (2 * temp) * (rho * (8 * T * K * _S_ * beta * (-u * Nx * Nz - v * Ny * Nz + w * (Nx * Nx + Ny * Ny) * aU) + Nz * (th2 * temp3 * a2 + th1 * temp4 * a1)));
+  double d55 = (-H / T + K / (K - 1)) * rho * aU;
+  double Fav1 = d11 * dU1 + d12 * dU2 + d13 * dU3 + d14 * dU4 + d15 * dU5;
+  double Fav2 = d21 * dU1 + d22 * dU2 + d23 * dU3 + d24 * dU4 + d25 * dU5;
+  double Fav3 = d31 * dU1 + d32 * dU2 + d33 * dU3 + d34 * dU4 + d35 * dU5;
+  double Fav4 = d41 * dU1 + d42 * dU2 + d43 * dU3 + d44 * dU4 + d45 * dU5;
+  double Fav5 = d51 * dU1 + d52 * dU2 + d53 * dU3 + d54 * dU4 + d55 * dU5;
+  F[0] = 0.5 * ((_rho * _U + rho_ * U_) - Ep * Fav1) / Jf;
+  F[1] = 0.5 * ((_rho * _u * _U + rho_ * u_ * U_ + n[0] * (_P + P_)) - Ep * Fav2) / Jf;
+  F[2] = 0.5 * ((_rho * _v * _U + rho_ * v_ * U_ + n[1] * (_P + P_)) - Ep * Fav3) / Jf;
+  F[3] = 0.5 * ((_rho * _w * _U + rho_ * w_ * U_ + n[2] * (_P + P_)) - Ep * Fav4) / Jf;
+  F[4] = 0.5 * ((_U * (3.5 * _P + 0.5 * _rho * _VV) + U_ * (3.5 * P_ + 0.5 * rho_ * VV_)) - Ep * Fav5) / Jf;
+  *Fav3_out = Fav3;
+  *Psum_out = _P + P_;
+}
+
+/* left (ML) and right (MR) states of face f along dir for the transverse
+ * cells at `base` (Flux_*_limiter.cpp MUSCL sections and seam overwrites).
+ * Jfl/Jfr: face Jacobian used to scale the left/right value (x, y). */
+static void limiter_states(const dims_t* d, int dir, double* const U_[5], const double* J, size_t base,
+                           size_t stride, int f, double Jfl, double Jfr, double L[5], double Rr[5]) {
+#define UV(v, c) U_[v][base + (size_t)(c) * stride]
+#define JC(c) J[base + (size_t)(c) * stride]
+  if (dir == 0) {
+    /* cell c = f gives ML, cell c = f+1 gives MR; 3-point near the x ends
+     * (Flux_X_limiter.cpp:133), never-filled ghosts read as zero */
+    int ge = d->gend;
+    for (int v = 0; v < 5; v++) {
+      int c = f;
+      L[v] = (c <= 3 || c >= ge) ? lim3_up(UV(v, c - 1), UV(v, c), UV(v, c + 1), 1) * Jfl
+                                 : lim5_up(UV(v, c - 2), UV(v, c - 1), UV(v, c), UV(v, c + 1), UV(v, c + 2), 1) * Jfl;
+      c = f + 1;
+      Rr[v] = (c <= 3 || c >= ge) ? lim3_dn(UV(v, c - 1), UV(v, c), UV(v, c + 1)) * Jfr
+                                  : lim5_dn(UV(v, c - 2), UV(v, c - 1), UV(v, c), UV(v, c + 1), UV(v, c + 2)) * Jfr;
+    }
+  } else if (dir == 1) {
+    int ny = d->ny;
+    for (int v = 0; v < 5; v++) {
+      /* left state: the wall face f = 1 is the J-weighted average of cells
+       * 1, 2 (Flux_Y_limiter.cpp:368-374); otherwise cell c = f */
+      int c = f;
+      if (f == 1) {
+        double temp = 1. / (1 / JC(1) + 1 / JC(2));
+        L[v] = temp * (UV(v, 1) + UV(v, 2));
+      } else if (c <= 2 || c >= ny - 1) {
+        L[v] = v == 0 ? lim3y_up(UV(v, c - 1), UV(v, c), UV(v, c + 1), JC(c))
+                      : lim3_up(UV(v, c - 1), UV(v, c), UV(v, c + 1), 1) * Jfl;
+      } else {
+        L[v] = lim5_up(UV(v, c - 2), UV(v, c - 1), UV(v, c), UV(v, c + 1), UV(v, c + 2), 1) * Jfl;
+      }
+      /* right state: face f = ny is the average of cells ny, ny+1
+       * (Flux_Y_limiter.cpp:434-440); otherwise cell c = f+1 */
+      c = f + 1;
+      if (f == ny) {
+        double temp = 1. / (1 / JC(ny) + 1 / JC(ny + 1));
+        Rr[v] = temp * (UV(v, ny) + UV(v, ny + 1));
+      } else if (c <= 2 || c >= ny - 1) {
+        Rr[v] = v == 0 ? lim3y_dn(UV(v, c - 1), UV(v, c), UV(v, c + 1), JC(c))
+                       : lim3_dn(UV(v, c - 1), UV(v, c), UV(v, c + 1)) * Jfr;
+      } else {
+        Rr[v] = lim5_dn(UV(v, c - 2), UV(v, c - 1), UV(v, c), UV(v, c + 1), UV(v, c + 2)) * Jfr;
+      }
+    }
+  } else {
+    int nz = d->nz, ny = d->ny;
+    const recipe_t* rl = (f == 1) ? &R_AVG : (f == 2 || f == nz) ? &R_O3L : NULL;
+    const recipe_t* rr = (f == 1 || f == nz - 1) ? &R_O3R : (f == nz) ? &R_AVG : NULL;
+    if (rl) reconstruct(rl, U_, J, base, stride, f, L);
+    if (rr) reconstruct(rr, U_, J, base, stride, f, Rr);
+    /* MUSCL cells k in [3, nz-1]; the 3-point branch is taken for k >= ny
+     * (Flux_Z_limiter.cpp:130 compares k with ny, B9), the left state uses
+     * u - 0.5 phi du and the face weights are 1/sum(w/J) unnormalised */
+    for (int side = 0; side < 2; side++) {
+      if (side == 0 && rl) continue;
+      if (side == 1 && rr) continue;
+      int c = side == 0 ? f : f + 1;
+      double jw;
+      int o3 = (c <= 2 || c >= ny);
+      if (o3) {
+        jw = side == 0 ? 1. / (-1 / JC(c - 1) + 5 / JC(c) + 2 / JC(c + 1))
+                       : 1. / (2 / JC(c - 1) + 5 / JC(c) - 1 / JC(c + 1));
+      } else {
+        jw = side == 0 ? 1. / (2 / JC(c - 2) - 13 / JC(c - 1) + 47 / JC(c) + 27 / JC(c + 1) - 3 / JC(c + 2))
+                       : 1. / (-3 / JC(c - 2) + 27 / JC(c - 1) + 47 / JC(c) - 13 / JC(c + 1) + 2 / JC(c + 2));
+      }
+      for (int v = 0; v < 5; v++) {
+        double s;
+        if (o3) s = side == 0 ? lim3_up(UV(v, c - 1), UV(v, c), UV(v, c + 1), -1) : lim3_dn(UV(v, c - 1), UV(v, c), UV(v, c + 1));
+        else s = side == 0 ? lim5_up(UV(v, c - 2), UV(v, c - 1), UV(v, c), UV(v, c + 1), UV(v, c + 2), -1)
+                           : lim5_dn(UV(v, c - 2), UV(v, c - 1), UV(v, c), UV(v, c + 1), UV(v, c + 2));
+        if (side == 0) L[v] = s * jw; else Rr[v] = s * jw;
+      }
+    }
+  }
+#undef UV
+#undef JC
+}
+
+void cfo_flux_limiter(const cfo_grid* g, int dir, double* const U_[5], const double* J, const double* Jf,
+                      double* const mf[9], const double* Ep, double* const inF[5]) {
+  dims_t d = dims(g);
+  size_t sI = (size_t)d.Ym * d.Zm, sJ = (size_t)d.Zm, sK = 1;
+  /* faces and transverse cells per direction (Flux_*_limiter.cpp flux loops) */
+  int f0, f1, a0, a1, b0, b1;
+  size_t stride, sa, sb;
+  if (dir == 0) { f0 = 2; f1 = d.gend; a0 = 2; a1 = d.ny; b0 = 2; b1 = d.nz; stride = sI; sa = sJ; sb = sK; }
+  else if (dir == 1) { f0 = 1; f1 = d.ny; a0 = 3; a1 = d.gend; b0 = 2; b1 = d.nz; stride = sJ; sa = sI; sb = sK; }
+  else { f0 = 1; f1 = d.nz; a0 = 3; a1 = d.gend; b0 = 2; b1 = d.ny; stride = sK; sa = sI; sb = sJ; }
+  const int mrow = 3 * dir;
+  for (int f = f0; f <= f1; f++)
+    for (int a = a0; a <= a1; a++)
+      for (int b = b0; b <= b1; b++) {
+        size_t base = (size_t)a * sa + (size_t)b * sb;
+        size_t o = (size_t)f * stride + (size_t)(a - 1) * sa + (size_t)(b - 1) * sb;
+        /* face J that scales ML / MR: J_u[c][j-1][k-1] and J_u[c-1][j-1][k-1]
+         * (x, Flux_X_limiter.cpp:125-126); J_v at faces f and f (y, the
+         * interior assignment of Flux_Y_limiter.cpp:231-232, also used near
+         * the walls where the reference leaves it unassigned) */
+        double Jfl = 0, Jfr = 0;
+        if (dir == 0) { Jfl = Jf[o]; Jfr = Jf[o]; }
+        else if (dir == 1) {
+          Jfl = Jf[o];
+          Jfr = Jf[o];
+        }
+        double L[5], Rr[5], F[5], fav3, psum;
+        limiter_states(&d, dir, U_, J, base, stride, f, Jfl, Jfr, L, Rr);
+        double n[3] = {mf[mrow][o], mf[mrow + 1][o], mf[mrow + 2][o]};
+        limiter_face_flux(L, Rr, n, Jf[o], Ep[o], F, &fav3, &psum);
+        if (dir == 1 && (f == 1 || f == d.ny)) {
+          /* wall faces (Flux_Y_limiter.cpp:658-666) */
+          F[0] = 0;
+          F[1] = 0;
+          F[2] = 0.5 * (n[1] * psum - Ep[o] * fav3) / Jf[o];
+          F[3] = 0;
+          F[4] = 0;
+        }
+        for (int v = 0; v < 5; v++) inF[v][o] = F[v];
+      }
+}

@@ -52,6 +52,14 @@ void cfo_halo_bc(const cfo_grid* g, double* const U_[5], const double* J);
 void cfo_flux(const cfo_grid* g, int dir, double* const U_[5], const double* J, const double* Jf,
               double* const mf[9], double* const inF[5]);
 
+/* Flux_X_limiter.cpp:104-547, Flux_Y_limiter.cpp:106-685 and
+ * Flux_Z_limiter.cpp:113-710, dir 0,1,2: limited MUSCL, mean-state
+ * preconditioned dissipation times the sensor Ep (face layout, like inF).
+ * Near the y walls the reference scales ML/MR 2..5 by a face J it never
+ * assigns there; this restatement uses the face J of the interior branch. */
+void cfo_flux_limiter(const cfo_grid* g, int dir, double* const U_[5], const double* J, const double* Jf,
+                      double* const mf[9], const double* Ep, double* const inF[5]);
+
 /* Viscous_terms.cpp:212-883; writes vF[0..3] = vF2..vF5 */
 void cfo_viscous(const cfo_grid* g, double* const U_[5], const double* J, const double* Ju,
                  double* const mu[9], const double* Jv, double* const mv[9], const double* Jw,

@@ -104,6 +104,13 @@ def flux(cfg, direction, U_, J, Jf, mf):
     return out
 
 
+def flux_limiter(cfg, direction, U_, J, Jf, mf, Ep):
+    out = zeros(cfg, 5)
+    lib().cfo_flux_limiter(C.byref(grid_of(cfg)), int(direction), _pp(U_), _p(J), _p(Jf), _pp(mf), _p(Ep),
+                           _pp(out))
+    return out
+
+
 def viscous(cfg, U_, met):
     out = zeros(cfg, 4)
     lib().cfo_viscous(C.byref(grid_of(cfg)), _pp(U_), _p(met["J"]), _p(met["J_u"]), _pp(met["m_u"]),

@@ -62,6 +62,22 @@ def masks(cfg):
     return out
 
 
+VAR_GRIDS = [(12, 10, 16), (8, 12, 9)]
+
+
+def limiter_mask(cfg, direction):
+    """Faces where the reference's limiter flux is defined.  Near the y walls
+    (faces 1, 2, ny-2, ny-1, ny) Flux_Y_limiter.cpp scales ML/MR 2..5 by jL/jR
+    that its near-wall branch never assigns (left over from another cell or
+    uninitialised), so the reference fixes no value there."""
+    m = masks(cfg)["xyz"[direction]].copy()
+    if direction == 1:
+        ny = cfg.ny
+        for f in (1, 2, ny - 2, ny - 1, ny):
+            m[:, f, :] = False
+    return m
+
+
 def rel_l2(a, b):
     a = np.asarray(a, dtype=np.float64).ravel()
     b = np.asarray(b, dtype=np.float64).ravel()

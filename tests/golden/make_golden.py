@@ -12,6 +12,7 @@ computes from them.
 from __future__ import annotations
 
 import json
+import os
 import subprocess
 import sys
 import tempfile
@@ -45,6 +46,21 @@ def write_inputs(d: Path, cfg: ChannelConfig, seed: int, iterate: bool):
     write_initial_bin(d / "initial.bin", Qn, Qm)
     if iterate:
         np.ascontiguousarray(poiseuille_state(cfg, fl, seed=seed, stream=2)).tofile(d / "iterate.bin")
+
+
+def perturbed_state(cfg: ChannelConfig, seed: int, stream: int) -> np.ndarray:
+    """Poiseuille state with every conserved variable perturbed: density and
+    total energy by +-2 %, momenta with the density (velocity noise as in
+    poiseuille_state).  No two neighbouring cells then hold equal values, so
+    the limiters' slope ratios are finite everywhere."""
+    Q = np.array(poiseuille_state(cfg, FlowParams(), seed=seed, stream=stream))
+    rng = np.random.default_rng(1000 * seed + stream)
+    f_rho = 1.0 + 0.02 * rng.uniform(-1.0, 1.0, Q[0].shape)
+    f_e = 1.0 + 0.02 * rng.uniform(-1.0, 1.0, Q[0].shape)
+    for v in range(4):
+        Q[v] *= f_rho
+    Q[4] *= f_e
+    return Q
 
 
 def load(d: Path, name: str, cfg: ChannelConfig) -> np.ndarray:
@@ -82,6 +98,38 @@ def unit_fixture(grid, seed):
     return name, meta
 
 
+VAR_GRIDS = [(12, 10, 16), (8, 12, 9)]
+
+
+def variant_fixture(grid, seed):
+    """Limiter fluxes, Filter sensor and RK3NODT stages on a perturbed state
+    (the unit fixtures' state has a uniform density along x and z, where the
+    limiters' slope ratios are 0/0)."""
+    cfg = ChannelConfig(X_out=grid[0], Y_out=grid[1], Z_out=grid[2])
+    exe = ref_driver(grid)
+    with tempfile.TemporaryDirectory() as td:
+        d = Path(td)
+        Qn, Qm = perturbed_state(cfg, seed, 0), perturbed_state(cfg, seed, 1)
+        write_initial_bin(d / "initial.bin", Qn, Qm)
+        np.ascontiguousarray(perturbed_state(cfg, seed, 2)).tofile(d / "iterate.bin")
+        subprocess.run([str(exe), "unit", str(d), repr(DT_UNIT)], check=True, stdout=subprocess.DEVNULL,
+                       env=dict(os.environ, OMP_NUM_THREADS="1"))
+        names = ["J", "J_u", "J_v", "J_w", "Y_point", "EpX", "EpY", "EpZ", "EpX_s", "EpY_s", "EpZ_s"]
+        for base, n in [("U", 5), ("Uq", 5), ("U_bc", 5), ("inFx", 5), ("inFy", 5), ("inFz", 5), ("vF", 4),
+                        ("inFxL", 5), ("inFyL", 5), ("inFzL", 5), ("inFxS", 5), ("inFyS", 5), ("inFzS", 5),
+                        ("U_rk1_", 5), ("U_rk2_", 5), ("U_rk3_", 5)]:
+            names += [f"{base}{v + 1}" for v in range(n)]
+        arrs = {n: load(d, n, cfg) for n in names}
+    meta = dict(grid=list(grid), seed=seed, deltaT=DT_UNIT, gamma_metric=1.5, f_drive=599.29876710,
+                state="perturbed_state (density, energy +-2 %)", omp_threads=1,
+                reference_calls=["halo", "Y/Z_boundary_condition", "Flux_X/Y/Z", "Viscous_terms",
+                                 "Filter(0.007, 1, 0)", "Flux_X/Y/Z_limiter (Filter sensor, then a smooth "
+                                 "sensor field)", "RK3NODT stages 1-3"])
+    name = f"var_g{grid[0]}x{grid[1]}x{grid[2]}"
+    np.savez_compressed(OUT / f"{name}.npz", **arrs)
+    return name, meta
+
+
 def traj_fixture(spec, keep_state: bool):
     grid = spec["grid"]
     cfg = ChannelConfig(X_out=grid[0], Y_out=grid[1], Z_out=grid[2])
@@ -108,11 +156,22 @@ def traj_fixture(spec, keep_state: bool):
 
 
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "variants":
+        # regenerate only the variant fixtures and merge them into the manifest
+        manifest = json.loads((OUT / "manifest.json").read_text())
+        for i, g in enumerate(VAR_GRIDS):
+            n, m = variant_fixture(g, seed=21 + i)
+            manifest["fixtures"][n] = m
+        (OUT / "manifest.json").write_text(json.dumps(manifest, indent=1))
+        return
     manifest = {"generator": "tests/golden/make_golden.py",
                 "reference": "ricklee209/Channel_Flow built by oracle/ref_build.sh (np=1, MPICH 3.3.2)",
                 "fixtures": {}}
     for i, g in enumerate(UNIT_GRIDS):
         n, m = unit_fixture(g, seed=11 + i)
+        manifest["fixtures"][n] = m
+    for i, g in enumerate(VAR_GRIDS):
+        n, m = variant_fixture(g, seed=21 + i)
         manifest["fixtures"][n] = m
     n, m = traj_fixture(TRAJ, keep_state=True)
     manifest["fixtures"][n] = m
