@@ -40,6 +40,7 @@ struct cf_ctx {
   double* NR = nullptr;   // (5)
   double* Pr = nullptr;   // viscous primitives U,V,W,T (4)
   double* VS = nullptr;   // viscous face fluxes LL/ML/NL (12)
+  double* Ep = nullptr;   // limiter sensors EpX, EpY, EpZ on the face layouts (3)
   double* Sm = nullptr;   // per-cell sweep summary (S_N)
   int* order = nullptr;   // sweep tiles in dependency order
   int* sync = nullptr;    // ticket, error flag, per-tile progress (x2 directions)
@@ -205,7 +206,8 @@ extern "C" int cf_create(const cf_params* p, cf_ctx** out) {
     double** ptr;
     int n;
   } fields[] = {{&c->Ui, 5}, {&c->Un, 5}, {&c->Uq, 5}, {&c->Fx, 5}, {&c->Fy, 5}, {&c->Fz, 5},
-                {&c->vF, 4}, {&c->MR, 5}, {&c->NL, 5}, {&c->NR, 5}, {&c->Pr, 4}, {&c->VS, 12}};
+                {&c->vF, 4}, {&c->MR, 5}, {&c->NL, 5}, {&c->NR, 5}, {&c->Pr, 4}, {&c->VS, 12},
+                {&c->Ep, 3}};
   for (auto& f : fields) {
     if (hipMalloc(f.ptr, nf * f.n * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
     // the reference leaves work arrays uninitialised; here they start at zero
@@ -244,7 +246,7 @@ extern "C" void cf_destroy(cf_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   double* ptrs[] = {c->Ui, c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->MR, c->NL,
-                    c->NR, c->Pr, c->VS, c->Sm, c->partial, c->er_dev, c->prof};
+                    c->NR, c->Pr, c->VS, c->Ep, c->Sm, c->partial, c->er_dev, c->prof};
   for (double* q : ptrs)
     if (q) (void)hipFree(q);
   if (c->order) (void)hipFree(c->order);
@@ -432,6 +434,35 @@ extern "C" int cf_flux_y(cf_ctx* c, const double* const U_[5], const double* J, 
 extern "C" int cf_flux_z(cf_ctx* c, const double* const U_[5], const double* J, const double* J_w,
                          const double* const m_w[9], double* const inFz[5]) {
   return flux_entry(c, 2, U_, J, J_w, m_w, inFz);
+}
+
+static int flux_limiter_entry(cf_ctx* c, int dir, const double* const U_[5], const double* J, const double* Jf,
+                              const double* const mf[9], const double* Ep, double* const inF[5]) {
+  if (!c || !U_ || !inF || !Ep) return CF_ERR_ARG;
+  int r;
+  const int set = dir == 0 ? cf::SET_U : dir == 1 ? cf::SET_V : cf::SET_W;
+  if ((r = ingest(c, J, cf::SET_C, 0, "J")) || (r = ingest_set(c, set, Jf, mf)) || (r = upload_prof(c))) return r;
+  if ((r = h2d_n(c, c->Ui, U_, 5))) return r;
+  double* E = c->Ep + (long)dir * c->g.nf;
+  if ((r = h2d_n(c, E, &Ep, 1))) return r;
+  double* F = dir == 0 ? c->Fx : dir == 1 ? c->Fy : c->Fz;
+  HIPCHK(c, hipMemsetAsync(F, 0, (size_t)c->g.nf * 5 * 8, c->stream));
+  cf::launch_flux_limiter(c->stream, dir, c->Ui, E, c->prof, F, c->g);
+  if ((r = d2h_n(c, inF, F, 5))) return r;
+  return sync(c);
+}
+
+extern "C" int cf_flux_x_limiter(cf_ctx* c, const double* const U_[5], const double* J, const double* J_u,
+                                 const double* const m_u[9], double* const inFx[5], const double* EpX) {
+  return flux_limiter_entry(c, 0, U_, J, J_u, m_u, EpX, inFx);
+}
+extern "C" int cf_flux_y_limiter(cf_ctx* c, const double* const U_[5], const double* J, const double* J_v,
+                                 const double* const m_v[9], double* const inFy[5], const double* EpY) {
+  return flux_limiter_entry(c, 1, U_, J, J_v, m_v, EpY, inFy);
+}
+extern "C" int cf_flux_z_limiter(cf_ctx* c, const double* const U_[5], const double* J, const double* J_w,
+                                 const double* const m_w[9], double* const inFz[5], const double* EpZ) {
+  return flux_limiter_entry(c, 2, U_, J, J_w, m_w, EpZ, inFz);
 }
 
 extern "C" int cf_viscous_terms(cf_ctx* c, const double* const U_[5], double* const vF[4], const double* J,

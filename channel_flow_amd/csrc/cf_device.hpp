@@ -546,4 +546,154 @@ __device__ __forceinline__ void visc_flux_z(const Coef& q, const DSet& dx, const
          Jf;
 }
 
+// ------------------------------------------------------------------------
+// Limiter fluxes (Flux_X_limiter.cpp, Flux_Y_limiter.cpp, Flux_Z_limiter.cpp)
+// ------------------------------------------------------------------------
+// the reference's min/max macros: (a<b)?a:b and (a>b)?a:b (a NaN in the
+// second place is returned as is)
+__device__ __forceinline__ double mac_min(double a, double b) { return a < b ? a : b; }
+__device__ __forceinline__ double mac_max(double a, double b) { return a > b ? a : b; }
+
+// limited 3-point value at a cell's upper face (before the face-J scaling);
+// SGN -1 is the z left state u - 0.5 phi du (Flux_Z_limiter.cpp:163)
+template <int SGN>
+__device__ __forceinline__ double lim3_up(double um, double u, double up) {
+  const double rL = (up - u) / (u - um);
+  const double bL = 1. / 3 * (1 + 2 * rL);
+  const double m = mac_min(2, mac_min(2 * rL, bL));
+  return SGN > 0 ? u + 0.5 * mac_max(0, m) * (u - um) : u - 0.5 * mac_max(0, m) * (u - um);
+}
+__device__ __forceinline__ double lim3_dn(double um, double u, double up) {
+  const double rR = (u - um) / (up - u);
+  const double bR = 1. / 3 * (1 + 2 * rR);
+  const double m = mac_min(2, mac_min(2 * rR, bR));
+  return u - 0.5 * mac_max(0, m) * (up - u);
+}
+template <int SGN>
+__device__ __forceinline__ double lim5_up(double umm, double um, double u, double up, double upp) {
+  const double irL = (u - um) / (um - umm);
+  const double rL = (up - u) / (u - um);
+  const double rLi = (upp - up) / (up - u);
+  const double bL = 1. / 30 * (-2. / irL + 11 + 24 * rL - 3 * rL * rLi);
+  const double m = mac_min(2, mac_min(2 * rL, bL));
+  return SGN > 0 ? u + 0.5 * mac_max(0, m) * (u - um) : u - 0.5 * mac_max(0, m) * (u - um);
+}
+__device__ __forceinline__ double lim5_dn(double umm, double um, double u, double up, double upp) {
+  const double rRi = (up - u) / (upp - up);
+  const double rR = (u - um) / (up - u);
+  const double irR = (um - umm) / (u - um);
+  const double bR = 1. / 30 * (-2. / rRi + 11 + 24 * rR - 3 * rR * irR);
+  const double m = mac_min(2, mac_min(2 * rR, bR));
+  return u - 0.5 * mac_max(0, m) * (up - u);
+}
+// near-wall y density (Flux_Y_limiter.cpp:146-168): 1e-7 in the denominators,
+// cell value scaled by J, slope not
+__device__ __forceinline__ double lim3y_up(double um, double u, double up, double Jc) {
+  const double rL = (up - u) / ((u - um) + 0.0000001);
+  const double bL = 1. / 3 * (1 + 2 * rL);
+  const double m = mac_min(2, mac_min(2 * rL, bL));
+  return u * Jc + 0.5 * mac_max(0, m) * (u - um);
+}
+__device__ __forceinline__ double lim3y_dn(double um, double u, double up, double Jc) {
+  const double rR = (u - um) / ((up - u) + 0.0000001);
+  const double bR = 1. / 3 * (1 + 2 * rR);
+  const double m = mac_min(2, mac_min(2 * rR, bR));
+  return u * Jc - 0.5 * mac_max(0, m) * (up - u);
+}
+
+// mean-state preconditioned dissipation D (5x5 on dP, du, dv, dw, dT) and
+// the face flux (Flux_X_limiter.cpp:361-531).  n = the face's metric row
+// along the flux direction; N = n/|n|.
+__device__ __forceinline__ void limiter_face_flux(const double L[5], const double R[5], const double n[3],
+                                                  double Jf, double Ep, double F[5], double* Fav3_out,
+                                                  double* Psum_out) {
+  constexpr double K = KG;
+  const double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  const double Nx = n[0] / nn, Ny = n[1] / nn, Nz = n[2] / nn;
+  const double _rho = L[0], _u = L[1] / _rho, _v = L[2] / _rho, _w = L[3] / _rho;
+  const double _U = n[0] * _u + n[1] * _v + n[2] * _w;
+  const double __U = Nx * _u + Ny * _v + Nz * _w;
+  const double _VV = _u * _u + _v * _v + _w * _w;
+  const double _P = (L[4] - 0.5 * _rho * _VV) * (K - 1);
+  const double _T = _P / _rho;
+  const double _C = K * _P / _rho;
+  const double _H = 0.5 * _VV + _C / (K - 1);
+  const double rho_ = R[0], u_ = R[1] / rho_, v_ = R[2] / rho_, w_ = R[3] / rho_;
+  const double U_ = n[0] * u_ + n[1] * v_ + n[2] * w_;
+  const double U__ = Nx * u_ + Ny * v_ + Nz * w_;
+  const double VV_ = u_ * u_ + v_ * v_ + w_ * w_;
+  const double P_ = (R[4] - 0.5 * rho_ * VV_) * (K - 1);
+  const double T_ = P_ / rho_;
+  const double C_ = K * P_ / rho_;
+  const double H_ = 0.5 * VV_ + C_ / (K - 1);
+  const double rho = 0.5 * (_rho + rho_);
+  const double u = 0.5 * (_u + u_), v = 0.5 * (_v + v_), w = 0.5 * (_w + w_);
+  const double U = 0.5 * (_U + U_);
+  const double _U_ = 0.5 * (__U + U__);
+  const double VV = u * u + v * v + w * w;
+  const double H = 0.5 * (_H + H_);
+  const double C = (H - 0.5 * VV) * (K - 1);
+  const double P = rho * C / K;
+  const double T = P / rho;
+  const double dU1 = P_ - _P, dU2 = u_ - _u, dU3 = v_ - _v, dU4 = w_ - _w, dU5 = T_ - _T;
+  const double beta = mac_max(VV / C, EPS_PRE);
+  const double S = sqrt(U * U * (beta - 1) * (beta - 1) + 4 * beta * C * (n[0] * n[0] + n[1] * n[1] + n[2] * n[2]));
+  const double _S_ = sqrt(_U_ * _U_ * (beta - 1) * (beta - 1) + 4 * beta * C * (Nx * Nx + Ny * Ny + Nz * Nz));
+  const double temp = 4 * K * _S_ * T * beta;
+  const double temp1 = S + U + U * beta;
+  const double temp2 = -S + U + U * beta;
+  const double temp3 = _S_ - _U_ + _U_ * beta;
+  const double temp4 = _S_ + _U_ - _U_ * beta;
+  const double temp5 = _S_ * _S_ - (beta - 1) * (beta - 1) * _U_ * _U_;
+  const double tu1 = 2 * T * Nx * K * beta + u * _S_ + u * (beta - 1) * _U_;
+  const double tu2 = 2 * T * Nx * K * beta - u * _S_ + u * (beta - 1) * _U_;
+  const double tv1 = 2 * T * Ny * K * beta + v * _S_ + v * (beta - 1) * _U_;
+  const double tv2 = 2 * T * Ny * K * beta - v * _S_ + v * (beta - 1) * _U_;
+  const double tw1 = 2 * T * Nz * K * beta + w * _S_ + w * (beta - 1) * _U_;
+  const double tw2 = 2 * T * Nz * K * beta - w * _S_ + w * (beta - 1) * _U_;
+  const double tuvw = 2 * T * K * beta * (u * Nx + v * Ny + w * Nz);
+  const double th1 = tuvw + H * _S_ + H * _U_ * beta - H * _U_;
+  const double th2 = tuvw - H * _S_ + H * _U_ * beta - H * _U_;
+  const double aU = fabs(U), a1 = fabs(temp1), a2 = fabs(temp2);
+  const double it = 1 / temp, i2t = 1 / (2 * temp), mi2t = -1 / (2 * temp);
+  const double d11 = it * (4 * (K - 1) * _S_ * beta * aU + temp3 * a1 + temp4 * a2);
+  const double d12 = mi2t * (Nx * rho * (a2 - a1) * temp5);
+  const double d13 = mi2t * (Ny * rho * (a2 - a1) * temp5);
+  const double d14 = mi2t * (Nz * rho * (a2 - a1) * temp5);
+  const double d15 = -rho * aU / T;
+  const double d21 = it * (u * _S_ * (4 * (K - 1) * beta * aU + a2 + a1) - (a2 - a1) * (2 * T * Nx * K * beta + u * _U_ * (beta - 1)));
+  const double d22 = i2t * (rho * (8 * T * K * _S_ * beta * (Ny * Ny + Nz * Nz) * aU + Nx * (temp3 * tu2 * a2 + temp4 * tu1 * a1)));
+  const double d23 = i2t * (rho * Ny * (-8 * T * K * _S_ * beta * Nx * aU + temp3 * tu2 * a2 + temp4 * tu1 * a1));
+  const double d24 = i2t * (rho * Nz * (-8 * T * K * _S_ * beta * Nx * aU + temp3 * tu2 * a2 + temp4 * tu1 * a1));
+  const double d25 = -u * rho * aU / T;
+  const double d31 = it * (v * _S_ * (4 * (K - 1) * beta * aU + a2 + a1) - (a2 - a1) * (2 * T * Ny * K * beta + v * _U_ * (beta - 1)));
+  const double d32 = i2t * (rho * Nx * (-8 * T * K * _S_ * beta * Ny * aU + temp3 * tv2 * a2 + temp4 * tv1 * a1));
+  const double d33 = i2t * (rho * (8 * T * K * _S_ * beta * (Nx * Nx + Nz * Nz) * aU + Ny * (temp3 * tv2 * a2 + temp4 * tv1 * a1)));
+  const double d34 = i2t * (rho * Nz * (-8 * T * K * _S_ * beta * Ny * aU + temp3 * tv2 * a2 + temp4 * tv1 * a1));
+  const double d35 = -v * rho * aU / T;
+  const double d41 = it * (w * _S_ * (4 * (K - 1) * beta * aU + a2 + a1) - (a2 - a1) * (2 * T * Nz * K * beta + w * _U_ * (beta - 1)));
+  const double d42 = i2t * (rho * Nx * (-8 * T * K * _S_ * beta * Nz * aU + temp3 * tw2 * a2 + temp4 * tw1 * a1));
+  const double d43 = i2t * (rho * Ny * (-8 * T * K * _S_ * beta * Nz * aU + temp3 * tw2 * a2 + temp4 * tw1 * a1));
+  const double d44 = i2t * (rho * (8 * T * K * _S_ * beta * (Nx * Nx + Ny * Ny) * aU + Nz * (temp3 * tw2 * a2 + temp4 * tw1 * a1)));
+  const double d45 = -w * rho * aU / T;
+  const double d51 = it * (_S_ * (4 * beta * (H * K - H - T * K) * aU + H * (a2 + a1)) - (a2 - a1) * (tuvw + H * _U_ * beta - H * _U_));
+  const double d52 = i2t * (rho * (8 * T * K * _S_ * beta * (-v * Nx * Ny - w * Nx * Nz + u * (Ny * Ny + Nz * Nz) * aU) + Nx * (th2 * temp3 * a2 + th1 * temp4 * a1)));
+  const double d53 = i2t * (rho * (8 * T * K * _S_ * beta * (-u * Nx * Ny - w * Ny * Nz + v * (Nx * Nx + Nz * Nz) * aU) + Ny * (th2 * temp3 * a2 + th1 * temp4 * a1)));
+  const double d54 = i2t * (rho * (8 * T * K * _S_ * beta * (-u * Nx * Nz - v * Ny * Nz + w * (Nx * Nx + Ny * Ny) * aU) + Nz * (th2 * temp3 * a2 + th1 * temp4 * a1)));
+  const double d55 = (-H / T + K / (K - 1)) * rho * aU;
+  const double Fav1 = d11 * dU1 + d12 * dU2 + d13 * dU3 + d14 * dU4 + d15 * dU5;
+  const double Fav2 = d21 * dU1 + d22 * dU2 + d23 * dU3 + d24 * dU4 + d25 * dU5;
+  const double Fav3 = d31 * dU1 + d32 * dU2 + d33 * dU3 + d34 * dU4 + d35 * dU5;
+  const double Fav4 = d41 * dU1 + d42 * dU2 + d43 * dU3 + d44 * dU4 + d45 * dU5;
+  const double Fav5 = d51 * dU1 + d52 * dU2 + d53 * dU3 + d54 * dU4 + d55 * dU5;
+  F[0] = 0.5 * ((_rho * _U + rho_ * U_) - Ep * Fav1) / Jf;
+  F[1] = 0.5 * ((_rho * _u * _U + rho_ * u_ * U_ + n[0] * (_P + P_)) - Ep * Fav2) / Jf;
+  F[2] = 0.5 * ((_rho * _v * _U + rho_ * v_ * U_ + n[1] * (_P + P_)) - Ep * Fav3) / Jf;
+  F[3] = 0.5 * ((_rho * _w * _U + rho_ * w_ * U_ + n[2] * (_P + P_)) - Ep * Fav4) / Jf;
+  F[4] = 0.5 * ((_U * (3.5 * _P + 0.5 * _rho * _VV) + U_ * (3.5 * P_ + 0.5 * rho_ * VV_)) - Ep * Fav5) / Jf;
+  *Fav3_out = Fav3;
+  *Psum_out = _P + P_;
+}
+
+
 }  // namespace cf

@@ -114,6 +114,118 @@ __global__ void __launch_bounds__(256) k_flux(const double* __restrict__ U, cons
 }
 
 // ---------------------------------------------------------------------------
+// limiter fluxes, one thread per face (Flux_X_limiter.cpp:104-547,
+// Flux_Y_limiter.cpp:106-685, Flux_Z_limiter.cpp:113-710)
+// ---------------------------------------------------------------------------
+template <int DIR>
+__global__ void __launch_bounds__(256) k_flux_lim(const double* __restrict__ U, const double* __restrict__ Ep,
+                                                  const double* __restrict__ prof, double* __restrict__ F, Geo g) {
+  // faces and transverse cells as k_flux
+  int x = blockIdx.x * blockDim.x + threadIdx.x;
+  int y = blockIdx.y * blockDim.y + threadIdx.y;
+  int z = blockIdx.z;
+  int f, i, j, k;
+  if (DIR == 0) { k = 2 + x; j = 2 + y; f = 2 + z; i = f; if (k > g.nz || j > g.ny) return; }
+  else if (DIR == 1) { k = 2 + x; f = 1 + y; i = 3 + z; j = f; if (k > g.nz || f > g.ny) return; }
+  else { f = 1 + x; j = 2 + y; i = 3 + z; k = f; if (f > g.nz || j > g.ny) return; }
+  const long stride = DIR == 0 ? g.plane : (DIR == 1 ? (long)g.Zp : 1L);
+  const long base = at(g, i, j, k);  // cell f along the direction
+  // cells f-2 .. f+3 (t = 0..5); ghosts the reference never fills are zero
+  double u[5][6], jj[6];
+#pragma unroll
+  for (int t = 0; t < 6; t++) {
+    const int c = f + t - 2;
+    const bool in = DIR == 0 ? (c >= 0 && c < g.Xnp) : (DIR == 1 ? (c >= 0 && c < g.Ym) : (c >= 0 && c < g.Zm));
+    const long p = base + (long)(t - 2) * stride;
+    jj[t] = pf(prof, g.Ym, SET_C, 0, DIR == 1 ? min(max(c, 0), g.Ym - 1) : j);
+#pragma unroll
+    for (int v = 0; v < 5; v++) u[v][t] = in ? U[v * g.nf + p] : 0.0;
+  }
+  long o;
+  int set, jp;
+  if (DIR == 0) { o = at(g, f, j - 1, k - 1); set = SET_U; jp = j - 1; }
+  else if (DIR == 1) { o = at(g, i - 1, f, k - 1); set = SET_V; jp = f; }
+  else { o = at(g, i - 1, j - 1, f); set = SET_W; jp = j - 1; }
+  const double Jf = pf(prof, g.Ym, set, 0, jp);
+  double L[5], R[5];
+  if (DIR == 0) {
+    // 3-point within two cells of the x ends (Flux_X_limiter.cpp:133)
+    const bool l3 = f <= 3 || f >= g.gend, r3 = f + 1 <= 3 || f + 1 >= g.gend;
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+      L[v] = (l3 ? lim3_up<1>(u[v][1], u[v][2], u[v][3]) : lim5_up<1>(u[v][0], u[v][1], u[v][2], u[v][3], u[v][4])) * Jf;
+      R[v] = (r3 ? lim3_dn(u[v][2], u[v][3], u[v][4]) : lim5_dn(u[v][1], u[v][2], u[v][3], u[v][4], u[v][5])) * Jf;
+    }
+  } else if (DIR == 1) {
+    // near-wall cells j <= 2, j >= ny-1 (Flux_Y_limiter.cpp:126); the face J
+    // of the interior branch also scales ML/MR 2..5 there (the reference
+    // leaves it unassigned); wall faces take J-weighted averages (:368-440)
+    const bool lw = f <= 2 || f >= g.ny - 1, rw = f + 1 <= 2 || f + 1 >= g.ny - 1;
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+      if (f == 1) {
+        const double temp = 1. / (1 / jj[2] + 1 / jj[3]);
+        L[v] = temp * (u[v][2] + u[v][3]);
+      } else if (lw) {
+        L[v] = v == 0 ? lim3y_up(u[v][1], u[v][2], u[v][3], jj[2]) : lim3_up<1>(u[v][1], u[v][2], u[v][3]) * Jf;
+      } else {
+        L[v] = lim5_up<1>(u[v][0], u[v][1], u[v][2], u[v][3], u[v][4]) * Jf;
+      }
+      if (f == g.ny) {
+        const double temp = 1. / (1 / jj[2] + 1 / jj[3]);
+        R[v] = temp * (u[v][2] + u[v][3]);
+      } else if (rw) {
+        R[v] = v == 0 ? lim3y_dn(u[v][2], u[v][3], u[v][4], jj[3]) : lim3_dn(u[v][2], u[v][3], u[v][4]) * Jf;
+      } else {
+        R[v] = lim5_dn(u[v][1], u[v][2], u[v][3], u[v][4], u[v][5]) * Jf;
+      }
+    }
+  } else {
+    // seam recipes (as Flux_Z.cpp) at k = 1, 2, nz; MUSCL cells k in
+    // [3, nz-1] with the 3-point branch for k >= ny (Flux_Z_limiter.cpp:130,
+    // B9), left state u - 0.5 phi du, face weights 1/sum(w/J) unnormalised
+    const int nz = g.nz;
+    if (f == 1) reconstruct(AVG, u, jj, L);
+    else if (f == 2 || f == nz) reconstruct(O3L, u, jj, L);
+    else {
+      const int c = f;
+      const bool o3 = c <= 2 || c >= g.ny;
+      const double jw = o3 ? 1. / (-1 / jj[1] + 5 / jj[2] + 2 / jj[3])
+                           : 1. / (2 / jj[0] - 13 / jj[1] + 47 / jj[2] + 27 / jj[3] - 3 / jj[4]);
+#pragma unroll
+      for (int v = 0; v < 5; v++)
+        L[v] = (o3 ? lim3_up<-1>(u[v][1], u[v][2], u[v][3]) : lim5_up<-1>(u[v][0], u[v][1], u[v][2], u[v][3], u[v][4])) * jw;
+    }
+    if (f == 1 || f == nz - 1) reconstruct(O3R, u, jj, R);
+    else if (f == nz) reconstruct(AVG, u, jj, R);
+    else {
+      const int c = f + 1;
+      const bool o3 = c <= 2 || c >= g.ny;
+      const double jw = o3 ? 1. / (2 / jj[2] + 5 / jj[3] - 1 / jj[4])
+                           : 1. / (-3 / jj[1] + 27 / jj[2] + 47 / jj[3] - 13 / jj[4] + 2 / jj[5]);
+#pragma unroll
+      for (int v = 0; v < 5; v++)
+        R[v] = (o3 ? lim3_dn(u[v][2], u[v][3], u[v][4]) : lim5_dn(u[v][1], u[v][2], u[v][3], u[v][4], u[v][5])) * jw;
+    }
+  }
+  const double n[3] = {pf(prof, g.Ym, set, 1 + 3 * DIR, jp), pf(prof, g.Ym, set, 2 + 3 * DIR, jp),
+                       pf(prof, g.Ym, set, 3 + 3 * DIR, jp)};
+  const double e = Ep[o];
+  double out[5], fav3, psum;
+  limiter_face_flux(L, R, n, Jf, e, out, &fav3, &psum);
+  if (DIR == 1 && (f == 1 || f == g.ny)) {
+    // wall faces (Flux_Y_limiter.cpp:658-666)
+    out[0] = 0;
+    out[1] = 0;
+    out[2] = 0.5 * (n[1] * psum - e * fav3) / Jf;
+    out[3] = 0;
+    out[4] = 0;
+  }
+#pragma unroll
+  for (int v = 0; v < 5; v++) F[v * g.nf + o] = out[v];
+}
+
+// ---------------------------------------------------------------------------
 // viscous terms: primitives on the stencil box, face fluxes, divergence
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_visc_prim(const double* __restrict__ U, const double* __restrict__ prof,
@@ -1317,6 +1429,17 @@ void launch_flux(hipStream_t st, int dir, const double* U, const double* prof, d
     hipLaunchKernelGGL(k_flux<1>, box_grid(g.nz - 1, g.ny, g.gend - 2, blk), blk, 0, st, U, prof, F, g);
   else
     hipLaunchKernelGGL(k_flux<2>, box_grid(g.nz, g.ny - 1, g.gend - 2, blk), blk, 0, st, U, prof, F, g);
+}
+
+void launch_flux_limiter(hipStream_t st, int dir, const double* U, const double* Ep, const double* prof, double* F,
+                         const Geo& g) {
+  dim3 blk(64, 4);
+  if (dir == 0)
+    hipLaunchKernelGGL(k_flux_lim<0>, box_grid(g.nz - 1, g.ny - 1, g.gend - 1, blk), blk, 0, st, U, Ep, prof, F, g);
+  else if (dir == 1)
+    hipLaunchKernelGGL(k_flux_lim<1>, box_grid(g.nz - 1, g.ny, g.gend - 2, blk), blk, 0, st, U, Ep, prof, F, g);
+  else
+    hipLaunchKernelGGL(k_flux_lim<2>, box_grid(g.nz, g.ny - 1, g.gend - 2, blk), blk, 0, st, U, Ep, prof, F, g);
 }
 
 void launch_viscous(hipStream_t st, const double* U, const double* prof, double* Pr, double* VS, double* vF,

@@ -6,6 +6,9 @@ namespace cf {
 
 void launch_halo_bc(hipStream_t st, double* U, const double* prof, const Geo& g, int x_periodic);
 void launch_flux(hipStream_t st, int dir, const double* U, const double* prof, double* F, const Geo& g);
+// limiter variants; Ep is the sensor on the face layout of F
+void launch_flux_limiter(hipStream_t st, int dir, const double* U, const double* Ep, const double* prof, double* F,
+                         const Geo& g);
 void launch_viscous(hipStream_t st, const double* U, const double* prof, double* Pr, double* VS, double* vF,
                     const Geo& g);
 void launch_rhs(hipStream_t st, const double* Ui, const double* Un, const double* Uq, const double* Fx,

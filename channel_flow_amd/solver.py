@@ -28,7 +28,8 @@ EXPORTS = [
     "cf_hyperplane", "cf_boundary_conditions", "cf_flux_x", "cf_flux_y", "cf_flux_z", "cf_viscous_terms",
     "cf_dual_time_stepping", "cf_load_state", "cf_load_initial", "cf_set_iterate", "cf_read_state",
     "cf_inner_iteration", "cf_physical_step", "cf_advance_levels", "cf_write_output", "cf_synchronize",
-    "cf_profile_inner_iteration", "cf_set_sweep_schedule",
+    "cf_profile_inner_iteration", "cf_set_sweep_schedule", "cf_flux_x_limiter", "cf_flux_y_limiter",
+    "cf_flux_z_limiter",
 ]
 
 
@@ -73,6 +74,8 @@ def load_library() -> C.CDLL:
     lib.cf_boundary_conditions.argtypes = [vp, PP, P]
     for n in ("cf_flux_x", "cf_flux_y", "cf_flux_z"):
         getattr(lib, n).argtypes = [vp, PP, P, P, PP, PP]
+    for n in ("cf_flux_x_limiter", "cf_flux_y_limiter", "cf_flux_z_limiter"):
+        getattr(lib, n).argtypes = [vp, PP, P, P, PP, PP, P]
     lib.cf_viscous_terms.argtypes = [vp, PP, PP, P, PP, P, PP, P, PP, P, PP]
     lib.cf_dual_time_stepping.argtypes = [vp, C.c_double, P, PP, PP, PP, P, PP, PP, PP, PP, PP, PP, PP, PP]
     lib.cf_load_state.argtypes = [vp, PP, PP]
@@ -188,6 +191,24 @@ class Solver:
     def Flux_Z(self, U_, J, J_w, m_w):
         out = self.zeros(5)
         self._chk(self.lib.cf_flux_z(self._h, _pp(U_), _p(J), _p(J_w), _pp(m_w), _pp(out)))
+        return out
+
+    def Flux_X_limiter(self, U_, J, J_u, m_u, EpX):
+        """Flux_X_limiter.cpp:104-547 (sensor EpX on the face layout)."""
+        out = self.zeros(5)
+        self._chk(self.lib.cf_flux_x_limiter(self._h, _pp(U_), _p(J), _p(J_u), _pp(m_u), _pp(out), _p(EpX)))
+        return out
+
+    def Flux_Y_limiter(self, U_, J, J_v, m_v, EpY):
+        """Flux_Y_limiter.cpp:106-685."""
+        out = self.zeros(5)
+        self._chk(self.lib.cf_flux_y_limiter(self._h, _pp(U_), _p(J), _p(J_v), _pp(m_v), _pp(out), _p(EpY)))
+        return out
+
+    def Flux_Z_limiter(self, U_, J, J_w, m_w, EpZ):
+        """Flux_Z_limiter.cpp:113-710."""
+        out = self.zeros(5)
+        self._chk(self.lib.cf_flux_z_limiter(self._h, _pp(U_), _p(J), _p(J_w), _pp(m_w), _pp(out), _p(EpZ)))
         return out
 
     def Viscous_terms(self, U_, met):

@@ -101,6 +101,18 @@ int cf_flux_y(cf_ctx* ctx, const double* const U_[5], const double* J, const dou
               const double* const m_v[9], double* const inFy[5]);
 int cf_flux_z(cf_ctx* ctx, const double* const U_[5], const double* J, const double* J_w,
               const double* const m_w[9], double* const inFz[5]);
+/* Flux_X/Y/Z_limiter (main.h:376-522; Flux_X_limiter.cpp:104-547,
+ * Flux_Y_limiter.cpp:106-685, Flux_Z_limiter.cpp:113-710): limited MUSCL + mean-state preconditioned
+ * dissipation scaled by the sensor EpX/EpY/EpZ (face layout of inF, as the
+ * reference's).  Near the y walls (faces 1, 2, ny-2, ny-1, ny) the reference
+ * multiplies ML/MR 2..5 by a face Jacobian its near-wall branch never
+ * assigns; these entry points use the face Jacobian of its interior branch. */
+int cf_flux_x_limiter(cf_ctx* ctx, const double* const U_[5], const double* J, const double* J_u,
+                      const double* const m_u[9], double* const inFx[5], const double* EpX);
+int cf_flux_y_limiter(cf_ctx* ctx, const double* const U_[5], const double* J, const double* J_v,
+                      const double* const m_v[9], double* const inFy[5], const double* EpY);
+int cf_flux_z_limiter(cf_ctx* ctx, const double* const U_[5], const double* J, const double* J_w,
+                      const double* const m_w[9], double* const inFz[5], const double* EpZ);
 /* Viscous_terms (main.h:525-651): writes vF[0..3] = vF2..vF5. */
 int cf_viscous_terms(cf_ctx* ctx, const double* const U_[5], double* const vF[4], const double* J,
                      const double* const m[9], const double* J_u, const double* const m_u[9],

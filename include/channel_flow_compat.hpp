@@ -12,7 +12,8 @@
 //   cf_compat::attach(ctx, params);        // once, before the first call
 //   Jacobian(myid, J, xidx, ...);          // as in main.cpp
 //
-// Differences from the reference, all in outputs nothing downstream reads:
+// Differences from the reference, all in outputs nothing downstream reads
+// (and one that the reference leaves undefined):
 //   - the MUSCL and viscous staging arrays (ML/MR, LR/LL/MR/ML/NR/NL of
 //     Viscous_terms), EpX/EpY/EpZ of the non-limiter fluxes and Residual1..5
 //     of Dual_time_stepping are accepted and left untouched (the device keeps
@@ -21,7 +22,10 @@
 //     and the periodic z ghosts together (the three are idempotent), and so
 //     does Z_boundary_condition;
 //   - a failing call prints cf_last_error() and aborts: the reference
-//     functions return void.
+//     functions return void;
+//   - Flux_Y_limiter near the walls uses the face Jacobian that the
+//     reference's interior branch assigns, where the reference reads one it
+//     never assigned.
 #ifndef CHANNEL_FLOW_COMPAT_HPP
 #define CHANNEL_FLOW_COMPAT_HPP
 
@@ -186,6 +190,27 @@ CF_FLUX(Flux_X, cf_flux_x, _u, inFx, EpX)
 CF_FLUX(Flux_Y, cf_flux_y, _v, inFy, EpY)
 CF_FLUX(Flux_Z, cf_flux_z, _w, inFz, EpZ)
 #undef CF_FLUX
+
+// the limiter variants take the same parameters and read the sensor
+#define CF_FLUX_LIM(NAME, FN, S, F, EP)                                                                          \
+  inline void NAME(int, CF_F(U1_), CF_F(U2_), CF_F(U3_), CF_F(U4_), CF_F(U5_), CF_F(MR1), CF_F(MR2), CF_F(MR3), \
+                   CF_F(MR4), CF_F(MR5), CF_F(ML1), CF_F(ML2), CF_F(ML3), CF_F(ML4), CF_F(ML5), CF_F(J),        \
+                   CF_F(J##S), CF_F(xidx##S), CF_F(xidy##S), CF_F(xidz##S), CF_F(etdx##S), CF_F(etdy##S),       \
+                   CF_F(etdz##S), CF_F(ztdx##S), CF_F(ztdy##S), CF_F(ztdz##S), CF_F(F##1), CF_F(F##2),          \
+                   CF_F(F##3), CF_F(F##4), CF_F(F##5), CF_F(EP)) {                                             \
+    (void)MR1, (void)MR2, (void)MR3, (void)MR4, (void)MR5, (void)ML1, (void)ML2, (void)ML3, (void)ML4, (void)ML5; \
+    const double* const U[5] = {CF_P(U1_), CF_P(U2_), CF_P(U3_), CF_P(U4_), CF_P(U5_)};                        \
+    const double* const m[9] = {CF_P(xidx##S), CF_P(xidy##S), CF_P(xidz##S), CF_P(etdx##S), CF_P(etdy##S),     \
+                                CF_P(etdz##S), CF_P(ztdx##S), CF_P(ztdy##S), CF_P(ztdz##S)};                   \
+    double* const out[5] = {CF_P(F##1), CF_P(F##2), CF_P(F##3), CF_P(F##4), CF_P(F##5)};                       \
+    cf_compat::check(FN(cf_compat::ctx(#NAME), U, CF_P(J), CF_P(J##S), m, out, CF_P(EP)), #NAME);              \
+  }
+
+// Flux_X_limiter.cpp:104-547, Flux_Y_limiter.cpp:106-685, Flux_Z_limiter.cpp:113-710
+CF_FLUX_LIM(Flux_X_limiter, cf_flux_x_limiter, _u, inFx, EpX)
+CF_FLUX_LIM(Flux_Y_limiter, cf_flux_y_limiter, _v, inFy, EpY)
+CF_FLUX_LIM(Flux_Z_limiter, cf_flux_z_limiter, _w, inFz, EpZ)
+#undef CF_FLUX_LIM
 
 #define CF_METRIC_ARGS(S)                                                                                   \
   CF_F(J##S), CF_F(xidx##S), CF_F(xidy##S), CF_F(xidz##S), CF_F(etdx##S), CF_F(etdy##S), CF_F(etdz##S), \

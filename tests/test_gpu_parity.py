@@ -11,7 +11,8 @@ bar: relative L2 <= 1e-12 per kernel, state after 10 steps <= 1e-10).
 import numpy as np
 import pytest
 
-from conftest import UNIT_GRIDS, fields, fixture_metrics, gpu_available, masks, rel_l2, unit_cfg
+from conftest import (UNIT_GRIDS, VAR_GRIDS, fields, fixture_metrics, gpu_available, limiter_mask, masks, rel_l2,
+                      unit_cfg)
 
 pytestmark = pytest.mark.gpu
 
@@ -71,6 +72,52 @@ def test_gpu_fluxes(grid, golden):
         for v, ref in enumerate(fields(z, "inF" + "xyz"[d])):
             assert rel_l2(outs[d][v][m], ref[m]) <= TOL_KERNEL
             np.testing.assert_array_equal(outs[d][v][m], ref[m], err_msg=f"inF{'xyz'[d]}{v + 1}")
+
+
+def _limiter_outputs(cfg, z, met, sensor):
+    U_ = fields(z, "U_bc")
+    sfx = "_s" if sensor == "S" else ""
+    Ep = [np.ascontiguousarray(z[f"Ep{a}{sfx}"]) for a in "XYZ"]
+    with solver(cfg) as s:
+        return [s.Flux_X_limiter(U_, met["J"], met["J_u"], met["m_u"], Ep[0]),
+                s.Flux_Y_limiter(U_, met["J"], met["J_v"], met["m_v"], Ep[1]),
+                s.Flux_Z_limiter(U_, met["J"], met["J_w"], met["m_w"], Ep[2])], U_, Ep
+
+
+@pytest.mark.parametrize("grid", VAR_GRIDS)
+@pytest.mark.parametrize("sensor", ["L", "S"])
+def test_gpu_limiter_fluxes(grid, sensor, golden):
+    """Flux_X/Y/Z_limiter on the perturbed state: bit for bit against the
+    reference on every face it defines, and against the oracle on every face
+    (the near-wall y faces follow the documented face-J definition)."""
+    from oracle import oracle as O
+    z = golden(f"var_g{grid[0]}x{grid[1]}x{grid[2]}")
+    met = fixture_metrics(golden(f"unit_g{grid[0]}x{grid[1]}x{grid[2]}"))
+    cfg = unit_cfg(grid)
+    outs, U_, Ep = _limiter_outputs(cfg, z, met, sensor)
+    Jf = [met["J_u"], met["J_v"], met["J_w"]]
+    mf = [met["m_u"], met["m_v"], met["m_w"]]
+    for d in range(3):
+        ref_o = O.flux_limiter(cfg, d, [u.copy() for u in U_], met["J"], Jf[d], mf[d], Ep[d])
+        m = limiter_mask(cfg, d)
+        mall = masks(cfg)["xyz"[d]]
+        for v, ref in enumerate(fields(z, "inF" + "xyz"[d] + sensor)):
+            np.testing.assert_array_equal(outs[d][v][m], ref[m], err_msg=f"inF{'xyz'[d]}{sensor}{v + 1}")
+            np.testing.assert_array_equal(outs[d][v][mall], ref_o[v][mall], err_msg=f"oracle {'xyz'[d]}{v + 1}")
+
+
+@pytest.mark.parametrize("grid", UNIT_GRIDS)
+def test_gpu_limiter_nan_on_flat_density(grid, golden):
+    """Constant density along x and z: 0/0 slope ratios, NaN fluxes on the
+    same faces as the reference (x and z)."""
+    z = golden(f"unit_g{grid[0]}x{grid[1]}x{grid[2]}")
+    cfg = unit_cfg(grid)
+    met = fixture_metrics(z)
+    outs, _, _ = _limiter_outputs(cfg, z, met, "S")
+    for d in (0, 2):
+        m = limiter_mask(cfg, d)
+        for v, ref in enumerate(fields(z, "inF" + "xyz"[d] + "S")):
+            np.testing.assert_array_equal(outs[d][v][m], ref[m], err_msg=f"inF{'xyz'[d]}S{v + 1}")
 
 
 @pytest.mark.parametrize("grid", UNIT_GRIDS)
