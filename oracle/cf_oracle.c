@@ -1254,3 +1254,202 @@ void cfo_flux_limiter(const cfo_grid* g, int dir, double* const U_[5], const dou
         for (int v = 0; v < 5; v++) inF[v][o] = F[v];
       }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Grid (Grid.cpp:50-141) and the ADM sensor Filter (Filter.cpp:12-1720)    */
+/* ------------------------------------------------------------------------ */
+
+void cfo_grid_points(const cfo_grid* g, double gamma, double* X_point, double* Y_point) {
+  dims_t d = dims(g);
+  for (int i = 0; i <= d.gend + 3; i++)
+    for (int j = 2; j <= d.ny; j++)
+      for (int k = 2; k <= d.nz; k++)
+        Y_point[IX(d, i, j, k)] = (0.5 * g->high) * (1 - 1. / tanh(gamma) * tanh(gamma * (1 - 2 * (j - 1.5) * g->deltaET)));
+  for (int j = 2; j <= d.ny; j++)
+    for (int k = 2; k <= d.nz; k++) {
+      Y_point[IX(d, 2, j, k)] = Y_point[IX(d, 3, j, k)];
+      Y_point[IX(d, d.gend + 1, j, k)] = Y_point[IX(d, d.gend, j, k)];
+    }
+  for (int i = 2; i <= d.gend + 1; i++)
+    for (int k = 2; k <= d.nz; k++) {
+      Y_point[IX(d, i, 1, k)] = -Y_point[IX(d, i, 2, k)];
+      Y_point[IX(d, i, d.ny + 1, k)] = Y_point[IX(d, i, d.ny, k)] + 2 * Y_point[IX(d, i, 2, k)];
+    }
+  for (int i = 2; i <= d.gend + 1; i++)
+    for (int j = 1; j <= d.ny + 1; j++) {
+      Y_point[IX(d, i, j, 1)] = Y_point[IX(d, i, j, 2)];
+      Y_point[IX(d, i, j, d.nz + 1)] = Y_point[IX(d, i, j, d.nz)];
+    }
+  if (X_point)
+    for (int i = 3; i <= d.gend; i++)
+      for (int j = 2; j <= d.ny; j++)
+        for (int k = 2; k <= d.nz; k++) X_point[IX(d, i, j, k)] = g->deltaXI * (i - 1.5);
+}
+
+/* one separable pass of the filter on components 0..3 over i in [3,gend],
+ * j in [2,ny], k in [2,nz]; the work arrays start as copies of U_, so every
+ * value read outside that box is U_'s own (Filter.cpp:121-135).
+ * wide = 0: the h filter (1,6,1)/8 (Filter.cpp:156-240); wide = 1: the 2h
+ * filter (1,2,1)/4 (Filter.cpp:831-915). */
+static void filter_pass(const dims_t* d, int wide, double* const U_[5], const double* src[4], double* Gx[4],
+                        double* Gy[4], double* out[4], const double* J, const double* Yp) {
+  for (int v = 0; v < 4; v++) {
+    for (int i = 3; i <= d->gend; i++)
+      for (int j = 2; j <= d->ny; j++)
+        for (int k = 2; k <= d->nz; k++) {
+          size_t p = IX(*d, i, j, k), pm = IX(*d, i - 1, j, k), pp = IX(*d, i + 1, j, k);
+          double a = (i - 1 < 3) ? U_[v][pm] : src[v][pm];
+          double b = src[v][p];
+          double c = (i + 1 > d->gend) ? U_[v][pp] : src[v][pp];
+          Gx[v][p] = wide ? 0.25 * (a + 2 * b + c) : 0.125 * (a + 6 * b + c);
+        }
+    for (int i = 3; i <= d->gend; i++)
+      for (int j = 2; j <= d->ny; j++)
+        for (int k = 2; k <= d->nz; k++) {
+          size_t p = IX(*d, i, j, k), pm = IX(*d, i, j - 1, k), pp = IX(*d, i, j + 1, k);
+          double A = Yp[pp] - Yp[pm];
+          double B = Yp[pp] - Yp[p];
+          double Cc = Yp[p] - Yp[pm];
+          double gm = (j - 1 < 2) ? U_[v][pm] : Gx[v][pm];
+          double gp = (j + 1 > d->ny) ? U_[v][pp] : Gx[v][pp];
+          Gy[v][p] = wide ? 0.5 * (B / A * gm * J[pm] + Gx[v][p] * J[p] + Cc / A * gp * J[pp]) / J[p]
+                          : 0.125 * (2 * B / A * gm * J[pm] + 6 * Gx[v][p] * J[p] + 2 * Cc / A * gp * J[pp]) / J[p];
+        }
+    for (int i = 3; i <= d->gend; i++)
+      for (int j = 2; j <= d->ny; j++)
+        for (int k = 2; k <= d->nz; k++) {
+          size_t p = IX(*d, i, j, k), pm = IX(*d, i, j, k - 1), pp = IX(*d, i, j, k + 1);
+          double a = (k - 1 < 2) ? U_[v][pm] : Gy[v][pm];
+          double c = (k + 1 > d->nz) ? U_[v][pp] : Gy[v][pp];
+          out[v][p] = wide ? 0.25 * (a + 2 * Gy[v][p] + c) : 0.125 * (a + 6 * Gy[v][p] + c);
+        }
+  }
+}
+
+/* E = |u - u_f|^2 with u_f from 6 G - 15 G^2 + 20 G^3 - 15 G^4 + 6 G^5 - G^6
+ * (Filter.cpp:763-800 and 1438-1475) */
+static void filter_energy(const dims_t* d, int wide, double* const U_[5], const double* J, const double* Yp,
+                          double* E, double* work) {
+  size_t nf = (size_t)(d->gend + 4) * d->Ym * d->Zm;
+  double *G[4], *Gn[4], *Gx[4], *Gy[4], *acc[4];
+  for (int v = 0; v < 4; v++) {
+    G[v] = work + (0 * 4 + v) * nf;
+    Gn[v] = work + (1 * 4 + v) * nf;
+    Gx[v] = work + (2 * 4 + v) * nf;
+    Gy[v] = work + (3 * 4 + v) * nf;
+    acc[v] = work + (4 * 4 + v) * nf;
+  }
+  static const double coef[6] = {6, -15, 20, -15, 6, -1};
+  const double* src[4] = {U_[0], U_[1], U_[2], U_[3]};
+  for (int pass = 0; pass < 6; pass++) {
+    filter_pass(d, wide, U_, src, Gx, Gy, Gn, J, Yp);
+    for (int v = 0; v < 4; v++)
+      for (int i = 3; i <= d->gend; i++)
+        for (int j = 2; j <= d->ny; j++)
+          for (int k = 2; k <= d->nz; k++) {
+            size_t p = IX(*d, i, j, k);
+            double t = Gn[v][p];
+            if (pass == 0) acc[v][p] = 6 * t;
+            else if (pass == 5) acc[v][p] = acc[v][p] - t;
+            else if (coef[pass] > 0) acc[v][p] = acc[v][p] + coef[pass] * t;
+            else acc[v][p] = acc[v][p] - (-coef[pass]) * t;
+            G[v][p] = t;
+          }
+    for (int v = 0; v < 4; v++) src[v] = G[v];
+  }
+  for (int i = 3; i <= d->gend; i++)
+    for (int j = 2; j <= d->ny; j++)
+      for (int k = 2; k <= d->nz; k++) {
+        size_t p = IX(*d, i, j, k);
+        double Ue = U_[1][p] / U_[0][p] - acc[1][p] / acc[0][p];
+        double Ve = U_[2][p] / U_[0][p] - acc[2][p] / acc[0][p];
+        double We = U_[3][p] / U_[0][p] - acc[3][p] / acc[0][p];
+        E[p] = Ue * Ue + Ve * Ve + We * We;
+      }
+}
+
+/* E at a cell of the sensor stencils: the planes next to the box copy the
+ * box's first/last cells (Filter.cpp:1490-1540) */
+static double e_at(const dims_t* d, const double* E, int i, int j, int k) {
+  if (i < 3) i = 3;
+  if (i > d->gend) i = d->gend;
+  if (j < 2) j = 2;
+  if (j > d->ny) j = d->ny;
+  if (k < 2) k = 2;
+  if (k > d->nz) k = d->nz;
+  return E[IX(*d, i, j, k)];
+}
+
+/* the sensor ratios themselves (test infrastructure: thresholds for the
+ * pinning fixtures are chosen from them) */
+static void filter_ratios(const dims_t* d, const double* Eh, const double* E2h, double* R[3]);
+
+void cfo_filter_ratios(const cfo_grid* g, double* const U_[5], const double* Y_point, const double* J,
+                       double* RX, double* RY, double* RZ) {
+  dims_t d = dims(g);
+  size_t nf = (size_t)(d.gend + 4) * d.Ym * d.Zm;
+  double* work = (double*)calloc(20 * nf, sizeof(double));
+  double* Eh = (double*)calloc(nf, sizeof(double));
+  double* E2h = (double*)calloc(nf, sizeof(double));
+  filter_energy(&d, 0, U_, J, Y_point, Eh, work);
+  filter_energy(&d, 1, U_, J, Y_point, E2h, work);
+  double* R[3] = {RX, RY, RZ};
+  filter_ratios(&d, Eh, E2h, R);
+  free(work);
+  free(Eh);
+  free(E2h);
+}
+
+void cfo_filter(const cfo_grid* g, double crit, double E_high, double E_low, double* const U_[5],
+                const double* Y_point, double* EpX, double* EpY, double* EpZ, const double* J) {
+  dims_t d = dims(g);
+  size_t nf = (size_t)(d.gend + 4) * d.Ym * d.Zm;
+  double* work = (double*)calloc(20 * nf, sizeof(double));
+  double* Eh = (double*)calloc(nf, sizeof(double));
+  double* E2h = (double*)calloc(nf, sizeof(double));
+  filter_energy(&d, 0, U_, J, Y_point, Eh, work);
+  filter_energy(&d, 1, U_, J, Y_point, E2h, work);
+  /* sensors on the face layouts (Filter.cpp:1550-1645); 0/0 compares false */
+  for (int i = 2; i <= d.gend; i++)
+    for (int j = 1; j < d.ny; j++)
+      for (int k = 1; k < d.nz; k++) {
+        double Em = (e_at(&d, Eh, i, j + 1, k + 1) + e_at(&d, Eh, i + 1, j + 1, k + 1)) /
+                    (e_at(&d, E2h, i, j + 1, k + 1) + e_at(&d, E2h, i + 1, j + 1, k + 1));
+        EpX[IX(d, i, j, k)] = Em >= crit ? E_high : E_low;
+      }
+  for (int i = 2; i <= d.gend - 1; i++)
+    for (int j = 1; j <= d.ny; j++)
+      for (int k = 1; k < d.nz; k++) {
+        double Em = (e_at(&d, Eh, i + 1, j, k + 1) + e_at(&d, Eh, i + 1, j + 1, k + 1)) /
+                    (e_at(&d, E2h, i + 1, j, k + 1) + e_at(&d, E2h, i + 1, j + 1, k + 1));
+        EpY[IX(d, i, j, k)] = Em >= crit ? E_high : E_low;
+      }
+  for (int i = 2; i <= d.gend - 1; i++)
+    for (int j = 1; j < d.ny; j++)
+      for (int k = 1; k <= d.nz; k++) {
+        double Em = (e_at(&d, Eh, i + 1, j + 1, k) + e_at(&d, Eh, i + 1, j + 1, k + 1)) /
+                    (e_at(&d, E2h, i + 1, j + 1, k) + e_at(&d, E2h, i + 1, j + 1, k + 1));
+        EpZ[IX(d, i, j, k)] = Em >= crit ? E_high : E_low;
+      }
+  free(work);
+  free(Eh);
+  free(E2h);
+}
+
+static void filter_ratios(const dims_t* d, const double* Eh, const double* E2h, double* R[3]) {
+  for (int i = 2; i <= d->gend; i++)
+    for (int j = 1; j < d->ny; j++)
+      for (int k = 1; k < d->nz; k++)
+        R[0][IX(*d, i, j, k)] = (e_at(d, Eh, i, j + 1, k + 1) + e_at(d, Eh, i + 1, j + 1, k + 1)) /
+                                (e_at(d, E2h, i, j + 1, k + 1) + e_at(d, E2h, i + 1, j + 1, k + 1));
+  for (int i = 2; i <= d->gend - 1; i++)
+    for (int j = 1; j <= d->ny; j++)
+      for (int k = 1; k < d->nz; k++)
+        R[1][IX(*d, i, j, k)] = (e_at(d, Eh, i + 1, j, k + 1) + e_at(d, Eh, i + 1, j + 1, k + 1)) /
+                                (e_at(d, E2h, i + 1, j, k + 1) + e_at(d, E2h, i + 1, j + 1, k + 1));
+  for (int i = 2; i <= d->gend - 1; i++)
+    for (int j = 1; j < d->ny; j++)
+      for (int k = 1; k <= d->nz; k++)
+        R[2][IX(*d, i, j, k)] = (e_at(d, Eh, i + 1, j + 1, k) + e_at(d, Eh, i + 1, j + 1, k + 1)) /
+                                (e_at(d, E2h, i + 1, j + 1, k) + e_at(d, E2h, i + 1, j + 1, k + 1));
+}

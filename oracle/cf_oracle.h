@@ -60,6 +60,20 @@ void cfo_flux(const cfo_grid* g, int dir, double* const U_[5], const double* J, 
 void cfo_flux_limiter(const cfo_grid* g, int dir, double* const U_[5], const double* J, const double* Jf,
                       double* const mf[9], const double* Ep, double* const inF[5]);
 
+/* Grid.cpp:50-141: Y_point (stretching gamma; the reference hard-codes 2.8)
+ * with its ghost planes, and X_point on the interior (may be NULL). */
+void cfo_grid_points(const cfo_grid* g, double gamma, double* X_point, double* Y_point);
+
+/* Filter.cpp:12-1720: ADM sensor.  Two chains of six separable filter
+ * passes (h and 2h) give E_h, E_2h = |u - u_filtered|^2; EpX/EpY/EpZ on the
+ * face layouts are E_high where (E_h + E_h')/(E_2h + E_2h') >= crit, else
+ * E_low (a 0/0 ratio gives E_low). */
+void cfo_filter(const cfo_grid* g, double crit, double E_high, double E_low, double* const U_[5],
+                const double* Y_point, double* EpX, double* EpY, double* EpZ, const double* J);
+/* the ratios the sensors threshold (face layouts) */
+void cfo_filter_ratios(const cfo_grid* g, double* const U_[5], const double* Y_point, const double* J,
+                       double* RX, double* RY, double* RZ);
+
 /* Viscous_terms.cpp:212-883; writes vF[0..3] = vF2..vF5 */
 void cfo_viscous(const cfo_grid* g, double* const U_[5], const double* J, const double* Ju,
                  double* const mu[9], const double* Jv, double* const mv[9], const double* Jw,

@@ -48,6 +48,10 @@ def lib():
         _lib.cfo_halo_bc.argtypes = [C.POINTER(Grid), PP, P]
         _lib.cfo_flux.argtypes = [C.POINTER(Grid), C.c_int, PP, P, P, PP, PP]
         _lib.cfo_viscous.argtypes = [C.POINTER(Grid), PP, P, P, PP, P, PP, P, PP, PP]
+        _lib.cfo_flux_limiter.argtypes = [C.POINTER(Grid), C.c_int, PP, P, P, PP, P, PP]
+        _lib.cfo_grid_points.argtypes = [C.POINTER(Grid), C.c_double, P, P]
+        _lib.cfo_filter.argtypes = [C.POINTER(Grid), C.c_double, C.c_double, C.c_double, PP, P, P, P, P, P]
+        _lib.cfo_filter_ratios.argtypes = [C.POINTER(Grid), PP, P, P, P, P, P]
         dt_args = [C.POINTER(Grid), C.c_double, PP, PP, PP, P, PP, PP, PP, PP, PP, PP, PP, PP, P]
         _lib.cfo_dual_time.argtypes = dt_args
         _lib.cfo_dual_time_hyperplane.argtypes = dt_args
@@ -109,6 +113,26 @@ def flux_limiter(cfg, direction, U_, J, Jf, mf, Ep):
     lib().cfo_flux_limiter(C.byref(grid_of(cfg)), int(direction), _pp(U_), _p(J), _p(Jf), _pp(mf), _p(Ep),
                            _pp(out))
     return out
+
+
+def grid_points(cfg, gamma=2.8):
+    X = np.zeros(cfg.shape)
+    Y = np.zeros(cfg.shape)
+    lib().cfo_grid_points(C.byref(grid_of(cfg)), float(gamma), _p(X), _p(Y))
+    return X, Y
+
+
+def filter_sensor(cfg, U_, Y_point, J, crit=0.007, E_high=1.0, E_low=0.0):
+    Ep = [np.zeros(cfg.shape) for _ in range(3)]
+    lib().cfo_filter(C.byref(grid_of(cfg)), float(crit), float(E_high), float(E_low), _pp(U_), _p(Y_point),
+                     _p(Ep[0]), _p(Ep[1]), _p(Ep[2]), _p(J))
+    return Ep
+
+
+def filter_ratios(cfg, U_, Y_point, J):
+    R = [np.zeros(cfg.shape) for _ in range(3)]
+    lib().cfo_filter_ratios(C.byref(grid_of(cfg)), _pp(U_), _p(Y_point), _p(J), _p(R[0]), _p(R[1]), _p(R[2]))
+    return R
 
 
 def viscous(cfg, U_, met):

@@ -331,6 +331,26 @@ int main(int argc, char** argv) {
       Output(step, myid, s.U[0], s.U[1], s.U[2], s.U[3], s.U[4], s.Uq[0], s.Uq[1], s.Uq[2], s.Uq[3], s.Uq[4], s.J);
     }
     fclose(f);
+  } else if (mode == "filter" && argc >= 5) {
+    // the reference Filter on the post-BC iterate once per threshold listed
+    // in <crit file> (one %la per line); EpX/EpY/EpZ of run r go to
+    // EpX_c<r>.bin ...  A threshold equal to a ratio computed elsewhere and
+    // its successor show whether the reference computes that same ratio.
+    load5("iterate.bin", s.U_, s.J);
+    halo_bc(s, myid);
+    FILE* cf = fopen(argv[4], "r");
+    if (!cf) { perror("crit file"); return 3; }
+    double crit;
+    int r = 0;
+    char name[64];
+    while (fscanf(cf, "%la", &crit) == 1) {
+      Filter(myid, crit, 1., 0., s.U_[0], s.U_[1], s.U_[2], s.U_[3], s.U_[4], s.Yp, s.EpX, s.EpY, s.EpZ, s.J);
+      snprintf(name, sizeof name, "EpX_c%d", r); dump(dir, name, s.EpX);
+      snprintf(name, sizeof name, "EpY_c%d", r); dump(dir, name, s.EpY);
+      snprintf(name, sizeof name, "EpZ_c%d", r); dump(dir, name, s.EpZ);
+      r++;
+    }
+    fclose(cf);
   } else if (mode == "time" && argc >= 5) {
     int nit = atoi(argv[4]);
     double er[5];

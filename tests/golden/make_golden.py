@@ -130,6 +130,51 @@ def variant_fixture(grid, seed):
     return name, meta
 
 
+def filter_pin_fixture(grid):
+    """The reference Filter's sensor patterns for thresholds placed at sensor
+    ratios: for sample faces the threshold is the ratio r that the oracle
+    computes there and the next double above it.  The reference marks the
+    face E_high at r and E_low at next(r) exactly when its own ratio is r, so
+    each pair pins one ratio bit for bit; the full patterns (bit-packed) pin
+    every face against every threshold."""
+    from oracle import oracle as O
+    cfg = ChannelConfig(X_out=grid[0], Y_out=grid[1], Z_out=grid[2])
+    z = np.load(OUT / f"var_g{grid[0]}x{grid[1]}x{grid[2]}.npz")
+    U = [np.ascontiguousarray(z[f"U_bc{v + 1}"]) for v in range(5)]
+    R = O.filter_ratios(cfg, U, np.ascontiguousarray(z["Y_point"]), np.ascontiguousarray(z["J"]))
+    g, ny, nz = cfg.gend, cfg.ny, cfg.nz
+    regions = [(slice(2, g + 1), slice(1, ny), slice(1, nz)), (slice(2, g), slice(1, ny + 1), slice(1, nz)),
+               (slice(2, g), slice(1, ny), slice(1, nz + 1))]
+    crits, faces = [], []
+    for a in range(3):
+        vals = R[a][regions[a]]
+        flat = np.sort(np.unique(vals[np.isfinite(vals)]))
+        for q in (0.02, 0.3, 0.7, 0.98):
+            r = flat[int(q * (len(flat) - 1))]
+            idx = np.argwhere(R[a] == r)
+            idx = [t for t in idx.tolist() if all(regions[a][d].start <= t[d] < regions[a][d].stop for d in range(3))][0]
+            faces.append([a] + idx)
+            crits += [float(r), float(np.nextafter(r, np.inf))]
+    exe = ref_driver(grid)
+    with tempfile.TemporaryDirectory() as td:
+        d = Path(td)
+        write_initial_bin(d / "initial.bin", perturbed_state(cfg, 21 + VAR_GRIDS.index(tuple(grid)), 0),
+                          perturbed_state(cfg, 21 + VAR_GRIDS.index(tuple(grid)), 1))
+        np.ascontiguousarray(perturbed_state(cfg, 21 + VAR_GRIDS.index(tuple(grid)), 2)).tofile(d / "iterate.bin")
+        (d / "crits.txt").write_text("".join(f"{c.hex()}\n" for c in crits))
+        subprocess.run([str(exe), "filter", str(d), repr(DT_UNIT), str(d / "crits.txt")], check=True,
+                       stdout=subprocess.DEVNULL, env=dict(os.environ, OMP_NUM_THREADS="1"))
+        pats = np.stack([np.stack([load(d, f"Ep{a}_c{r}", cfg) for a in "XYZ"]) for r in range(len(crits))])
+    assert np.array_equal(np.unique(pats), [0.0, 1.0]) or set(np.unique(pats)) <= {0.0, 1.0}
+    arrs = dict(crits=np.array(crits), faces=np.array(faces, dtype=np.int64),
+                patterns=np.packbits(pats.astype(bool), axis=-1), pattern_shape=np.array(pats.shape))
+    name = f"fpin_g{grid[0]}x{grid[1]}x{grid[2]}"
+    np.savez_compressed(OUT / f"{name}.npz", **arrs)
+    meta = dict(grid=list(grid), state=f"var_g{grid[0]}x{grid[1]}x{grid[2]} iterate", omp_threads=1,
+                reference_calls=["Filter(crit, 1, 0) for each listed crit"], runs=len(crits))
+    return name, meta
+
+
 def traj_fixture(spec, keep_state: bool):
     grid = spec["grid"]
     cfg = ChannelConfig(X_out=grid[0], Y_out=grid[1], Z_out=grid[2])
@@ -164,6 +209,13 @@ def main():
             manifest["fixtures"][n] = m
         (OUT / "manifest.json").write_text(json.dumps(manifest, indent=1))
         return
+    if len(sys.argv) > 1 and sys.argv[1] == "filterpin":
+        manifest = json.loads((OUT / "manifest.json").read_text())
+        for g in VAR_GRIDS:
+            n, m = filter_pin_fixture(g)
+            manifest["fixtures"][n] = m
+        (OUT / "manifest.json").write_text(json.dumps(manifest, indent=1))
+        return
     manifest = {"generator": "tests/golden/make_golden.py",
                 "reference": "ricklee209/Channel_Flow built by oracle/ref_build.sh (np=1, MPICH 3.3.2)",
                 "fixtures": {}}
@@ -172,6 +224,9 @@ def main():
         manifest["fixtures"][n] = m
     for i, g in enumerate(VAR_GRIDS):
         n, m = variant_fixture(g, seed=21 + i)
+        manifest["fixtures"][n] = m
+    for g in VAR_GRIDS:
+        n, m = filter_pin_fixture(g)
         manifest["fixtures"][n] = m
     n, m = traj_fixture(TRAJ, keep_state=True)
     manifest["fixtures"][n] = m

@@ -181,3 +181,51 @@ def test_first_iteration_64cubed_matches_reference(golden):
     Ui = [a.copy() for a in U]
     er = O.inner_iteration(cfg, Ui, U, [a.copy() for a in U], met)
     np.testing.assert_array_equal(er, z["er"][0, 2:7])
+
+
+def _fpin(z):
+    sh = tuple(z["pattern_shape"])
+    return np.unpackbits(z["patterns"], axis=-1)[..., :sh[-1]].reshape(sh).astype(np.float64)
+
+
+@pytest.mark.parametrize("prefix", ["unit", "var"])
+@pytest.mark.parametrize("grid", VAR_GRIDS)
+def test_grid_y_point_matches_reference(prefix, grid, golden):
+    """Grid.cpp:50-141 with the hard-coded stretching 2.8 (Y_point and its
+    ghost planes, the cells Filter reads)."""
+    z = golden(f"{prefix}_g{grid[0]}x{grid[1]}x{grid[2]}")
+    cfg = unit_cfg(grid)
+    _, Y = O.grid_points(cfg, 2.8)
+    m = masks(cfg)["bc"]
+    np.testing.assert_array_equal(Y[m], z["Y_point"][m])
+
+
+@pytest.mark.parametrize("prefix", ["unit", "var"])
+@pytest.mark.parametrize("grid", VAR_GRIDS)
+def test_filter_sensor_matches_reference(prefix, grid, golden):
+    """Filter(0.007, 1, 0): EpX/EpY/EpZ on their face layouts."""
+    z = golden(f"{prefix}_g{grid[0]}x{grid[1]}x{grid[2]}")
+    cfg = unit_cfg(grid)
+    Ep = O.filter_sensor(cfg, fields(z, "U_bc"), np.ascontiguousarray(z["Y_point"]), np.ascontiguousarray(z["J"]))
+    for a in range(3):
+        m = masks(cfg)["xyz"[a]]
+        np.testing.assert_array_equal(Ep[a][m], z["Ep" + "XYZ"[a]][m])
+
+
+@pytest.mark.parametrize("grid", VAR_GRIDS)
+def test_filter_ratios_pinned_by_thresholds(grid, golden):
+    """The reference's sensor patterns at thresholds r and next(r) for sample
+    faces: every face agrees at every threshold, and each sample face flips
+    between the two, so the reference's ratio there is r bit for bit."""
+    z = golden(f"var_g{grid[0]}x{grid[1]}x{grid[2]}")
+    pin = golden(f"fpin_g{grid[0]}x{grid[1]}x{grid[2]}")
+    cfg = unit_cfg(grid)
+    pats = _fpin(pin)
+    U_, Yp, J = fields(z, "U_bc"), np.ascontiguousarray(z["Y_point"]), np.ascontiguousarray(z["J"])
+    for r, crit in enumerate(pin["crits"]):
+        Ep = O.filter_sensor(cfg, U_, Yp, J, crit=float(crit))
+        for a in range(3):
+            m = masks(cfg)["xyz"[a]]
+            np.testing.assert_array_equal(Ep[a][m], pats[r][a][m], err_msg=f"run {r} Ep{'XYZ'[a]}")
+    for s, (a, i, j, k) in enumerate(pin["faces"]):
+        assert pats[2 * s][a][i, j, k] == 1.0 and pats[2 * s + 1][a][i, j, k] == 0.0
