@@ -41,6 +41,8 @@ struct cf_ctx {
   double* Pr = nullptr;   // viscous primitives U,V,W,T (4)
   double* VS = nullptr;   // viscous face fluxes LL/ML/NL (12)
   double* Ep = nullptr;   // limiter sensors EpX, EpY, EpZ on the face layouts (3)
+  double* Fw = nullptr;   // Filter workspace (FILTER_WORK_FIELDS), allocated on first use
+  double* ypt = nullptr;  // Y_point j-profile (Y_m)
   double* Sm = nullptr;   // per-cell sweep summary (S_N)
   int* order = nullptr;   // sweep tiles in dependency order
   int* sync = nullptr;    // ticket, error flag, per-tile progress (x2 directions)
@@ -236,6 +238,8 @@ extern "C" int cf_create(const cf_params* p, cf_ctx** out) {
   if (hipMalloc(&c->partial, (size_t)nb * 5 * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
   if (hipMalloc(&c->er_dev, 8 * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
   if (hipMalloc(&c->prof, c->hprof.size() * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
+  if (hipMalloc(&c->ypt, (size_t)g.Ym * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
+  if (hipMemsetAsync(c->ypt, 0, (size_t)g.Ym * sizeof(double), c->stream) != hipSuccess) return bail(CF_ERR_HIP);
   if (hipMemsetAsync(c->prof, 0, c->hprof.size() * sizeof(double), c->stream) != hipSuccess) return bail(CF_ERR_HIP);
   if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(CF_ERR_HIP);
   *out = c;
@@ -245,8 +249,8 @@ extern "C" int cf_create(const cf_params* p, cf_ctx** out) {
 extern "C" void cf_destroy(cf_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  double* ptrs[] = {c->Ui, c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->MR, c->NL,
-                    c->NR, c->Pr, c->VS, c->Ep, c->Sm, c->partial, c->er_dev, c->prof};
+  double* ptrs[] = {c->Ui, c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->MR,  c->NL,     c->NR,
+                    c->Pr, c->VS, c->Ep, c->Fw, c->ypt, c->Sm, c->partial, c->er_dev, c->prof};
   for (double* q : ptrs)
     if (q) (void)hipFree(q);
   if (c->order) (void)hipFree(c->order);
@@ -463,6 +467,75 @@ extern "C" int cf_flux_y_limiter(cf_ctx* c, const double* const U_[5], const dou
 extern "C" int cf_flux_z_limiter(cf_ctx* c, const double* const U_[5], const double* J, const double* J_w,
                                  const double* const m_w[9], double* const inFz[5], const double* EpZ) {
   return flux_limiter_entry(c, 2, U_, J, J_w, m_w, EpZ, inFz);
+}
+
+// ---------------------------------------------------------------------------
+// Grid (Grid.cpp:50-141) and Filter (Filter.cpp:12-1720)
+// ---------------------------------------------------------------------------
+extern "C" int cf_grid(cf_ctx* c, double gamma, double* X_point, double* Y_point) {
+  if (!c) return CF_ERR_ARG;
+  const Geo& g = c->g;
+  const int gend = g.gend, ny = g.ny, nz = g.nz, Ym = g.Ym, Zm = g.Zm;
+  // cell-centre heights of the tanh-stretched grid, a function of j only
+  std::vector<double> yp(Ym, 0.0);
+  for (int j = 2; j <= ny; j++)
+    yp[j] = (0.5 * c->p.high) * (1 - 1. / tanh(gamma) * tanh(gamma * (1 - 2 * (j - 1.5) * c->p.deltaET)));
+  const double ylo = -yp[2], yhi = yp[ny] + 2 * yp[2];  // wall ghosts
+  auto A = [&](int i, int j, int k) -> double& { return Y_point[((size_t)i * Ym + j) * Zm + k]; };
+  if (Y_point) {
+    for (int i = 0; i <= gend + 3; i++)
+      for (int j = 2; j <= ny; j++)
+        for (int k = 2; k <= nz; k++) A(i, j, k) = yp[j];
+    for (int i = 2; i <= gend + 1; i++)
+      for (int k = 2; k <= nz; k++) {
+        A(i, 1, k) = ylo;
+        A(i, ny + 1, k) = yhi;
+      }
+    for (int i = 2; i <= gend + 1; i++)
+      for (int j = 1; j <= ny + 1; j++) {
+        A(i, j, 1) = A(i, j, 2);
+        A(i, j, nz + 1) = A(i, j, nz);
+      }
+  }
+  if (X_point)
+    for (int i = 3; i <= gend; i++)
+      for (int j = 2; j <= ny; j++)
+        for (int k = 2; k <= nz; k++) X_point[((size_t)i * Ym + j) * Zm + k] = c->p.deltaXI * (i - 1.5);
+  yp[1] = ylo;
+  yp[ny + 1] = yhi;
+  HIPCHK(c, hipMemcpyAsync(c->ypt, yp.data(), Ym * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  return sync(c);
+}
+
+extern "C" int cf_filter(cf_ctx* c, double Roe_criterion, double E_high, double E_low, const double* const U_[5],
+                         const double* Y_point, double* EpX, double* EpY, double* EpZ, const double* J) {
+  if (!c || !U_ || !Y_point || !EpX || !EpY || !EpZ) return CF_ERR_ARG;
+  const Geo& g = c->g;
+  int r;
+  if ((r = ingest(c, J, cf::SET_C, 0, "J")) || (r = upload_prof(c))) return r;
+  // Y_point as read by the filter: j in [1, ny+1] over the interior i, k
+  std::vector<double> yp(g.Ym, 0.0);
+  for (int j = 1; j <= g.ny + 1; j++) {
+    const double v0 = Y_point[((size_t)3 * g.Ym + j) * g.Zm + 2];
+    for (int i = 3; i <= g.gend; i++)
+      for (int k = 2; k <= g.nz; k++)
+        if (Y_point[((size_t)i * g.Ym + j) * g.Zm + k] != v0)
+          return fail(c, CF_ERR_METRICS, "Y_point varies along i/k at (%d,%d,%d)", i, j, k);
+    yp[j] = v0;
+  }
+  HIPCHK(c, hipMemcpyAsync(c->ypt, yp.data(), g.Ym * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (!c->Fw) {
+    if (hipMalloc(&c->Fw, (size_t)g.nf * cf::FILTER_WORK_FIELDS * sizeof(double)) != hipSuccess)
+      return fail(c, CF_ERR_HIP, "hipMalloc of the filter workspace failed");
+  }
+  if ((r = h2d_n(c, c->Ui, U_, 5))) return r;
+  // the sensors keep the caller's values outside the face ranges
+  double* const ep[3] = {EpX, EpY, EpZ};
+  if ((r = h2d_n(c, c->Ep, (const double* const*)ep, 3))) return r;
+  cf::launch_filter(c->stream, c->Ui, c->prof, c->ypt, c->Fw, Roe_criterion, E_high, E_low, c->Ep, c->Ep + g.nf,
+                    c->Ep + 2 * g.nf, g);
+  if ((r = d2h_n(c, ep, c->Ep, 3))) return r;
+  return sync(c);
 }
 
 extern "C" int cf_viscous_terms(cf_ctx* c, const double* const U_[5], double* const vF[4], const double* J,

@@ -226,6 +226,133 @@ __global__ void __launch_bounds__(256) k_flux_lim(const double* __restrict__ U, 
 }
 
 // ---------------------------------------------------------------------------
+// ADM sensor (Filter.cpp:12-1720): two chains (h, 2h) of six separable
+// 3-point filter passes over the interior box on rho and the momenta; every
+// value read outside the box is U_'s own (the reference's work arrays start
+// as copies of U_ and no pass writes outside the box).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool in_box(const Geo& g, int i, int j, int k) {
+  return i >= 3 && i <= g.gend && j >= 2 && j <= g.ny && k >= 2 && k <= g.nz;
+}
+
+template <int WIDE>
+__global__ void __launch_bounds__(256) k_filt_x(const double* __restrict__ src, const double* __restrict__ U,
+                                                double* __restrict__ Gx, Geo g) {
+  const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int i = 3 + blockIdx.z;
+  if (k > g.nz || j > g.ny) return;
+  const long p = at(g, i, j, k);
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const double* s = (i - 1 < 3 ? U : src) + v * g.nf;
+    const double* t = (i + 1 > g.gend ? U : src) + v * g.nf;
+    const double a = s[p - g.plane], b = src[v * g.nf + p], c = t[p + g.plane];
+    Gx[v * g.nf + p] = WIDE ? 0.25 * (a + 2 * b + c) : 0.125 * (a + 6 * b + c);
+  }
+}
+
+template <int WIDE>
+__global__ void __launch_bounds__(256) k_filt_y(const double* __restrict__ Gx, const double* __restrict__ U,
+                                                const double* __restrict__ prof, const double* __restrict__ Yp,
+                                                double* __restrict__ Gy, Geo g) {
+  const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int i = 3 + blockIdx.z;
+  if (k > g.nz || j > g.ny) return;
+  const long p = at(g, i, j, k);
+  const double A = Yp[j + 1] - Yp[j - 1];  // H_L + H_R
+  const double B = Yp[j + 1] - Yp[j];      // H_R
+  const double C = Yp[j] - Yp[j - 1];      // H_L
+  const double Jm = pf(prof, g.Ym, SET_C, 0, j - 1), J0 = pf(prof, g.Ym, SET_C, 0, j), Jp = pf(prof, g.Ym, SET_C, 0, j + 1);
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const double gm = (j - 1 < 2 ? U : Gx)[v * g.nf + p - g.Zp];
+    const double gp = (j + 1 > g.ny ? U : Gx)[v * g.nf + p + g.Zp];
+    const double gx = Gx[v * g.nf + p];
+    Gy[v * g.nf + p] = WIDE ? 0.5 * (B / A * gm * Jm + gx * J0 + C / A * gp * Jp) / J0
+                            : 0.125 * (2 * B / A * gm * Jm + 6 * gx * J0 + 2 * C / A * gp * Jp) / J0;
+  }
+}
+
+// z pass, then the running sum 6 G - 15 G^2 + 20 G^3 - 15 G^4 + 6 G^5 - G^6
+// in the reference's left-to-right order (Filter.cpp:763-775)
+template <int WIDE>
+__global__ void __launch_bounds__(256) k_filt_z(const double* __restrict__ Gy, const double* __restrict__ U,
+                                                double* __restrict__ G, double* __restrict__ acc, int pass, Geo g) {
+  const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int i = 3 + blockIdx.z;
+  if (k > g.nz || j > g.ny) return;
+  const long p = at(g, i, j, k);
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const double a = (k - 1 < 2 ? U : Gy)[v * g.nf + p - 1];
+    const double c = (k + 1 > g.nz ? U : Gy)[v * g.nf + p + 1];
+    const double y = Gy[v * g.nf + p];
+    const double t = WIDE ? 0.25 * (a + 2 * y + c) : 0.125 * (a + 6 * y + c);
+    G[v * g.nf + p] = t;
+    double& s = acc[v * g.nf + p];
+    switch (pass) {
+      case 0: s = 6 * t; break;
+      case 1: s = s - 15 * t; break;
+      case 2: s = s + 20 * t; break;
+      case 3: s = s - 15 * t; break;
+      case 4: s = s + 6 * t; break;
+      default: s = s - t; break;
+    }
+  }
+}
+
+// E = |u - u_f|^2 (Filter.cpp:790-800, 1465-1475)
+__global__ void __launch_bounds__(256) k_filt_energy(const double* __restrict__ U, const double* __restrict__ acc,
+                                                     double* __restrict__ E, Geo g) {
+  const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int i = 3 + blockIdx.z;
+  if (k > g.nz || j > g.ny) return;
+  const long p = at(g, i, j, k), n = g.nf;
+  const double Ue = U[n + p] / U[p] - acc[n + p] / acc[p];
+  const double Ve = U[2 * n + p] / U[p] - acc[2 * n + p] / acc[p];
+  const double We = U[3 * n + p] / U[p] - acc[3 * n + p] / acc[p];
+  E[p] = Ue * Ue + Ve * Ve + We * We;
+}
+
+// E at a cell of the sensor stencils: the planes next to the box copy the
+// box's first/last cells (Filter.cpp:1490-1540)
+__device__ __forceinline__ double e_at(const double* __restrict__ E, const Geo& g, int i, int j, int k) {
+  i = min(max(i, 3), g.gend);
+  j = min(max(j, 2), g.ny);
+  k = min(max(k, 2), g.nz);
+  return E[at(g, i, j, k)];
+}
+
+// sensors on the face layouts (Filter.cpp:1550-1645); a 0/0 ratio compares
+// false and gives E_low
+template <int DIR>
+__global__ void __launch_bounds__(256) k_filt_sensor(const double* __restrict__ Eh, const double* __restrict__ E2h,
+                                                     double crit, double E_high, double E_low,
+                                                     double* __restrict__ Ep, Geo g) {
+  const int k = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int i = 2 + blockIdx.z;
+  if (DIR == 0 && (k >= g.nz || j >= g.ny || i > g.gend)) return;
+  if (DIR == 1 && (k >= g.nz || j > g.ny || i > g.gend - 1)) return;
+  if (DIR == 2 && (k > g.nz || j >= g.ny || i > g.gend - 1)) return;
+  double r;
+  if (DIR == 0)
+    r = (e_at(Eh, g, i, j + 1, k + 1) + e_at(Eh, g, i + 1, j + 1, k + 1)) /
+        (e_at(E2h, g, i, j + 1, k + 1) + e_at(E2h, g, i + 1, j + 1, k + 1));
+  else if (DIR == 1)
+    r = (e_at(Eh, g, i + 1, j, k + 1) + e_at(Eh, g, i + 1, j + 1, k + 1)) /
+        (e_at(E2h, g, i + 1, j, k + 1) + e_at(E2h, g, i + 1, j + 1, k + 1));
+  else
+    r = (e_at(Eh, g, i + 1, j + 1, k) + e_at(Eh, g, i + 1, j + 1, k + 1)) /
+        (e_at(E2h, g, i + 1, j + 1, k) + e_at(E2h, g, i + 1, j + 1, k + 1));
+  Ep[at(g, i, j, k)] = r >= crit ? E_high : E_low;
+}
+
+// ---------------------------------------------------------------------------
 // viscous terms: primitives on the stencil box, face fluxes, divergence
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_visc_prim(const double* __restrict__ U, const double* __restrict__ prof,
@@ -1440,6 +1567,39 @@ void launch_flux_limiter(hipStream_t st, int dir, const double* U, const double*
     hipLaunchKernelGGL(k_flux_lim<1>, box_grid(g.nz - 1, g.ny, g.gend - 2, blk), blk, 0, st, U, Ep, prof, F, g);
   else
     hipLaunchKernelGGL(k_flux_lim<2>, box_grid(g.nz, g.ny - 1, g.gend - 2, blk), blk, 0, st, U, Ep, prof, F, g);
+}
+
+void launch_filter(hipStream_t st, const double* U, const double* prof, const double* Yp, double* W, double crit,
+                   double E_high, double E_low, double* EpX, double* EpY, double* EpZ, const Geo& g) {
+  // W: 22 fields = Gx(4) Gy(4) G0(4) G1(4) acc(4) E_h E_2h
+  const long n = g.nf;
+  double *Gx = W, *Gy = W + 4 * n, *Gb[2] = {W + 8 * n, W + 12 * n}, *acc = W + 16 * n;
+  double *Eh = W + 20 * n, *E2h = W + 21 * n;
+  dim3 blk(64, 4);
+  dim3 gr = box_grid(g.nz - 1, g.ny - 1, g.gend - 2, blk);
+  for (int wide = 0; wide < 2; wide++) {
+    const double* src = U;
+    for (int pass = 0; pass < 6; pass++) {
+      double* out = Gb[pass & 1];
+      if (wide) {
+        hipLaunchKernelGGL(k_filt_x<1>, gr, blk, 0, st, src, U, Gx, g);
+        hipLaunchKernelGGL(k_filt_y<1>, gr, blk, 0, st, Gx, U, prof, Yp, Gy, g);
+        hipLaunchKernelGGL(k_filt_z<1>, gr, blk, 0, st, Gy, U, out, acc, pass, g);
+      } else {
+        hipLaunchKernelGGL(k_filt_x<0>, gr, blk, 0, st, src, U, Gx, g);
+        hipLaunchKernelGGL(k_filt_y<0>, gr, blk, 0, st, Gx, U, prof, Yp, Gy, g);
+        hipLaunchKernelGGL(k_filt_z<0>, gr, blk, 0, st, Gy, U, out, acc, pass, g);
+      }
+      src = out;
+    }
+    hipLaunchKernelGGL(k_filt_energy, gr, blk, 0, st, U, acc, wide ? E2h : Eh, g);
+  }
+  hipLaunchKernelGGL(k_filt_sensor<0>, box_grid(g.nz - 1, g.ny - 1, g.gend - 1, blk), blk, 0, st, Eh, E2h, crit,
+                     E_high, E_low, EpX, g);
+  hipLaunchKernelGGL(k_filt_sensor<1>, box_grid(g.nz - 1, g.ny, g.gend - 2, blk), blk, 0, st, Eh, E2h, crit, E_high,
+                     E_low, EpY, g);
+  hipLaunchKernelGGL(k_filt_sensor<2>, box_grid(g.nz, g.ny - 1, g.gend - 2, blk), blk, 0, st, Eh, E2h, crit, E_high,
+                     E_low, EpZ, g);
 }
 
 void launch_viscous(hipStream_t st, const double* U, const double* prof, double* Pr, double* VS, double* vF,

@@ -9,6 +9,11 @@ void launch_flux(hipStream_t st, int dir, const double* U, const double* prof, d
 // limiter variants; Ep is the sensor on the face layout of F
 void launch_flux_limiter(hipStream_t st, int dir, const double* U, const double* Ep, const double* prof, double* F,
                          const Geo& g);
+// ADM sensor: Yp = Y_point j-profile, W = 22-field workspace; the sensors are
+// written on the face layouts
+constexpr int FILTER_WORK_FIELDS = 22;
+void launch_filter(hipStream_t st, const double* U, const double* prof, const double* Yp, double* W, double crit,
+                   double E_high, double E_low, double* EpX, double* EpY, double* EpZ, const Geo& g);
 void launch_viscous(hipStream_t st, const double* U, const double* prof, double* Pr, double* VS, double* vF,
                     const Geo& g);
 void launch_rhs(hipStream_t st, const double* Ui, const double* Un, const double* Uq, const double* Fx,

@@ -29,7 +29,7 @@ EXPORTS = [
     "cf_dual_time_stepping", "cf_load_state", "cf_load_initial", "cf_set_iterate", "cf_read_state",
     "cf_inner_iteration", "cf_physical_step", "cf_advance_levels", "cf_write_output", "cf_synchronize",
     "cf_profile_inner_iteration", "cf_set_sweep_schedule", "cf_flux_x_limiter", "cf_flux_y_limiter",
-    "cf_flux_z_limiter",
+    "cf_flux_z_limiter", "cf_grid", "cf_filter",
 ]
 
 
@@ -76,6 +76,8 @@ def load_library() -> C.CDLL:
         getattr(lib, n).argtypes = [vp, PP, P, P, PP, PP]
     for n in ("cf_flux_x_limiter", "cf_flux_y_limiter", "cf_flux_z_limiter"):
         getattr(lib, n).argtypes = [vp, PP, P, P, PP, PP, P]
+    lib.cf_grid.argtypes = [vp, C.c_double, P, P]
+    lib.cf_filter.argtypes = [vp, C.c_double, C.c_double, C.c_double, PP, P, P, P, P, P]
     lib.cf_viscous_terms.argtypes = [vp, PP, PP, P, PP, P, PP, P, PP, P, PP]
     lib.cf_dual_time_stepping.argtypes = [vp, C.c_double, P, PP, PP, PP, P, PP, PP, PP, PP, PP, PP, PP, PP]
     lib.cf_load_state.argtypes = [vp, PP, PP]
@@ -192,6 +194,19 @@ class Solver:
         out = self.zeros(5)
         self._chk(self.lib.cf_flux_z(self._h, _pp(U_), _p(J), _p(J_w), _pp(m_w), _pp(out)))
         return out
+
+    def Grid(self, gamma=2.8):
+        """Grid.cpp:50-141: (X_point, Y_point) in the reference layout."""
+        X, Y = self.zeros(1)[0], self.zeros(1)[0]
+        self._chk(self.lib.cf_grid(self._h, float(gamma), _p(X), _p(Y)))
+        return X, Y
+
+    def Filter(self, Roe_criterion, E_high, E_low, U_, Y_point, J, Ep=None):
+        """Filter.cpp:12-1720: (EpX, EpY, EpZ); Ep gives the arrays to update."""
+        Ep = [self.zeros(1)[0] for _ in range(3)] if Ep is None else Ep
+        self._chk(self.lib.cf_filter(self._h, float(Roe_criterion), float(E_high), float(E_low), _pp(U_), _p(Y_point),
+                                     _p(Ep[0]), _p(Ep[1]), _p(Ep[2]), _p(J)))
+        return Ep
 
     def Flux_X_limiter(self, U_, J, J_u, m_u, EpX):
         """Flux_X_limiter.cpp:104-547 (sensor EpX on the face layout)."""

@@ -113,6 +113,17 @@ int cf_flux_y_limiter(cf_ctx* ctx, const double* const U_[5], const double* J, c
                       const double* const m_v[9], double* const inFy[5], const double* EpY);
 int cf_flux_z_limiter(cf_ctx* ctx, const double* const U_[5], const double* J, const double* J_w,
                       const double* const m_w[9], double* const inFz[5], const double* EpZ);
+/* Grid (main.h:10-20, Grid.cpp:50-141): Y_point of the tanh-stretched grid
+ * for stretching gamma (the reference hard-codes 2.8) with its ghost planes,
+ * and X_point on the interior; either pointer may be NULL.  Also keeps the
+ * Y_point profile for cf_filter. */
+int cf_grid(cf_ctx* ctx, double gamma, double* X_point, double* Y_point);
+/* Filter (main.h:946-972, Filter.cpp:12-1720): the ADM sensor.  EpX/EpY/EpZ
+ * become E_high where the ratio of the filtered-velocity energies at scales
+ * h and 2h is >= Roe_criterion, else E_low, on the face layouts of inFx/y/z;
+ * elements outside those ranges keep their values. */
+int cf_filter(cf_ctx* ctx, double Roe_criterion, double E_high, double E_low, const double* const U_[5],
+              const double* Y_point, double* EpX, double* EpY, double* EpZ, const double* J);
 /* Viscous_terms (main.h:525-651): writes vF[0..3] = vF2..vF5. */
 int cf_viscous_terms(cf_ctx* ctx, const double* const U_[5], double* const vF[4], const double* J,
                      const double* const m[9], const double* J_u, const double* const m_u[9],

@@ -46,7 +46,8 @@ typedef double (*field)[Y_m][Z_m];
 struct State {
   cf_ctx* ctx = nullptr;
   cf_params p{};
-  double gamma = 1.5;  // Jacobian*.cpp hard-code
+  double gamma = 1.5;       // Jacobian*.cpp hard-code
+  double gamma_grid = 2.8;  // Grid.cpp hard-code
 };
 
 inline State& state() {
@@ -156,6 +157,22 @@ inline void Hyperplane(int, int (*Nijk)[2], int (*ijk)[3]) {
     Nijk[m][0] = ii;
     Nijk[m][1] = mi;
   }
+}
+
+// Grid.cpp:50-141 (Z_point is not written, as in the reference)
+inline void Grid(int, CF_F(X_point), CF_F(Y_point), CF_F(Z_point)) {
+  (void)Z_point;
+  cf_compat::check(cf_grid(cf_compat::ctx("Grid"), cf_compat::state().gamma_grid, CF_P(X_point), CF_P(Y_point)),
+                   "Grid");
+}
+
+// Filter.cpp:12-1720
+inline void Filter(int, double Roe_criterion, double E_high, double E_low, CF_F(U1_), CF_F(U2_), CF_F(U3_),
+                   CF_F(U4_), CF_F(U5_), CF_F(Y_point), CF_F(EpX), CF_F(EpY), CF_F(EpZ), CF_F(J)) {
+  const double* const U[5] = {CF_P(U1_), CF_P(U2_), CF_P(U3_), CF_P(U4_), CF_P(U5_)};
+  cf_compat::check(cf_filter(cf_compat::ctx("Filter"), Roe_criterion, E_high, E_low, U, CF_P(Y_point), CF_P(EpX),
+                             CF_P(EpY), CF_P(EpZ), CF_P(J)),
+                   "Filter");
 }
 
 // main.cpp:156-250 x ghosts, Y_boundary_condition.cpp:48-120 and

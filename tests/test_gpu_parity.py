@@ -261,3 +261,34 @@ def test_gpu_sweep_schedules_identical(grid):
     np.testing.assert_array_equal(outs[0][1], outs[2][1])
     np.testing.assert_array_equal(outs[0][0], outs[2][0])
     np.testing.assert_allclose(outs[0][0], outs[1][0], rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("grid", VAR_GRIDS)
+def test_gpu_grid_and_filter(grid, golden):
+    """Grid (Y_point, stretching 2.8) and Filter(0.007, 1, 0) bit for bit
+    against the reference, on both the flat-density and the perturbed state,
+    and the sensor patterns at the pinning thresholds (each sample face
+    flips between a ratio r and the next double, so the GPU's ratio there is
+    the reference's bit for bit)."""
+    cfg = unit_cfg(grid)
+    pin = golden(f"fpin_g{grid[0]}x{grid[1]}x{grid[2]}")
+    sh = tuple(pin["pattern_shape"])
+    pats = np.unpackbits(pin["patterns"], axis=-1)[..., :sh[-1]].reshape(sh).astype(np.float64)
+    with solver(cfg) as s:
+        for prefix in ("unit", "var"):
+            z = golden(f"{prefix}_g{grid[0]}x{grid[1]}x{grid[2]}")
+            _, Yp = s.Grid(2.8)
+            m = masks(cfg)["bc"]
+            np.testing.assert_array_equal(Yp[m], z["Y_point"][m])
+            Ep = s.Filter(0.007, 1.0, 0.0, fields(z, "U_bc"), np.ascontiguousarray(z["Y_point"]),
+                          np.ascontiguousarray(z["J"]))
+            for a in range(3):
+                mk = masks(cfg)["xyz"[a]]
+                np.testing.assert_array_equal(Ep[a][mk], z["Ep" + "XYZ"[a]][mk], err_msg=f"{prefix} Ep{'XYZ'[a]}")
+        z = golden(f"var_g{grid[0]}x{grid[1]}x{grid[2]}")
+        for r, crit in enumerate(pin["crits"]):
+            Ep = s.Filter(float(crit), 1.0, 0.0, fields(z, "U_bc"), np.ascontiguousarray(z["Y_point"]),
+                          np.ascontiguousarray(z["J"]))
+            for a in range(3):
+                mk = masks(cfg)["xyz"[a]]
+                np.testing.assert_array_equal(Ep[a][mk], pats[r][a][mk], err_msg=f"run {r} Ep{'XYZ'[a]}")
