@@ -176,6 +176,21 @@ def main():
                                           if fields[k] and v > 0 else None)} for k, v in stage_ms.items()}
     log(f"stages: {json.dumps(per_stage)}")
 
+    # variant kernels off the dual-time path (Filter sensor, limiter fluxes) on
+    # the same resident state; algorithmic fields per cell: Filter reads rho
+    # and the momenta (4) and writes 3 sensors, each limiter flux reads U_ 5
+    # and its sensor, writes 5
+    s.Grid(2.8)
+    vt = {}
+    for _ in range(3):
+        for k, v in s.profile_variants(0.007).items():
+            vt.setdefault(k, []).append(v)
+    vfields = {"filter": 7, "flux_x_limiter": 11, "flux_y_limiter": 11, "flux_z_limiter": 11}
+    variants = {k: {"ms": round(float(np.median(v)), 4),
+                    "GBps_algorithmic": round(vfields[k] * 8 * cfg.cells / (float(np.median(v)) * 1e-3) / 1e9, 1)}
+                for k, v in vt.items()}
+    log(f"variants: {json.dumps(variants)}")
+
     conv = None
     if not args.no_converge:
         # one physical step with inner iterations until max_k er_k/er_k(1) <= 1e-8 (k = 1..4)
@@ -214,6 +229,7 @@ def main():
                    "parallelism": "single GPU"},
         "phys_step_to_1e-8": conv,
         "stages_ms": per_stage,
+        "variants_ms": variants,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -809,6 +809,38 @@ extern "C" int cf_profile_inner_iteration(cf_ctx* c, double er[5], double stage_
   return CF_OK;
 }
 
+extern "C" int cf_profile_variants(cf_ctx* c, double Roe_criterion, double ms[CF_VARIANT_STAGES]) {
+  if (!c || !ms) return CF_ERR_ARG;
+  if (!c->have_state) return fail(c, CF_ERR_STATE, "no state loaded");
+  const Geo& g = c->g;
+  if (!c->Fw) {
+    if (hipMalloc(&c->Fw, (size_t)g.nf * cf::FILTER_WORK_FIELDS * sizeof(double)) != hipSuccess)
+      return fail(c, CF_ERR_HIP, "hipMalloc of the filter workspace failed");
+  }
+  hipEvent_t ev[CF_VARIANT_STAGES + 1];
+  for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
+  // on the resident iterate with its ghosts set, as in an inner iteration
+  cf::launch_halo_bc(c->stream, c->Ui, c->prof, g, c->p.x_periodic);
+  HIPCHK(c, hipEventRecord(ev[0], c->stream));
+  cf::launch_filter(c->stream, c->Ui, c->prof, c->ypt, c->Fw, Roe_criterion, 1.0, 0.0, c->Ep, c->Ep + g.nf,
+                    c->Ep + 2 * g.nf, g);
+  HIPCHK(c, hipEventRecord(ev[1], c->stream));
+  double* F[3] = {c->Fx, c->Fy, c->Fz};
+  for (int d = 0; d < 3; d++) {
+    cf::launch_flux_limiter(c->stream, d, c->Ui, c->Ep + (long)d * g.nf, c->prof, F[d], g);
+    HIPCHK(c, hipEventRecord(ev[2 + d], c->stream));
+  }
+  int r = sync(c);
+  if (r) return r;
+  for (int q = 0; q < CF_VARIANT_STAGES; q++) {
+    float t = 0;
+    HIPCHK(c, hipEventElapsedTime(&t, ev[q], ev[q + 1]));
+    ms[q] = t;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  return CF_OK;
+}
+
 extern "C" int cf_set_sweep_schedule(cf_ctx* c, int schedule) {
   if (!c || schedule < 0 || schedule > 2) return fail(c, CF_ERR_ARG, "bad schedule");
   c->schedule = schedule;

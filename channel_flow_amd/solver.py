@@ -29,7 +29,7 @@ EXPORTS = [
     "cf_dual_time_stepping", "cf_load_state", "cf_load_initial", "cf_set_iterate", "cf_read_state",
     "cf_inner_iteration", "cf_physical_step", "cf_advance_levels", "cf_write_output", "cf_synchronize",
     "cf_profile_inner_iteration", "cf_set_sweep_schedule", "cf_flux_x_limiter", "cf_flux_y_limiter",
-    "cf_flux_z_limiter", "cf_grid", "cf_filter",
+    "cf_flux_z_limiter", "cf_grid", "cf_filter", "cf_profile_variants",
 ]
 
 
@@ -77,6 +77,7 @@ def load_library() -> C.CDLL:
     for n in ("cf_flux_x_limiter", "cf_flux_y_limiter", "cf_flux_z_limiter"):
         getattr(lib, n).argtypes = [vp, PP, P, P, PP, PP, P]
     lib.cf_grid.argtypes = [vp, C.c_double, P, P]
+    lib.cf_profile_variants.argtypes = [vp, C.c_double, P]
     lib.cf_filter.argtypes = [vp, C.c_double, C.c_double, C.c_double, PP, P, P, P, P, P]
     lib.cf_viscous_terms.argtypes = [vp, PP, PP, P, PP, P, PP, P, PP, P, PP]
     lib.cf_dual_time_stepping.argtypes = [vp, C.c_double, P, PP, PP, PP, P, PP, PP, PP, PP, PP, PP, PP, PP]
@@ -194,6 +195,14 @@ class Solver:
         out = self.zeros(5)
         self._chk(self.lib.cf_flux_z(self._h, _pp(U_), _p(J), _p(J_w), _pp(m_w), _pp(out)))
         return out
+
+    VARIANTS = ["filter", "flux_x_limiter", "flux_y_limiter", "flux_z_limiter"]
+
+    def profile_variants(self, Roe_criterion=0.007):
+        """HIP-event times (ms) of the variant kernels on the resident iterate."""
+        ms = (C.c_double * len(self.VARIANTS))()
+        self._chk(self.lib.cf_profile_variants(self._h, float(Roe_criterion), ms))
+        return {n: ms[q] for q, n in enumerate(self.VARIANTS)}
 
     def Grid(self, gamma=2.8):
         """Grid.cpp:50-141: (X_point, Y_point) in the reference layout."""

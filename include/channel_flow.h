@@ -173,6 +173,13 @@ int cf_write_output(cf_ctx* ctx, int step, const char* dir);
 #define CF_PROFILE_STAGES 11
 int cf_profile_inner_iteration(cf_ctx* ctx, double er[5], double stage_ms[CF_PROFILE_STAGES],
                                int launches[CF_PROFILE_STAGES]);
+/* Times the variant kernels on the resident iterate (after its ghosts are
+ * set) with HIP events on the context's stream: ms[0] Filter (sensor with
+ * E_high = 1, E_low = 0 into the device sensor arrays; cf_grid must have
+ * set Y_point), ms[1..3] Flux_X/Y/Z_limiter with those sensors.  No
+ * counterpart in the reference. */
+#define CF_VARIANT_STAGES 4
+int cf_profile_variants(cf_ctx* ctx, double Roe_criterion, double ms[CF_VARIANT_STAGES]);
 /* Visiting order of the LU-SGS sweeps; all give identical bits because each
  * cell depends only on its three upstream neighbours.  schedule 0 (default):
  * one persistent launch per sweep direction, 8x8-column tiles walking k in
