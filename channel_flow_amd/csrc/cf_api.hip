@@ -48,6 +48,7 @@ struct cf_ctx {
   int* sync = nullptr;    // ticket, error flag, per-tile progress (x2 directions)
   int ntiles = 0;
   int schedule = 0;       // 0: persistent tiles, 1: one hyperplane per launch
+  int solver = 0;         // implicit part: 0 LU-SGS (Dual_time_stepping), 1 DPLUSGS
   long spin_limit = 1L << 22;
   double* partial = nullptr;
   double* er_dev = nullptr;
@@ -604,8 +605,26 @@ static int finish_iteration(cf_ctx* c, double er[5]) {
 }
 
 // the implicit part of one inner iteration on device-resident fields
+// DPLUSGS on device-resident fields: the same RHS, four Jacobi sweeps, then
+// the update and norms (DPLUSGS.cpp:189-1361)
+static int dplusgs_device(cf_ctx* c, double deltaT, double er[5]) {
+  const Geo& g = c->g;
+  cf::launch_rhs(c->stream, c->Ui, c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->prof, c->MR, c->NL, g, deltaT,
+                 c->p.f_drive);
+  cf::launch_dplusgs(c->stream, c->Ui, c->MR, c->NL, c->NR, c->prof, g, deltaT);
+  double DN = 1. / ((double)c->p.X_out * c->p.Y_out * c->p.Z_out);
+  cf::launch_update(c->stream, c->Ui, c->NR, c->prof, c->partial, c->er_dev, g, DN);
+  HIPCHK(c, hipMemcpyAsync(er, c->er_dev, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  int r = sync(c);
+  if (r) return r;
+  for (int v = 0; v < 5; v++)
+    if (!std::isfinite(er[v])) return fail(c, CF_ERR_NONFINITE, "residual norm er%d is not finite", v + 1);
+  return CF_OK;
+}
+
 static int dual_time_device(cf_ctx* c, double deltaT, double er[5], bool want_nl) {
   const Geo& g = c->g;
+  if (c->solver == 1) return dplusgs_device(c, deltaT, er);
   cf::launch_rhs(c->stream, c->Ui, c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->prof, c->MR,
                  want_nl ? c->NL : nullptr, g, deltaT, c->p.f_drive);
   sweeps(c, deltaT);
@@ -617,7 +636,29 @@ static int dual_time_device(cf_ctx* c, double deltaT, double er[5], bool want_nl
   return finish_iteration(c, er);
 }
 
+static int implicit_entry(cf_ctx* c, int solver, double deltaT, double er[5], double* const U_[5],
+                          const double* const U[5], const double* const Uq[5], const double* J,
+                          const double* const m[9], const double* const inFx[5], const double* const inFy[5],
+                          const double* const inFz[5], const double* const vF[4], double* const MR[5],
+                          double* const NR[5], double* const NL[5]);
+
+extern "C" int cf_dplusgs(cf_ctx* c, double deltaT, double er[5], double* const U_[5], const double* const U[5],
+                          const double* const Uq[5], const double* J, const double* const m[9],
+                          const double* const inFx[5], const double* const inFy[5], const double* const inFz[5],
+                          const double* const vF[4], double* const MR[5], double* const NR[5], double* const NL[5]) {
+  return implicit_entry(c, 1, deltaT, er, U_, U, Uq, J, m, inFx, inFy, inFz, vF, MR, NR, NL);
+}
+
 extern "C" int cf_dual_time_stepping(cf_ctx* c, double deltaT, double er[5], double* const U_[5],
+                                     const double* const U[5], const double* const Uq[5], const double* J,
+                                     const double* const m[9], const double* const inFx[5],
+                                     const double* const inFy[5], const double* const inFz[5],
+                                     const double* const vF[4], double* const MR[5], double* const NR[5],
+                                     double* const NL[5]) {
+  return implicit_entry(c, 0, deltaT, er, U_, U, Uq, J, m, inFx, inFy, inFz, vF, MR, NR, NL);
+}
+
+static int implicit_entry(cf_ctx* c, int solver, double deltaT, double er[5], double* const U_[5],
                                      const double* const U[5], const double* const Uq[5], const double* J,
                                      const double* const m[9], const double* const inFx[5],
                                      const double* const inFy[5], const double* const inFz[5],
@@ -635,7 +676,11 @@ extern "C" int cf_dual_time_stepping(cf_ctx* c, double deltaT, double er[5], dou
   HIPCHK(c, hipMemsetAsync(c->NL, 0, (size_t)c->g.nf * 5 * 8, c->stream));
   HIPCHK(c, hipMemsetAsync(c->NR, 0, (size_t)c->g.nf * 5 * 8, c->stream));
   HIPCHK(c, hipMemsetAsync(c->MR, 0, (size_t)c->g.nf * 5 * 8, c->stream));
-  if ((r = dual_time_device(c, deltaT, er, true))) {
+  const int keep = c->solver;
+  c->solver = solver;
+  r = dual_time_device(c, deltaT, er, true);
+  c->solver = keep;
+  if (r) {
     d2h_n(c, U_, c->Ui, 5);
     sync(c);
     return r;
@@ -838,6 +883,12 @@ extern "C" int cf_profile_variants(cf_ctx* c, double Roe_criterion, double ms[CF
     ms[q] = t;
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
+  return CF_OK;
+}
+
+extern "C" int cf_set_implicit_solver(cf_ctx* c, int solver) {
+  if (!c || solver < 0 || solver > 1) return fail(c, CF_ERR_ARG, "bad solver");
+  c->solver = solver;
   return CF_OK;
 }
 

@@ -1393,6 +1393,68 @@ __global__ void __launch_bounds__(256) k_sweep_pipe(const double* __restrict__ S
   }
 }
 
+// ---------------------------------------------------------------------------
+// DPLUSGS (DPLUSGS.cpp:488-1140): point-Jacobi sweeps over all interior
+// cells at once, NR = D^-1 (MR - L NL - U NL) with the split Jacobians of
+// the neighbours; the upper neighbours' NL is read at j+1, k+1 (where the
+// reference reads their states; its NL indices j_, k_ are never assigned)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_jacobi(const double* __restrict__ Ui, const double* __restrict__ MR,
+                                                const double* __restrict__ NLin, double* __restrict__ NRout,
+                                                const double* __restrict__ prof, Geo g, double deltaT) {
+  const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int i = 3 + blockIdx.z;
+  if (k > g.nz || j > g.ny) return;
+  const long nf = g.nf;
+  const long c = at(g, i, j, k);
+  const double dd[3] = {g.dXI, g.dET, g.dZT};
+  double lo[3][5], hi[3][5];
+#pragma unroll
+  for (int side = 0; side < 2; side++) {
+#pragma unroll
+    for (int dir = 0; dir < 3; dir++) {
+      double(&acc)[5] = side == 0 ? lo[dir] : hi[dir];
+      const long st = dir == 0 ? g.plane : (dir == 1 ? (long)g.Zp : 1L);
+      const long nb = side == 0 ? c - st : c + st;
+      const bool inside = side == 0 ? (dir == 0 ? i - 1 >= 3 : dir == 1 ? j - 1 >= 2 : k - 1 >= 2)
+                                    : (dir == 0 ? i + 1 <= g.gend : dir == 1 ? j + 1 <= g.ny : k + 1 <= g.nz);
+      if (!inside) {
+        // ghost neighbours: NL is zero there
+#pragma unroll
+        for (int v = 0; v < 5; v++) acc[v] = 0.0;
+        continue;
+      }
+      const int jn = dir == 1 ? (side == 0 ? j - 1 : j + 1) : j;
+      double u5[5], N[5];
+      ld5(Ui, nf, nb, u5);
+      ld5(NLin, nf, nb, N);
+      PState s = prim_dt(u5[0], u5[1], u5[2], u5[3], u5[4], pf(prof, g.Ym, SET_C, 0, jn));
+      Split a = split_jac(s, pf(prof, g.Ym, SET_C, 1 + 3 * dir, jn), pf(prof, g.Ym, SET_C, 2 + 3 * dir, jn),
+                          pf(prof, g.Ym, SET_C, 3 + 3 * dir, jn), dir);
+      if (side == 0) lower_sum(a, N, acc); else upper_sum(a, N, acc);
+    }
+  }
+  double LL[5], UU[5];
+#pragma unroll
+  for (int v = 0; v < 5; v++) {
+    LL[v] = lo[0][v] / dd[0] + lo[1][v] / dd[1] + lo[2][v] / dd[2];
+    UU[v] = hi[0][v] / dd[0] + hi[1][v] / dd[1] + hi[2][v] / dd[2];
+  }
+  double u5[5], m9[9], d11, d22, d51;
+  ld5(Ui, nf, c, u5);
+  PState s0 = prim_dt(u5[0], u5[1], u5[2], u5[3], u5[4], pf(prof, g.Ym, SET_C, 0, j));
+  cell_metric(prof, g.Ym, j, m9);
+  dinv(s0, m9, g.dXI, g.dET, g.dZT, deltaT, &d11, &d22, &d51);
+  double M[5];
+  ld5(MR, nf, c, M);
+  NRout[c] = d11 * (M[0] - LL[0] - UU[0]);
+  NRout[nf + c] = d22 * (M[1] - LL[1] - UU[1]);
+  NRout[2 * nf + c] = d22 * (M[2] - LL[2] - UU[2]);
+  NRout[3 * nf + c] = d22 * (M[3] - LL[3] - UU[3]);
+  NRout[4 * nf + c] = d51 * (M[0] - LL[0] - UU[0]) + d22 * (M[4] - LL[4] - UU[4]);
+}
+
 __global__ void k_copy_interior(const double* __restrict__ src, double* __restrict__ dst, Geo g) {
   int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
   int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
@@ -1723,6 +1785,19 @@ void launch_sweeps_planes(hipStream_t st, const double* Ui, const double* MR, do
 void launch_copy_interior(hipStream_t st, const double* src, double* dst, const Geo& g) {
   dim3 blk(64, 4);
   hipLaunchKernelGGL(k_copy_interior, box_grid(g.nz - 1, g.ny - 1, g.gend - 2, blk), blk, 0, st, src, dst, g);
+}
+
+void launch_dplusgs(hipStream_t st, const double* Ui, const double* MR, double* NL, double* NR, const double* prof,
+                    const Geo& g, double deltaT) {
+  dim3 blk(64, 4);
+  dim3 gr = box_grid(g.nz - 1, g.ny - 1, g.gend - 2, blk);
+  // four sweeps, ping-pong NL -> NR -> NL -> NR -> NL; both end with the last
+  for (int sweep = 0; sweep < 4; sweep++) {
+    const double* in = (sweep & 1) ? NR : NL;
+    double* out = (sweep & 1) ? NL : NR;
+    hipLaunchKernelGGL(k_jacobi, gr, blk, 0, st, Ui, MR, in, out, prof, g, deltaT);
+  }
+  launch_copy_interior(st, NL, NR, g);
 }
 
 long update_blocks(const Geo& g) {

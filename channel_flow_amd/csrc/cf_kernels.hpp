@@ -37,6 +37,10 @@ void launch_sweep_outputs(hipStream_t st, const double* Sbuf, double* NL, double
 void launch_sweeps_planes(hipStream_t st, const double* Ui, const double* MR, double* NL, double* NR,
                           const double* prof, const Geo& g, double deltaT, int n_sweeps,
                           const hipEvent_t* marks = nullptr);
+// DPLUSGS: four point-Jacobi sweeps from NL (= D^-1 MR); NL and NR both end
+// with the last sweep's values (ghosts of both must be zero)
+void launch_dplusgs(hipStream_t st, const double* Ui, const double* MR, double* NL, double* NR, const double* prof,
+                    const Geo& g, double deltaT);
 void launch_copy_interior(hipStream_t st, const double* src, double* dst, const Geo& g);
 long update_blocks(const Geo& g);
 void launch_update(hipStream_t st, double* Ui, const double* NR, const double* prof, double* partial, double* er_dev,

@@ -29,7 +29,7 @@ EXPORTS = [
     "cf_dual_time_stepping", "cf_load_state", "cf_load_initial", "cf_set_iterate", "cf_read_state",
     "cf_inner_iteration", "cf_physical_step", "cf_advance_levels", "cf_write_output", "cf_synchronize",
     "cf_profile_inner_iteration", "cf_set_sweep_schedule", "cf_flux_x_limiter", "cf_flux_y_limiter",
-    "cf_flux_z_limiter", "cf_grid", "cf_filter", "cf_profile_variants",
+    "cf_flux_z_limiter", "cf_grid", "cf_filter", "cf_profile_variants", "cf_dplusgs", "cf_set_implicit_solver",
 ]
 
 
@@ -78,6 +78,8 @@ def load_library() -> C.CDLL:
         getattr(lib, n).argtypes = [vp, PP, P, P, PP, PP, P]
     lib.cf_grid.argtypes = [vp, C.c_double, P, P]
     lib.cf_profile_variants.argtypes = [vp, C.c_double, P]
+    lib.cf_dplusgs.argtypes = lib.cf_dual_time_stepping.argtypes
+    lib.cf_set_implicit_solver.argtypes = [vp, C.c_int]
     lib.cf_filter.argtypes = [vp, C.c_double, C.c_double, C.c_double, PP, P, P, P, P, P]
     lib.cf_viscous_terms.argtypes = [vp, PP, PP, P, PP, P, PP, P, PP, P, PP]
     lib.cf_dual_time_stepping.argtypes = [vp, C.c_double, P, PP, PP, PP, P, PP, PP, PP, PP, PP, PP, PP, PP]
@@ -251,6 +253,19 @@ class Solver:
                                                  _pp(met["m"]), _pp(inFx), _pp(inFy), _pp(inFz), _pp(vF), _pp(MR),
                                                  _pp(NR), _pp(NL)))
         return np.array(er[:]), MR, NR, NL
+
+    def DPLUSGS(self, U_, U, Uq, met, inFx, inFy, inFz, vF, deltaT=None):
+        """DPLUSGS.cpp:189-1361 (Jacobi sweeps); as Dual_time_stepping."""
+        er = (C.c_double * 5)()
+        MR, NR, NL = self.zeros(5), self.zeros(5), self.zeros(5)
+        dt = self.cfg.deltaT if deltaT is None else deltaT
+        self._chk(self.lib.cf_dplusgs(self._h, dt, er, _pp(U_), _pp(U), _pp(Uq), _p(met["J"]), _pp(met["m"]),
+                                      _pp(inFx), _pp(inFy), _pp(inFz), _pp(vF), _pp(MR), _pp(NR), _pp(NL)))
+        return np.array(er[:]), MR, NR, NL
+
+    def set_implicit_solver(self, solver: int = 0):
+        """0: LU-SGS (Dual_time_stepping, default); 1: DPLUSGS."""
+        self._chk(self.lib.cf_set_implicit_solver(self._h, int(solver)))
 
     # --- device-resident path --------------------------------------------------
     def load_state(self, Qn, Qnm1):

@@ -139,6 +139,17 @@ int cf_dual_time_stepping(cf_ctx* ctx, double deltaT, double er[5], double* cons
                           const double* const vF[4], double* const MR[5], double* const NR[5],
                           double* const NL[5]);
 
+/* DPLUSGS (main.h:766-870, DPLUSGS.cpp:189-1361): the RHS and D^-1 of
+ * Dual_time_stepping, then four point-Jacobi sweeps NR = D^-1 (MR - L NL -
+ * U NL), NL <- NR, then the same update and norms; same arguments as
+ * cf_dual_time_stepping.  The reference reads the upper neighbours' NL at
+ * indices j_, k_ it never assigns (undefined behaviour); this entry point
+ * reads them at j+1, k+1, where the reference reads their states. */
+int cf_dplusgs(cf_ctx* ctx, double deltaT, double er[5], double* const U_[5], const double* const U[5],
+               const double* const Uq[5], const double* J, const double* const m[9], const double* const inFx[5],
+               const double* const inFy[5], const double* const inFz[5], const double* const vF[4],
+               double* const MR[5], double* const NR[5], double* const NL[5]);
+
 /* --- device-resident path ------------------------------------------------ */
 /* Loads level n and n-1 from interior Q arrays ([X_out][Y_out][Z_out] per
  * variable, rho, rho u, rho v, rho w, rho E; Initial_condition.cpp:87-116)
@@ -173,6 +184,9 @@ int cf_write_output(cf_ctx* ctx, int step, const char* dir);
 #define CF_PROFILE_STAGES 11
 int cf_profile_inner_iteration(cf_ctx* ctx, double er[5], double stage_ms[CF_PROFILE_STAGES],
                                int launches[CF_PROFILE_STAGES]);
+/* Implicit part of cf_inner_iteration / cf_physical_step: 0 (default)
+ * Dual_time_stepping's LU-SGS sweeps, 1 DPLUSGS's Jacobi sweeps. */
+int cf_set_implicit_solver(cf_ctx* ctx, int solver);
 /* Times the variant kernels on the resident iterate (after its ghosts are
  * set) with HIP events on the context's stream: ms[0] Filter (sensor with
  * E_high = 1, E_low = 0 into the device sensor arrays; cf_grid must have

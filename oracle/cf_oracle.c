@@ -832,15 +832,63 @@ static void bwd_cell(const sweep_ctx* x, int i, int j, int k) {
   x->NR[4][c] = x->NL[4][c] - (d51 * UUv[0] + d22 * UUv[4]);
 }
 
-static void dual_time_impl(const cfo_grid* g, double deltaT, double* const U_[5], double* const U[5],
-                           double* const Uq[5], const double* J, double* const m[9], double* const inFx[5],
-                           double* const inFy[5], double* const inFz[5], double* const vF[4],
-                           double* const MR[5], double* const NL[5], double* const NR[5], double er[5],
-                           int hyperplane) {
-  dims_t d = dims(g);
-  int gend = d.gend, ny = d.ny, nz = d.nz, nyy = ny + 1, nzz = nz + 1;
-  size_t sI = (size_t)d.Ym * d.Zm, sJ = (size_t)d.Zm;
-  /* zero implicit coupling through every ghost layer (:195-316) */
+/* DPLUSGS.cpp:573-1121: one point-Jacobi update of a cell from the previous
+ * iterate NL on both sides, NR = D^-1 (MR - LL - UU).  The upper neighbours'
+ * states and metrics are read at j+1 and k+1 (:874-887, :931-938); their NL
+ * is read at j_ and k_, which the reference never assigns (:995-1051), and
+ * is taken at j+1 and k+1 here. */
+static void jacobi_cell(const sweep_ctx* x, int i, int j, int k) {
+  const dims_t d = x->d;
+  size_t c = IX(d, i, j, k);
+  size_t lo[3] = {c - (size_t)d.Ym * d.Zm, c - (size_t)d.Zm, c - 1};
+  size_t hi[3] = {c + (size_t)d.Ym * d.Zm, c + (size_t)d.Zm, c + 1};
+  double lsum[3][5], usum[3][5];
+  for (int dir = 0; dir < 3; dir++) {
+    pstate_t s = prim_dt(x->U_, x->J, lo[dir]);
+    double n[3] = {x->m[3 * dir][lo[dir]], x->m[3 * dir + 1][lo[dir]], x->m[3 * dir + 2][lo[dir]]};
+    split_t a = split_jac(&s, n);
+    split_p5(&a, &s, n, dir);
+    double N1 = x->NL[0][lo[dir]], N2 = x->NL[1][lo[dir]], N3 = x->NL[2][lo[dir]], N4 = x->NL[3][lo[dir]],
+           N5 = x->NL[4][lo[dir]];
+    lsum[dir][0] = -a.p11 * N1 - a.p12 * N2 - a.p13 * N3 - a.p14 * N4;
+    lsum[dir][1] = -a.p21 * N1 - a.p22 * N2;
+    lsum[dir][2] = -a.p31 * N1 - a.p33 * N3;
+    lsum[dir][3] = -a.p41 * N1 - a.p44 * N4;
+    lsum[dir][4] = -a.p51 * N1 - a.p52 * N2 - a.p53 * N3 - a.p54 * N4 - a.p55 * N5;
+  }
+  for (int dir = 0; dir < 3; dir++) {
+    pstate_t s = prim_dt(x->U_, x->J, hi[dir]);
+    double n[3] = {x->m[3 * dir][hi[dir]], x->m[3 * dir + 1][hi[dir]], x->m[3 * dir + 2][hi[dir]]};
+    split_t a = split_jac(&s, n);
+    split_p5(&a, &s, n, dir);
+    double N1 = x->NL[0][hi[dir]], N2 = x->NL[1][hi[dir]], N3 = x->NL[2][hi[dir]], N4 = x->NL[3][hi[dir]],
+           N5 = x->NL[4][hi[dir]];
+    usum[dir][0] = +a.m11 * N1 + a.p12 * N2 + a.p13 * N3 + a.p14 * N4;
+    usum[dir][1] = +a.p21 * N1 + a.m22 * N2;
+    usum[dir][2] = +a.p31 * N1 + a.m33 * N3;
+    usum[dir][3] = +a.p41 * N1 + a.m44 * N4;
+    usum[dir][4] = +a.p51 * N1 + a.p52 * N2 + a.p53 * N3 + a.p54 * N4 + a.m55 * N5;
+  }
+  double LLv[5], UUv[5];
+  for (int v = 0; v < 5; v++) {
+    LLv[v] = lsum[0][v] / x->g->deltaXI + lsum[1][v] / x->g->deltaET + lsum[2][v] / x->g->deltaZT;
+    UUv[v] = usum[0][v] / x->g->deltaXI + usum[1][v] / x->g->deltaET + usum[2][v] / x->g->deltaZT;
+  }
+  pstate_t s0 = prim_dt(x->U_, x->J, c);
+  double m9[9], d11, d22, d51;
+  load_m9(x->m, c, m9);
+  dinv(&s0, m9, x->g, x->deltaT, &d11, &d22, &d51);
+  x->NR[0][c] = d11 * (x->MR[0][c] - LLv[0] - UUv[0]);
+  x->NR[1][c] = d22 * (x->MR[1][c] - LLv[1] - UUv[1]);
+  x->NR[2][c] = d22 * (x->MR[2][c] - LLv[2] - UUv[2]);
+  x->NR[3][c] = d22 * (x->MR[3][c] - LLv[3] - UUv[3]);
+  x->NR[4][c] = d51 * (x->MR[0][c] - LLv[0] - UUv[0]) + d22 * (x->MR[4][c] - LLv[4] - UUv[4]);
+}
+
+/* zero implicit coupling through every ghost layer
+ * (Dual_time_stepping.cpp:195-316) */
+static void zero_sweep_ghosts(dims_t d, double* const NL[5], double* const NR[5]) {
+  int gend = d.gend, nyy = d.ny + 1, nzz = d.nz + 1;
   for (int v = 0; v < 5; v++) {
     for (int j = 1; j <= nyy; j++)
       for (int k = 1; k <= nzz; k++) {
@@ -858,6 +906,78 @@ static void dual_time_impl(const cfo_grid* g, double deltaT, double* const U_[5]
       }
     }
   }
+}
+
+/* NL = D^-1 MR at one cell (Dual_time_stepping.cpp:480-507); NR = NL for
+ * the LU-SGS sweeps (mode 0, 1), left alone for the Jacobi sweeps (mode 2) */
+static void init_nl_cell(const cfo_grid* g, dims_t d, double deltaT, double* const U_[5], const double* J,
+                         double* const m[9], double* const MR[5], double* const NL[5], double* const NR[5],
+                         size_t c, int mode) {
+  pstate_t s = prim_dt(U_, J, c);
+  double m9[9], e11, e22, e51;
+  load_m9(m, c, m9);
+  dinv(&s, m9, g, deltaT, &e11, &e22, &e51);
+  NL[0][c] = e11 * MR[0][c];
+  NL[1][c] = e22 * MR[1][c];
+  NL[2][c] = e22 * MR[2][c];
+  NL[3][c] = e22 * MR[3][c];
+  NL[4][c] = e51 * MR[0][c] + e22 * MR[4][c];
+  if (mode != 2)
+    for (int q = 0; q < 5; q++) NR[q][c] = NL[q][c];
+  (void)d;
+}
+
+/* symmetric Gauss-Seidel sweeps (Dual_time_stepping.cpp:514-1277): mode 0
+ * lexicographic, mode 1 plane by plane; mode 2: the four Jacobi sweeps of
+ * DPLUSGS.cpp:488-1140.  Each sweep ends with NL <- NR. */
+static void run_sweeps(const sweep_ctx* x, int mode) {
+  const dims_t d = x->d;
+  const int gend = d.gend, ny = d.ny, nz = d.nz, mmax = gend + ny + nz;
+  const int nsw = mode == 2 ? 4 : x->g->n_sweeps;
+  double* const* NL = x->NL;
+  double* const* NR = x->NR;
+  for (int sweep = 1; sweep <= nsw; sweep++) {
+    if (mode == 2) {
+      for (int i = 3; i <= gend; i++)
+        for (int j = 2; j <= ny; j++)
+          for (int k = 2; k <= nz; k++) jacobi_cell(x, i, j, k);
+    } else if (mode == 0) {
+      for (int i = 3; i <= gend; i++)
+        for (int j = 2; j <= ny; j++)
+          for (int k = 2; k <= nz; k++) fwd_cell(x, i, j, k);
+      for (int i = gend; i >= 3; i--)
+        for (int j = ny; j >= 2; j--)
+          for (int k = nz; k >= 2; k--) bwd_cell(x, i, j, k);
+    } else {
+      for (int mm = 7; mm <= mmax; mm++)
+        for (int i = 3; i <= gend; i++)
+          for (int j = 2; j <= ny; j++) {
+            int k = mm - i - j;
+            if (k >= 2 && k <= nz) fwd_cell(x, i, j, k);
+          }
+      for (int mm = mmax; mm >= 7; mm--)
+        for (int i = 3; i <= gend; i++)
+          for (int j = 2; j <= ny; j++) {
+            int k = mm - i - j;
+            if (k >= 2 && k <= nz) bwd_cell(x, i, j, k);
+          }
+    }
+    for (int q = 0; q < 5; q++)
+      for (int i = 3; i <= gend; i++)
+        for (int j = 2; j <= ny; j++)
+          for (int k = 2; k <= nz; k++) NL[q][IX(d, i, j, k)] = NR[q][IX(d, i, j, k)];
+  }
+}
+
+static void dual_time_impl(const cfo_grid* g, double deltaT, double* const U_[5], double* const U[5],
+                           double* const Uq[5], const double* J, double* const m[9], double* const inFx[5],
+                           double* const inFy[5], double* const inFz[5], double* const vF[4],
+                           double* const MR[5], double* const NL[5], double* const NR[5], double er[5],
+                           int hyperplane) {
+  dims_t d = dims(g);
+  int gend = d.gend, ny = d.ny, nz = d.nz;
+  size_t sI = (size_t)d.Ym * d.Zm, sJ = (size_t)d.Zm;
+  zero_sweep_ghosts(d, NL, NR);
   /* unsteady residual, preconditioned primitive transform and D^-1 (:338-507) */
   for (int i = 3; i <= gend; i++)
     for (int j = 2; j <= ny; j++)
@@ -896,46 +1016,10 @@ static void dual_time_impl(const cfo_grid* g, double deltaT, double* const U_[5]
         MR[2][c] = (d31 * Rk[0] + 0 * Rk[1] + d22 * Rk[2] + 0 * Rk[3] + 0 * Rk[4]);
         MR[3][c] = (d41 * Rk[0] + 0 * Rk[1] + 0 * Rk[2] + d22 * Rk[3] + 0 * Rk[4]);
         MR[4][c] = (d51 * Rk[0] + d52 * Rk[1] + d53 * Rk[2] + d54 * Rk[3] + d55 * Rk[4]);
-        double m9[9], e11, e22, e51;
-        load_m9(m, c, m9);
-        dinv(&s, m9, g, deltaT, &e11, &e22, &e51);
-        NL[0][c] = e11 * MR[0][c];
-        NL[1][c] = e22 * MR[1][c];
-        NL[2][c] = e22 * MR[2][c];
-        NL[3][c] = e22 * MR[3][c];
-        NL[4][c] = e51 * MR[0][c] + e22 * MR[4][c];
-        for (int q = 0; q < 5; q++) NR[q][c] = NL[q][c];
+        init_nl_cell(g, d, deltaT, U_, J, m, MR, NL, NR, c, hyperplane);
       }
-  /* symmetric Gauss-Seidel sweeps (:514-1277) */
   sweep_ctx x = {g, d, deltaT, U_, J, m, MR, NL, NR};
-  int mmax = gend + ny + nz;
-  for (int sweep = 1; sweep <= g->n_sweeps; sweep++) {
-    if (!hyperplane) {
-      for (int i = 3; i <= gend; i++)
-        for (int j = 2; j <= ny; j++)
-          for (int k = 2; k <= nz; k++) fwd_cell(&x, i, j, k);
-      for (int i = gend; i >= 3; i--)
-        for (int j = ny; j >= 2; j--)
-          for (int k = nz; k >= 2; k--) bwd_cell(&x, i, j, k);
-    } else {
-      for (int mm = 7; mm <= mmax; mm++)
-        for (int i = 3; i <= gend; i++)
-          for (int j = 2; j <= ny; j++) {
-            int k = mm - i - j;
-            if (k >= 2 && k <= nz) fwd_cell(&x, i, j, k);
-          }
-      for (int mm = mmax; mm >= 7; mm--)
-        for (int i = 3; i <= gend; i++)
-          for (int j = 2; j <= ny; j++) {
-            int k = mm - i - j;
-            if (k >= 2 && k <= nz) bwd_cell(&x, i, j, k);
-          }
-    }
-    for (int q = 0; q < 5; q++)
-      for (int i = 3; i <= gend; i++)
-        for (int j = 2; j <= ny; j++)
-          for (int k = 2; k <= nz; k++) NL[q][IX(d, i, j, k)] = NR[q][IX(d, i, j, k)];
-  }
+  run_sweeps(&x, hyperplane);
   /* primitive update and residual norms (:1290-1352) */
   double e[5] = {0, 0, 0, 0, 0};
   double DN = 1. / ((double)g->X_out * g->Y_out * g->Z_out);
@@ -981,6 +1065,24 @@ void cfo_dual_time_hyperplane(const cfo_grid* g, double deltaT, double* const U_
                               double* const inFy[5], double* const inFz[5], double* const vF[4],
                               double* const MR[5], double* const NL[5], double* const NR[5], double er[5]) {
   dual_time_impl(g, deltaT, U_, U, Uq, J, m, inFx, inFy, inFz, vF, MR, NL, NR, er, 1);
+}
+
+void cfo_dplusgs(const cfo_grid* g, double deltaT, double* const U_[5], double* const U[5], double* const Uq[5],
+                 const double* J, double* const m[9], double* const inFx[5], double* const inFy[5],
+                 double* const inFz[5], double* const vF[4], double* const MR[5], double* const NL[5],
+                 double* const NR[5], double er[5]) {
+  dual_time_impl(g, deltaT, U_, U, Uq, J, m, inFx, inFy, inFz, vF, MR, NL, NR, er, 2);
+}
+
+void cfo_sweeps_apply(const cfo_grid* g, double deltaT, int mode, double* const U_[5], const double* J,
+                      double* const m[9], double* const MR[5], double* const NL[5], double* const NR[5]) {
+  dims_t d = dims(g);
+  zero_sweep_ghosts(d, NL, NR);
+  for (int i = 3; i <= d.gend; i++)
+    for (int j = 2; j <= d.ny; j++)
+      for (int k = 2; k <= d.nz; k++) init_nl_cell(g, d, deltaT, U_, J, m, MR, NL, NR, IX(d, i, j, k), mode);
+  sweep_ctx x = {g, d, deltaT, U_, J, m, MR, NL, NR};
+  run_sweeps(&x, mode);
 }
 
 /* ------------------------------------------------------------------------ */

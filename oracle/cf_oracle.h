@@ -74,6 +74,25 @@ void cfo_filter(const cfo_grid* g, double crit, double E_high, double E_low, dou
 void cfo_filter_ratios(const cfo_grid* g, double* const U_[5], const double* Y_point, const double* J,
                        double* RX, double* RY, double* RZ);
 
+/* DPLUSGS.cpp:189-1361: the RHS and D^-1 of Dual_time_stepping, then four
+ * point-Jacobi sweeps NR = D^-1 (MR - L NL - U NL), NL <- NR, then the
+ * same update and norms.  The reference reads the upper neighbours' NL at
+ * indices it never assigns (UB); this restatement reads them at j+1, k+1,
+ * where it reads their states and metrics.  Parity with the reference is
+ * not pinned for the sweeps (the RHS, D^-1 and update are those of
+ * cfo_dual_time). */
+void cfo_dplusgs(const cfo_grid* g, double deltaT, double* const U_[5], double* const U[5], double* const Uq[5],
+                 const double* J, double* const m[9], double* const inFx[5], double* const inFy[5],
+                 double* const inFz[5], double* const vF[4], double* const MR[5], double* const NL[5],
+                 double* const NR[5], double er[5]);
+
+/* The implicit sweeps alone, on a given MR: ghosts of NL/NR zeroed,
+ * NL = D^-1 MR, then mode 0 (LU-SGS, lexicographic), 1 (LU-SGS, plane by
+ * plane) or 2 (DPLUSGS Jacobi).  A test hook: it lets the tests read the
+ * sweeps as linear maps of MR. */
+void cfo_sweeps_apply(const cfo_grid* g, double deltaT, int mode, double* const U_[5], const double* J,
+                      double* const m[9], double* const MR[5], double* const NL[5], double* const NR[5]);
+
 /* Viscous_terms.cpp:212-883; writes vF[0..3] = vF2..vF5 */
 void cfo_viscous(const cfo_grid* g, double* const U_[5], const double* J, const double* Ju,
                  double* const mu[9], const double* Jv, double* const mv[9], const double* Jw,

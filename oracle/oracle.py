@@ -55,6 +55,8 @@ def lib():
         dt_args = [C.POINTER(Grid), C.c_double, PP, PP, PP, P, PP, PP, PP, PP, PP, PP, PP, PP, P]
         _lib.cfo_dual_time.argtypes = dt_args
         _lib.cfo_dual_time_hyperplane.argtypes = dt_args
+        _lib.cfo_dplusgs.argtypes = dt_args
+        _lib.cfo_sweeps_apply.argtypes = [C.POINTER(Grid), C.c_double, C.c_int, PP, P, PP, PP, PP, PP]
     return _lib
 
 
@@ -142,25 +144,37 @@ def viscous(cfg, U_, met):
     return out
 
 
-def dual_time(cfg, U_, U, Uq, met, inFx, inFy, inFz, vF, hyperplane=False, deltaT=None):
-    """Updates U_ in place; returns (er[5], MR, NL, NR)."""
+def dual_time(cfg, U_, U, Uq, met, inFx, inFy, inFz, vF, hyperplane=False, deltaT=None, dplusgs=False):
+    """Updates U_ in place; returns (er[5], MR, NL, NR).  dplusgs: the
+    DPLUSGS Jacobi sweeps instead of LU-SGS."""
     MR, NL, NR = zeros(cfg, 5), zeros(cfg, 5), zeros(cfg, 5)
     er = (C.c_double * 5)()
-    fn = lib().cfo_dual_time_hyperplane if hyperplane else lib().cfo_dual_time
+    fn = lib().cfo_dplusgs if dplusgs else (lib().cfo_dual_time_hyperplane if hyperplane else lib().cfo_dual_time)
     fn(C.byref(grid_of(cfg)), cfg.deltaT if deltaT is None else deltaT, _pp(U_), _pp(U), _pp(Uq),
        _p(met["J"]), _pp(met["m"]), _pp(inFx), _pp(inFy), _pp(inFz), _pp(vF), _pp(MR), _pp(NL), _pp(NR), er)
     return np.array(er[:]), MR, NL, NR
 
 
-def inner_iteration(cfg, U_, U, Uq, met):
+def sweeps_apply(cfg, mode, U_, met, MR, deltaT=None):
+    """The implicit sweeps alone on a given MR (0: LU-SGS lexicographic,
+    1: LU-SGS by planes, 2: DPLUSGS Jacobi); returns NR."""
+    MR = [np.ascontiguousarray(a, dtype=np.float64) for a in MR]
+    NL, NR = zeros(cfg, 5), zeros(cfg, 5)
+    lib().cfo_sweeps_apply(C.byref(grid_of(cfg)), cfg.deltaT if deltaT is None else deltaT, int(mode), _pp(U_),
+                           _p(met["J"]), _pp(met["m"]), _pp(MR), _pp(NL), _pp(NR))
+    return NR
+
+
+def inner_iteration(cfg, U_, U, Uq, met, dplusgs=False):
     """One inner iteration of the dual-time path (main.cpp:156-353 with the DT
-    calls enabled): ghosts, three fluxes, viscous terms, dual time step."""
+    calls enabled): ghosts, three fluxes, viscous terms, dual time step
+    (dplusgs: DPLUSGS instead of Dual_time_stepping)."""
     halo_bc(cfg, U_, met["J"])
     inFx = flux(cfg, 0, U_, met["J"], met["J_u"], met["m_u"])
     inFy = flux(cfg, 1, U_, met["J"], met["J_v"], met["m_v"])
     inFz = flux(cfg, 2, U_, met["J"], met["J_w"], met["m_w"])
     vF = viscous(cfg, U_, met)
-    er, MR, NL, NR = dual_time(cfg, U_, U, Uq, met, inFx, inFy, inFz, vF)
+    er, MR, NL, NR = dual_time(cfg, U_, U, Uq, met, inFx, inFy, inFz, vF, dplusgs=dplusgs)
     return er
 
 

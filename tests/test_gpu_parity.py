@@ -150,6 +150,56 @@ def test_gpu_dual_time(grid, golden):
     np.testing.assert_allclose(er, z["er_dt"], rtol=TOL_KERNEL, atol=0)
 
 
+@pytest.mark.parametrize("grid", UNIT_GRIDS)
+def test_gpu_dplusgs(grid, golden):
+    """DPLUSGS: MR bit for bit against the reference's golden RHS; the four
+    Jacobi sweeps, NL/NR and the update bit for bit against the oracle; the
+    norms to 1e-12 (tree reduction on the GPU)."""
+    from oracle import oracle as O
+    z = golden(f"unit_g{grid[0]}x{grid[1]}x{grid[2]}")
+    cfg = unit_cfg(grid)
+    args = (fields(z, "U"), fields(z, "Uq"), fixture_metrics(z), fields(z, "inFx"), fields(z, "inFy"),
+            fields(z, "inFz"), fields(z, "vF", 4))
+    U_o = [a.copy() for a in fields(z, "U_bc")]
+    er_o, MR_o, NL_o, NR_o = O.dual_time(cfg, U_o, *args, deltaT=5e-8, dplusgs=True)
+    U_ = [a.copy() for a in fields(z, "U_bc")]
+    with solver(cfg) as s:
+        er, MR, NR, NL = s.DPLUSGS(U_, *args, deltaT=5e-8)
+    m = masks(cfg)["int"]
+    for v in range(5):
+        np.testing.assert_array_equal(MR[v][m], z[f"MR_dt{v + 1}"][m])
+        np.testing.assert_array_equal(NR[v][m], NR_o[v][m])
+        np.testing.assert_array_equal(NL[v][m], NL_o[v][m])
+        np.testing.assert_array_equal(U_[v][m], U_o[v][m])
+    np.testing.assert_allclose(er, er_o, rtol=TOL_KERNEL, atol=0)
+
+
+def test_gpu_inner_iteration_dplusgs_16x12x16():
+    """The device-resident inner iteration with the DPLUSGS solver selected,
+    three iterations against the oracle: the iterate bit for bit, the norms
+    to 1e-12."""
+    from oracle import oracle as O
+    from channel_flow_amd.config import ChannelConfig, FlowParams, poiseuille_state
+    from channel_flow_amd.solver import Solver
+    cfg = ChannelConfig(16, 12, 16, gamma_metric=1.5, deltaT=5e-8)
+    Qn = poiseuille_state(cfg, FlowParams(), seed=7, stream=0)
+    Qm = poiseuille_state(cfg, FlowParams(), seed=7, stream=1)
+    met = O.metrics(cfg)
+    U, Uq = O.state_from_Q(cfg, Qn, met["J"]), O.state_from_Q(cfg, Qm, met["J"])
+    Ui = [a.copy() for a in U]
+    er_o = [O.inner_iteration(cfg, Ui, U, Uq, met, dplusgs=True) for _ in range(3)]
+    sl = (slice(3, cfg.gend + 1), slice(2, cfg.ny + 1), slice(2, cfg.nz + 1))
+    qi_o = np.stack([Ui[v][sl] * met["J"][sl] for v in range(5)])
+    with Solver(cfg) as s:
+        s.set_implicit_solver(1)
+        s.Jacobian_all(1.5, want_arrays=False)
+        s.load_state(Qn, Qm)
+        er = [s.inner_iteration() for _ in range(3)]
+        qi = s.read_state()[2]
+    np.testing.assert_array_equal(qi, qi_o)
+    np.testing.assert_allclose(np.array(er), np.array(er_o), rtol=TOL_KERNEL, atol=0)
+
+
 def gpu_trajectory(cfg, Qn, Qm, steps):
     rows = []
     with solver(cfg) as s:
