@@ -78,11 +78,11 @@ def load_library() -> C.CDLL:
         getattr(lib, n).argtypes = [vp, PP, P, P, PP, PP, P]
     lib.cf_grid.argtypes = [vp, C.c_double, P, P]
     lib.cf_profile_variants.argtypes = [vp, C.c_double, P]
-    lib.cf_dplusgs.argtypes = lib.cf_dual_time_stepping.argtypes
     lib.cf_set_implicit_solver.argtypes = [vp, C.c_int]
     lib.cf_filter.argtypes = [vp, C.c_double, C.c_double, C.c_double, PP, P, P, P, P, P]
     lib.cf_viscous_terms.argtypes = [vp, PP, PP, P, PP, P, PP, P, PP, P, PP]
     lib.cf_dual_time_stepping.argtypes = [vp, C.c_double, P, PP, PP, PP, P, PP, PP, PP, PP, PP, PP, PP, PP]
+    lib.cf_dplusgs.argtypes = lib.cf_dual_time_stepping.argtypes
     lib.cf_load_state.argtypes = [vp, PP, PP]
     lib.cf_load_initial.argtypes = [vp, C.c_char_p]
     lib.cf_set_iterate.argtypes = [vp, PP]

@@ -37,6 +37,29 @@ def test_python_mirror_binds_every_export():
     assert sorted(EXPORTS) == declared_functions()
 
 
+def declared_arity():
+    """name -> number of parameters, from the prototypes in the header"""
+    txt = (ROOT / "include" / "channel_flow.h").read_text()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    out = {}
+    for name, params in re.findall(r"(cf_[a-z0-9_]+)\s*\(([^;{]*?)\)\s*;", txt, flags=re.S):
+        params = params.strip()
+        out[name] = 0 if params in ("", "void") else params.count(",") + 1
+    return out
+
+
+def test_python_mirror_declares_every_signature():
+    """Each bound export carries argtypes of the header's arity, so ctypes
+    converts doubles and pointers instead of guessing."""
+    from channel_flow_amd.solver import EXPORTS, load_library
+    lib = load_library()
+    arity = declared_arity()
+    for n in EXPORTS:
+        at = getattr(lib, n).argtypes
+        assert at is not None, n
+        assert len(at) == arity[n], (n, len(at), arity[n])
+
+
 def test_solver_refuses_without_library(tmp_path, monkeypatch):
     """The product path fails loudly instead of falling back to anything."""
     import channel_flow_amd.solver as S
