@@ -49,6 +49,7 @@ struct cf_ctx {
   int ntiles = 0;
   int schedule = 0;       // 0: persistent tiles, 1: one hyperplane per launch
   int solver = 0;         // implicit part: 0 LU-SGS (Dual_time_stepping), 1 DPLUSGS
+  int visc_schedule = 0;  // 0: fused viscous kernel, 1: three kernels through Pr/VS (allocated on first use)
   long spin_limit = 1L << 22;
   double* partial = nullptr;
   double* er_dev = nullptr;
@@ -209,8 +210,7 @@ extern "C" int cf_create(const cf_params* p, cf_ctx** out) {
     double** ptr;
     int n;
   } fields[] = {{&c->Ui, 5}, {&c->Un, 5}, {&c->Uq, 5}, {&c->Fx, 5}, {&c->Fy, 5}, {&c->Fz, 5},
-                {&c->vF, 4}, {&c->MR, 5}, {&c->NL, 5}, {&c->NR, 5}, {&c->Pr, 4}, {&c->VS, 12},
-                {&c->Ep, 3}};
+                {&c->vF, 4}, {&c->MR, 5}, {&c->NL, 5}, {&c->NR, 5}, {&c->Ep, 3}};
   for (auto& f : fields) {
     if (hipMalloc(f.ptr, nf * f.n * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
     // the reference leaves work arrays uninitialised; here they start at zero
@@ -539,6 +539,43 @@ extern "C" int cf_filter(cf_ctx* c, double Roe_criterion, double E_high, double 
   return sync(c);
 }
 
+// the fused viscous kernel drops the coefficient products of the off-diagonal
+// face metrics, which must then be exactly zero in all three face sets
+static bool diagonal_face_metrics(const cf_ctx* c) {
+  const double* P = c->hprof.data();
+  const int off[6] = {2, 3, 4, 6, 7, 8};  // xidy, xidz, etdx, etdz, ztdx, ztdy
+  for (int set = cf::SET_U; set <= cf::SET_W; set++)
+    for (int q = 0; q < 6; q++)
+      for (int j = 0; j < c->g.Ym; j++)
+        if (P[(size_t)(set * 10 + off[q]) * c->g.Ym + j] != 0.0) return false;
+  return true;
+}
+
+static int ensure_viscous_staging(cf_ctx* c) {
+  if (c->Pr) return CF_OK;
+  const size_t nf = (size_t)c->g.nf;
+  if (hipMalloc(&c->Pr, nf * 4 * sizeof(double)) != hipSuccess ||
+      hipMalloc(&c->VS, nf * 12 * sizeof(double)) != hipSuccess)
+    return fail(c, CF_ERR_HIP, "out of device memory for the viscous staging fields");
+  HIPCHK(c, hipMemsetAsync(c->Pr, 0, nf * 4 * sizeof(double), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->VS, 0, nf * 12 * sizeof(double), c->stream));
+  return CF_OK;
+}
+
+// Viscous_terms.cpp:212-883 on the device state: the fused kernel for
+// diagonal metrics (schedule 0), otherwise the three staged kernels
+static bool viscous_fused(const cf_ctx* c) { return c->visc_schedule == 0 && diagonal_face_metrics(c); }
+
+static int run_viscous(cf_ctx* c) {
+  const bool fused = viscous_fused(c);
+  if (!fused) {
+    int r = ensure_viscous_staging(c);
+    if (r) return r;
+  }
+  cf::launch_viscous(c->stream, c->Ui, c->prof, c->Pr, c->VS, c->vF, c->g, fused);
+  return CF_OK;
+}
+
 extern "C" int cf_viscous_terms(cf_ctx* c, const double* const U_[5], double* const vF[4], const double* J,
                                 const double* const m[9], const double* J_u, const double* const m_u[9],
                                 const double* J_v, const double* const m_v[9], const double* J_w,
@@ -551,7 +588,7 @@ extern "C" int cf_viscous_terms(cf_ctx* c, const double* const U_[5], double* co
   (void)m;
   if ((r = h2d_n(c, c->Ui, U_, 5))) return r;
   HIPCHK(c, hipMemsetAsync(c->vF, 0, (size_t)c->g.nf * 4 * 8, c->stream));
-  cf::launch_viscous(c->stream, c->Ui, c->prof, c->Pr, c->VS, c->vF, c->g);
+  if (int rv = run_viscous(c)) return rv;
   if ((r = d2h_n(c, vF, c->vF, 4))) return r;
   return sync(c);
 }
@@ -785,7 +822,7 @@ extern "C" int cf_inner_iteration(cf_ctx* c, double er[5]) {
   cf::launch_flux(c->stream, 0, c->Ui, c->prof, c->Fx, g);
   cf::launch_flux(c->stream, 1, c->Ui, c->prof, c->Fy, g);
   cf::launch_flux(c->stream, 2, c->Ui, c->prof, c->Fz, g);
-  cf::launch_viscous(c->stream, c->Ui, c->prof, c->Pr, c->VS, c->vF, g);
+  if (int rv = run_viscous(c)) return rv;
   return dual_time_device(c, c->p.deltaT, er, false);
 }
 
@@ -818,7 +855,7 @@ extern "C" int cf_profile_inner_iteration(cf_ctx* c, double er[5], double stage_
   HIPCHK(c, hipEventRecord(ev[3], c->stream));
   cf::launch_flux(c->stream, 2, c->Ui, c->prof, c->Fz, g);
   HIPCHK(c, hipEventRecord(ev[4], c->stream));
-  cf::launch_viscous(c->stream, c->Ui, c->prof, c->Pr, c->VS, c->vF, g);
+  if (int rv = run_viscous(c)) return rv;
   HIPCHK(c, hipEventRecord(ev[5], c->stream));
   cf::launch_rhs(c->stream, c->Ui, c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->prof, c->MR, nullptr, g,
                  c->p.deltaT, c->p.f_drive);
@@ -847,7 +884,7 @@ extern "C" int cf_profile_inner_iteration(cf_ctx* c, double er[5], double stage_
   if ((r = span(mk[nm - 1], ev[7 + nm], &stage_ms[10]))) return r;
   const bool tiled = effective_schedule(c) == 0;
   const int nplanes = g.gend + g.ny + g.nz - 6;
-  const int nl[CF_PROFILE_STAGES] = {3, 1, 1, 1, 3, 1, tiled ? 1 : 0, tiled ? ns : 0, tiled ? ns : nplanes * ns,
+  const int nl[CF_PROFILE_STAGES] = {3, 1, 1, 1, viscous_fused(c) ? 1 : 3, 1, tiled ? 1 : 0, tiled ? ns : 0, tiled ? ns : nplanes * ns,
                                      tiled ? ns : nplanes * ns, 2};
   for (int q = 0; q < CF_PROFILE_STAGES; q++) launches[q] = nl[q];
   for (auto& e : ev) (void)hipEventDestroy(e);
@@ -889,6 +926,12 @@ extern "C" int cf_profile_variants(cf_ctx* c, double Roe_criterion, double ms[CF
 extern "C" int cf_set_implicit_solver(cf_ctx* c, int solver) {
   if (!c || solver < 0 || solver > 1) return fail(c, CF_ERR_ARG, "bad solver");
   c->solver = solver;
+  return CF_OK;
+}
+
+extern "C" int cf_set_viscous_schedule(cf_ctx* c, int schedule) {
+  if (!c || schedule < 0 || schedule > 1) return fail(c, CF_ERR_ARG, "bad viscous schedule");
+  c->visc_schedule = schedule;
   return CF_OK;
 }
 

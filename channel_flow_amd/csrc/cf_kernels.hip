@@ -460,6 +460,211 @@ __global__ void __launch_bounds__(256) k_visc_div(const double* __restrict__ VS,
   }
 }
 
+// Fused viscous terms for diagonal metrics.  One block owns a VJ (j) x VK (k) tile of cells and a
+// run of VC planes in i.  The primitives of three consecutive planes, with a
+// one-cell rim in j and k, live in LDS.  Each of the VJ*VK main threads
+// computes the x face above its cell and the y and z faces below it; a
+// fifth wave computes the tile's extra y-face row (j0+VJ) and z-face column
+// (k0+VK).  The x face below a cell is the one computed at the previous
+// plane.  The face arithmetic and the divergence are those of k_visc_faces
+// and k_visc_div, operand for operand, so vF is unchanged bit for bit while
+// the primitives and the twelve face-flux fields never reach HBM.
+constexpr int VJ = 8, VK = 32, VC = 16, VR = VJ + 2, VKK = VK + 2, VTHREADS = VJ * VK + 64;
+
+struct VLds {
+  double P[3][4][VR][VKK];
+  double ML[4][VJ + 1][VK];
+  double NL[4][VJ][VK + 1];
+};
+
+__device__ __forceinline__ Prim vp(const VLds& L, int s, int r, int c) {
+  Prim q;
+  q.U = L.P[s][0][r][c];
+  q.V = L.P[s][1][r][c];
+  q.W = L.P[s][2][r][c];
+  q.T = L.P[s][3][r][c];
+  return q;
+}
+
+__device__ __forceinline__ void vload_plane(VLds& L, const double* __restrict__ U, const double* __restrict__ prof,
+                                            const Geo& g, int ip, int s, int j0, int k0, int tid) {
+  for (int idx = tid; idx < VR * VKK; idx += VTHREADS) {
+    const int r = idx / VKK, c = idx - r * VKK;
+    const int jj = j0 - 1 + r, kk = k0 - 1 + c;
+    Prim q{0., 0., 0., 0.};
+    if (jj <= g.ny + 1 && kk <= g.nz + 1) {
+      const long p = at(g, ip, jj, kk);
+      q = prim_visc(U[p], U[g.nf + p], U[2 * g.nf + p], U[3 * g.nf + p], U[4 * g.nf + p],
+                    pf(prof, g.Ym, SET_C, 0, jj));
+    }
+    L.P[s][0][r][c] = q.U;
+    L.P[s][1][r][c] = q.V;
+    L.P[s][2][r][c] = q.W;
+    L.P[s][3][r][c] = q.T;
+  }
+}
+
+// The three faces for diagonal metrics (xidy = xidz = etdx = etdz = ztdx =
+// ztdy = 0 in every face set).  Of the coefficient groups a..f of
+// Viscous_terms.cpp:471-526 only a1..a4, b1..b4, c1..c4, d5, d7, e6, e9, f8
+// and f10 are then non-zero; the fluxes of Viscous_terms.cpp:533-547,
+// 681-695 and 827-841 keep their remaining terms in the reference's order, so
+// each result differs from the full sum at most in the sign of a zero.  Each
+// face needs only the tangential derivatives that meet a non-zero
+// coefficient, which is also why the B6 typo (it alters dy.uv, dy.vw, dy.uw
+// of the z face) does not reach the z-face fluxes.
+struct DiagCoef {
+  double X, E, Z;  // xidx, etdy, ztdz of the face set
+};
+
+__device__ __forceinline__ DiagCoef diag_face(const double* __restrict__ prof, int Ym, int set, int j) {
+  DiagCoef q;
+  q.X = pf(prof, Ym, set, 1, j);
+  q.E = pf(prof, Ym, set, 5, j);
+  q.Z = pf(prof, Ym, set, 9, j);
+  return q;
+}
+
+__device__ __forceinline__ double lp(const VLds& L, int s, int f, int r, int c) { return L.P[s][f][r][c]; }
+
+// x face between planes (slot sm) and (slot s0) at LDS cell (r, c), row j
+__device__ __forceinline__ void vface_x(const VLds& L, const double* __restrict__ prof, const Geo& g, int s0, int sm,
+                                        int r, int c, int j, double invXI, double inv4ET, double inv4ZT,
+                                        double o[4]) {
+  const Prim P0 = vp(L, s0, r, c), iP = vp(L, sm, r, c);
+  const DSet dx = d_two(P0, iP, invXI);
+  const double dyu = (lp(L, s0, 0, r + 1, c) + lp(L, sm, 0, r + 1, c) - lp(L, s0, 0, r - 1, c) - lp(L, sm, 0, r - 1, c)) * inv4ET;
+  const double dyv = (lp(L, s0, 1, r + 1, c) + lp(L, sm, 1, r + 1, c) - lp(L, s0, 1, r - 1, c) - lp(L, sm, 1, r - 1, c)) * inv4ET;
+  const double dzu = (lp(L, s0, 0, r, c + 1) + lp(L, sm, 0, r, c + 1) - lp(L, s0, 0, r, c - 1) - lp(L, sm, 0, r, c - 1)) * inv4ZT;
+  const double dzw = (lp(L, s0, 2, r, c + 1) + lp(L, sm, 2, r, c + 1) - lp(L, s0, 2, r, c - 1) - lp(L, sm, 2, r, c - 1)) * inv4ZT;
+  const DiagCoef m = diag_face(prof, g.Ym, SET_U, j - 1);
+  const double Jf = pf(prof, g.Ym, SET_U, 0, j - 1);
+  const double a1 = (4. / 3) * m.X * m.X, a2 = m.X * m.X, a3 = m.X * m.X, a4 = m.X * m.X;
+  const double d5 = m.X * m.E, d7 = -((2. / 3) * m.X * m.E);
+  const double e6 = m.X * m.Z, e9 = -((2. / 3) * m.X * m.Z);
+  const double Ux = 0.5 * (P0.U + iP.U), Vx = 0.5 * (P0.V + iP.V), Wx = 0.5 * (P0.W + iP.W);
+  o[0] = MU_L * (a1 * dx.u + d7 * dyv + e9 * dzw) / Jf;
+  o[1] = MU_L * (a2 * dx.v + d5 * dyu) / Jf;
+  o[2] = MU_L * (a3 * dx.w + e6 * dzu) / Jf;
+  o[3] = MU_L * (0.5 * a1 * dx.uu + 0.5 * a2 * dx.vv + 0.5 * a3 * dx.ww + CF_CVK_PR(a4, dx.T) + d5 * Vx * dyu +
+                 d7 * Ux * dyv + e6 * Wx * dzu + e9 * Ux * dzw) /
+         Jf;
+}
+
+// y face between rows r-1 and r of plane slot s0
+__device__ __forceinline__ void vface_y(const VLds& L, const double* __restrict__ prof, const Geo& g, int sp, int s0,
+                                        int sm, int r, int c, int j, double invET, double inv4XI, double inv4ZT,
+                                        double o[4]) {
+  const Prim P0 = vp(L, s0, r, c), jP = vp(L, s0, r - 1, c);
+  const DSet dy = d_two(P0, jP, invET);
+  const double dxu = (lp(L, sp, 0, r, c) + lp(L, sp, 0, r - 1, c) - lp(L, sm, 0, r, c) - lp(L, sm, 0, r - 1, c)) * inv4XI;
+  const double dxv = (lp(L, sp, 1, r, c) + lp(L, sp, 1, r - 1, c) - lp(L, sm, 1, r, c) - lp(L, sm, 1, r - 1, c)) * inv4XI;
+  const double dzv = (lp(L, s0, 1, r, c + 1) + lp(L, s0, 1, r - 1, c + 1) - lp(L, s0, 1, r, c - 1) - lp(L, s0, 1, r - 1, c - 1)) * inv4ZT;
+  const double dzw = (lp(L, s0, 2, r, c + 1) + lp(L, s0, 2, r - 1, c + 1) - lp(L, s0, 2, r, c - 1) - lp(L, s0, 2, r - 1, c - 1)) * inv4ZT;
+  const DiagCoef m = diag_face(prof, g.Ym, SET_V, j - 1);
+  const double Jf = pf(prof, g.Ym, SET_V, 0, j - 1);
+  const double b1 = m.E * m.E, b2 = (4. / 3) * m.E * m.E, b3 = m.E * m.E, b4 = m.E * m.E;
+  const double d5 = m.X * m.E, d7 = -((2. / 3) * m.X * m.E);
+  const double f8 = m.E * m.Z, f10 = -((2. / 3) * m.E * m.Z);
+  const double Uy = 0.5 * (P0.U + jP.U), Vy = 0.5 * (P0.V + jP.V), Wy = 0.5 * (P0.W + jP.W);
+  o[0] = MU_L * (d5 * dxv + b1 * dy.u) / Jf;
+  o[1] = MU_L * (d7 * dxu + b2 * dy.v + f10 * dzw) / Jf;
+  o[2] = MU_L * (b3 * dy.w + f8 * dzv) / Jf;
+  o[3] = MU_L * (0.5 * b1 * dy.uu + 0.5 * b2 * dy.vv + 0.5 * b3 * dy.ww + CF_CVK_PR(b4, dy.T) + d5 * Uy * dxv +
+                 d7 * Vy * dxu + f8 * Wy * dzv + f10 * Vy * dzw) /
+         Jf;
+}
+
+// z face between columns c-1 and c of plane slot s0
+__device__ __forceinline__ void vface_z(const VLds& L, const double* __restrict__ prof, const Geo& g, int sp, int s0,
+                                        int sm, int r, int c, int j, double invZT, double inv4XI, double inv4ET,
+                                        double o[4]) {
+  const Prim P0 = vp(L, s0, r, c), kP = vp(L, s0, r, c - 1);
+  const DSet dz = d_two(P0, kP, invZT);
+  const double dxu = (lp(L, sp, 0, r, c) + lp(L, sp, 0, r, c - 1) - lp(L, sm, 0, r, c) - lp(L, sm, 0, r, c - 1)) * inv4XI;
+  const double dxw = (lp(L, sp, 2, r, c) + lp(L, sp, 2, r, c - 1) - lp(L, sm, 2, r, c) - lp(L, sm, 2, r, c - 1)) * inv4XI;
+  const double dyv = (lp(L, s0, 1, r + 1, c) + lp(L, s0, 1, r + 1, c - 1) - lp(L, s0, 1, r - 1, c) - lp(L, s0, 1, r - 1, c - 1)) * inv4ET;
+  const double dyw = (lp(L, s0, 2, r + 1, c) + lp(L, s0, 2, r + 1, c - 1) - lp(L, s0, 2, r - 1, c) - lp(L, s0, 2, r - 1, c - 1)) * inv4ET;
+  const DiagCoef m = diag_face(prof, g.Ym, SET_W, j - 1);
+  const double Jf = pf(prof, g.Ym, SET_W, 0, j - 1);
+  const double c1 = m.Z * m.Z, c2 = m.Z * m.Z, c3 = (4. / 3) * m.Z * m.Z, c4 = m.Z * m.Z;
+  const double e6 = m.X * m.Z, e9 = -((2. / 3) * m.X * m.Z);
+  const double f8 = m.E * m.Z, f10 = -((2. / 3) * m.E * m.Z);
+  const double Uz = 0.5 * (P0.U + kP.U), Vz = 0.5 * (P0.V + kP.V), Wz = 0.5 * (P0.W + kP.W);
+  o[0] = MU_L * (e6 * dxw + c1 * dz.u) / Jf;
+  o[1] = MU_L * (f8 * dyw + c2 * dz.v) / Jf;
+  o[2] = MU_L * (e9 * dxu + f10 * dyv + c3 * dz.w) / Jf;
+  o[3] = MU_L * (0.5 * c1 * dz.uu + 0.5 * c2 * dz.vv + 0.5 * c3 * dz.ww + CF_CVK_PR(c4, dz.T) + e6 * Uz * dxw +
+                 e9 * Wz * dxu + f8 * Vz * dyw + f10 * Wz * dyv) /
+         Jf;
+}
+
+__global__ void __launch_bounds__(VTHREADS) k_visc_fused(const double* __restrict__ U,
+                                                         const double* __restrict__ prof, double* __restrict__ vF,
+                                                         Geo g, double invXI, double invET, double invZT,
+                                                         double inv4XI, double inv4ET, double inv4ZT) {
+  __shared__ VLds L;
+  const int tid = threadIdx.x;
+  const int k0 = 2 + blockIdx.x * VK, j0 = 2 + blockIdx.y * VJ;
+  const int ib = 3 + blockIdx.z * VC, ie = min(ib + VC - 1, g.gend);
+  const bool main_t = tid < VJ * VK;
+  const int ty = main_t ? tid / VK : 0, tx = main_t ? tid % VK : 0;
+  const int j = j0 + ty, k = k0 + tx;
+  const bool cell = main_t && j <= g.ny && k <= g.nz;
+  const bool doML = main_t && j <= g.ny + 1 && k <= g.nz;
+  const bool doNL = main_t && j <= g.ny && k <= g.nz + 1;
+  // the fifth wave: lanes [0,VK) the y faces of row j0+VJ, lanes [VK,VK+VJ)
+  // the z faces of column k0+VK
+  const int e = tid - VJ * VK;
+  const bool exML = !main_t && e < VK && j0 + VJ <= g.ny + 1 && k0 + e <= g.nz;
+  const bool exNL = !main_t && e >= VK && e < VK + VJ && j0 + (e - VK) <= g.ny && k0 + VK <= g.nz + 1;
+  const long nf = g.nf;
+
+  vload_plane(L, U, prof, g, ib - 1, (ib - 1) % 3, j0, k0, tid);
+  vload_plane(L, U, prof, g, ib, ib % 3, j0, k0, tid);
+  __syncthreads();
+  double LLc[4] = {0., 0., 0., 0.};
+  if (cell) vface_x(L, prof, g, ib % 3, (ib - 1) % 3, ty + 1, tx + 1, j, invXI, inv4ET, inv4ZT, LLc);
+  for (int i = ib; i <= ie; i++) {
+    const int sm = (i - 1) % 3, s0 = i % 3, sp = (i + 1) % 3;
+    vload_plane(L, U, prof, g, i + 1, sp, j0, k0, tid);
+    __syncthreads();
+    double LLn[4] = {0., 0., 0., 0.}, o[4];
+    if (cell) vface_x(L, prof, g, sp, s0, ty + 1, tx + 1, j, invXI, inv4ET, inv4ZT, LLn);
+    if (doML) {
+      vface_y(L, prof, g, sp, s0, sm, ty + 1, tx + 1, j, invET, inv4XI, inv4ZT, o);
+#pragma unroll
+      for (int v = 0; v < 4; v++) L.ML[v][ty][tx] = o[v];
+    }
+    if (doNL) {
+      vface_z(L, prof, g, sp, s0, sm, ty + 1, tx + 1, j, invZT, inv4XI, inv4ET, o);
+#pragma unroll
+      for (int v = 0; v < 4; v++) L.NL[v][ty][tx] = o[v];
+    }
+    if (exML) {
+      vface_y(L, prof, g, sp, s0, sm, VJ + 1, e + 1, j0 + VJ, invET, inv4XI, inv4ZT, o);
+#pragma unroll
+      for (int v = 0; v < 4; v++) L.ML[v][VJ][e] = o[v];
+    }
+    if (exNL) {
+      const int r = e - VK;
+      vface_z(L, prof, g, sp, s0, sm, r + 1, VK + 1, j0 + r, invZT, inv4XI, inv4ET, o);
+#pragma unroll
+      for (int v = 0; v < 4; v++) L.NL[v][r][VK] = o[v];
+    }
+    __syncthreads();
+    if (cell) {
+      const long c = at(g, i, j, k);
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        vF[v * nf + c] = (LLn[v] - LLc[v]) * invXI + (L.ML[v][ty + 1][tx] - L.ML[v][ty][tx]) * invET +
+                         (L.NL[v][ty][tx + 1] - L.NL[v][ty][tx]) * invZT;
+        LLc[v] = LLn[v];
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // right-hand side: Dual_time_stepping.cpp:338-507
 // ---------------------------------------------------------------------------
@@ -1665,10 +1870,16 @@ void launch_filter(hipStream_t st, const double* U, const double* prof, const do
 }
 
 void launch_viscous(hipStream_t st, const double* U, const double* prof, double* Pr, double* VS, double* vF,
-                    const Geo& g) {
+                    const Geo& g, bool fused_viscous) {
   dim3 blk(64, 4);
   double invXI = 1. / (g.dXI), invET = 1. / (g.dET), invZT = 1. / (g.dZT);
   double inv4XI = 1. / (4 * g.dXI), inv4ET = 1. / (4 * g.dET), inv4ZT = 1. / (4 * g.dZT);
+  if (fused_viscous) {
+    dim3 gr((g.nz - 1 + VK - 1) / VK, (g.ny - 1 + VJ - 1) / VJ, (g.gend - 2 + VC - 1) / VC);
+    hipLaunchKernelGGL(k_visc_fused, gr, dim3(VTHREADS), 0, st, U, prof, vF, g, invXI, invET, invZT, inv4XI, inv4ET,
+                       inv4ZT);
+    return;
+  }
   hipLaunchKernelGGL(k_visc_prim, box_grid(g.nz + 1, g.ny + 1, g.gend, blk), blk, 0, st, U, prof, Pr, g);
   hipLaunchKernelGGL(k_visc_faces, box_grid(g.nz, g.ny, g.gend - 1, blk), blk, 0, st, Pr, prof, VS, g, invXI, invET,
                      invZT, inv4XI, inv4ET, inv4ZT);

@@ -14,8 +14,11 @@ void launch_flux_limiter(hipStream_t st, int dir, const double* U, const double*
 constexpr int FILTER_WORK_FIELDS = 22;
 void launch_filter(hipStream_t st, const double* U, const double* prof, const double* Yp, double* W, double crit,
                    double E_high, double E_low, double* EpX, double* EpY, double* EpZ, const Geo& g);
+// fused: one kernel with the primitives and face fluxes in LDS (default);
+// otherwise primitives, face fluxes and divergence as three kernels through
+// the staging fields Pr (4) and VS (12)
 void launch_viscous(hipStream_t st, const double* U, const double* prof, double* Pr, double* VS, double* vF,
-                    const Geo& g);
+                    const Geo& g, bool fused = true);
 void launch_rhs(hipStream_t st, const double* Ui, const double* Un, const double* Uq, const double* Fx,
                 const double* Fy, const double* Fz, const double* vF, const double* prof, double* MR, double* NL,
                 const Geo& g, double deltaT, double f_drive);

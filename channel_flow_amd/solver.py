@@ -30,6 +30,7 @@ EXPORTS = [
     "cf_inner_iteration", "cf_physical_step", "cf_advance_levels", "cf_write_output", "cf_synchronize",
     "cf_profile_inner_iteration", "cf_set_sweep_schedule", "cf_flux_x_limiter", "cf_flux_y_limiter",
     "cf_flux_z_limiter", "cf_grid", "cf_filter", "cf_profile_variants", "cf_dplusgs", "cf_set_implicit_solver",
+    "cf_set_viscous_schedule",
 ]
 
 
@@ -92,6 +93,7 @@ def load_library() -> C.CDLL:
     lib.cf_advance_levels.argtypes = [vp]
     lib.cf_profile_inner_iteration.argtypes = [vp, P, P, C.POINTER(C.c_int)]
     lib.cf_set_sweep_schedule.argtypes = [vp, C.c_int]
+    lib.cf_set_viscous_schedule.argtypes = [vp, C.c_int]
     lib.cf_write_output.argtypes = [vp, C.c_int, C.c_char_p]
     lib.cf_synchronize.argtypes = [vp]
     _lib = lib
@@ -322,6 +324,11 @@ class Solver:
         """0: pipelined tiled sweep (default); 1: one launch per hyperplane;
         2: single-wave tiles.  All three give the same bits."""
         self._chk(self.lib.cf_set_sweep_schedule(self._h, int(schedule)))
+
+    def set_viscous_schedule(self, schedule: int = 0):
+        """0: one fused viscous kernel (default); 1: three kernels through
+        staging fields.  Both give the same bits."""
+        self._chk(self.lib.cf_set_viscous_schedule(self._h, int(schedule)))
 
     def synchronize(self):
         self._chk(self.lib.cf_synchronize(self._h))

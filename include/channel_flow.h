@@ -202,6 +202,12 @@ int cf_profile_variants(cf_ctx* ctx, double Roe_criterion, double ms[CF_VARIANT_
  * Hyperplane.cpp:50-78); schedule 2: the tiles of schedule 0 as single waves
  * that load their own inputs. */
 int cf_set_sweep_schedule(cf_ctx* ctx, int schedule);
+/* Kernel schedule of Viscous_terms (Viscous_terms.cpp:212-883); both give
+ * identical bits.  0 (default): one kernel that keeps the primitives and
+ * face fluxes of a tile in LDS; 1: primitives, face fluxes and divergence as
+ * three kernels through 16 staging fields in HBM (allocated here on first
+ * selection, like the reference's staging arrays). */
+int cf_set_viscous_schedule(cf_ctx* ctx, int schedule);
 /* Synchronises the context's stream. */
 int cf_synchronize(cf_ctx* ctx);
 
