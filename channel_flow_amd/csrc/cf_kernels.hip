@@ -1230,7 +1230,7 @@ __global__ void __launch_bounds__(64) k_sweep_tiles(const double* __restrict__ S
 #define CF_PIPE_QS 3
 #endif
 #ifndef CF_PIPE_QD
-#define CF_PIPE_QD 4
+#define CF_PIPE_QD 2
 #endif
 constexpr int PIPE_QS = CF_PIPE_QS;  // own-field lookahead (steps)
 constexpr int PIPE_QD = CF_PIPE_QD;  // halo lookahead (steps)
@@ -1242,13 +1242,24 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-struct PipeLds {
+struct PipeData {
   double own[3][PO_N][64];  // D11 D22 D51, MR or NL (5), z SplitCoef (6) of the lane cells
   double xc[3][6][72];      // x SplitCoef: lanes, then the x halo by b
   double yc[3][6][72];      // y SplitCoef: lanes, then the y halo by a
   double N[2][5][80];       // outputs of the previous step: lanes, x halo by b (64+), y halo by a (72+)
   int ticket;
 };
+
+// One tile per CU.  The block reserves more than half of the CU's 160 KB of
+// LDS so that no second sweep block shares the CU: co-resident tiles slow
+// each other's compute wave, which is on the critical path, by more than the
+// extra tiles in flight gain (256x192x256, one forward sweep: 3.46 ms with
+// three tiles per CU, 2.79 ms with two, 2.24 ms with one).
+constexpr int PIPE_LDS_BYTES = 82 * 1024;
+struct PipeLds : PipeData {
+  char reserve[PIPE_LDS_BYTES - sizeof(PipeData)];
+};
+static_assert(sizeof(PipeLds) > 80 * 1024, "one sweep block per CU");
 
 __device__ __forceinline__ void coef_store(double (&row)[6][72], int col, const SplitCoef& c) {
   row[0][col] = c.c11;
@@ -1376,7 +1387,7 @@ __device__ void pipe_compute(PipeLds& L, double* __restrict__ SRC, double* __res
 // only for the oldest slot; the guarded remainder runs after it.
 template <int Q, class Consume, class Issue>
 __device__ __forceinline__ void pipe_loader_loop(int nsteps, Consume consume, Issue issue) {
-  static_assert(Q >= 1 && Q <= 6, "lookahead");
+  static_assert(Q >= 1 && Q <= 8, "lookahead");
   int t0 = 0;
   for (; t0 + Q <= nsteps; t0 += Q) {
 #pragma unroll
@@ -1477,24 +1488,28 @@ __device__ void pipe_own_loader(PipeLds& L, const double* __restrict__ S, const 
   pipe_loader_loop<QS>(nsteps, consume, issue);
 }
 
-// wave 3: the halo of the next step QD steps ahead.  Lanes 0..7 serve the
-// x-edge consumers (b = lane), lanes 8..15 the y-edge consumers (a = lane-8);
-// each loads its remote cell's summary along the direction (7 values) and
-// NL/NR (5 values).  The other lanes load their own tile's cells (discarded).
+// wave 3: the halo of the next step QD steps ahead.  There are 16 consumer
+// lanes of the compute wave: 0..7 on the x edge (b = c) and 8..15 on the y
+// edge (a = c - 8).  Each needs its remote cell's summary along the
+// direction (P T beta U lam pd p51) and NL/NR (5 values).  Lane l loads
+// values 3g..3g+2 (g = l >> 4) for consumer l & 15, so all 64 lanes carry
+// three values per step; after the sentinel poll, lanes 0..15 gather their
+// consumer's twelve values with cross-lane shuffles.
 template <bool FWD, int QD>
 __device__ void pipe_halo_loader(PipeLds& L, const double* __restrict__ S, const double* __restrict__ DST,
                                  const double* __restrict__ prof, const Geo& g, const Sk& q, int tile, int TX,
                                  int TY, int* err, long spin_limit) {
   const int lane = threadIdx.x & 63;
+  const int cid = lane & 15, grp = lane >> 4;
   const int nsteps = q.nsteps, Nz = q.Nz;
   const long nf = q.nsk;
-  const int dir = lane < 8 ? 0 : 1;
-  const int idx = lane & 7;
+  const int dir = cid < 8 ? 0 : 1;
+  const int idx = cid & 7;
   const int a = dir == 0 ? (FWD ? 0 : TI - 1) : idx;  // consumer lane (a, b)
   const int b = dir == 0 ? idx : (FWD ? 0 : TJ - 1);
   const int i = 3 + TX * TI + a, j = 2 + TY * TJ + b;
   const bool dom = dir == 0 ? (FWD ? i - 1 >= 3 : i + 1 <= g.gend) : (FWD ? j - 1 >= 2 : j + 1 <= g.ny);
-  const bool ok = lane < 16 && i <= g.gend && j <= g.ny && dom;
+  const bool ok = i <= g.gend && j <= g.ny && dom;
   const int off = FWD ? a + b : (TI - 1 - a) + (TJ - 1 - b);
   const long sx = FWD ? (-(long)nsteps * 64 + 7 * 64 + 7) : ((long)nsteps * 64 - 7 * 64 - 7);
   const long sy = FWD ? (-(long)q.nTI * nsteps * 64 + 7 * 64 + 56) : ((long)q.nTI * nsteps * 64 - 7 * 64 - 56);
@@ -1513,52 +1528,64 @@ __device__ void pipe_halo_loader(PipeLds& L, const double* __restrict__ S, const
     const int sc = min(sig, nsteps - 1);
     return own0 + (live(sig) ? rdiff : 0L) + (long)(FWD ? sc : nsteps - 1 - sc) * 64;
   };
-  // values: P T beta U lam pd p51 (summary), N0..N4 (dynamic)
-  auto field = [&](int f) -> const double* {
-    if (f < 3) return S + (long)(G_P + f) * nf;
-    if (f < 7) return S + (long)(dU + f - 3) * nf;
-    return DST + (long)(f - 7) * nf;
-  };
-  Slots<QD, 12> buf;
+  // this lane's three values f = 3 grp + r: P T beta U lam pd p51 N0..N4
+  const double* fld[3];
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    const int f = 3 * grp + r;
+    fld[r] = f < 3 ? S + (long)(G_P + f) * nf : (f < 7 ? S + (long)(dU + f - 3) * nf : DST + (long)(f - 7) * nf);
+  }
+  const bool dyn[3] = {3 * grp >= 7, 3 * grp + 1 >= 7, 3 * grp + 2 >= 7};
+  Slots<QD, 3> buf;
   auto issue = [&](int sig, int sl) {
     const long c = cell(sig);
-    double d[12];
+    double d[3];
 #pragma unroll
-    for (int f = 0; f < 12; f++) d[f] = ld_coherent(field(f) + c);
+    for (int r = 0; r < 3; r++) d[r] = ld_coherent(fld[r] + c);
     buf.put(sl, d);
   };
   auto consume = [&](int sig, int sl) {
     if (sig >= nsteps) return;
-    double d[12];
+    double d[3];
     buf.get(sl, d);
-    if (live(sig)) {
-      long cnt = 0;
-      const long c = cell(sig);
-      bool pend = false;
+    const bool lv = live(sig);
+    bool pend = false;
 #pragma unroll
-      for (int v = 0; v < 5; v++) pend |= is_sentinel(d[7 + v]);
-      while (pend) {
-        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || ++cnt > spin_limit) {
-          __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
+    for (int r = 0; r < 3; r++) pend |= lv && dyn[r] && is_sentinel(d[r]);
+    long cnt = 0;
+    while (__any(pend)) {
+      const int e = __builtin_amdgcn_readfirstlane(__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      if (e != 0 || ++cnt > spin_limit) {
+        if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      if (pend) {
+        const long c = cell(sig);
         pend = false;
 #pragma unroll
-        for (int v = 0; v < 5; v++) {
-          d[7 + v] = ld_coherent(field(7 + v) + c);
-          pend |= is_sentinel(d[7 + v]);
-        }
+        for (int r = 0; r < 3; r++)
+          if (dyn[r]) {
+            d[r] = ld_coherent(fld[r] + c);
+            pend |= is_sentinel(d[r]);
+          }
       }
+    }
+    double v[12];
+#pragma unroll
+    for (int gq = 0; gq < 4; gq++)
+#pragma unroll
+      for (int r = 0; r < 3; r++) v[3 * gq + r] = __shfl(d[r], cid + 16 * gq, 64);
+    if (lane < 16 && lv) {
       const int sp = (sig + 2) % 3, np = (sig & 1) ^ 1;
       if (dir == 0) {
-        coef_store(L.xc[sp], 64 + idx, diag_coef<0, FWD>(d[0], d[1], d[2], d[3], d[4], d[5], d[6], n));
+        coef_store(L.xc[sp], 64 + idx, diag_coef<0, FWD>(v[0], v[1], v[2], v[3], v[4], v[5], v[6], n));
 #pragma unroll
-        for (int v = 0; v < 5; v++) L.N[np][v][64 + idx] = d[7 + v];
+        for (int w = 0; w < 5; w++) L.N[np][w][64 + idx] = v[7 + w];
       } else {
-        coef_store(L.yc[sp], 64 + idx, diag_coef<1, FWD>(d[0], d[1], d[2], d[3], d[4], d[5], d[6], n));
+        coef_store(L.yc[sp], 64 + idx, diag_coef<1, FWD>(v[0], v[1], v[2], v[3], v[4], v[5], v[6], n));
 #pragma unroll
-        for (int v = 0; v < 5; v++) L.N[np][v][72 + idx] = d[7 + v];
+        for (int w = 0; w < 5; w++) L.N[np][w][72 + idx] = v[7 + w];
       }
     }
   };
