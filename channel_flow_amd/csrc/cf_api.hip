@@ -19,7 +19,7 @@
 using cf::Geo;
 
 namespace {
-constexpr int NPROF = 40;  // 4 sets x (J + 9 metrics)
+constexpr int NPROF = cf::PROF_METRIC_ROWS;  // 4 sets x (J + 9 metrics), uploaded from the host
 }
 
 struct cf_ctx {
@@ -154,6 +154,8 @@ static int ingest(cf_ctx* c, const double* A, int set, int comp, const char* nam
 
 static int upload_prof(cf_ctx* c) {
   HIPCHK(c, hipMemcpyAsync(c->prof, c->hprof.data(), c->hprof.size() * 8, hipMemcpyHostToDevice, c->stream));
+  cf::launch_flux_tables(c->stream, c->prof, c->g);
+  HIPCHK(c, hipGetLastError());
   return CF_OK;
 }
 
@@ -195,9 +197,10 @@ extern "C" int cf_create(const cf_params* p, cf_ctx** out) {
   g.dXI = p->deltaXI;
   g.dET = p->deltaET;
   g.dZT = p->deltaZT;
-  g.rXI = 1.0 / p->deltaXI;
-  g.rET = 1.0 / p->deltaET;
-  g.rZT = 1.0 / p->deltaZT;
+  // 0 where the residual-corrected division is not guaranteed: plain division
+  g.rXI = cf::rcp_checked(p->deltaXI);
+  g.rET = cf::rcp_checked(p->deltaET);
+  g.rZT = cf::rcp_checked(p->deltaZT);
   c->hprof.assign((size_t)NPROF * g.Ym, 0.0);
   auto bail = [&](int code) {
     cf_destroy(c);
@@ -238,10 +241,11 @@ extern "C" int cf_create(const cf_params* p, cf_ctx** out) {
   long nb = std::max(cf::update_blocks(g), cf::sweep_update_blocks(g));
   if (hipMalloc(&c->partial, (size_t)nb * 5 * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
   if (hipMalloc(&c->er_dev, 8 * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
-  if (hipMalloc(&c->prof, c->hprof.size() * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
+  if (hipMalloc(&c->prof, (size_t)cf::PROF_ROWS * g.Ym * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
   if (hipMalloc(&c->ypt, (size_t)g.Ym * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
   if (hipMemsetAsync(c->ypt, 0, (size_t)g.Ym * sizeof(double), c->stream) != hipSuccess) return bail(CF_ERR_HIP);
-  if (hipMemsetAsync(c->prof, 0, c->hprof.size() * sizeof(double), c->stream) != hipSuccess) return bail(CF_ERR_HIP);
+  if (hipMemsetAsync(c->prof, 0, (size_t)cf::PROF_ROWS * g.Ym * sizeof(double), c->stream) != hipSuccess)
+    return bail(CF_ERR_HIP);
   if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(CF_ERR_HIP);
   *out = c;
   return CF_OK;

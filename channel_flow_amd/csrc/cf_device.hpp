@@ -27,7 +27,7 @@ struct Geo {
   long plane;   // Ym * Zp
   long nf;      // Xnp * plane
   double dXI, dET, dZT;
-  double rXI, rET, rZT;  // RN(1/d) of the spacings, for div_const
+  double rXI, rET, rZT;  // rcp_checked of the spacings, for div_prof
 };
 
 __device__ __forceinline__ long at(const Geo& g, int i, int j, int k) {
@@ -37,9 +37,48 @@ __device__ __forceinline__ long at(const Geo& g, int i, int j, int k) {
 // j-profiles of the metrics: prof[(set*10 + comp) * Ym + j]; set 0 = cell
 // centres, 1/2/3 = x/y/z faces; comp 0 = J, 1..9 = xidx..ztdz.
 enum { SET_C = 0, SET_U = 1, SET_V = 2, SET_W = 3 };
+// The profile buffer holds PROF_ROWS rows of Ym values: 4 sets x (J + 9
+// metrics), then rows derived from them for the fluxes (FT_*).
+constexpr int PROF_METRIC_ROWS = 40;
+constexpr int FT_REC = PROF_METRIC_ROWS;      // 3 directions x 7 recipes: 1 / sum(w/J)
+constexpr int FT_NRM = FT_REC + 21;           // 3 directions x (insqr, N0, N1, N2) of the face normal
+constexpr int FT_RCP = FT_NRM + 12;          // 3 directions x (rcp_checked of Jf, of insqr)
+constexpr int PROF_ROWS = FT_RCP + 6;
 __device__ __forceinline__ double pf(const double* __restrict__ prof, int Ym, int set, int comp, int j) {
   return prof[(set * 10 + comp) * Ym + j];
 }
+
+// row `row` of the profile buffer (the derived FT_* rows), at j
+__device__ __forceinline__ double prow(const double* __restrict__ prof, int Ym, int row, int j) {
+  return prof[(long)row * Ym + j];
+}
+
+// x/d for a divisor d fixed by the grid, with r = RN(1/d):
+//   q0 = x r,  q = q0 + fma(-q0, d, x) r  (both steps single roundings).
+// When |r d - 1| <= 2^-54, q0 is within one ulp of x/d for every x, and
+// Markstein's theorem (IBM J. Res. Dev. 34(1), 1990) makes q the correctly
+// rounded x/d: the same bits as the division, except that -0/d gives +0.
+__device__ __forceinline__ double div_const(double x, double d, double r) {
+  const double q0 = x * r;
+  const double e = fma(-q0, d, x);
+  return fma(e, r, q0);
+}
+
+// RN(1/d) when it meets the bound above (r d - 1 is exact in one fma), else 0
+__host__ __device__ __forceinline__ double rcp_checked(double d) {
+  const double r = 1.0 / d;
+  const double e = fma(r, d, -1.0);
+  return fabs(e) <= 0x1p-54 ? r : 0.0;
+}
+
+// x/d with a reciprocal from rcp_checked (0: plain division)
+__device__ __forceinline__ double div_prof(double x, double d, double r) {
+  return r != 0.0 ? div_const(x, d, r) : x / d;
+}
+
+// 1/(K-1); |RKG (K-1) - 1| = 2^-54.3 meets the bound (checked on the host
+// by tests/test_division_kat.py)
+constexpr double RKG = 1.0 / (KG - 1);
 
 // ------------------------------------------------------------------------
 // MUSCL recipes (Flux_X.cpp:92-215, Flux_Y.cpp:91-240, Flux_Z.cpp:96-300)
@@ -98,6 +137,54 @@ __device__ __forceinline__ void rec_w(const double (&u)[5][6], const double (&jj
   }
 }
 
+// the weighted sums of rec_w with temp = 1/sum(w/J) taken from the profile
+// rows (it depends only on the recipe and the J of the stencil, hence on j)
+template <int FIRST, int N>
+__device__ __forceinline__ void rec_w_t(const double (&u)[5][6], const double (&w)[N], double temp, double q[5]) {
+  constexpr int t0 = FIRST + 2;
+#pragma unroll
+  for (int v = 0; v < 5; v++) {
+    double a = w[0] * u[v][t0];
+#pragma unroll
+    for (int t = 1; t < N; t++) a += w[t] * u[v][t0 + t];
+    q[v] = temp * a;
+  }
+}
+
+__device__ __forceinline__ void reconstruct_t(int r, const double (&u)[5][6], double temp, double q[5]) {
+  switch (r) {
+    case O5L: { const double w[5] = {2, -13, 47, 27, -3}; rec_w_t<-2, 5>(u, w, temp, q); break; }
+    case O5R: { const double w[5] = {-3, 27, 47, -13, 2}; rec_w_t<-1, 5>(u, w, temp, q); break; }
+    case O3L: { const double w[3] = {-1, 5, 2}; rec_w_t<-1, 3>(u, w, temp, q); break; }
+    case O3R: { const double w[3] = {2, 5, -1}; rec_w_t<0, 3>(u, w, temp, q); break; }
+    case AVG: { const double w[2] = {1, 1}; rec_w_t<0, 2>(u, w, temp, q); break; }
+    case AVGM: { const double w[2] = {1, 1}; rec_w_t<-1, 2>(u, w, temp, q); break; }
+    default: { const double w[2] = {1, 1}; rec_w_t<1, 2>(u, w, temp, q); break; }  // AVG1
+  }
+}
+
+// 1/sum(w/J) of recipe r, formed exactly as rec_w forms it
+template <int FIRST, int N>
+__device__ __forceinline__ double rec_temp_w(const double (&jj)[6], const double (&w)[N]) {
+  constexpr int t0 = FIRST + 2;
+  double s = w[0] / jj[t0];
+#pragma unroll
+  for (int t = 1; t < N; t++) s += w[t] / jj[t0 + t];
+  return 1. / s;
+}
+
+__device__ __forceinline__ double rec_temp(int r, const double (&jj)[6]) {
+  switch (r) {
+    case O5L: { const double w[5] = {2, -13, 47, 27, -3}; return rec_temp_w<-2, 5>(jj, w); }
+    case O5R: { const double w[5] = {-3, 27, 47, -13, 2}; return rec_temp_w<-1, 5>(jj, w); }
+    case O3L: { const double w[3] = {-1, 5, 2}; return rec_temp_w<-1, 3>(jj, w); }
+    case O3R: { const double w[3] = {2, 5, -1}; return rec_temp_w<0, 3>(jj, w); }
+    case AVG: { const double w[2] = {1, 1}; return rec_temp_w<0, 2>(jj, w); }
+    case AVGM: { const double w[2] = {1, 1}; return rec_temp_w<-1, 2>(jj, w); }
+    default: { const double w[2] = {1, 1}; return rec_temp_w<1, 2>(jj, w); }  // AVG1
+  }
+}
+
 __device__ __forceinline__ void reconstruct(int r, const double (&u)[5][6], const double (&jj)[6], double q[5]) {
   switch (r) {
     case O5L: { const double w[5] = {2, -13, 47, 27, -3}; rec_w<-2, 5>(u, jj, w, q); break; }
@@ -114,22 +201,33 @@ __device__ __forceinline__ void reconstruct(int r, const double (&u)[5][6], cons
 // Flux_Z.cpp:307-507).  energy_norm selects deltaP*V*insqr (Flux_Y.cpp:441)
 // over deltaP*U (Flux_X.cpp:412, Flux_Z.cpp:499); wall turns on the
 // impermeable-wall face of Flux_Y.cpp:452-460.
-__device__ __forceinline__ void roe_flux(const double L[5], const double Rr[5], const double n[3], double Jf,
-                                         bool energy_norm, bool wall, double F[5]) {
-  double insqr = 1.0 / sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-  double N0 = n[0] * insqr, N1 = n[1] * insqr, N2 = n[2] * insqr;
+// nt = (insqr, N0, N1, N2) = (1/|n|, n/|n|) of the face, a function of j
+// formed once in the profile rows as 1.0 / sqrt(n.n) and n * insqr
+// FAST: divide by Jf and insqr through their reciprocals rJf, rin
+// (rcp_checked, both non-zero); otherwise plainly
+template <bool FAST>
+__device__ __forceinline__ double div_face(double x, double d, double r) {
+  if (FAST) return div_const(x, d, r);
+  return x / d;
+}
+
+template <bool FAST>
+__device__ __forceinline__ void roe_flux(const double L[5], const double Rr[5], const double n[3], const double nt[4],
+                                         double Jf, double rJf, double rin, bool energy_norm, bool wall,
+                                         double F[5]) {
+  const double insqr = nt[0], N0 = nt[1], N1 = nt[2], N2 = nt[3];
   double _rho = L[0], _u = L[1] / _rho, _v = L[2] / _rho, _w = L[3] / _rho;
   double _U = n[0] * _u + n[1] * _v + n[2] * _w;
   double _VV = _u * _u + _v * _v + _w * _w;
   double _P = (L[4] - 0.5 * _rho * _VV) * (KG - 1);
   double _C = KG * _P / _rho;
-  double _H = 0.5 * _VV + _C / (KG - 1);
+  double _H = 0.5 * _VV + div_const(_C, KG - 1, RKG);
   double rho_ = Rr[0], u_ = Rr[1] / rho_, v_ = Rr[2] / rho_, w_ = Rr[3] / rho_;
   double U_ = n[0] * u_ + n[1] * v_ + n[2] * w_;
   double VV_ = u_ * u_ + v_ * v_ + w_ * w_;
   double P_ = (Rr[4] - 0.5 * rho_ * VV_) * (KG - 1);
   double C_ = KG * P_ / rho_;
-  double H_ = 0.5 * VV_ + C_ / (KG - 1);
+  double H_ = 0.5 * VV_ + div_const(C_, KG - 1, RKG);
   double t5 = sqrt(_rho), t6 = sqrt(rho_), t4 = t5 + t6;
   double rho = sqrt(_rho * rho_);
   double u = (t5 * _u + t6 * u_) / t4;
@@ -143,7 +241,7 @@ __device__ __forceinline__ void roe_flux(const double L[5], const double Rr[5], 
   double dU2 = rho_ * u_ - _rho * _u;
   double dU3 = rho_ * v_ - _rho * _v;
   double dU4 = rho_ * w_ - _rho * _w;
-  double dU5 = (P_ / (KG - 1) + 0.5 * rho_ * VV_) - (_P / (KG - 1) + 0.5 * _rho * _VV);
+  double dU5 = (div_const(P_, KG - 1, RKG) + 0.5 * rho_ * VV_) - (div_const(_P, KG - 1, RKG) + 0.5 * _rho * _VV);
   double S = sqrt(C);
   double temp1 = (P_ - _P) / rho / C;
   double temp2 = U * insqr / S * (U_ - _U) * insqr;
@@ -158,16 +256,17 @@ __device__ __forceinline__ void roe_flux(const double L[5], const double Rr[5], 
   if (wall) {
     F[0] = 0;
     F[1] = 0;
-    F[2] = 0.5 * (n[1] * (_P + P_) - Fav3 / insqr) / Jf;
+    F[2] = div_face<FAST>(0.5 * (n[1] * (_P + P_) - div_face<FAST>(Fav3, insqr, rin)), Jf, rJf);
     F[3] = 0;
     F[4] = 0;
     return;
   }
-  F[0] = 0.5 * ((_rho * _U + rho_ * U_) - Fav1 / insqr) / Jf;
-  F[1] = 0.5 * ((_rho * _u * _U + rho_ * u_ * U_ + n[0] * (_P + P_)) - Fav2 / insqr) / Jf;
-  F[2] = 0.5 * ((_rho * _v * _U + rho_ * v_ * U_ + n[1] * (_P + P_)) - Fav3 / insqr) / Jf;
-  F[3] = 0.5 * ((_rho * _w * _U + rho_ * w_ * U_ + n[2] * (_P + P_)) - Fav4 / insqr) / Jf;
-  F[4] = 0.5 * ((_U * (3.5 * _P + 0.5 * _rho * _VV) + U_ * (3.5 * P_ + 0.5 * rho_ * VV_)) - Fav5 / insqr) / Jf;
+  F[0] = div_face<FAST>(0.5 * ((_rho * _U + rho_ * U_) - div_face<FAST>(Fav1, insqr, rin)), Jf, rJf);
+  F[1] = div_face<FAST>(0.5 * ((_rho * _u * _U + rho_ * u_ * U_ + n[0] * (_P + P_)) - div_face<FAST>(Fav2, insqr, rin)), Jf, rJf);
+  F[2] = div_face<FAST>(0.5 * ((_rho * _v * _U + rho_ * v_ * U_ + n[1] * (_P + P_)) - div_face<FAST>(Fav3, insqr, rin)), Jf, rJf);
+  F[3] = div_face<FAST>(0.5 * ((_rho * _w * _U + rho_ * w_ * U_ + n[2] * (_P + P_)) - div_face<FAST>(Fav4, insqr, rin)), Jf, rJf);
+  F[4] = div_face<FAST>(0.5 * ((_U * (3.5 * _P + 0.5 * _rho * _VV) + U_ * (3.5 * P_ + 0.5 * rho_ * VV_)) - div_face<FAST>(Fav5, insqr, rin)),
+                  Jf, rJf);
 }
 
 // ------------------------------------------------------------------------
@@ -367,14 +466,6 @@ __device__ __forceinline__ void upper_diag(const DNb& c, double n, const double 
   diag_apply<D, false>(diag_coef<D, false>(c.P, c.T, c.beta, c.U, c.lam, c.pd, c.p51, n), N, o);
 }
 
-// x/d for a fixed divisor d with r = RN(1/d): q0 = x*r is within one ulp of
-// x/d, and one residual step q0 + fma(-q0,d,x)*r rounds correctly
-// (P. Markstein, IBM J. Res. Dev. 34(1), 1990).  Same bits as x/d.
-__device__ __forceinline__ double div_const(double x, double d, double r) {
-  const double q0 = x * r;
-  const double e = fma(-q0, d, x);
-  return fma(e, r, q0);
-}
 
 // lower (forward) contribution -A+ N of one neighbour, Dual_time_stepping.cpp:751-811
 __device__ __forceinline__ void lower_sum(const Split& a, const double N[5], double o[5]) {

@@ -81,24 +81,24 @@ __global__ void __launch_bounds__(256) k_flux(const double* __restrict__ U, cons
   auto lo_of = [](int r) { return r == O5L ? -2 : (r == O5R || r == O3L || r == AVGM) ? -1 : (r == AVG1 ? 1 : 0); };
   auto hi_of = [](int r) { return r == O5L ? 2 : r == O5R ? 3 : r == O3L ? 1 : r == O3R ? 2 : r == AVGM ? 0 : r == AVG1 ? 2 : 1; };
   int tlo = min(lo_of(rl), lo_of(rr)) + 2, thi = max(hi_of(rl), hi_of(rr)) + 2;
-  double u[5][6], jj[6];
+  double u[5][6];
 #pragma unroll
   for (int t = 0; t < 6; t++) {
     if (t < tlo || t > thi) {
-      jj[t] = 1.0;
 #pragma unroll
       for (int v = 0; v < 5; v++) u[v][t] = 0.0;
       continue;
     }
     long p = base + (long)(t - 2) * stride;
-    int jc = DIR == 1 ? f + t - 2 : j;
-    jj[t] = pf(prof, g.Ym, SET_C, 0, jc);
 #pragma unroll
     for (int v = 0; v < 5; v++) u[v][t] = U[v * g.nf + p];
   }
+  // 1/sum(w/J) varies with j only: with the cell's j along x and z, with the
+  // face index along y
+  const int rj = DIR == 1 ? f : j;
   double L[5], R[5], out[5];
-  reconstruct(rl, u, jj, L);
-  reconstruct(rr, u, jj, R);
+  reconstruct_t(rl, u, prow(prof, g.Ym, FT_REC + 7 * DIR + rl, rj), L);
+  reconstruct_t(rr, u, prow(prof, g.Ym, FT_REC + 7 * DIR + rr, rj), R);
   long o;
   int set, jp;
   if (DIR == 0) { o = at(g, f, j - 1, k - 1); set = SET_U; jp = j - 1; }
@@ -106,9 +106,17 @@ __global__ void __launch_bounds__(256) k_flux(const double* __restrict__ U, cons
   else { o = at(g, i - 1, j - 1, f); set = SET_W; jp = j - 1; }
   double n[3] = {pf(prof, g.Ym, set, 1 + 3 * DIR, jp), pf(prof, g.Ym, set, 2 + 3 * DIR, jp),
                  pf(prof, g.Ym, set, 3 + 3 * DIR, jp)};
+  double nt[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) nt[c] = prow(prof, g.Ym, FT_NRM + 4 * DIR + c, jp);
   double Jf = pf(prof, g.Ym, set, 0, jp);
   bool wall = DIR == 1 && (f == 1 || f == g.ny);
-  roe_flux(L, R, n, Jf, DIR == 1, wall, out);
+  const double rJf = prow(prof, g.Ym, FT_RCP + 2 * DIR, jp), rin = prow(prof, g.Ym, FT_RCP + 2 * DIR + 1, jp);
+  // a wave covers one face row, so this vote is one branch per wave
+  if (__all(rJf != 0.0 && rin != 0.0))
+    roe_flux<true>(L, R, n, nt, Jf, rJf, rin, DIR == 1, wall, out);
+  else
+    roe_flux<false>(L, R, n, nt, Jf, rJf, rin, DIR == 1, wall, out);
 #pragma unroll
   for (int v = 0; v < 5; v++) F[v * g.nf + o] = out[v];
 }
@@ -1150,8 +1158,8 @@ __global__ void __launch_bounds__(64) k_sweep_tiles(const double* __restrict__ S
       double L[5];
 #pragma unroll
       for (int v = 0; v < 5; v++)
-        L[v] = div_const(acc[0][v], g.dXI, g.rXI) + div_const(acc[1][v], g.dET, g.rET) +
-               div_const(acc[2][v], g.dZT, g.rZT);
+        L[v] = div_prof(acc[0][v], g.dXI, g.rXI) + div_prof(acc[1][v], g.dET, g.rET) +
+               div_prof(acc[2][v], g.dZT, g.rZT);
       const double d11 = cur.own[G_D11], d22 = cur.own[G_D22], d51 = cur.own[G_D51];
       const double* M = cur.src;
       double out[5];
@@ -1274,8 +1282,15 @@ __device__ __forceinline__ SplitCoef coef_load(const double (&row)[6][72], int c
   return SplitCoef{row[0][col], row[1][col], row[2][col], row[3][col], row[4][col], row[5][col]};
 }
 
+// x/d by a spacing: FD when all three spacings have checked reciprocals
+template <bool FD>
+__device__ __forceinline__ double div_sp(double x, double d, double r) {
+  if (FD) return div_const(x, d, r);
+  return x / d;
+}
+
 // wave 0: one cell per lane and step, inputs from LDS only
-template <bool FWD>
+template <bool FWD, bool FD>
 __device__ void pipe_compute(PipeLds& L, double* __restrict__ SRC, double* __restrict__ DST,
                              double* __restrict__ AUX, const Geo& g, const Sk& q, int tile, int TX, int TY) {
   const int lane = threadIdx.x & 63;
@@ -1336,8 +1351,8 @@ __device__ void pipe_compute(PipeLds& L, double* __restrict__ SRC, double* __res
       double Lv[5];
 #pragma unroll
       for (int v = 0; v < 5; v++)
-        Lv[v] = div_const(acc[0][v], g.dXI, g.rXI) + div_const(acc[1][v], g.dET, g.rET) +
-                div_const(acc[2][v], g.dZT, g.rZT);
+        Lv[v] = div_sp<FD>(acc[0][v], g.dXI, g.rXI) + div_sp<FD>(acc[1][v], g.dET, g.rET) +
+                div_sp<FD>(acc[2][v], g.dZT, g.rZT);
       double out[5];
       if (FWD) {
         out[0] = d11 * (M[0] - Lv[0]);
@@ -1615,7 +1630,11 @@ __global__ void __launch_bounds__(256) k_sweep_pipe(const double* __restrict__ S
   const int tile = order[FWD ? t : q.ntiles - 1 - t];
   const int TX = tile % q.nTI, TY = tile / q.nTI;
   if (wave == 0) {
-    pipe_compute<FWD>(L, SRC, DST, AUX, g, q, tile, TX, TY);
+    // the kernel arguments make this choice uniform: one scalar branch
+    if (g.rXI != 0.0 && g.rET != 0.0 && g.rZT != 0.0)
+      pipe_compute<FWD, true>(L, SRC, DST, AUX, g, q, tile, TX, TY);
+    else
+      pipe_compute<FWD, false>(L, SRC, DST, AUX, g, q, tile, TX, TY);
   } else if (wave == 1) {
     pipe_own_loader<FWD, true, QS>(L, S, SRC, prof, g, q, tile, TY);
   } else if (wave == 2) {
@@ -1840,6 +1859,40 @@ void launch_halo_bc(hipStream_t st, double* U, const double* prof, const Geo& g,
   hipLaunchKernelGGL(k_halo_x, dim3((unsigned)((g.plane + 255) / 256), 5, 2), dim3(256), 0, st, U, g, src_lo, src_hi);
   hipLaunchKernelGGL(k_wall, dim3((g.nz - 1 + 63) / 64, g.gend, 2), dim3(64), 0, st, U, prof, g);
   hipLaunchKernelGGL(k_zper, dim3((g.ny + 3 + 63) / 64, g.gend, 5), dim3(64), 0, st, U, g);
+}
+
+// derived profile rows for the fluxes, recomputed whenever the metric rows
+// change: FT_REC + 7 d + r holds 1/sum(w/J) of recipe r in direction d (at
+// the cell's j along x and z, at face j along y; stencil J clamped to the
+// array for recipes a face does not use), FT_NRM + 4 d the face normal's
+// (insqr, N0, N1, N2) at the face row the fluxes read, FT_RCP + 2 d the
+// checked reciprocals of that row's face J and insqr
+__global__ void k_flux_tables(double* __restrict__ prof, Geo g) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= 24 * g.Ym) return;
+  const int row = idx / g.Ym, j = idx - row * g.Ym;
+  if (row < 21) {
+    const int d = row / 7, r = row % 7;
+    double jj[6];
+#pragma unroll
+    for (int t = 0; t < 6; t++) jj[t] = pf(prof, g.Ym, SET_C, 0, d == 1 ? min(max(j + t - 2, 0), g.Ym - 1) : j);
+    prof[(long)(FT_REC + row) * g.Ym + j] = rec_temp(r, jj);
+  } else {
+    const int d = row - 21, set = SET_U + d;
+    const double n0 = pf(prof, g.Ym, set, 1 + 3 * d, j), n1 = pf(prof, g.Ym, set, 2 + 3 * d, j),
+                 n2 = pf(prof, g.Ym, set, 3 + 3 * d, j);
+    const double insqr = 1.0 / sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+    prof[(long)(FT_NRM + 4 * d) * g.Ym + j] = insqr;
+    prof[(long)(FT_NRM + 4 * d + 1) * g.Ym + j] = n0 * insqr;
+    prof[(long)(FT_NRM + 4 * d + 2) * g.Ym + j] = n1 * insqr;
+    prof[(long)(FT_NRM + 4 * d + 3) * g.Ym + j] = n2 * insqr;
+    prof[(long)(FT_RCP + 2 * d) * g.Ym + j] = rcp_checked(pf(prof, g.Ym, set, 0, j));
+    prof[(long)(FT_RCP + 2 * d + 1) * g.Ym + j] = rcp_checked(insqr);
+  }
+}
+
+void launch_flux_tables(hipStream_t st, double* prof, const Geo& g) {
+  hipLaunchKernelGGL(k_flux_tables, dim3((24 * g.Ym + 127) / 128), dim3(128), 0, st, prof, g);
 }
 
 void launch_flux(hipStream_t st, int dir, const double* U, const double* prof, double* F, const Geo& g) {

@@ -6,6 +6,8 @@ namespace cf {
 
 void launch_halo_bc(hipStream_t st, double* U, const double* prof, const Geo& g, int x_periodic);
 void launch_flux(hipStream_t st, int dir, const double* U, const double* prof, double* F, const Geo& g);
+// rebuilds the derived flux rows of the profile buffer from its metric rows
+void launch_flux_tables(hipStream_t st, double* prof, const Geo& g);
 // limiter variants; Ep is the sensor on the face layout of F
 void launch_flux_limiter(hipStream_t st, int dir, const double* U, const double* Ep, const double* prof, double* F,
                          const Geo& g);

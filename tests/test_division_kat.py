@@ -1,13 +1,19 @@
-"""The sweep kernels divide by the grid spacings with a residual-corrected
+"""The kernels divide by grid-fixed divisors (the spacings in the sweeps;
+K-1, the face J and 1/|n| in the Roe flux) with a residual-corrected
 reciprocal (Markstein, IBM J. Res. Dev. 34(1), 1990):
 
     q0 = x * r,  r = RN(1/d);   q = fma(fma(-q0, d, x), r, q0)
 
-which must equal the IEEE quotient x/d bit for bit for the reference's
-division `(...)/deltaXI` to be reproduced.  This checks the identity with C's
-correctly rounded fma() on the spacings of every grid used here, over operands
+which must equal the IEEE quotient x/d bit for bit.  When |r d - 1| <= 2^-54,
+q0 = RN(x r) is within one ulp of x/d for every x, and Markstein's theorem
+makes q the correctly rounded quotient.  The library checks that bound for
+each divisor (rcp_checked) and divides plainly where it fails.  Here the bound
+is checked exactly, with rationals, for K-1 and the spacings of every grid
+used, and the identity itself with C's correctly rounded fma() over operands
 spanning 2^-40 .. 2^23 of both signs.
 """
+from fractions import Fraction
+import math
 import subprocess
 import textwrap
 
@@ -23,9 +29,11 @@ SRC = textwrap.dedent(r"""
       const double hi = 0.005, pi = 3.14159265358979323846;
       const int grids[][3] = {{256,192,256},{64,64,64},{12,10,16},{8,12,9},{16,12,16},{512,384,512},{21,13,18}};
       long bad = 0, n = 0;
-      for (unsigned g = 0; g < sizeof grids / sizeof grids[0]; g++) {
-        double ds[3] = {2 * pi * hi / grids[g][0], 1.0 / grids[g][1], pi * hi / grids[g][2]};
-        for (int t = 0; t < 3; t++) {
+      for (unsigned g = 0; g <= sizeof grids / sizeof grids[0]; g++) {
+        const int last = g == sizeof grids / sizeof grids[0];
+        double ds[3] = {0.3999999999999999, 0, 0};  /* K - 1 = 1.4 - 1 in FP64 */
+        if (!last) { ds[0] = 2 * pi * hi / grids[g][0]; ds[1] = 1.0 / grids[g][1]; ds[2] = pi * hi / grids[g][2]; }
+        for (int t = 0; t < (last ? 1 : 3); t++) {
           double d = ds[t], r = 1.0 / d;
           for (long i = 0; i < 1000000; i++) {
             uint64_t u = xr();
@@ -51,4 +59,36 @@ def test_residual_corrected_division_is_exact(tmp_path):
     subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-o", str(exe), str(c), "-lm"], check=True)
     r = subprocess.run([str(exe)], capture_output=True, text=True)
     n, bad = map(int, r.stdout.split())
-    assert n == 21_000_000 and bad == 0, r.stdout
+    assert n == 22_000_000 and bad == 0, r.stdout
+
+
+def _bound_ok(d):
+    r = 1.0 / d
+    return abs(Fraction(r) * Fraction(d) - 1) <= Fraction(1, 2 ** 54)
+
+
+def test_reciprocal_bound_for_fixed_divisors():
+    """|RN(1/d) d - 1| <= 2^-54, exactly, for K-1 and the spacings of the
+    grids used here (the library also checks it at run time)."""
+    kg1 = 1.4 - 1
+    assert kg1 == 0.3999999999999999
+    assert _bound_ok(kg1)
+    hi = 0.005
+    for X, Y, Z in [(256, 192, 256), (64, 64, 64), (12, 10, 16), (8, 12, 9), (16, 12, 16), (512, 384, 512),
+                    (21, 13, 18)]:
+        for d in (2 * math.pi * hi / X, 1.0 / Y, math.pi * hi / Z):
+            assert _bound_ok(d), d
+
+
+def test_bound_check_matches_fma_form():
+    """rcp_checked's test fma(r, d, -1) is exact, so it decides the same as
+    the rational bound; about half of all divisors pass it."""
+    rng = __import__("numpy").random.default_rng(5)
+    passed = 0
+    for d in rng.uniform(1e-3, 1e3, 2000):
+        d = float(d)
+        r = 1.0 / d
+        e = Fraction(r) * Fraction(d) - 1
+        assert Fraction(float(e)) == e  # representable, so fma returns it exactly
+        passed += _bound_ok(d)
+    assert 500 < passed < 1900
