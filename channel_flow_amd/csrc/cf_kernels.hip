@@ -901,28 +901,11 @@ __device__ __forceinline__ bool sk_cell(const Sk& q, const Geo& g, int tile, int
   return s < q.nsteps && kk >= 0 && kk < q.Nz && *i <= g.gend && *j <= g.ny;
 }
 
-// diagonal-metric summary (DiagField) and MR in sweep order, gathered from
-// the natural layout: a block is 16 steps x 64 lanes of one tile, so a lane's
-// 16 consecutive steps read 16 consecutive k of its column
-__global__ void __launch_bounds__(1024) k_sweep_prep(const double* __restrict__ Ui, const double* __restrict__ MR,
-                                                     const double* __restrict__ prof, double* __restrict__ S,
-                                                     double* __restrict__ MRs, double* __restrict__ NLs, Geo g, Sk q,
-                                                     double deltaT) {
-  const int lane = threadIdx.x & 63;
-  const int s = blockIdx.x * 16 + (threadIdx.x >> 6);
-  const int tile = blockIdx.y;
-  if (s < q.nsteps && ((lane & 7) == TI - 1 || (lane >> 3) == TJ - 1)) {
-    // slots the first forward sweep hands across tiles start as the sentinel
-    const double sv = __longlong_as_double((long long)SENTINEL);
-#pragma unroll
-    for (int v = 0; v < 5; v++) NLs[v * q.nsk + sk_at(q, tile, s, lane)] = sv;
-  }
-  int i, j, k;
-  if (!sk_cell(q, g, tile, s, lane, &i, &j, &k)) return;
-  const long nf = g.nf;
-  const long c = at(g, i, j, k);
-  const long o = sk_at(q, tile, s, lane);
-  PState st = prim_dt(Ui[c], Ui[nf + c], Ui[2 * nf + c], Ui[3 * nf + c], Ui[4 * nf + c], pf(prof, g.Ym, SET_C, 0, j));
+// diagonal-metric summary (DiagField) of one cell (Dual_time_stepping.cpp:
+// 343-372 primitives, 464-489 D^-1, 751-811 split-Jacobian ingredients)
+__device__ __forceinline__ void prep_summary(const double u5[5], const double* __restrict__ prof, const Geo& g, int j,
+                                             double deltaT, double v[G_N]) {
+  PState st = prim_dt(u5[0], u5[1], u5[2], u5[3], u5[4], pf(prof, g.Ym, SET_C, 0, j));
   double m9[9];
 #pragma unroll
   for (int r = 0; r < 9; r++) m9[r] = pf(prof, g.Ym, SET_C, 1 + r, j);
@@ -932,7 +915,6 @@ __global__ void __launch_bounds__(1024) k_sweep_prep(const double* __restrict__ 
   L3[2] = lambda_dir(st, m9[6], m9[7], m9[8], &U3[2]);
   double d11, d22, d51;
   dinv_lam(st, L3[0], L3[1], L3[2], g.dXI, g.dET, g.dZT, deltaT, &d11, &d22, &d51);
-  double v[G_N];
   v[G_P] = st.P;
   v[G_T] = st.T;
   v[G_BETA] = st.beta;
@@ -947,10 +929,68 @@ __global__ void __launch_bounds__(1024) k_sweep_prep(const double* __restrict__ 
     v[G_PDX + 4 * d] = 0.5 * n / st.rho;
     v[G_P51X + 4 * d] = 0.5 * U3[d] * (KG - 1) * (st.beta - 1) / st.rho / KG;
   }
+}
+
+// A prep block covers PC steps of one tile.  Column (a, b) of the tile meets
+// cells k-2 = s - a - b at steps s0 .. s0+PC-1: one run of PC consecutive k,
+// which the block stages in LDS with coalesced loads; slot (s, lane) then
+// reads its cell at LDS index s - s0 whatever the column.
+constexpr int PC = 24, PCP = PC + 1;  // steps per block; LDS column stride
+
+__device__ __forceinline__ void prep_stage(double (*L)[64][PCP], const double* __restrict__ src, const Geo& g,
+                                           const Sk& q, int tile, int s0) {
+  const int TX = tile % q.nTI, TY = tile / q.nTI;
+  for (int idx = threadIdx.x; idx < 64 * PC; idx += blockDim.x) {
+    const int col = idx / PC, t = idx - col * PC;
+    const int a = col & 7, b = col >> 3;
+    const int i = 3 + TX * TI + a, j = 2 + TY * TJ + b, kk = s0 - a - b + t;
+    if (i > g.gend || j > g.ny || kk < 0 || kk >= q.Nz) continue;
+    const long c = at(g, i, j, 2 + kk);
 #pragma unroll
-  for (int r = 0; r < G_N; r++) S[r * q.nsk + o] = v[r];
+    for (int f = 0; f < 5; f++) L[f][col][t] = src[f * g.nf + c];
+  }
+}
+
+// the summary and MR in sweep order, and the NL hand-off slots seeded with the
+// sentinel; each wave writes whole 512-byte rows
+__global__ void __launch_bounds__(256) k_sweep_prep(const double* __restrict__ Ui, const double* __restrict__ MR,
+                                                    const double* __restrict__ prof, double* __restrict__ S,
+                                                    double* __restrict__ MRs, double* __restrict__ NLs, Geo g, Sk q,
+                                                    double deltaT) {
+  __shared__ double L[5][64][PCP];
+  const int tile = blockIdx.y, s0 = blockIdx.x * PC;
+  prep_stage(L, Ui, g, q, tile, s0);
+  __syncthreads();
+  const double sv = __longlong_as_double((long long)SENTINEL);
+  for (int idx = threadIdx.x; idx < PC * 64; idx += blockDim.x) {
+    const int t = idx >> 6, lane = idx & 63, s = s0 + t;
+    if (s >= q.nsteps) continue;
+    const long o = sk_at(q, tile, s, lane);
+    if ((lane & 7) == TI - 1 || (lane >> 3) == TJ - 1) {
+      // slots the first forward sweep hands across tiles start as the sentinel
 #pragma unroll
-  for (int r = 0; r < 5; r++) MRs[r * q.nsk + o] = MR[r * nf + c];
+      for (int v = 0; v < 5; v++) NLs[v * q.nsk + o] = sv;
+    }
+    int i, j, k;
+    if (!sk_cell(q, g, tile, s, lane, &i, &j, &k)) continue;
+    double u5[5], v[G_N];
+#pragma unroll
+    for (int f = 0; f < 5; f++) u5[f] = L[f][lane][t];
+    prep_summary(u5, prof, g, j, deltaT, v);
+#pragma unroll
+    for (int r = 0; r < G_N; r++) S[r * q.nsk + o] = v[r];
+  }
+  __syncthreads();
+  prep_stage(L, MR, g, q, tile, s0);
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < PC * 64; idx += blockDim.x) {
+    const int t = idx >> 6, lane = idx & 63, s = s0 + t;
+    int i, j, k;
+    if (!sk_cell(q, g, tile, s, lane, &i, &j, &k)) continue;
+    const long o = sk_at(q, tile, s, lane);
+#pragma unroll
+    for (int r = 0; r < 5; r++) MRs[r * q.nsk + o] = L[r][lane][t];
+  }
 }
 
 // sweep-order field(s) back to the natural layout
@@ -2010,8 +2050,8 @@ void launch_sweeps(hipStream_t st, double* Ui, const double* MR, double* Sbuf, d
   double* MRs = Sbuf + (long)G_N * q.nsk;
   double* NLs = MRs + 5 * q.nsk;
   double* NRs = NLs + 5 * q.nsk;
-  hipLaunchKernelGGL(k_sweep_prep, dim3((q.nsteps + 15) / 16, q.ntiles), dim3(1024), 0, st, Ui, MR, prof, S, MRs, NLs,
-                     g, q, deltaT);
+  hipLaunchKernelGGL(k_sweep_prep, dim3((q.nsteps + PC - 1) / PC, q.ntiles), dim3(256), 0, st, Ui, MR, prof, S, MRs,
+                     NLs, g, q, deltaT);
   if (marks) (void)hipEventRecord(marks[0], st);
   const int nt = q.ntiles;
   for (int sweep = 0; sweep < n_sweeps; sweep++) {
