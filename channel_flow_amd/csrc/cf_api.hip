@@ -733,6 +733,28 @@ static int implicit_entry(cf_ctx* c, int solver, double deltaT, double er[5], do
   return sync(c);
 }
 
+extern "C" int cf_rk3nodt(cf_ctx* c, int RK, double deltaT, double er[5], double* const U_[5],
+                          const double* const U[5], const double* const Uq[5], const double* J,
+                          const double* const m[9], const double* const inFx[5], const double* const inFy[5],
+                          const double* const inFz[5], const double* const vF[4], double* const MR[5]) {
+  if (!c || !U_ || !U || !inFx || !inFy || !inFz || !vF) return fail(c, CF_ERR_ARG, "null argument");
+  if (RK < 1 || RK > 3) return fail(c, CF_ERR_ARG, "RK stage must be 1, 2 or 3");
+  (void)er;  // RK3NODT.cpp leaves er1..er5 untouched
+  (void)Uq;
+  int r;
+  if ((r = ingest_set(c, cf::SET_C, J, m)) || (r = upload_prof(c))) return r;
+  if ((r = h2d_n(c, c->Ui, (const double* const*)U_, 5)) || (r = h2d_n(c, c->Un, U, 5)) ||
+      (r = h2d_n(c, c->Fx, inFx, 5)) || (r = h2d_n(c, c->Fy, inFy, 5)) || (r = h2d_n(c, c->Fz, inFz, 5)) ||
+      (r = h2d_n(c, c->vF, vF, 4)))
+    return r;
+  cf::launch_rk3(c->stream, RK, c->Ui, c->Un, c->Fx, c->Fy, c->Fz, c->vF, c->prof, c->MR, c->g, deltaT,
+                 c->p.f_drive);
+  HIPCHK(c, hipGetLastError());
+  if ((r = d2h_n(c, U_, c->Ui, 5))) return r;
+  if (MR && (r = d2h_n(c, MR, c->MR, 5))) return r;
+  return sync(c);
+}
+
 // ---------------------------------------------------------------------------
 // device-resident path
 // ---------------------------------------------------------------------------
@@ -931,6 +953,24 @@ extern "C" int cf_set_implicit_solver(cf_ctx* c, int solver) {
   if (!c || solver < 0 || solver > 1) return fail(c, CF_ERR_ARG, "bad solver");
   c->solver = solver;
   return CF_OK;
+}
+
+extern "C" int cf_rk3_step(cf_ctx* c, int viscous) {
+  if (!c) return CF_ERR_ARG;
+  if (!c->have_state) return fail(c, CF_ERR_STATE, "no state loaded");
+  const Geo& g = c->g;
+  for (int RK = 1; RK <= 3; RK++) {
+    cf::launch_halo_bc(c->stream, c->Ui, c->prof, g, c->p.x_periodic);
+    cf::launch_flux(c->stream, 0, c->Ui, c->prof, c->Fx, g);
+    cf::launch_flux(c->stream, 1, c->Ui, c->prof, c->Fy, g);
+    cf::launch_flux(c->stream, 2, c->Ui, c->prof, c->Fz, g);
+    if (viscous)
+      if (int rv = run_viscous(c)) return rv;
+    cf::launch_rk3(c->stream, RK, c->Ui, c->Un, c->Fx, c->Fy, c->Fz, viscous ? c->vF : nullptr, c->prof, nullptr, g,
+                   c->p.deltaT, c->p.f_drive);
+  }
+  HIPCHK(c, hipGetLastError());
+  return sync(c);
 }
 
 extern "C" int cf_set_viscous_schedule(cf_ctx* c, int schedule) {

@@ -2007,6 +2007,58 @@ void launch_viscous(hipStream_t st, const double* U, const double* prof, double*
                      invZT);
 }
 
+// RK3NODT.cpp:115-183: one stage of the explicit three-stage update on the
+// interior, U_ <- c1 U + c2 U_ + c3 (dt Rk), in place (each cell reads its
+// own U_ before writing it; the fluxes are separate arrays).  vF may be null:
+// the terms are then +0, as for the never-assigned (zero) vF arrays of a
+// reference build that skips Viscous_terms.
+__global__ void __launch_bounds__(256) k_rk3(double* __restrict__ Ui, const double* __restrict__ Un,
+                                             const double* __restrict__ Fx, const double* __restrict__ Fy,
+                                             const double* __restrict__ Fz, const double* __restrict__ vF,
+                                             const double* __restrict__ prof, double* __restrict__ MR, Geo g,
+                                             double deltaT, double f_drive, double c1, double c2, double c3) {
+  const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int i = 3 + blockIdx.z;
+  if (k > g.nz || j > g.ny) return;
+  const long nf = g.nf;
+  const long c = at(g, i, j, k);
+  const long fx = at(g, i, j - 1, k - 1), fxm = fx - g.plane;
+  const long fy = at(g, i - 1, j, k - 1), fym = fy - g.Zp;
+  const long fz = at(g, i - 1, j - 1, k), fzm = fz - 1;
+  const double J = pf(prof, g.Ym, SET_C, 0, j);
+  const double tmp = Ui[nf + c] / Ui[c];
+  double Rk[5];
+#pragma unroll
+  for (int q = 0; q < 5; q++)
+    Rk[q] = -((Fx[q * nf + fx] - Fx[q * nf + fxm]) / g.dXI + (Fy[q * nf + fy] - Fy[q * nf + fym]) / g.dET +
+              (Fz[q * nf + fz] - Fz[q * nf + fzm]) / g.dZT);
+  const double v2 = vF ? vF[c] : 0.0, v3 = vF ? vF[nf + c] : 0.0, v4 = vF ? vF[2 * nf + c] : 0.0,
+               v5 = vF ? vF[3 * nf + c] : 0.0;
+  Rk[1] = Rk[1] + v2 + f_drive / J;
+  Rk[2] = Rk[2] + v3;
+  Rk[3] = Rk[3] + v4;
+  Rk[4] = Rk[4] + v5 + f_drive * tmp / J;
+#pragma unroll
+  for (int q = 0; q < 5; q++) {
+    const double m = deltaT * Rk[q];
+    if (MR) MR[q * nf + c] = m;
+    Ui[q * nf + c] = c1 * Un[q * nf + c] + c2 * Ui[q * nf + c] + c3 * m;
+  }
+}
+
+void launch_rk3(hipStream_t st, int RK, double* Ui, const double* Un, const double* Fx, const double* Fy,
+                const double* Fz, const double* vF, const double* prof, double* MR, const Geo& g, double deltaT,
+                double f_drive) {
+  // RK3NODT.cpp:112-114
+  const double c1[3] = {1.0, 0.75, 1.0 / 3.0};
+  const double c2[3] = {0.0, 0.25, 2.0 / 3.0};
+  const double c3[3] = {1.0, 0.25, 2.0 / 3.0};
+  dim3 blk(64, 4);
+  hipLaunchKernelGGL(k_rk3, box_grid(g.nz - 1, g.ny - 1, g.gend - 2, blk), blk, 0, st, Ui, Un, Fx, Fy, Fz, vF, prof,
+                     MR, g, deltaT, f_drive, c1[RK - 1], c2[RK - 1], c3[RK - 1]);
+}
+
 void launch_rhs(hipStream_t st, const double* Ui, const double* Un, const double* Uq, const double* Fx,
                 const double* Fy, const double* Fz, const double* vF, const double* prof, double* MR, double* NL,
                 const Geo& g, double deltaT, double f_drive) {

@@ -21,6 +21,10 @@ void launch_filter(hipStream_t st, const double* U, const double* prof, const do
 // the staging fields Pr (4) and VS (12)
 void launch_viscous(hipStream_t st, const double* U, const double* prof, double* Pr, double* VS, double* vF,
                     const Geo& g, bool fused = true);
+// RK3NODT stage RK (1..3) on the interior; vF and MR may be null
+void launch_rk3(hipStream_t st, int RK, double* Ui, const double* Un, const double* Fx, const double* Fy,
+                const double* Fz, const double* vF, const double* prof, double* MR, const Geo& g, double deltaT,
+                double f_drive);
 void launch_rhs(hipStream_t st, const double* Ui, const double* Un, const double* Uq, const double* Fx,
                 const double* Fy, const double* Fz, const double* vF, const double* prof, double* MR, double* NL,
                 const Geo& g, double deltaT, double f_drive);

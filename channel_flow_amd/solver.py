@@ -30,7 +30,7 @@ EXPORTS = [
     "cf_inner_iteration", "cf_physical_step", "cf_advance_levels", "cf_write_output", "cf_synchronize",
     "cf_profile_inner_iteration", "cf_set_sweep_schedule", "cf_flux_x_limiter", "cf_flux_y_limiter",
     "cf_flux_z_limiter", "cf_grid", "cf_filter", "cf_profile_variants", "cf_dplusgs", "cf_set_implicit_solver",
-    "cf_set_viscous_schedule",
+    "cf_set_viscous_schedule", "cf_rk3nodt", "cf_rk3_step",
 ]
 
 
@@ -94,6 +94,8 @@ def load_library() -> C.CDLL:
     lib.cf_profile_inner_iteration.argtypes = [vp, P, P, C.POINTER(C.c_int)]
     lib.cf_set_sweep_schedule.argtypes = [vp, C.c_int]
     lib.cf_set_viscous_schedule.argtypes = [vp, C.c_int]
+    lib.cf_rk3nodt.argtypes = [vp, C.c_int, C.c_double, P, PP, PP, PP, P, PP, PP, PP, PP, PP, PP]
+    lib.cf_rk3_step.argtypes = [vp, C.c_int]
     lib.cf_write_output.argtypes = [vp, C.c_int, C.c_char_p]
     lib.cf_synchronize.argtypes = [vp]
     _lib = lib
@@ -264,6 +266,19 @@ class Solver:
         self._chk(self.lib.cf_dplusgs(self._h, dt, er, _pp(U_), _pp(U), _pp(Uq), _p(met["J"]), _pp(met["m"]),
                                       _pp(inFx), _pp(inFy), _pp(inFz), _pp(vF), _pp(MR), _pp(NR), _pp(NL)))
         return np.array(er[:]), MR, NR, NL
+
+    def RK3NODT(self, RK, U_, U, Uq, met, inFx, inFy, inFz, vF, deltaT=None):
+        """RK3NODT.cpp:115-183, stage RK; updates U_ in place, returns MR = dt Rk."""
+        er = (C.c_double * 5)()
+        MR = self.zeros(5)
+        dt = self.cfg.deltaT if deltaT is None else deltaT
+        self._chk(self.lib.cf_rk3nodt(self._h, int(RK), dt, er, _pp(U_), _pp(U), _pp(Uq), _p(met["J"]),
+                                      _pp(met["m"]), _pp(inFx), _pp(inFy), _pp(inFz), _pp(vF), _pp(MR)))
+        return MR
+
+    def rk3_step(self, viscous: bool = True):
+        """Three RK3NODT stages on the device state (main.cpp:152-377)."""
+        self._chk(self.lib.cf_rk3_step(self._h, int(bool(viscous))))
 
     def set_implicit_solver(self, solver: int = 0):
         """0: LU-SGS (Dual_time_stepping, default); 1: DPLUSGS."""

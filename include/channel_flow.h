@@ -202,6 +202,23 @@ int cf_profile_variants(cf_ctx* ctx, double Roe_criterion, double ms[CF_VARIANT_
  * Hyperplane.cpp:50-78); schedule 2: the tiles of schedule 0 as single waves
  * that load their own inputs. */
 int cf_set_sweep_schedule(cf_ctx* ctx, int schedule);
+/* RK3NODT.cpp:115-183 (main.h RK3NODT; the explicit path of the shipped
+ * main.cpp): stage RK (1, 2, 3) of the SSP-RK3 update
+ * U_ <- c1 U + c2 U_ + c3 dt Rk with (c1, c2, c3) = (1, 0, 1), (3/4, 1/4,
+ * 1/4), (1/3, 2/3, 2/3), Rk the flux divergence + vF + driving force, on
+ * reference-layout host arrays.  MR receives dt Rk (may be NULL); er, Uq
+ * and the metrics other than J are accepted for the reference's parameter
+ * list and not used (RK3NODT.cpp does not touch er1..er5). */
+int cf_rk3nodt(cf_ctx* ctx, int RK, double deltaT, double er[5], double* const U_[5], const double* const U[5],
+               const double* const Uq[5], const double* J, const double* const m[9], const double* const inFx[5],
+               const double* const inFy[5], const double* const inFz[5], const double* const vF[4],
+               double* const MR[5]);
+/* One pass of main.cpp:152-377 on the device state with the RK3NODT path:
+ * three stages of ghosts, Flux_X/Y/Z, Viscous_terms (viscous != 0; with 0
+ * the viscous terms are zero, as in the shipped main.cpp, where the call is
+ * commented out and vF is never assigned) and RK3NODT, from the iterate
+ * with level n as the base.  cf_advance_levels then shifts the levels. */
+int cf_rk3_step(cf_ctx* ctx, int viscous);
 /* Kernel schedule of Viscous_terms (Viscous_terms.cpp:212-883); both give
  * identical bits.  0 (default): one kernel that keeps the primitives and
  * face fluxes of a tile in LDS; 1: primitives, face fluxes and divergence as

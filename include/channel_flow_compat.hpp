@@ -281,6 +281,28 @@ inline void Dual_time_stepping(int, double deltaT, double* er1, double* er2, dou
   *er1 = er[0], *er2 = er[1], *er3 = er[2], *er4 = er[3], *er5 = er[4];
 }
 
+// RK3NODT.cpp:16-183 (main.h): stage RK of the explicit SSP-RK3 update;
+// er1..er5 are left untouched, as the reference leaves them
+inline void RK3NODT(int, int RK, double deltaT, double* er1, double* er2, double* er3, double* er4, double* er5,
+                    CF_F(U1_), CF_F(U2_), CF_F(U3_), CF_F(U4_), CF_F(U5_), CF_F(U1), CF_F(U2), CF_F(U3), CF_F(U4),
+                    CF_F(U5), CF_F(U1q), CF_F(U2q), CF_F(U3q), CF_F(U4q), CF_F(U5q), CF_METRIC_ARGS(),
+                    CF_FIVE(inFx), CF_FIVE(inFy), CF_FIVE(inFz), CF_F(vF2), CF_F(vF3), CF_F(vF4), CF_F(vF5),
+                    CF_FIVE(MR)) {
+  (void)er1, (void)er2, (void)er3, (void)er4, (void)er5;
+  double* const Ui[5] = {CF_P(U1_), CF_P(U2_), CF_P(U3_), CF_P(U4_), CF_P(U5_)};
+  const double* const Un[5] = {CF_P(U1), CF_P(U2), CF_P(U3), CF_P(U4), CF_P(U5)};
+  const double* const Uq[5] = {CF_P(U1q), CF_P(U2q), CF_P(U3q), CF_P(U4q), CF_P(U5q)};
+  const double* const m[9] = CF_METRIC_PTRS();
+  const double* const fx[5] = CF_FIVE_P(inFx);
+  const double* const fy[5] = CF_FIVE_P(inFy);
+  const double* const fz[5] = CF_FIVE_P(inFz);
+  const double* const vF[4] = {CF_P(vF2), CF_P(vF3), CF_P(vF4), CF_P(vF5)};
+  double* const MRp[5] = CF_FIVE_P(MR);
+  double er[5];
+  cf_compat::check(cf_rk3nodt(cf_compat::ctx("RK3NODT"), RK, deltaT, er, Ui, Un, Uq, CF_P(J), m, fx, fy, fz, vF, MRp),
+                   "RK3NODT");
+}
+
 // DPLUSGS.cpp:189-1361 (the Dual_time_stepping parameters without Nijk/ijk);
 // the upper neighbours' NL is read at j+1, k+1 (the reference's j_, k_ are
 // never assigned)

@@ -1074,6 +1074,41 @@ void cfo_dplusgs(const cfo_grid* g, double deltaT, double* const U_[5], double* 
   dual_time_impl(g, deltaT, U_, U, Uq, J, m, inFx, inFy, inFz, vF, MR, NL, NR, er, 2);
 }
 
+/* RK3NODT.cpp:115-183: one stage RK (1..3) of the explicit three-stage
+ * update U_ <- c1 U + c2 U_ + c3 dt Rk, Rk the flux divergence plus viscous
+ * terms and the driving force; MR receives dt Rk. */
+void cfo_rk3nodt(const cfo_grid* g, int RK, double deltaT, double* const U_[5], double* const U[5], const double* J,
+                 double* const inFx[5], double* const inFy[5], double* const inFz[5], double* const vF[4],
+                 double* const MR[5]) {
+  const double c1[3] = {1.0, 0.75, 1.0 / 3.0};
+  const double c2[3] = {0.0, 0.25, 2.0 / 3.0};
+  const double c3[3] = {1.0, 0.25, 2.0 / 3.0};
+  dims_t d = dims(g);
+  size_t sI = (size_t)d.Ym * d.Zm, sJ = (size_t)d.Zm;
+  for (int i = 3; i <= d.gend; i++)
+    for (int j = 2; j <= d.ny; j++)
+      for (int k = 2; k <= d.nz; k++) {
+        size_t c = IX(d, i, j, k);
+        size_t fx = IX(d, i, j - 1, k - 1), fxm = fx - sI;  /* inFx[i][j-1][_k], [i-1] */
+        size_t fy = IX(d, i - 1, j, k - 1), fym = fy - sJ;  /* inFy[i-1][j][_k], [j-1] */
+        size_t fz = IX(d, i - 1, j - 1, k), fzm = fz - 1;   /* inFz[i-1][j-1][k], [_k] */
+        double tmp = U_[1][c] / U_[0][c];
+        double Rk[5];
+        for (int q = 0; q < 5; q++)
+          Rk[q] = -((inFx[q][fx] - inFx[q][fxm]) / g->deltaXI + (inFy[q][fy] - inFy[q][fym]) / g->deltaET +
+                    (inFz[q][fz] - inFz[q][fzm]) / g->deltaZT);
+        Rk[1] = Rk[1] + vF[0][c] + g->f_drive / J[c];
+        Rk[2] = Rk[2] + vF[1][c];
+        Rk[3] = Rk[3] + vF[2][c];
+        Rk[4] = Rk[4] + vF[3][c] + g->f_drive * tmp / J[c];
+        for (int q = 0; q < 5; q++) {
+          MR[q][c] = deltaT * Rk[q];
+          U_[q][c] = c1[RK - 1] * U[q][c] + c2[RK - 1] * U_[q][c] + c3[RK - 1] * MR[q][c];
+        }
+      }
+  (void)sJ;
+}
+
 void cfo_sweeps_apply(const cfo_grid* g, double deltaT, int mode, double* const U_[5], const double* J,
                       double* const m[9], double* const MR[5], double* const NL[5], double* const NR[5]) {
   dims_t d = dims(g);

@@ -10,13 +10,20 @@
  * Y_m = Y_out+4, Z_m = Z_out+4, interior i in [3,gend], j in [2,ny],
  * k in [2,nz] with gend = X_out+2, ny = Y_out+1, nz = Z_out+1.
  *
- * Parity status per function: cfo_metrics, cfo_hyperplane_counts,
- * cfo_halo_bc, cfo_flux, cfo_viscous, cfo_dual_time (both sweep orders) and
- * cfo_inner_iteration are pinned bit for bit by golden vectors that the
+ * Parity status per function.  Pinned bit for bit by golden vectors that the
  * reference itself wrote (built from its own sources by oracle/ref_build.sh,
- * driven by oracle/ref_driver.cpp; fixtures in tests/golden/): per-function
- * outputs at two non-cubic grids, an 18-step trajectory at 16x12x16 and the
- * first inner iteration at 64^3.
+ * driven by oracle/ref_driver.cpp; fixtures in tests/golden/): cfo_metrics,
+ * cfo_hyperplane_counts, cfo_halo_bc, cfo_flux, cfo_viscous, cfo_dual_time
+ * (both sweep orders), cfo_rk3nodt (three stages), cfo_grid_points and
+ * cfo_filter (at the reference's threshold, with the sensor ratios pinned
+ * by threshold runs) on two non-cubic grids, an 18-step trajectory at
+ * 16x12x16 and the first inner iteration at 64^3; cfo_flux_limiter on flat
+ * and perturbed states except the y near-wall faces, where the reference
+ * reads unassigned values (parity unpinned there).  cfo_dplusgs: its
+ * right-hand side is golden-pinned; its Jacobi sweeps are pinned through
+ * the golden-pinned LU-SGS sweeps, whose matrices tests/test_dplusgs.py
+ * reads off cfo_sweeps_apply; the upper neighbours' iterate index (j+1,
+ * k+1) is this restatement's choice where the reference is undefined.
  */
 #ifndef CF_ORACLE_H
 #define CF_ORACLE_H
@@ -78,13 +85,19 @@ void cfo_filter_ratios(const cfo_grid* g, double* const U_[5], const double* Y_p
  * point-Jacobi sweeps NR = D^-1 (MR - L NL - U NL), NL <- NR, then the
  * same update and norms.  The reference reads the upper neighbours' NL at
  * indices it never assigns (UB); this restatement reads them at j+1, k+1,
- * where it reads their states and metrics.  Parity with the reference is
- * not pinned for the sweeps (the RHS, D^-1 and update are those of
- * cfo_dual_time). */
+ * where it reads their states and metrics.  The RHS, D^-1 and update are
+ * those of cfo_dual_time. */
 void cfo_dplusgs(const cfo_grid* g, double deltaT, double* const U_[5], double* const U[5], double* const Uq[5],
                  const double* J, double* const m[9], double* const inFx[5], double* const inFy[5],
                  double* const inFz[5], double* const vF[4], double* const MR[5], double* const NL[5],
                  double* const NR[5], double er[5]);
+
+/* RK3NODT.cpp:115-183: stage RK (1..3) of the explicit SSP-RK3 update,
+ * U_ <- c1 U + c2 U_ + c3 (dt Rk), c = (1, 0, 1), (3/4, 1/4, 1/4),
+ * (1/3, 2/3, 2/3); MR <- dt Rk. */
+void cfo_rk3nodt(const cfo_grid* g, int RK, double deltaT, double* const U_[5], double* const U[5], const double* J,
+                 double* const inFx[5], double* const inFy[5], double* const inFz[5], double* const vF[4],
+                 double* const MR[5]);
 
 /* The implicit sweeps alone, on a given MR: ghosts of NL/NR zeroed,
  * NL = D^-1 MR, then mode 0 (LU-SGS, lexicographic), 1 (LU-SGS, plane by

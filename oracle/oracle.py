@@ -57,6 +57,7 @@ def lib():
         _lib.cfo_dual_time_hyperplane.argtypes = dt_args
         _lib.cfo_dplusgs.argtypes = dt_args
         _lib.cfo_sweeps_apply.argtypes = [C.POINTER(Grid), C.c_double, C.c_int, PP, P, PP, PP, PP, PP]
+        _lib.cfo_rk3nodt.argtypes = [C.POINTER(Grid), C.c_int, C.c_double, PP, PP, P, PP, PP, PP, PP, PP]
     return _lib
 
 
@@ -163,6 +164,27 @@ def sweeps_apply(cfg, mode, U_, met, MR, deltaT=None):
     lib().cfo_sweeps_apply(C.byref(grid_of(cfg)), cfg.deltaT if deltaT is None else deltaT, int(mode), _pp(U_),
                            _p(met["J"]), _pp(met["m"]), _pp(MR), _pp(NL), _pp(NR))
     return NR
+
+
+def rk3nodt(cfg, RK, U_, U, J, inFx, inFy, inFz, vF, deltaT=None):
+    """RK3NODT stage RK on U_ in place; returns MR = dt Rk."""
+    MR = zeros(cfg, 5)
+    lib().cfo_rk3nodt(C.byref(grid_of(cfg)), int(RK), cfg.deltaT if deltaT is None else deltaT, _pp(U_), _pp(U),
+                      _p(J), _pp(inFx), _pp(inFy), _pp(inFz), _pp(vF), _pp(MR))
+    return MR
+
+
+def rk3_step(cfg, U_, U, met, with_viscous=True, deltaT=None):
+    """One pass of main.cpp:152-377 with the RK3NODT path: three stages of
+    ghosts, fluxes, viscous terms (vF = 0 when with_viscous is False, as when
+    main.cpp leaves Viscous_terms commented out) and RK3NODT."""
+    for RK in (1, 2, 3):
+        halo_bc(cfg, U_, met["J"])
+        inFx = flux(cfg, 0, U_, met["J"], met["J_u"], met["m_u"])
+        inFy = flux(cfg, 1, U_, met["J"], met["J_v"], met["m_v"])
+        inFz = flux(cfg, 2, U_, met["J"], met["J_w"], met["m_w"])
+        vF = viscous(cfg, U_, met) if with_viscous else zeros(cfg, 4)
+        rk3nodt(cfg, RK, U_, U, met["J"], inFx, inFy, inFz, vF, deltaT=deltaT)
 
 
 def inner_iteration(cfg, U_, U, Uq, met, dplusgs=False):

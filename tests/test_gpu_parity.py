@@ -200,6 +200,56 @@ def test_gpu_inner_iteration_dplusgs_16x12x16():
     np.testing.assert_allclose(np.array(er), np.array(er_o), rtol=TOL_KERNEL, atol=0)
 
 
+@pytest.mark.parametrize("grid", UNIT_GRIDS)
+@pytest.mark.parametrize("prefix", ["unit", "var"])
+def test_gpu_rk3nodt(grid, prefix, golden):
+    """RK3NODT stages 1-3 bit for bit against the reference; MR = dt Rk
+    bit for bit against the oracle."""
+    from oracle import oracle as O
+    z = golden(f"{prefix}_g{grid[0]}x{grid[1]}x{grid[2]}")
+    # the perturbed-state fixtures share the flat-state fixture's grid metrics
+    met = fixture_metrics(golden(f"unit_g{grid[0]}x{grid[1]}x{grid[2]}"))
+    cfg = unit_cfg(grid)
+    m = masks(cfg)["int"]
+    with solver(cfg) as s:
+        for RK in (1, 2, 3):
+            U_ = [a.copy() for a in fields(z, "U_bc")]
+            MR = s.RK3NODT(RK, U_, fields(z, "U"), fields(z, "Uq"), met, fields(z, "inFx"),
+                           fields(z, "inFy"), fields(z, "inFz"), fields(z, "vF", 4), deltaT=5e-8)
+            Uo = [a.copy() for a in fields(z, "U_bc")]
+            MRo = O.rk3nodt(cfg, RK, Uo, fields(z, "U"), np.ascontiguousarray(z["J"]), fields(z, "inFx"),
+                            fields(z, "inFy"), fields(z, "inFz"), fields(z, "vF", 4), deltaT=5e-8)
+            for v in range(5):
+                np.testing.assert_array_equal(U_[v][m], z[f"U_rk{RK}_{v + 1}"][m])
+                np.testing.assert_array_equal(MR[v][m], MRo[v][m])
+
+
+@pytest.mark.parametrize("viscous", [True, False])
+def test_gpu_rk3_step_16x12x16(viscous):
+    """Two passes of the device-resident RK3NODT path (three stages of
+    ghosts, fluxes, viscous terms or none, and the update) against the
+    oracle, bit for bit."""
+    from oracle import oracle as O
+    from channel_flow_amd.config import ChannelConfig, FlowParams, poiseuille_state
+    from channel_flow_amd.solver import Solver
+    cfg = ChannelConfig(16, 12, 16, gamma_metric=1.5, deltaT=5e-8)
+    Qn = poiseuille_state(cfg, FlowParams(), seed=11, stream=0)
+    met = O.metrics(cfg)
+    U = O.state_from_Q(cfg, Qn, met["J"])
+    Ui = [a.copy() for a in U]
+    for _ in range(2):
+        O.rk3_step(cfg, Ui, U, met, with_viscous=viscous)
+    sl = (slice(3, cfg.gend + 1), slice(2, cfg.ny + 1), slice(2, cfg.nz + 1))
+    qi_o = np.stack([Ui[v][sl] * met["J"][sl] for v in range(5)])
+    with Solver(cfg) as s:
+        s.Jacobian_all(1.5, want_arrays=False)
+        s.load_state(Qn, Qn)
+        for _ in range(2):
+            s.rk3_step(viscous)
+        qi = s.read_state()[2]
+    np.testing.assert_array_equal(qi, qi_o)
+
+
 def gpu_trajectory(cfg, Qn, Qm, steps):
     rows = []
     with solver(cfg) as s:

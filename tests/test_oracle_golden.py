@@ -229,3 +229,50 @@ def test_filter_ratios_pinned_by_thresholds(grid, golden):
             np.testing.assert_array_equal(Ep[a][m], pats[r][a][m], err_msg=f"run {r} Ep{'XYZ'[a]}")
     for s, (a, i, j, k) in enumerate(pin["faces"]):
         assert pats[2 * s][a][i, j, k] == 1.0 and pats[2 * s + 1][a][i, j, k] == 0.0
+
+
+@pytest.mark.parametrize("grid", UNIT_GRIDS)
+@pytest.mark.parametrize("prefix", ["unit", "var"])
+def test_rk3nodt_matches_reference(grid, prefix, golden):
+    """RK3NODT.cpp:115-183, stages 1-3 from the same post-BC state, fluxes
+    and viscous terms, bit for bit against the reference's U_."""
+    z = golden(f"{prefix}_g{grid[0]}x{grid[1]}x{grid[2]}")
+    cfg = unit_cfg(grid)
+    m = masks(cfg)["int"]
+    for RK in (1, 2, 3):
+        U_ = [a.copy() for a in fields(z, "U_bc")]
+        MR = O.rk3nodt(cfg, RK, U_, fields(z, "U"), np.ascontiguousarray(z["J"]), fields(z, "inFx"),
+                       fields(z, "inFy"), fields(z, "inFz"), fields(z, "vF", 4), deltaT=5e-8)
+        for v in range(5):
+            np.testing.assert_array_equal(U_[v][m], z[f"U_rk{RK}_{v + 1}"][m], err_msg=f"stage {RK} U{v + 1}_")
+        # stage 1 is U + dt Rk exactly as formed: MR = U_ - U there
+        if RK == 1:
+            for v in range(5):
+                np.testing.assert_array_equal(U_[v][m], (1.0 * fields(z, "U")[v] + 0.0 * z[f"U_bc{v + 1}"]
+                                                         + 1.0 * MR[v])[m])
+
+
+def test_rk3_step_stage_structure():
+    """The oracle's three-stage pass: stage 1 from U_ = U is U + dt R(U), and
+    with the viscous terms off the pass differs from the viscous one only
+    through vF."""
+    from channel_flow_amd.config import ChannelConfig, FlowParams, poiseuille_state
+    cfg = ChannelConfig(10, 8, 9, gamma_metric=1.5, deltaT=5e-8)
+    met = O.metrics(cfg)
+    U = O.state_from_Q(cfg, poiseuille_state(cfg, FlowParams(), seed=2, stream=0), met["J"])
+    a, b = [x.copy() for x in U], [x.copy() for x in U]
+    O.rk3_step(cfg, a, U, met, with_viscous=True)
+    O.rk3_step(cfg, b, U, met, with_viscous=False)
+    m = masks(cfg)["int"]
+    for v in range(5):
+        assert np.isfinite(a[v][m]).all() and np.isfinite(b[v][m]).all()
+    assert any(not np.array_equal(a[v][m], b[v][m]) for v in range(1, 5))
+    # within one stage the mass equation carries no viscous term
+    O.halo_bc(cfg, U, met["J"])
+    F = [O.flux(cfg, d, U, met["J"], met[f"J_{x}"], met[f"m_{x}"]) for d, x in enumerate("uvw")]
+    rng = np.random.default_rng(4)
+    c, d = [x.copy() for x in U], [x.copy() for x in U]
+    O.rk3nodt(cfg, 1, c, U, met["J"], *F, [rng.standard_normal(U[0].shape) for _ in range(4)])
+    O.rk3nodt(cfg, 1, d, U, met["J"], *F, O.zeros(cfg, 4))
+    np.testing.assert_array_equal(c[0][m], d[0][m])
+    assert all(not np.array_equal(c[v][m], d[v][m]) for v in range(1, 5))
