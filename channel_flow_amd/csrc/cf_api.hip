@@ -53,6 +53,7 @@ struct cf_ctx {
   long spin_limit = 1L << 22;
   double* partial = nullptr;
   double* er_dev = nullptr;
+  double* stat = nullptr;  // running per-j sums of Statistic's mean profiles (7 * Ym)
   double* prof = nullptr;  // metric j-profiles (NPROF * Ym)
   std::vector<double> hprof;
   std::vector<double> staging;  // reference-layout staging for 2D copies
@@ -241,6 +242,8 @@ extern "C" int cf_create(const cf_params* p, cf_ctx** out) {
   long nb = std::max(cf::update_blocks(g), cf::sweep_update_blocks(g));
   if (hipMalloc(&c->partial, (size_t)nb * 5 * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
   if (hipMalloc(&c->er_dev, 8 * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
+  if (hipMalloc(&c->stat, (size_t)7 * g.Ym * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
+  if (hipMemsetAsync(c->stat, 0, (size_t)7 * g.Ym * sizeof(double), c->stream) != hipSuccess) return bail(CF_ERR_HIP);
   if (hipMalloc(&c->prof, (size_t)cf::PROF_ROWS * g.Ym * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
   if (hipMalloc(&c->ypt, (size_t)g.Ym * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
   if (hipMemsetAsync(c->ypt, 0, (size_t)g.Ym * sizeof(double), c->stream) != hipSuccess) return bail(CF_ERR_HIP);
@@ -255,7 +258,7 @@ extern "C" void cf_destroy(cf_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   double* ptrs[] = {c->Ui, c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->MR,  c->NL,     c->NR,
-                    c->Pr, c->VS, c->Ep, c->Fw, c->ypt, c->Sm, c->partial, c->er_dev, c->prof};
+                    c->Pr, c->VS, c->Ep, c->Fw, c->ypt, c->Sm, c->partial, c->er_dev, c->prof, c->stat};
   for (double* q : ptrs)
     if (q) (void)hipFree(q);
   if (c->order) (void)hipFree(c->order);
@@ -971,6 +974,36 @@ extern "C" int cf_rk3_step(cf_ctx* c, int viscous) {
   }
   HIPCHK(c, hipGetLastError());
   return sync(c);
+}
+
+extern "C" int cf_statistic(cf_ctx* c, int step, int statistic_step, const double* const U_[5],
+                            const char* directory) {
+  if (!c || statistic_step < 1) return fail(c, CF_ERR_ARG, "statistic_step must be >= 1");
+  const Geo& g = c->g;
+  int r;
+  if (U_) {
+    if ((r = h2d_n(c, c->Ui, U_, 5))) return r;
+  } else if (!c->have_state) {
+    return fail(c, CF_ERR_STATE, "no state loaded");
+  }
+  cf::launch_stat(c->stream, c->Ui, c->stat, g);
+  HIPCHK(c, hipGetLastError());
+  if (step % statistic_step != 0) return sync(c);
+  std::vector<double> h((size_t)7 * g.Ym);
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->stat, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->stat, 0, h.size() * 8, c->stream));
+  if ((r = sync(c))) return r;
+  // Statistic.cpp:1117-1146: average, one "%.16f" line per j = 2..ny for
+  // U, V, W, UU, VV, WW, UV, into st%05d.dat
+  const double inv = 1. / statistic_step / c->p.X_out / c->p.Z_out;
+  char path[4096];
+  snprintf(path, sizeof path, "%s/st%05d.dat", directory ? directory : ".", step);
+  FILE* f = fopen(path, "w");
+  if (!f) return fail(c, CF_ERR_IO, "cannot open %s", path);
+  for (int q = 0; q < 7; q++)
+    for (int j = 2; j <= g.ny; j++) fprintf(f, "%.16f\n", h[(size_t)q * g.Ym + j] * inv);
+  if (fclose(f) != 0) return fail(c, CF_ERR_IO, "cannot write %s", path);
+  return CF_OK;
 }
 
 extern "C" int cf_set_viscous_schedule(cf_ctx* c, int schedule) {

@@ -2059,6 +2059,46 @@ void launch_rk3(hipStream_t st, int RK, double* Ui, const double* Un, const doub
                      MR, g, deltaT, f_drive, c1[RK - 1], c2[RK - 1], c3[RK - 1]);
 }
 
+// Statistic.cpp:1043-1071: one block per j adds the state's sums over i and
+// k of U, V, W, UU, VV, WW, UV to sums[q * Ym + j].  Each thread sums its
+// cells in (i, k) order, then the block combines the 256 partial sums in a
+// fixed tree, so repeated runs give the same bits (the reference's single
+// running sum is matched to rounding, not bit for bit).
+__global__ void __launch_bounds__(256) k_stat(const double* __restrict__ U, double* __restrict__ sums, Geo g) {
+  __shared__ double red[7][256];
+  const int j = 2 + blockIdx.x;
+  const int t = threadIdx.x;
+  const int X = g.gend - 2, Z = g.nz - 1;
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (long c = t; c < (long)X * Z; c += 256) {
+    const int i = 3 + (int)(c / Z), k = 2 + (int)(c % Z);
+    const long p = at(g, i, j, k);
+    const double r = U[p];
+    const double u = U[g.nf + p] / r, v = U[2 * g.nf + p] / r, w = U[3 * g.nf + p] / r;
+    acc[0] += u;
+    acc[1] += v;
+    acc[2] += w;
+    acc[3] += u * u;
+    acc[4] += v * v;
+    acc[5] += w * w;
+    acc[6] += u * v;
+  }
+#pragma unroll
+  for (int q = 0; q < 7; q++) red[q][t] = acc[q];
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h)
+#pragma unroll
+      for (int q = 0; q < 7; q++) red[q][t] += red[q][t + h];
+    __syncthreads();
+  }
+  if (t < 7) sums[t * g.Ym + j] += red[t][0];
+}
+
+void launch_stat(hipStream_t st, const double* U, double* sums, const Geo& g) {
+  hipLaunchKernelGGL(k_stat, dim3(g.ny - 1), dim3(256), 0, st, U, sums, g);
+}
+
 void launch_rhs(hipStream_t st, const double* Ui, const double* Un, const double* Uq, const double* Fx,
                 const double* Fy, const double* Fz, const double* vF, const double* prof, double* MR, double* NL,
                 const Geo& g, double deltaT, double f_drive) {

@@ -31,6 +31,7 @@ EXPORTS = [
     "cf_profile_inner_iteration", "cf_set_sweep_schedule", "cf_flux_x_limiter", "cf_flux_y_limiter",
     "cf_flux_z_limiter", "cf_grid", "cf_filter", "cf_profile_variants", "cf_dplusgs", "cf_set_implicit_solver",
     "cf_set_viscous_schedule", "cf_rk3nodt", "cf_rk3_step",
+    "cf_statistic",
 ]
 
 
@@ -96,6 +97,7 @@ def load_library() -> C.CDLL:
     lib.cf_set_viscous_schedule.argtypes = [vp, C.c_int]
     lib.cf_rk3nodt.argtypes = [vp, C.c_int, C.c_double, P, PP, PP, PP, P, PP, PP, PP, PP, PP, PP]
     lib.cf_rk3_step.argtypes = [vp, C.c_int]
+    lib.cf_statistic.argtypes = [vp, C.c_int, C.c_int, PP, C.c_char_p]
     lib.cf_write_output.argtypes = [vp, C.c_int, C.c_char_p]
     lib.cf_synchronize.argtypes = [vp]
     _lib = lib
@@ -275,6 +277,13 @@ class Solver:
         self._chk(self.lib.cf_rk3nodt(self._h, int(RK), dt, er, _pp(U_), _pp(U), _pp(Uq), _p(met["J"]),
                                       _pp(met["m"]), _pp(inFx), _pp(inFy), _pp(inFz), _pp(vF), _pp(MR)))
         return MR
+
+    def Statistic(self, step, statistic_step, U_=None, directory="."):
+        """Statistic.cpp:1043-1154 mean profiles; writes st%05d.dat when
+        step % statistic_step == 0.  U_: host arrays, or None for the
+        device iterate."""
+        ptr = _pp([np.ascontiguousarray(a, dtype=np.float64) for a in U_]) if U_ is not None else None
+        self._chk(self.lib.cf_statistic(self._h, int(step), int(statistic_step), ptr, str(directory).encode()))
 
     def rk3_step(self, viscous: bool = True):
         """Three RK3NODT stages on the device state (main.cpp:152-377)."""

@@ -219,6 +219,17 @@ int cf_rk3nodt(cf_ctx* ctx, int RK, double deltaT, double er[5], double* const U
  * commented out and vF is never assigned) and RK3NODT, from the iterate
  * with level n as the base.  cf_advance_levels then shifts the levels. */
 int cf_rk3_step(cf_ctx* ctx, int viscous);
+/* Statistic.cpp:1043-1154 (main.h Statistic), the mean profiles: adds the
+ * per-j sums over i and k of U, V, W, UU, VV, WW, UV (U = U2_/U1_ ...) of
+ * the state to running sums kept in the context; when step % statistic_step
+ * == 0 writes <directory>/st%05d.dat (the sums / statistic_step / X_out /
+ * Z_out, seven blocks of one "%.16f" line per j = 2..ny, the reference's
+ * format) and resets the sums.  U_: reference-layout host arrays, or NULL
+ * for the device iterate.  The per-j sums are combined in a fixed tree: the
+ * means equal the reference's to rounding (not bit for bit).  The
+ * reference's other Statistic outputs (ND%05d.dat, Shear_stress.dat) are
+ * not produced. */
+int cf_statistic(cf_ctx* ctx, int step, int statistic_step, const double* const U_[5], const char* directory);
 /* Kernel schedule of Viscous_terms (Viscous_terms.cpp:212-883); both give
  * identical bits.  0 (default): one kernel that keeps the primitives and
  * face fluxes of a tile in LDS; 1: primitives, face fluxes and divergence as

@@ -303,6 +303,20 @@ inline void RK3NODT(int, int RK, double deltaT, double* er1, double* er2, double
                    "RK3NODT");
 }
 
+// Statistic.cpp:1043-1154 (main.h): the mean profiles and st%05d.dat in the
+// working directory, like the reference; the numerical-dissipation and
+// shear-stress statistics of Statistic.cpp:100-1040 are not produced
+inline void Statistic(int, int step, int iteration, int statistic_step, CF_F(U1_), CF_F(U2_), CF_F(U3_), CF_F(U4_),
+                      CF_F(U5_), CF_F(J), CF_F(J_v), CF_F(xidx_v), CF_F(xidy_v), CF_F(xidz_v), CF_F(etdx_v),
+                      CF_F(etdy_v), CF_F(etdz_v), CF_F(ztdx_v), CF_F(ztdy_v), CF_F(ztdz_v), CF_FIVE(MR), CF_FIVE(ML),
+                      CF_F(EpY)) {
+  (void)iteration, (void)J, (void)J_v, (void)xidx_v, (void)xidy_v, (void)xidz_v, (void)etdx_v, (void)etdy_v;
+  (void)etdz_v, (void)ztdx_v, (void)ztdy_v, (void)ztdz_v, (void)MR1, (void)MR2, (void)MR3, (void)MR4, (void)MR5;
+  (void)ML1, (void)ML2, (void)ML3, (void)ML4, (void)ML5, (void)EpY;
+  const double* const U[5] = {CF_P(U1_), CF_P(U2_), CF_P(U3_), CF_P(U4_), CF_P(U5_)};
+  cf_compat::check(cf_statistic(cf_compat::ctx("Statistic"), step, statistic_step, U, "."), "Statistic");
+}
+
 // DPLUSGS.cpp:189-1361 (the Dual_time_stepping parameters without Nijk/ijk);
 // the upper neighbours' NL is read at j+1, k+1 (the reference's j_, k_ are
 // never assigned)

@@ -1109,6 +1109,28 @@ void cfo_rk3nodt(const cfo_grid* g, int RK, double deltaT, double* const U_[5], 
   (void)sJ;
 }
 
+/* Statistic.cpp:1043-1071: adds one state's sums over i and k of
+ * U, V, W, UU, VV, WW, UV (velocities U_2/U_1 ...) to sums[q * Y_m + j],
+ * q = 0..6, in the reference's order (j, then i, then k). */
+void cfo_statistic_accum(const cfo_grid* g, double* const U_[5], double* sums) {
+  dims_t d = dims(g);
+  for (int j = 2; j <= d.ny; j++)
+    for (int i = 3; i <= d.gend; i++)
+      for (int k = 2; k <= d.nz; k++) {
+        size_t c = IX(d, i, j, k);
+        double U = U_[1][c] / U_[0][c];
+        double V = U_[2][c] / U_[0][c];
+        double W = U_[3][c] / U_[0][c];
+        sums[0 * d.Ym + j] = sums[0 * d.Ym + j] + U;
+        sums[1 * d.Ym + j] = sums[1 * d.Ym + j] + V;
+        sums[2 * d.Ym + j] = sums[2 * d.Ym + j] + W;
+        sums[3 * d.Ym + j] = sums[3 * d.Ym + j] + U * U;
+        sums[4 * d.Ym + j] = sums[4 * d.Ym + j] + V * V;
+        sums[5 * d.Ym + j] = sums[5 * d.Ym + j] + W * W;
+        sums[6 * d.Ym + j] = sums[6 * d.Ym + j] + U * V;
+      }
+}
+
 void cfo_sweeps_apply(const cfo_grid* g, double deltaT, int mode, double* const U_[5], const double* J,
                       double* const m[9], double* const MR[5], double* const NL[5], double* const NR[5]) {
   dims_t d = dims(g);

@@ -99,6 +99,10 @@ void cfo_rk3nodt(const cfo_grid* g, int RK, double deltaT, double* const U_[5], 
                  double* const inFx[5], double* const inFy[5], double* const inFz[5], double* const vF[4],
                  double* const MR[5]);
 
+/* Statistic.cpp:1043-1071: adds the state's sums over i, k of U, V, W, UU,
+ * VV, WW, UV to sums[q * Y_m + j] (q = 0..6), in the reference's order. */
+void cfo_statistic_accum(const cfo_grid* g, double* const U_[5], double* sums);
+
 /* The implicit sweeps alone, on a given MR: ghosts of NL/NR zeroed,
  * NL = D^-1 MR, then mode 0 (LU-SGS, lexicographic), 1 (LU-SGS, plane by
  * plane) or 2 (DPLUSGS Jacobi).  A test hook: it lets the tests read the

@@ -57,6 +57,7 @@ def lib():
         _lib.cfo_dual_time_hyperplane.argtypes = dt_args
         _lib.cfo_dplusgs.argtypes = dt_args
         _lib.cfo_sweeps_apply.argtypes = [C.POINTER(Grid), C.c_double, C.c_int, PP, P, PP, PP, PP, PP]
+        _lib.cfo_statistic_accum.argtypes = [C.POINTER(Grid), PP, P]
         _lib.cfo_rk3nodt.argtypes = [C.POINTER(Grid), C.c_int, C.c_double, PP, PP, P, PP, PP, PP, PP, PP]
     return _lib
 
@@ -164,6 +165,31 @@ def sweeps_apply(cfg, mode, U_, met, MR, deltaT=None):
     lib().cfo_sweeps_apply(C.byref(grid_of(cfg)), cfg.deltaT if deltaT is None else deltaT, int(mode), _pp(U_),
                            _p(met["J"]), _pp(met["m"]), _pp(MR), _pp(NL), _pp(NR))
     return NR
+
+
+class Statistic:
+    """Statistic.cpp:1043-1154, the mean profiles: per-j sums accumulated over
+    the calls of statistic_step steps, then averaged (1/statistic_step/
+    X_out/Z_out) and written as st%05d.dat (seven blocks of "%.16f" lines,
+    j = 2..ny: U, V, W, UU, VV, WW, UV), then reset."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        self.sums = np.zeros(7 * cfg.shape[1])
+
+    def __call__(self, step, statistic_step, U_):
+        lib().cfo_statistic_accum(C.byref(grid_of(self.cfg)), _pp(U_), _p(self.sums))
+        if step % statistic_step != 0:
+            return None
+        c = self.cfg
+        Ym = c.shape[1]
+        inv = 1. / statistic_step / c.X_out / c.Z_out
+        lines = []
+        for q in range(7):
+            for j in range(2, c.ny + 1):
+                lines.append("%.16f\n" % (self.sums[q * Ym + j] * inv))
+        self.sums[:] = 0
+        return "".join(lines)
 
 
 def rk3nodt(cfg, RK, U_, U, J, inFx, inFy, inFz, vF, deltaT=None):

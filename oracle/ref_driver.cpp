@@ -16,6 +16,8 @@
 //   ref_driver traj <dir> <deltaT> <steps> <iteration_end_step>
 //       the main.cpp step loop; writes er.txt (one line per inner iteration:
 //       step it er1..er5 raw and normalised) and LES%05d.bin via Output().
+//   ref_driver stat <dir> <deltaT>
+//       Statistic's mean profiles over two steps and over one (st*.dat).
 //   ref_driver time <dir> <deltaT> <inner_iterations>
 //       times the inner iteration (halo+BC, fluxes, viscous, dual time).
 #include <stdlib.h>
@@ -199,6 +201,12 @@ static void dplusgs(State& s, int myid, double deltaT, double er[5]) {
           s.NL[4], s.Res[0], s.Res[1], s.Res[2], s.Res[3], s.Res[4]);
 }
 
+static void statistic(State& s, int myid, F3 u[5], int step, int statistic_step) {
+  Statistic(myid, step, 1, statistic_step, u[0], u[1], u[2], u[3], u[4], s.J, s.Jv, s.mv[0], s.mv[1], s.mv[2],
+            s.mv[3], s.mv[4], s.mv[5], s.mv[6], s.mv[7], s.mv[8], s.MR[0], s.MR[1], s.MR[2], s.MR[3], s.MR[4],
+            s.ML[0], s.ML[1], s.ML[2], s.ML[3], s.ML[4], s.EpY);
+}
+
 static void copy5(F3 dst[5], F3 src[5]) {
   for (int v = 0; v < 5; v++) memcpy(dst[v], src[v], g_n * sizeof(double));
 }
@@ -351,6 +359,16 @@ int main(int argc, char** argv) {
       r++;
     }
     fclose(cf);
+  } else if (mode == "stat") {
+    // Statistic.cpp:1043-1154, the mean profiles: step 1 accumulates the
+    // post-BC iterate and step 2 the level-n state with statistic_step = 2
+    // (st00002.dat, their average); step 3 with statistic_step = 1 writes the
+    // iterate alone (st00003.dat)
+    load5("iterate.bin", s.U_, s.J);
+    halo_bc(s, myid);
+    statistic(s, myid, s.U_, 1, 2);
+    statistic(s, myid, s.U, 2, 2);
+    statistic(s, myid, s.U_, 3, 1);
   } else if (mode == "time" && argc >= 5) {
     int nit = atoi(argv[4]);
     double er[5];

@@ -200,12 +200,43 @@ def traj_fixture(spec, keep_state: bool):
     return name, meta
 
 
+def stat_fixture(grid, seed):
+    """Statistic's mean profiles (st%05d.dat, Statistic.cpp:1043-1154) from
+    the reference: the variant fixture's post-BC iterate (step 1) and
+    level-n state (step 2) averaged over statistic_step = 2, then the
+    iterate alone with statistic_step = 1.  The files are kept as text."""
+    cfg = ChannelConfig(X_out=grid[0], Y_out=grid[1], Z_out=grid[2])
+    exe = ref_driver(grid)
+    with tempfile.TemporaryDirectory() as td:
+        d = Path(td)
+        Qn, Qm = perturbed_state(cfg, seed, 0), perturbed_state(cfg, seed, 1)
+        write_initial_bin(d / "initial.bin", Qn, Qm)
+        np.ascontiguousarray(perturbed_state(cfg, seed, 2)).tofile(d / "iterate.bin")
+        subprocess.run([str(exe), "stat", str(d), repr(DT_UNIT)], check=True, stdout=subprocess.DEVNULL,
+                       env=dict(os.environ, OMP_NUM_THREADS="1"))
+        arrs = {f"st{n}": np.frombuffer((d / f"st0000{n}.dat").read_bytes(), dtype=np.uint8) for n in (2, 3)}
+    meta = dict(grid=list(grid), seed=seed, inputs=f"var_g{grid[0]}x{grid[1]}x{grid[2]} U_bc (iterate) and U",
+                reference_calls=["Statistic(step 1, statistic_step 2, iterate)",
+                                 "Statistic(step 2, statistic_step 2, level n) -> st00002.dat",
+                                 "Statistic(step 3, statistic_step 1, iterate) -> st00003.dat"])
+    name = f"stat_g{grid[0]}x{grid[1]}x{grid[2]}"
+    np.savez_compressed(OUT / f"{name}.npz", **arrs)
+    return name, meta
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "variants":
         # regenerate only the variant fixtures and merge them into the manifest
         manifest = json.loads((OUT / "manifest.json").read_text())
         for i, g in enumerate(VAR_GRIDS):
             n, m = variant_fixture(g, seed=21 + i)
+            manifest["fixtures"][n] = m
+        (OUT / "manifest.json").write_text(json.dumps(manifest, indent=1))
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "stat":
+        manifest = json.loads((OUT / "manifest.json").read_text())
+        for i, g in enumerate(VAR_GRIDS):
+            n, m = stat_fixture(g, seed=21 + i)
             manifest["fixtures"][n] = m
         (OUT / "manifest.json").write_text(json.dumps(manifest, indent=1))
         return
@@ -227,6 +258,9 @@ def main():
         manifest["fixtures"][n] = m
     for g in VAR_GRIDS:
         n, m = filter_pin_fixture(g)
+        manifest["fixtures"][n] = m
+    for i, g in enumerate(VAR_GRIDS):
+        n, m = stat_fixture(g, seed=21 + i)
         manifest["fixtures"][n] = m
     n, m = traj_fixture(TRAJ, keep_state=True)
     manifest["fixtures"][n] = m

@@ -250,6 +250,50 @@ def test_gpu_rk3_step_16x12x16(viscous):
     np.testing.assert_array_equal(qi, qi_o)
 
 
+def _st_values(text):
+    return np.array([float(x) for x in text.split()])
+
+
+@pytest.mark.parametrize("grid", VAR_GRIDS)
+def test_gpu_statistic(grid, golden, tmp_path):
+    """Statistic's mean profiles on the GPU against the reference's
+    st00002.dat / st00003.dat: same lines and format, values to 1e-13 of the
+    velocity scale (the per-j sums are combined in a fixed tree, not in the
+    reference's single running sum)."""
+    z = golden(f"var_g{grid[0]}x{grid[1]}x{grid[2]}")
+    st = golden(f"stat_g{grid[0]}x{grid[1]}x{grid[2]}")
+    cfg = unit_cfg(grid)
+    with solver(cfg) as s:
+        s.Statistic(1, 2, fields(z, "U_bc"), tmp_path)
+        assert not (tmp_path / "st00001.dat").exists()
+        s.Statistic(2, 2, fields(z, "U"), tmp_path)
+        s.Statistic(3, 1, fields(z, "U_bc"), tmp_path)
+    for n in (2, 3):
+        ref = st[f"st{n}"].tobytes().decode()
+        got = (tmp_path / f"st0000{n}.dat").read_text()
+        assert got.count("\n") == ref.count("\n") == 7 * (cfg.ny - 1)
+        assert all(len(x.split(".")[1]) == 16 for x in got.split())
+        a, b = _st_values(got), _st_values(ref)
+        np.testing.assert_allclose(a, b, rtol=1e-13, atol=1e-13 * np.abs(b).max())
+
+
+def test_gpu_statistic_device_state(tmp_path):
+    """The device iterate (U_ = NULL) gives the oracle's profiles."""
+    from oracle import oracle as O
+    from channel_flow_amd.config import ChannelConfig, FlowParams, poiseuille_state
+    from channel_flow_amd.solver import Solver
+    cfg = ChannelConfig(16, 12, 16, gamma_metric=1.5, deltaT=5e-8)
+    Qn = poiseuille_state(cfg, FlowParams(), seed=5, stream=0)
+    met = O.metrics(cfg)
+    ref = O.Statistic(cfg)(4, 1, O.state_from_Q(cfg, Qn, met["J"]))
+    with Solver(cfg) as s:
+        s.Jacobian_all(1.5, want_arrays=False)
+        s.load_state(Qn, Qn)
+        s.Statistic(4, 1, None, tmp_path)
+    a, b = _st_values((tmp_path / "st00004.dat").read_text()), _st_values(ref)
+    np.testing.assert_allclose(a, b, rtol=1e-13, atol=1e-13 * np.abs(b).max())
+
+
 def gpu_trajectory(cfg, Qn, Qm, steps):
     rows = []
     with solver(cfg) as s:

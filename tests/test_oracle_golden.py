@@ -276,3 +276,16 @@ def test_rk3_step_stage_structure():
     O.rk3nodt(cfg, 1, d, U, met["J"], *F, O.zeros(cfg, 4))
     np.testing.assert_array_equal(c[0][m], d[0][m])
     assert all(not np.array_equal(c[v][m], d[v][m]) for v in range(1, 5))
+
+
+@pytest.mark.parametrize("grid", VAR_GRIDS)
+def test_statistic_matches_reference(grid, golden):
+    """Statistic.cpp:1043-1154: st00002.dat (iterate and level n averaged
+    over two steps) and st00003.dat (the iterate alone) byte for byte, so
+    the sum order, the scaling and the "%.16f" format are the reference's."""
+    z = golden(f"var_g{grid[0]}x{grid[1]}x{grid[2]}")
+    st = golden(f"stat_g{grid[0]}x{grid[1]}x{grid[2]}")
+    S = O.Statistic(unit_cfg(grid))
+    assert S(1, 2, fields(z, "U_bc")) is None
+    assert S(2, 2, fields(z, "U")) == st["st2"].tobytes().decode()
+    assert S(3, 1, fields(z, "U_bc")) == st["st3"].tobytes().decode()
