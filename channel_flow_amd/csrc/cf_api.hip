@@ -976,6 +976,37 @@ extern "C" int cf_rk3_step(cf_ctx* c, int viscous) {
   return sync(c);
 }
 
+extern "C" int cf_driven_force(cf_ctx* c, double Q0, const double* U2_, const double* J, double* Qall,
+                               double* ER_Q) {
+  if (!c || !Qall || !ER_Q) return fail(c, CF_ERR_ARG, "null argument");
+  const Geo& g = c->g;
+  int r;
+  if (J && ((r = ingest(c, J, cf::SET_C, 0, "J")) || (r = upload_prof(c)))) return r;
+  const double* U2 = c->Ui + g.nf;
+  if (U2_) {
+    // reference layout [X_np][Y_m][Z_m] -> the device rows of Zp
+    if ((r = h2d(c, c->Ui + g.nf, U2_))) return r;
+  } else if (!c->have_state) {
+    return fail(c, CF_ERR_STATE, "no state loaded");
+  }
+  // Driven_force.cpp:45-47, in the reference's order
+  double Jsum = 0;
+  for (int j = 2; j <= g.ny; j++) Jsum = Jsum + 1. / c->hprof[j];
+  const int X = g.gend - 2;
+  if ((size_t)X > (size_t)cf::update_blocks(g) * 5) return fail(c, CF_ERR_ARG, "grid too large for the partials");
+  cf::launch_bulk_flux(c->stream, U2, c->prof, Jsum, c->partial, g);
+  HIPCHK(c, hipGetLastError());
+  std::vector<double> part((size_t)X);
+  HIPCHK(c, hipMemcpyAsync(part.data(), c->partial, part.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  if ((r = sync(c))) return r;
+  double Q = 0;
+  for (int i = 0; i < X; i++) Q = Q + part[i];
+  Q = Q / c->p.X_out / c->p.Z_out;
+  *Qall = Q;
+  *ER_Q = fabs(Q0 - Q) / Q0;
+  return CF_OK;
+}
+
 extern "C" int cf_statistic(cf_ctx* c, int step, int statistic_step, const double* const U_[5],
                             const char* directory) {
   if (!c || statistic_step < 1) return fail(c, CF_ERR_ARG, "statistic_step must be >= 1");

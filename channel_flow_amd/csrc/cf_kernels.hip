@@ -2099,6 +2099,33 @@ void launch_stat(hipStream_t st, const double* U, double* sums, const Geo& g) {
   hipLaunchKernelGGL(k_stat, dim3(g.ny - 1), dim3(256), 0, st, U, sums, g);
 }
 
+// Driven_force.cpp:49-61: one block per i-plane sums ((U2_ J) 1) / J / Jsum
+// over (j, k) (per thread in order, then a fixed tree) into part[i - 3]
+__global__ void __launch_bounds__(256) k_bulk_flux(const double* __restrict__ U2, const double* __restrict__ prof,
+                                                   double Jsum, double* __restrict__ part, Geo g) {
+  __shared__ double red[256];
+  const int i = 3 + blockIdx.x, t = threadIdx.x;
+  const int Y = g.ny - 1, Z = g.nz - 1;
+  double acc = 0;
+  for (long c = t; c < (long)Y * Z; c += 256) {
+    const int j = 2 + (int)(c / Z), k = 2 + (int)(c % Z);
+    const double J = pf(prof, g.Ym, SET_C, 0, j);
+    acc += (U2[at(g, i, j, k)] * J) * 1. / J / Jsum;
+  }
+  red[t] = acc;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h) red[t] += red[t + h];
+    __syncthreads();
+  }
+  if (t == 0) part[blockIdx.x] = red[0];
+}
+
+void launch_bulk_flux(hipStream_t st, const double* U2, const double* prof, double Jsum, double* part,
+                      const Geo& g) {
+  hipLaunchKernelGGL(k_bulk_flux, dim3(g.gend - 2), dim3(256), 0, st, U2, prof, Jsum, part, g);
+}
+
 void launch_rhs(hipStream_t st, const double* Ui, const double* Un, const double* Uq, const double* Fx,
                 const double* Fy, const double* Fz, const double* vF, const double* prof, double* MR, double* NL,
                 const Geo& g, double deltaT, double f_drive) {

@@ -27,6 +27,9 @@ void launch_rk3(hipStream_t st, int RK, double* Ui, const double* Un, const doub
                 double f_drive);
 // adds the per-j sums of U, V, W, UU, VV, WW, UV of the interior to sums[7][Ym]
 void launch_stat(hipStream_t st, const double* U, double* sums, const Geo& g);
+// per-i-plane sums of the bulk flux integrand of Driven_force into part[X_out]
+void launch_bulk_flux(hipStream_t st, const double* U2, const double* prof, double Jsum, double* part,
+                      const Geo& g);
 void launch_rhs(hipStream_t st, const double* Ui, const double* Un, const double* Uq, const double* Fx,
                 const double* Fy, const double* Fz, const double* vF, const double* prof, double* MR, double* NL,
                 const Geo& g, double deltaT, double f_drive);

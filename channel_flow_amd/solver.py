@@ -31,7 +31,7 @@ EXPORTS = [
     "cf_profile_inner_iteration", "cf_set_sweep_schedule", "cf_flux_x_limiter", "cf_flux_y_limiter",
     "cf_flux_z_limiter", "cf_grid", "cf_filter", "cf_profile_variants", "cf_dplusgs", "cf_set_implicit_solver",
     "cf_set_viscous_schedule", "cf_rk3nodt", "cf_rk3_step",
-    "cf_statistic",
+    "cf_statistic", "cf_driven_force",
 ]
 
 
@@ -98,6 +98,7 @@ def load_library() -> C.CDLL:
     lib.cf_rk3nodt.argtypes = [vp, C.c_int, C.c_double, P, PP, PP, PP, P, PP, PP, PP, PP, PP, PP]
     lib.cf_rk3_step.argtypes = [vp, C.c_int]
     lib.cf_statistic.argtypes = [vp, C.c_int, C.c_int, PP, C.c_char_p]
+    lib.cf_driven_force.argtypes = [vp, C.c_double, P, P, P, P]
     lib.cf_write_output.argtypes = [vp, C.c_int, C.c_char_p]
     lib.cf_synchronize.argtypes = [vp]
     _lib = lib
@@ -284,6 +285,15 @@ class Solver:
         device iterate."""
         ptr = _pp([np.ascontiguousarray(a, dtype=np.float64) for a in U_]) if U_ is not None else None
         self._chk(self.lib.cf_statistic(self._h, int(step), int(statistic_step), ptr, str(directory).encode()))
+
+    def Driven_force(self, Q0=20.464969232814912, U2_=None, J=None):
+        """Driven_force.cpp:38-73: (Qall, ER_Q) of U2_ (host array) or of the
+        device iterate; Q0 defaults to prm.h's."""
+        Q, E = C.c_double(), C.c_double()
+        u = _p(np.ascontiguousarray(U2_, dtype=np.float64)) if U2_ is not None else None
+        jj = _p(np.ascontiguousarray(J, dtype=np.float64)) if J is not None else None
+        self._chk(self.lib.cf_driven_force(self._h, float(Q0), u, jj, C.byref(Q), C.byref(E)))
+        return Q.value, E.value
 
     def rk3_step(self, viscous: bool = True):
         """Three RK3NODT stages on the device state (main.cpp:152-377)."""

@@ -219,6 +219,14 @@ int cf_rk3nodt(cf_ctx* ctx, int RK, double deltaT, double er[5], double* const U
  * commented out and vF is never assigned) and RK3NODT, from the iterate
  * with level n as the base.  cf_advance_levels then shifts the levels. */
 int cf_rk3_step(cf_ctx* ctx, int viscous);
+/* Driven_force.cpp:38-73 (main.h Driven_force): the bulk flux
+ * Qall = sum over the interior of (U2_ J) 1/J / Jsum / X_out / Z_out with
+ * Jsum = sum_j 1/J, and ER_Q = |Q0 - Qall| / Q0 (the reference prints both
+ * and takes Q0 from prm.h:15, 20.464969232814912).  U2_: reference-layout
+ * host array, or NULL for the device iterate; J: the cell J, or NULL for the
+ * context's.  The sum is formed per i-plane in a fixed tree and the planes
+ * are added in order: Qall equals the reference's to rounding. */
+int cf_driven_force(cf_ctx* ctx, double Q0, const double* U2_, const double* J, double* Qall, double* ER_Q);
 /* Statistic.cpp:1043-1154 (main.h Statistic), the mean profiles: adds the
  * per-j sums over i and k of U, V, W, UU, VV, WW, UV (U = U2_/U1_ ...) of
  * the state to running sums kept in the context; when step % statistic_step

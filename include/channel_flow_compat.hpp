@@ -303,6 +303,15 @@ inline void RK3NODT(int, int RK, double deltaT, double* er1, double* er2, double
                    "RK3NODT");
 }
 
+// Driven_force.cpp:38-73 (main.h): prints the bulk flux line as the
+// reference does, with prm.h's Q0
+inline void Driven_force(int myid, CF_F(U2_), CF_F(J)) {
+  const double Q0 = 20.464969232814912;  // prm.h:15
+  double Q = 0, E = 0;
+  cf_compat::check(cf_driven_force(cf_compat::ctx("Driven_force"), Q0, CF_P(U2_), CF_P(J), &Q, &E), "Driven_force");
+  if (myid == 0) std::printf("Q0=%f\tQall=%f\tER_Q=%f\n", Q0, Q, E);
+}
+
 // Statistic.cpp:1043-1154 (main.h): the mean profiles and st%05d.dat in the
 // working directory, like the reference; the numerical-dissipation and
 // shear-stress statistics of Statistic.cpp:100-1040 are not produced

@@ -1109,6 +1109,25 @@ void cfo_rk3nodt(const cfo_grid* g, int RK, double deltaT, double* const U_[5], 
   (void)sJ;
 }
 
+/* Driven_force.cpp:38-73: bulk flux Qall = sum over the interior of
+ * (U2_ J) 1/J / Jsum, Jsum = sum_j 1/J (at i = k = 3), divided by X_out and
+ * Z_out; ER_Q = |Q0 - Qall| / Q0.  out = {Qall, ER_Q}. */
+void cfo_driven_force(const cfo_grid* g, const double* U2_, const double* J, double Q0, double out[2]) {
+  dims_t d = dims(g);
+  double Jsum = 0, Q = 0;
+  for (int j = 2; j <= d.ny; j++) Jsum = Jsum + 1. / J[IX(d, 3, j, 3)];
+  for (int i = 3; i <= d.gend; i++)
+    for (int j = 2; j <= d.ny; j++)
+      for (int k = 2; k <= d.nz; k++) {
+        size_t c = IX(d, i, j, k);
+        double temp = (U2_[c] * J[c]) * 1. / J[c] / Jsum;
+        Q = Q + temp;
+      }
+  Q = Q / g->X_out / g->Z_out;
+  out[0] = Q;
+  out[1] = fabs(Q0 - Q) / Q0;
+}
+
 /* Statistic.cpp:1043-1071: adds one state's sums over i and k of
  * U, V, W, UU, VV, WW, UV (velocities U_2/U_1 ...) to sums[q * Y_m + j],
  * q = 0..6, in the reference's order (j, then i, then k). */

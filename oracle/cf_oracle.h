@@ -99,6 +99,10 @@ void cfo_rk3nodt(const cfo_grid* g, int RK, double deltaT, double* const U_[5], 
                  double* const inFx[5], double* const inFy[5], double* const inFz[5], double* const vF[4],
                  double* const MR[5]);
 
+/* Driven_force.cpp:38-73: out = {Qall, ER_Q}, Qall the bulk flux
+ * sum (U2_ J) 1/J / Jsum / X_out / Z_out in the reference's order. */
+void cfo_driven_force(const cfo_grid* g, const double* U2_, const double* J, double Q0, double out[2]);
+
 /* Statistic.cpp:1043-1071: adds the state's sums over i, k of U, V, W, UU,
  * VV, WW, UV to sums[q * Y_m + j] (q = 0..6), in the reference's order. */
 void cfo_statistic_accum(const cfo_grid* g, double* const U_[5], double* sums);

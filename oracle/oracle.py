@@ -58,6 +58,7 @@ def lib():
         _lib.cfo_dplusgs.argtypes = dt_args
         _lib.cfo_sweeps_apply.argtypes = [C.POINTER(Grid), C.c_double, C.c_int, PP, P, PP, PP, PP, PP]
         _lib.cfo_statistic_accum.argtypes = [C.POINTER(Grid), PP, P]
+        _lib.cfo_driven_force.argtypes = [C.POINTER(Grid), P, P, C.c_double, P]
         _lib.cfo_rk3nodt.argtypes = [C.POINTER(Grid), C.c_int, C.c_double, PP, PP, P, PP, PP, PP, PP, PP]
     return _lib
 
@@ -165,6 +166,16 @@ def sweeps_apply(cfg, mode, U_, met, MR, deltaT=None):
     lib().cfo_sweeps_apply(C.byref(grid_of(cfg)), cfg.deltaT if deltaT is None else deltaT, int(mode), _pp(U_),
                            _p(met["J"]), _pp(met["m"]), _pp(MR), _pp(NL), _pp(NR))
     return NR
+
+
+Q0_REFERENCE = 20.464969232814912  # prm.h:15
+
+
+def driven_force(cfg, U2_, J, Q0=Q0_REFERENCE):
+    """Driven_force.cpp:38-73: (Qall, ER_Q) and the line the reference prints."""
+    out = (C.c_double * 2)()
+    lib().cfo_driven_force(C.byref(grid_of(cfg)), _p(np.ascontiguousarray(U2_)), _p(np.ascontiguousarray(J)), Q0, out)
+    return out[0], out[1], "Q0=%f\tQall=%f\tER_Q=%f\n" % (Q0, out[0], out[1])
 
 
 class Statistic:

@@ -16,6 +16,8 @@
 //   ref_driver traj <dir> <deltaT> <steps> <iteration_end_step>
 //       the main.cpp step loop; writes er.txt (one line per inner iteration:
 //       step it er1..er5 raw and normalised) and LES%05d.bin via Output().
+//   ref_driver force <dir> <deltaT>
+//       Driven_force's printed line for the iterate and for level n.
 //   ref_driver stat <dir> <deltaT>
 //       Statistic's mean profiles over two steps and over one (st*.dat).
 //   ref_driver time <dir> <deltaT> <inner_iterations>
@@ -369,6 +371,13 @@ int main(int argc, char** argv) {
     statistic(s, myid, s.U_, 1, 2);
     statistic(s, myid, s.U, 2, 2);
     statistic(s, myid, s.U_, 3, 1);
+  } else if (mode == "force") {
+    // Driven_force.cpp:38-73 on the post-BC iterate, then on level n: each
+    // call prints its Q0 / Qall / ER_Q line
+    load5("iterate.bin", s.U_, s.J);
+    halo_bc(s, myid);
+    Driven_force(myid, s.U_[1], s.J);
+    Driven_force(myid, s.U[1], s.J);
   } else if (mode == "time" && argc >= 5) {
     int nit = atoi(argv[4]);
     double er[5];

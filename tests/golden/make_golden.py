@@ -224,12 +224,40 @@ def stat_fixture(grid, seed):
     return name, meta
 
 
+def force_fixture(grid, seed):
+    """Driven_force's printed lines (Driven_force.cpp:72) for the variant
+    fixture's post-BC iterate and level-n state."""
+    cfg = ChannelConfig(X_out=grid[0], Y_out=grid[1], Z_out=grid[2])
+    exe = ref_driver(grid)
+    with tempfile.TemporaryDirectory() as td:
+        d = Path(td)
+        Qn, Qm = perturbed_state(cfg, seed, 0), perturbed_state(cfg, seed, 1)
+        write_initial_bin(d / "initial.bin", Qn, Qm)
+        np.ascontiguousarray(perturbed_state(cfg, seed, 2)).tofile(d / "iterate.bin")
+        out = subprocess.run([str(exe), "force", str(d), repr(DT_UNIT)], check=True, capture_output=True,
+                             env=dict(os.environ, OMP_NUM_THREADS="1")).stdout
+    lines = [ln for ln in out.decode().splitlines(keepends=True) if ln.startswith("Q0=")]
+    assert len(lines) == 2, out
+    meta = dict(grid=list(grid), seed=seed, inputs=f"var_g{grid[0]}x{grid[1]}x{grid[2]} U_bc (iterate) and U",
+                reference_calls=["Driven_force(iterate)", "Driven_force(level n)"])
+    name = f"force_g{grid[0]}x{grid[1]}x{grid[2]}"
+    np.savez_compressed(OUT / f"{name}.npz", lines=np.frombuffer("".join(lines).encode(), dtype=np.uint8))
+    return name, meta
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "variants":
         # regenerate only the variant fixtures and merge them into the manifest
         manifest = json.loads((OUT / "manifest.json").read_text())
         for i, g in enumerate(VAR_GRIDS):
             n, m = variant_fixture(g, seed=21 + i)
+            manifest["fixtures"][n] = m
+        (OUT / "manifest.json").write_text(json.dumps(manifest, indent=1))
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "force":
+        manifest = json.loads((OUT / "manifest.json").read_text())
+        for i, g in enumerate(VAR_GRIDS):
+            n, m = force_fixture(g, seed=21 + i)
             manifest["fixtures"][n] = m
         (OUT / "manifest.json").write_text(json.dumps(manifest, indent=1))
         return
@@ -261,6 +289,8 @@ def main():
         manifest["fixtures"][n] = m
     for i, g in enumerate(VAR_GRIDS):
         n, m = stat_fixture(g, seed=21 + i)
+        manifest["fixtures"][n] = m
+        n, m = force_fixture(g, seed=21 + i)
         manifest["fixtures"][n] = m
     n, m = traj_fixture(TRAJ, keep_state=True)
     manifest["fixtures"][n] = m

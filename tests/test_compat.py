@@ -30,6 +30,7 @@ DRIVER = textwrap.dedent(r"""
       keep(&Y_boundary_condition); keep(&Z_boundary_condition); keep(&Flux_X); keep(&Flux_Y); keep(&Flux_Z);
       keep(&Flux_X_limiter); keep(&Flux_Y_limiter); keep(&Flux_Z_limiter); keep(&Grid); keep(&Filter);
       keep(&DPLUSGS); keep(&RK3NODT); keep(&Statistic);
+      keep(&Driven_force);
       keep(&Viscous_terms); keep(&Dual_time_stepping); keep(&Initial_condition); keep(&Output);
       cf_params p{};
       p.X_out = XOUT; p.Y_out = YOUT; p.Z_out = ZOUT;

@@ -294,6 +294,25 @@ def test_gpu_statistic_device_state(tmp_path):
     np.testing.assert_allclose(a, b, rtol=1e-13, atol=1e-13 * np.abs(b).max())
 
 
+@pytest.mark.parametrize("grid", VAR_GRIDS)
+def test_gpu_driven_force(grid, golden):
+    """Driven_force on the GPU: Qall to 1e-13 of the oracle's (the planes are
+    summed in a fixed tree), ER_Q to what that allows, and the reference's
+    printed line."""
+    from oracle import oracle as O
+    z = golden(f"var_g{grid[0]}x{grid[1]}x{grid[2]}")
+    want = golden(f"force_g{grid[0]}x{grid[1]}x{grid[2]}")["lines"].tobytes().decode().splitlines(keepends=True)
+    cfg = unit_cfg(grid)
+    with solver(cfg) as s:
+        for name, line in (("U_bc2", want[0]), ("U2", want[1])):
+            Q, E = s.Driven_force(O.Q0_REFERENCE, z[name], z["J"])
+            Qo, Eo, _ = O.driven_force(cfg, z[name], z["J"])
+            np.testing.assert_allclose(Q, Qo, rtol=1e-13, atol=0)
+            # ER_Q = |Q0 - Q| / Q0 cancels: bounded through Q's own tolerance
+            assert abs(E - Eo) <= 1e-13 * Qo / O.Q0_REFERENCE
+            assert "Q0=%f\tQall=%f\tER_Q=%f\n" % (O.Q0_REFERENCE, Q, E) == line
+
+
 def gpu_trajectory(cfg, Qn, Qm, steps):
     rows = []
     with solver(cfg) as s:

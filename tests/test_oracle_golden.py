@@ -289,3 +289,32 @@ def test_statistic_matches_reference(grid, golden):
     assert S(1, 2, fields(z, "U_bc")) is None
     assert S(2, 2, fields(z, "U")) == st["st2"].tobytes().decode()
     assert S(3, 1, fields(z, "U_bc")) == st["st3"].tobytes().decode()
+
+
+@pytest.mark.parametrize("grid", VAR_GRIDS)
+def test_driven_force_matches_reference(grid, golden):
+    """Driven_force.cpp:38-73: the lines the reference prints for the
+    iterate and for level n (its only output)."""
+    z = golden(f"var_g{grid[0]}x{grid[1]}x{grid[2]}")
+    want = golden(f"force_g{grid[0]}x{grid[1]}x{grid[2]}")["lines"].tobytes().decode()
+    cfg = unit_cfg(grid)
+    got = O.driven_force(cfg, z["U_bc2"], z["J"])[2] + O.driven_force(cfg, z["U2"], z["J"])[2]
+    assert got == want
+
+
+def test_driven_force_closed_form():
+    """A uniform rho*u gives Qall = rho*u (the weights 1/J / Jsum sum to one);
+    a plane Poiseuille profile gives the discrete bulk flux."""
+    cfg = unit_cfg((12, 10, 16))
+    met = O.metrics(cfg)
+    J = met["J"]
+    U2 = np.zeros(cfg.shape)
+    sl = (slice(3, cfg.gend + 1), slice(2, cfg.ny + 1), slice(2, cfg.nz + 1))
+    U2[sl] = 0.3 / J[sl]
+    Q, E, _ = O.driven_force(cfg, U2, J, Q0=0.3)
+    assert abs(Q - 0.3) <= 1e-14 and E <= 1e-13
+    w = 1.0 / J[3, 2:cfg.ny + 1, 3]
+    prof = np.linspace(0.1, 1.0, cfg.ny - 1)
+    U2[sl] = (prof[None, :, None] / J[sl])
+    Q, _, _ = O.driven_force(cfg, U2, J)
+    assert abs(Q - (prof * w).sum() / w.sum()) <= 1e-13
