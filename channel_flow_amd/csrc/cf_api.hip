@@ -8,6 +8,7 @@
 
 #include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -974,6 +975,97 @@ extern "C" int cf_rk3_step(cf_ctx* c, int viscous) {
   }
   HIPCHK(c, hipGetLastError());
   return sync(c);
+}
+
+// ---------------------------------------------------------------------------
+// exact checkpoint: the device state as stored (U = Q/J), the running maxima
+// of er and the statistic sums, so that a restarted run continues bit for bit
+// ---------------------------------------------------------------------------
+static const char CKPT_MAGIC[8] = {'C', 'F', 'C', 'K', 'P', 'T', '0', '1'};
+
+static int ckpt_field(cf_ctx* c, FILE* f, double* dev, bool write) {
+  const Geo& g = c->g;
+  std::vector<double> buf((size_t)g.nf, 0.0), in((size_t)c->p.Y_out * c->p.Z_out);
+  const size_t X = c->p.X_out, Y = c->p.Y_out, Z = c->p.Z_out;
+  // loading overwrites the interior only; the ghosts are rebuilt by every iteration
+  HIPCHK(c, hipMemcpy(buf.data(), dev, buf.size() * 8, hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < X; i++) {
+    if (!write && fread(in.data(), 8, in.size(), f) != in.size()) return fail(c, CF_ERR_IO, "short checkpoint");
+    for (size_t j = 0; j < Y; j++)
+      for (size_t k = 0; k < Z; k++) {
+        double& v = buf[(i + 3) * g.plane + (j + 2) * g.Zp + (k + 2)];
+        if (write) in[j * Z + k] = v;
+        else v = in[j * Z + k];
+      }
+    if (write && fwrite(in.data(), 8, in.size(), f) != in.size()) return fail(c, CF_ERR_IO, "checkpoint write failed");
+  }
+  if (!write) HIPCHK(c, hipMemcpy(dev, buf.data(), buf.size() * 8, hipMemcpyHostToDevice));
+  return CF_OK;
+}
+
+extern "C" int cf_save_checkpoint(cf_ctx* c, const char* path) {
+  if (!c || !path) return CF_ERR_ARG;
+  if (!c->have_state) return fail(c, CF_ERR_STATE, "no state loaded");
+  int r;
+  if ((r = sync(c))) return r;
+  FILE* f = fopen(path, "wb");
+  if (!f) return fail(c, CF_ERR_IO, "cannot open %s", path);
+  const int32_t dims[4] = {c->p.X_out, c->p.Y_out, c->p.Z_out, 0};
+  std::vector<double> st((size_t)7 * c->g.Ym);
+  HIPCHK(c, hipMemcpy(st.data(), c->stat, st.size() * 8, hipMemcpyDeviceToHost));
+  bool ok = fwrite(CKPT_MAGIC, 1, 8, f) == 8 && fwrite(dims, 4, 4, f) == 4 && fwrite(c->E, 8, 5, f) == 5 &&
+            fwrite(st.data(), 8, st.size(), f) == st.size();
+  double* fields[15];
+  for (int v = 0; v < 5; v++) {
+    fields[v] = c->Ui + v * c->g.nf;
+    fields[5 + v] = c->Un + v * c->g.nf;
+    fields[10 + v] = c->Uq + v * c->g.nf;
+  }
+  for (int q = 0; ok && q < 15; q++)
+    if ((r = ckpt_field(c, f, fields[q], true))) {
+      fclose(f);
+      return r;
+    }
+  if (fclose(f) != 0 || !ok) return fail(c, CF_ERR_IO, "checkpoint write failed: %s", path);
+  return CF_OK;
+}
+
+extern "C" int cf_load_checkpoint(cf_ctx* c, const char* path) {
+  if (!c || !path) return CF_ERR_ARG;
+  // the context's stream does not synchronise with the blocking copies below
+  if (int r0 = sync(c)) return r0;
+  FILE* f = fopen(path, "rb");
+  if (!f) return fail(c, CF_ERR_IO, "cannot open %s", path);
+  char magic[8];
+  int32_t dims[4];
+  double E[5];
+  std::vector<double> st((size_t)7 * c->g.Ym);
+  bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, CKPT_MAGIC, 8) == 0 && fread(dims, 4, 4, f) == 4;
+  if (!ok || dims[0] != c->p.X_out || dims[1] != c->p.Y_out || dims[2] != c->p.Z_out) {
+    fclose(f);
+    return fail(c, CF_ERR_IO, "%s is not a checkpoint of this grid", path);
+  }
+  if (fread(E, 8, 5, f) != 5 || fread(st.data(), 8, st.size(), f) != st.size()) {
+    fclose(f);
+    return fail(c, CF_ERR_IO, "short checkpoint");
+  }
+  int r;
+  double* fields[15];
+  for (int v = 0; v < 5; v++) {
+    fields[v] = c->Ui + v * c->g.nf;
+    fields[5 + v] = c->Un + v * c->g.nf;
+    fields[10 + v] = c->Uq + v * c->g.nf;
+  }
+  for (int q = 0; q < 15; q++)
+    if ((r = ckpt_field(c, f, fields[q], false))) {
+      fclose(f);
+      return r;
+    }
+  fclose(f);
+  HIPCHK(c, hipMemcpy(c->stat, st.data(), st.size() * 8, hipMemcpyHostToDevice));
+  for (int v = 0; v < 5; v++) c->E[v] = E[v];
+  c->have_state = true;
+  return CF_OK;
 }
 
 extern "C" int cf_driven_force(cf_ctx* c, double Q0, const double* U2_, const double* J, double* Qall,

@@ -31,7 +31,7 @@ EXPORTS = [
     "cf_profile_inner_iteration", "cf_set_sweep_schedule", "cf_flux_x_limiter", "cf_flux_y_limiter",
     "cf_flux_z_limiter", "cf_grid", "cf_filter", "cf_profile_variants", "cf_dplusgs", "cf_set_implicit_solver",
     "cf_set_viscous_schedule", "cf_rk3nodt", "cf_rk3_step",
-    "cf_statistic", "cf_driven_force",
+    "cf_statistic", "cf_driven_force", "cf_save_checkpoint", "cf_load_checkpoint",
 ]
 
 
@@ -99,6 +99,8 @@ def load_library() -> C.CDLL:
     lib.cf_rk3_step.argtypes = [vp, C.c_int]
     lib.cf_statistic.argtypes = [vp, C.c_int, C.c_int, PP, C.c_char_p]
     lib.cf_driven_force.argtypes = [vp, C.c_double, P, P, P, P]
+    lib.cf_save_checkpoint.argtypes = [vp, C.c_char_p]
+    lib.cf_load_checkpoint.argtypes = [vp, C.c_char_p]
     lib.cf_write_output.argtypes = [vp, C.c_int, C.c_char_p]
     lib.cf_synchronize.argtypes = [vp]
     _lib = lib
@@ -294,6 +296,13 @@ class Solver:
         jj = _p(np.ascontiguousarray(J, dtype=np.float64)) if J is not None else None
         self._chk(self.lib.cf_driven_force(self._h, float(Q0), u, jj, C.byref(Q), C.byref(E)))
         return Q.value, E.value
+
+    def save_checkpoint(self, path):
+        """Exact restart file: state as stored, E_k, statistic sums."""
+        self._chk(self.lib.cf_save_checkpoint(self._h, str(path).encode()))
+
+    def load_checkpoint(self, path):
+        self._chk(self.lib.cf_load_checkpoint(self._h, str(path).encode()))
 
     def rk3_step(self, viscous: bool = True):
         """Three RK3NODT stages on the device state (main.cpp:152-377)."""

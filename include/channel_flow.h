@@ -175,6 +175,14 @@ int cf_physical_step(cf_ctx* ctx, int* iterations, double er[5]);
 int cf_advance_levels(cf_ctx* ctx);
 /* Output (main.h:872-896, Output.cpp:56-190): writes <dir>/LES%05d.bin. */
 int cf_write_output(cf_ctx* ctx, int step, const char* dir);
+/* Exact restart, no counterpart in the reference (whose restart reads
+ * LES%05d.bin, Q = U J, and starts E_k and the statistics afresh): the
+ * interior of the iterate and of levels n and n-1 as stored (U = Q/J), the
+ * running maxima E_k of cf_physical_step and the statistic sums.  Format:
+ * "CFCKPT01", int32 X_out Y_out Z_out 0, double E[5], double sums[7][Y_m],
+ * then 15 fields (U_, U, Uq) of X_out*Y_out*Z_out doubles, [x][y][z]. */
+int cf_save_checkpoint(cf_ctx* ctx, const char* path);
+int cf_load_checkpoint(cf_ctx* ctx, const char* path);
 /* Runs one inner iteration like cf_inner_iteration and times its stages
  * with HIP events recorded on the context's stream.  stage_ms: ghosts,
  * Flux_X, Flux_Y, Flux_Z, Viscous_terms, RHS, sweep summary pre-pass, sweep

@@ -313,6 +313,40 @@ def test_gpu_driven_force(grid, golden):
             assert "Q0=%f\tQall=%f\tER_Q=%f\n" % (O.Q0_REFERENCE, Q, E) == line
 
 
+def test_gpu_checkpoint_restart_is_exact(tmp_path):
+    """Two physical steps, a checkpoint, two more steps with statistics;
+    a new context restarted from the checkpoint repeats the last two steps
+    bit for bit: iteration counts, normalised residuals, state and st file."""
+    from channel_flow_amd.config import ChannelConfig, FlowParams, poiseuille_state
+    from channel_flow_amd.solver import Solver
+    cfg = ChannelConfig(16, 12, 16, gamma_metric=1.5, deltaT=5e-8, iteration_end_step=6)
+    Qn = poiseuille_state(cfg, FlowParams(), seed=13, stream=0)
+    ck = tmp_path / "run.ckpt"
+    runs = []
+    for restart in (False, True):
+        out = tmp_path / ("b" if restart else "a")
+        out.mkdir()
+        with Solver(cfg) as s:
+            s.Jacobian_all(1.5, want_arrays=False)
+            if restart:
+                s.load_checkpoint(ck)
+            else:
+                s.load_state(Qn, Qn)
+                for step in (1, 2):
+                    s.physical_step()
+                    s.Statistic(step, 4, None, out)
+                s.save_checkpoint(ck)
+            hist = []
+            for step in (3, 4):
+                it, ern = s.physical_step()
+                hist.append((it, *ern))
+                s.Statistic(step, 4, None, out)
+            runs.append((np.array(hist), np.concatenate(s.read_state()), (out / "st00004.dat").read_text()))
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    np.testing.assert_array_equal(runs[0][1], runs[1][1])
+    assert runs[0][2] == runs[1][2]
+
+
 def gpu_trajectory(cfg, Qn, Qm, steps):
     rows = []
     with solver(cfg) as s:
