@@ -1811,59 +1811,71 @@ __global__ void __launch_bounds__(256) k_update(double* __restrict__ Ui, const d
   }
 }
 
-// the same update with NR read from the sweep order (Sk)
-__global__ void __launch_bounds__(256) k_update_from_sweep(double* __restrict__ Ui, const double* __restrict__ NR,
-                                                const double* __restrict__ prof, double* __restrict__ partial, Geo g,
-                                                        Sk q) {
+// The primitive update (Dual_time_stepping.cpp:1290-1352) with NR read from
+// the sweep order through LDS: a block
+// covers PC steps of one tile (as k_sweep_prep does), loads those NR rows
+// coalesced, then walks each column's run of PC consecutive k in the natural
+// layout.  Per-block partial sums of the residuals, in a fixed order.
+__global__ void __launch_bounds__(256) k_update_sk(double* __restrict__ Ui, const double* __restrict__ NR,
+                                                   const double* __restrict__ prof, double* __restrict__ partial,
+                                                   Geo g, Sk q) {
+  __shared__ double L[5][64][PCP];
   __shared__ double red[5][256];
-  int tid = threadIdx.y * blockDim.x + threadIdx.x;
-  int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-  int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
-  int i = 3 + blockIdx.z;
+  const int tid = threadIdx.x;
+  const int tile = blockIdx.y, s0 = blockIdx.x * PC;
+  const int TX = tile % q.nTI, TY = tile / q.nTI;
+  for (int idx = tid; idx < PC * 64; idx += blockDim.x) {
+    const int t = idx >> 6, lane = idx & 63, s = s0 + t;
+    if (s >= q.nsteps) continue;
+    const long o = sk_at(q, tile, s, lane);
+#pragma unroll
+    for (int f = 0; f < 5; f++) L[f][lane][t] = NR[f * q.nsk + o];
+  }
+  __syncthreads();
   double r[5] = {0, 0, 0, 0, 0};
-  if (k <= g.nz && j <= g.ny) {
-    const long nf = g.nf;
-    long c = at(g, i, j, k);
-    double J = pf(prof, g.Ym, SET_C, 0, j);
-    double U1 = Ui[c], U2 = Ui[nf + c], U3 = Ui[2 * nf + c], U4 = Ui[3 * nf + c], U5 = Ui[4 * nf + c];
-    double rhoold = U1 * J;
-    double Uold = U2 / U1, Vold = U3 / U1, Wold = U4 / U1;
-    double VVold = Uold * Uold + Vold * Vold + Wold * Wold;
-    double Pold = (U5 * J - 0.5 * rhoold * VVold) * (KG - 1);
-    double Told = Pold / rhoold;
-    // NR of this cell in the sweep order
-    const int a = (i - 3) % TI, b = (j - 2) % TJ;
-    const int tile = (i - 3) / TI + ((j - 2) / TJ) * q.nTI;
-    const long o = sk_at(q, tile, (k - 2) + a + b, a + 8 * b);
-    double P = Pold + NR[o] * J;
-    double Uu = Uold + NR[q.nsk + o] * J;
-    double V = Vold + NR[2 * q.nsk + o] * J;
-    double W = Wold + NR[3 * q.nsk + o] * J;
-    double T = Told + NR[4 * q.nsk + o] * J;
-    double rho = P / T;
+  const long nf = g.nf;
+  for (int idx = tid; idx < 64 * PC; idx += blockDim.x) {
+    const int col = idx / PC, t = idx - col * PC;
+    const int a = col & 7, b = col >> 3;
+    const int i = 3 + TX * TI + a, j = 2 + TY * TJ + b, kk = s0 - a - b + t;
+    if (i > g.gend || j > g.ny || kk < 0 || kk >= q.Nz) continue;
+    const long c = at(g, i, j, 2 + kk);
+    const double J = pf(prof, g.Ym, SET_C, 0, j);
+    const double U1 = Ui[c], U2 = Ui[nf + c], U3 = Ui[2 * nf + c], U4 = Ui[3 * nf + c], U5 = Ui[4 * nf + c];
+    const double rhoold = U1 * J;
+    const double Uold = U2 / U1, Vold = U3 / U1, Wold = U4 / U1;
+    const double VVold = Uold * Uold + Vold * Vold + Wold * Wold;
+    const double Pold = (U5 * J - 0.5 * rhoold * VVold) * (KG - 1);
+    const double Told = Pold / rhoold;
+    const double P = Pold + L[0][col][t] * J;
+    const double Uu = Uold + L[1][col][t] * J;
+    const double V = Vold + L[2][col][t] * J;
+    const double W = Wold + L[3][col][t] * J;
+    const double T = Told + L[4][col][t] * J;
+    const double rho = P / T;
     Ui[c] = rho / J;
     Ui[nf + c] = rho * Uu / J;
     Ui[2 * nf + c] = rho * V / J;
     Ui[3 * nf + c] = rho * W / J;
     Ui[4 * nf + c] = (P / (KG - 1) + 0.5 * rho * (Uu * Uu + V * V + W * W)) / J;
-    double q[5] = {(P - Pold) * (P - Pold), (Uu - Uold) * (Uu - Uold), (V - Vold) * (V - Vold),
-                   (W - Wold) * (W - Wold), (T - Told) * (T - Told)};
+    const double d[5] = {(P - Pold) * (P - Pold), (Uu - Uold) * (Uu - Uold), (V - Vold) * (V - Vold),
+                         (W - Wold) * (W - Wold), (T - Told) * (T - Told)};
 #pragma unroll
-    for (int v = 0; v < 5; v++) r[v] = q[v] * q[v];
+    for (int v = 0; v < 5; v++) r[v] += d[v] * d[v];
   }
 #pragma unroll
   for (int v = 0; v < 5; v++) red[v][tid] = r[v];
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (tid < s)
+  for (int h = 128; h > 0; h >>= 1) {
+    if (tid < h)
 #pragma unroll
-      for (int v = 0; v < 5; v++) red[v][tid] += red[v][tid + s];
+      for (int v = 0; v < 5; v++) red[v][tid] += red[v][tid + h];
     __syncthreads();
   }
   if (tid == 0) {
-    long b = ((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    const long blk = (long)blockIdx.y * gridDim.x + blockIdx.x;
 #pragma unroll
-    for (int v = 0; v < 5; v++) partial[b * 5 + v] = red[v][0];
+    for (int v = 0; v < 5; v++) partial[blk * 5 + v] = red[v][0];
   }
 }
 
@@ -2192,10 +2204,9 @@ void launch_sweeps(hipStream_t st, double* Ui, const double* MR, double* Sbuf, d
     if (marks) (void)hipEventRecord(marks[3 + 3 * sweep], st);
   }
   // primitive update reading NR from the sweep order
-  dim3 blk(64, 4);
-  dim3 gr = box_grid(g.nz - 1, g.ny - 1, g.gend - 2, blk);
-  hipLaunchKernelGGL(k_update_from_sweep, gr, blk, 0, st, Ui, NRs, prof, partial, g, q);
-  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, st, partial, (long)gr.x * gr.y * gr.z, er_dev, DN);
+  dim3 gr((q.nsteps + PC - 1) / PC, q.ntiles);
+  hipLaunchKernelGGL(k_update_sk, gr, dim3(256), 0, st, Ui, NRs, prof, partial, g, q);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, st, partial, (long)gr.x * gr.y, er_dev, DN);
   (void)NL;
   (void)NR;
 }
