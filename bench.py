@@ -31,12 +31,13 @@ GRID = (256, 192, 256)
 HBM_PEAK_GBS = 8000.0
 # algorithmic FP64 fields read + written per interior cell and inner
 # iteration (geometry excluded, each field counted once): fluxes U_ 5 -> F 5;
-# viscous U_ 5 -> vF 4; RHS U_,U,Uq,inF x3,vF -> MR, NL; forward sweep U_,MR,NL
-# -> NL and backward sweep U_,NL,NR -> NR, 20 each per sweep; update U_,NR ->
-# U_ 15.  The summary pre-pass and the sync reset are implementation work with
-# no algorithmic bytes of their own.
+# viscous U_ 5 -> vF 4; RHS U_,U,Uq,inF x3,vF -> MR (39: the first NL =
+# D^-1 MR is formed in the sweep order, never written in the natural layout);
+# forward sweep U_,MR,NL -> NL and backward sweep U_,NL,NR -> NR, 20 each per
+# sweep; update U_,NR -> U_ 15.  The summary pre-pass and the sync reset are
+# implementation work with no algorithmic bytes of their own.
 def stage_fields(n_sweeps):
-    return {"ghosts": 0, "flux_x": 10, "flux_y": 10, "flux_z": 10, "viscous": 9, "rhs": 44, "sweep_prep": 0,
+    return {"ghosts": 0, "flux_x": 10, "flux_y": 10, "flux_z": 10, "viscous": 9, "rhs": 39, "sweep_prep": 0,
             "sweep_sync": 0, "lusgs_fwd": 20 * n_sweeps, "lusgs_bwd": 20 * n_sweeps, "update_norms": 15}
 
 
