@@ -44,7 +44,7 @@ def stage_fields(n_sweeps):
 # stages that are one kernel per launch, and that kernel's name in rocprofv3
 STAGE_KERNEL = {"flux_x": "cf::k_flux<0>", "flux_y": "cf::k_flux<1>", "flux_z": "cf::k_flux<2>", "rhs": "cf::k_rhs",
                 "lusgs_fwd": "cf::k_sweep_pipe<true, 3, 2>", "lusgs_bwd": "cf::k_sweep_pipe<false, 3, 2>",
-                "sweep_prep": "cf::k_sweep_prep"}
+                "sweep_prep": "cf::k_sweep_prep_rhs"}
 TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_256x192x256.md"
 
 
@@ -161,6 +161,9 @@ def main():
             stages.setdefault(k, []).append((ms, nl))
     stage_ms = {k: float(np.median([m for m, _ in v])) for k, v in stages.items()}
     launches = {k: v[0][1] for k, v in stages.items()}
+    if not launches.get("rhs"):
+        # the RHS runs inside the sweep pre-pass (k_sweep_prep_rhs)
+        fields["sweep_prep"], fields["rhs"] = fields["rhs"], 0
     # dominant kernel: the single-kernel stage with the largest time per iteration
     dom = max((k for k in STAGE_KERNEL if launches.get(k)), key=stage_ms.get)
     per_launch_ms = stage_ms[dom] / launches[dom]

@@ -616,16 +616,24 @@ static bool diagonal_metrics(const cf_ctx* c) {
 static int effective_schedule(const cf_ctx* c) { return (c->schedule != 1 && diagonal_metrics(c)) ? 0 : 1; }
 
 // LU-SGS forward/backward sweeps (Dual_time_stepping.cpp:514-1277)
-static void sweeps(cf_ctx* c, double deltaT, const hipEvent_t* marks = nullptr) {
+// fuse_rhs: the tiled sweeps' pre-pass forms MR itself (the RHS kernel is
+// not launched and MR is not written in the natural layout)
+static void sweeps(cf_ctx* c, double deltaT, const hipEvent_t* marks = nullptr, bool fuse_rhs = false) {
   const Geo& g = c->g;
   if (effective_schedule(c) == 1) {
     cf::launch_sweeps_planes(c->stream, c->Ui, c->MR, c->NL, c->NR, c->prof, g, deltaT, c->p.n_sweeps, marks);
     return;
   }
   double DN = 1. / ((double)c->p.X_out * c->p.Y_out * c->p.Z_out);
+  const cf::RhsIn rin{c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->p.f_drive};
   cf::launch_sweeps(c->stream, c->Ui, c->MR, c->Sm, c->NL, c->NR, c->prof, g, deltaT, c->p.n_sweeps, c->order,
-                    c->sync, c->partial, c->er_dev, DN, c->spin_limit, marks, c->schedule != 2);
+                    c->sync, c->partial, c->er_dev, DN, c->spin_limit, marks, c->schedule != 2,
+                    fuse_rhs ? &rin : nullptr);
 }
+
+// the RHS runs inside the tiled sweeps' pre-pass when the caller needs
+// neither MR nor NL in the natural layout
+static bool rhs_fused(const cf_ctx* c, bool want_nl) { return !want_nl && effective_schedule(c) == 0; }
 
 // primitive update + norms after the sweeps; the tiled schedule applies them
 // inside its last backward sweep
@@ -670,9 +678,11 @@ static int dplusgs_device(cf_ctx* c, double deltaT, double er[5]) {
 static int dual_time_device(cf_ctx* c, double deltaT, double er[5], bool want_nl) {
   const Geo& g = c->g;
   if (c->solver == 1) return dplusgs_device(c, deltaT, er);
-  cf::launch_rhs(c->stream, c->Ui, c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->prof, c->MR,
-                 want_nl ? c->NL : nullptr, g, deltaT, c->p.f_drive);
-  sweeps(c, deltaT);
+  const bool fuse = rhs_fused(c, want_nl);
+  if (!fuse)
+    cf::launch_rhs(c->stream, c->Ui, c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->prof, c->MR,
+                   want_nl ? c->NL : nullptr, g, deltaT, c->p.f_drive);
+  sweeps(c, deltaT, nullptr, fuse);
   if (want_nl) {
     if (effective_schedule(c) == 0) cf::launch_sweep_outputs(c->stream, c->Sm, c->NL, c->NR, g);
     cf::launch_copy_interior(c->stream, c->NR, c->NL, g);
@@ -887,10 +897,12 @@ extern "C" int cf_profile_inner_iteration(cf_ctx* c, double er[5], double stage_
   HIPCHK(c, hipEventRecord(ev[4], c->stream));
   if (int rv = run_viscous(c)) return rv;
   HIPCHK(c, hipEventRecord(ev[5], c->stream));
-  cf::launch_rhs(c->stream, c->Ui, c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->prof, c->MR, nullptr, g,
-                 c->p.deltaT, c->p.f_drive);
+  const bool fuse = rhs_fused(c, false);
+  if (!fuse)
+    cf::launch_rhs(c->stream, c->Ui, c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->prof, c->MR, nullptr, g,
+                   c->p.deltaT, c->p.f_drive);
   HIPCHK(c, hipEventRecord(ev[6], c->stream));
-  sweeps(c, c->p.deltaT, mk);
+  sweeps(c, c->p.deltaT, mk, fuse);
   update(c);
   HIPCHK(c, hipEventRecord(ev[7 + nm], c->stream));
   int r = finish_iteration(c, er);
@@ -914,7 +926,7 @@ extern "C" int cf_profile_inner_iteration(cf_ctx* c, double er[5], double stage_
   if ((r = span(mk[nm - 1], ev[7 + nm], &stage_ms[10]))) return r;
   const bool tiled = effective_schedule(c) == 0;
   const int nplanes = g.gend + g.ny + g.nz - 6;
-  const int nl[CF_PROFILE_STAGES] = {3, 1, 1, 1, viscous_fused(c) ? 1 : 3, 1, tiled ? 1 : 0, tiled ? ns : 0, tiled ? ns : nplanes * ns,
+  const int nl[CF_PROFILE_STAGES] = {3, 1, 1, 1, viscous_fused(c) ? 1 : 3, fuse ? 0 : 1, tiled ? 1 : 0, tiled ? ns : 0, tiled ? ns : nplanes * ns,
                                      tiled ? ns : nplanes * ns, 2};
   for (int q = 0; q < CF_PROFILE_STAGES; q++) launches[q] = nl[q];
   for (auto& e : ev) (void)hipEventDestroy(e);

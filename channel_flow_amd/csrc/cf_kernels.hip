@@ -681,16 +681,14 @@ __device__ __forceinline__ void cell_metric(const double* __restrict__ prof, int
   for (int c = 0; c < 9; c++) m9[c] = pf(prof, Ym, SET_C, 1 + c, j);
 }
 
-__global__ void __launch_bounds__(256) k_rhs(const double* __restrict__ Ui, const double* __restrict__ Un,
-                                             const double* __restrict__ Uq, const double* __restrict__ Fx,
-                                             const double* __restrict__ Fy, const double* __restrict__ Fz,
-                                             const double* __restrict__ vF, const double* __restrict__ prof,
-                                             double* __restrict__ MR, double* __restrict__ NL, Geo g, double deltaT,
-                                             double f_drive) {
-  int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-  int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
-  int i = 3 + blockIdx.z;
-  if (k > g.nz || j > g.ny) return;
+// MR = Gamma^-1 Rk of interior cell (i, j, k) (Dual_time_stepping.cpp:
+// 338-436); s receives the cell's primitive state
+__device__ __forceinline__ void rhs_cell(const double* __restrict__ Ui, const double* __restrict__ Un,
+                                         const double* __restrict__ Uq, const double* __restrict__ Fx,
+                                         const double* __restrict__ Fy, const double* __restrict__ Fz,
+                                         const double* __restrict__ vF, const double* __restrict__ prof,
+                                         const Geo& g, double deltaT, double f_drive, int i, int j, int k,
+                                         double M[5], PState& s) {
   const long nf = g.nf;
   long c = at(g, i, j, k);
   long fx = at(g, i, j - 1, k - 1), fxm = fx - g.plane;
@@ -700,7 +698,7 @@ __global__ void __launch_bounds__(256) k_rhs(const double* __restrict__ Ui, cons
   double u5[5];
 #pragma unroll
   for (int v = 0; v < 5; v++) u5[v] = Ui[v * nf + c];
-  PState s = prim_dt(u5[0], u5[1], u5[2], u5[3], u5[4], J);
+  s = prim_dt(u5[0], u5[1], u5[2], u5[3], u5[4], J);
   double beta = s.beta, rho = s.rho, u = s.u, v = s.v, w = s.w, VV = s.VV, T = s.T, H = s.H;
   double temp = rho * KG;
   double d11 = -(H * (KG - 1) + VV - KG * (T + VV)) * beta;
@@ -724,12 +722,28 @@ __global__ void __launch_bounds__(256) k_rhs(const double* __restrict__ Ui, cons
   Rk[2] = Rk[2] + vF[nf + c];
   Rk[3] = Rk[3] + vF[2 * nf + c];
   Rk[4] = Rk[4] + vF[3 * nf + c] + f_drive * u / J;
-  double M[5];
   M[0] = (d11 * Rk[0] + d12 * Rk[1] + d13 * Rk[2] + d14 * Rk[3] + d15 * Rk[4]);
   M[1] = (d21 * Rk[0] + d22 * Rk[1] + 0 * Rk[2] + 0 * Rk[3] + 0 * Rk[4]);
   M[2] = (d31 * Rk[0] + 0 * Rk[1] + d22 * Rk[2] + 0 * Rk[3] + 0 * Rk[4]);
   M[3] = (d41 * Rk[0] + 0 * Rk[1] + 0 * Rk[2] + d22 * Rk[3] + 0 * Rk[4]);
   M[4] = (d51 * Rk[0] + d52 * Rk[1] + d53 * Rk[2] + d54 * Rk[3] + d55 * Rk[4]);
+}
+
+__global__ void __launch_bounds__(256) k_rhs(const double* __restrict__ Ui, const double* __restrict__ Un,
+                                             const double* __restrict__ Uq, const double* __restrict__ Fx,
+                                             const double* __restrict__ Fy, const double* __restrict__ Fz,
+                                             const double* __restrict__ vF, const double* __restrict__ prof,
+                                             double* __restrict__ MR, double* __restrict__ NL, Geo g, double deltaT,
+                                             double f_drive) {
+  int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  int i = 3 + blockIdx.z;
+  if (k > g.nz || j > g.ny) return;
+  const long nf = g.nf;
+  const long c = at(g, i, j, k);
+  double M[5];
+  PState s;
+  rhs_cell(Ui, Un, Uq, Fx, Fy, Fz, vF, prof, g, deltaT, f_drive, i, j, k, M, s);
 #pragma unroll
   for (int q = 0; q < 5; q++) MR[q * nf + c] = M[q];
   if (NL) {
@@ -990,6 +1004,57 @@ __global__ void __launch_bounds__(256) k_sweep_prep(const double* __restrict__ U
     const long o = sk_at(q, tile, s, lane);
 #pragma unroll
     for (int r = 0; r < 5; r++) MRs[r * q.nsk + o] = L[r][lane][t];
+  }
+}
+
+// k_rhs and k_sweep_prep in one pass: while a block stages its tile's k-runs
+// (coalesced, natural order) it forms MR of each cell with rhs_cell, so MR
+// never goes to HBM in the natural layout and U_ is read once.  PR steps per
+// block keep the ten staged fields within 66 KB of LDS.
+constexpr int PR = 12, PRP = PR + 1;
+
+__global__ void __launch_bounds__(256) k_sweep_prep_rhs(const double* __restrict__ Ui, RhsIn rin,
+                                                        const double* __restrict__ prof, double* __restrict__ S,
+                                                        double* __restrict__ MRs, double* __restrict__ NLs, Geo g,
+                                                        Sk q, double deltaT) {
+  __shared__ double L[10][64][PRP];  // U_ 5, MR 5
+  const int tile = blockIdx.y, s0 = blockIdx.x * PR;
+  const int TX = tile % q.nTI, TY = tile / q.nTI;
+  for (int idx = threadIdx.x; idx < 64 * PR; idx += blockDim.x) {
+    const int col = idx / PR, t = idx - col * PR;
+    const int a = col & 7, b = col >> 3;
+    const int i = 3 + TX * TI + a, j = 2 + TY * TJ + b, kk = s0 - a - b + t;
+    if (i > g.gend || j > g.ny || kk < 0 || kk >= q.Nz) continue;
+    const long c = at(g, i, j, 2 + kk);
+    double M[5];
+    PState st;
+    rhs_cell(Ui, rin.Un, rin.Uq, rin.Fx, rin.Fy, rin.Fz, rin.vF, prof, g, deltaT, rin.f_drive, i, j, 2 + kk, M, st);
+#pragma unroll
+    for (int f = 0; f < 5; f++) {
+      L[f][col][t] = Ui[f * g.nf + c];
+      L[5 + f][col][t] = M[f];
+    }
+  }
+  __syncthreads();
+  const double sv = __longlong_as_double((long long)SENTINEL);
+  for (int idx = threadIdx.x; idx < PR * 64; idx += blockDim.x) {
+    const int t = idx >> 6, lane = idx & 63, s = s0 + t;
+    if (s >= q.nsteps) continue;
+    const long o = sk_at(q, tile, s, lane);
+    if ((lane & 7) == TI - 1 || (lane >> 3) == TJ - 1) {
+#pragma unroll
+      for (int v = 0; v < 5; v++) NLs[v * q.nsk + o] = sv;
+    }
+    int i, j, k;
+    if (!sk_cell(q, g, tile, s, lane, &i, &j, &k)) continue;
+    double u5[5], v[G_N];
+#pragma unroll
+    for (int f = 0; f < 5; f++) u5[f] = L[f][lane][t];
+    prep_summary(u5, prof, g, j, deltaT, v);
+#pragma unroll
+    for (int r = 0; r < G_N; r++) S[r * q.nsk + o] = v[r];
+#pragma unroll
+    for (int r = 0; r < 5; r++) MRs[r * q.nsk + o] = L[5 + r][lane][t];
   }
 }
 
@@ -2174,15 +2239,19 @@ long sweep_update_blocks(const Geo& g) {
 void launch_sweeps(hipStream_t st, double* Ui, const double* MR, double* Sbuf, double* NL, double* NR,
                    const double* prof, const Geo& g, double deltaT, int n_sweeps, const int* order, int* sync,
                    double* partial, double* er_dev, double DN, long spin_limit, const hipEvent_t* marks,
-                   bool pipelined) {
+                   bool pipelined, const RhsIn* rin) {
   // Sbuf holds G_N summary fields, then MR, NL, NR (5 each) in sweep order
   Sk q = sweep_layout(g);
   double* S = Sbuf;
   double* MRs = Sbuf + (long)G_N * q.nsk;
   double* NLs = MRs + 5 * q.nsk;
   double* NRs = NLs + 5 * q.nsk;
-  hipLaunchKernelGGL(k_sweep_prep, dim3((q.nsteps + PC - 1) / PC, q.ntiles), dim3(256), 0, st, Ui, MR, prof, S, MRs,
-                     NLs, g, q, deltaT);
+  if (rin)
+    hipLaunchKernelGGL(k_sweep_prep_rhs, dim3((q.nsteps + PR - 1) / PR, q.ntiles), dim3(256), 0, st, Ui, *rin, prof,
+                       S, MRs, NLs, g, q, deltaT);
+  else
+    hipLaunchKernelGGL(k_sweep_prep, dim3((q.nsteps + PC - 1) / PC, q.ntiles), dim3(256), 0, st, Ui, MR, prof, S,
+                       MRs, NLs, g, q, deltaT);
   if (marks) (void)hipEventRecord(marks[0], st);
   const int nt = q.ntiles;
   for (int sweep = 0; sweep < n_sweeps; sweep++) {

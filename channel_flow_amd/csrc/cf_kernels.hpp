@@ -41,10 +41,15 @@ long sweep_update_blocks(const Geo& g);
 // sweep.
 // persistent tiled sweeps (diagonal metrics); Sbuf: G_N + 15 fields of
 // sweep_field_elems doubles; ends with the primitive update of Ui and er
+// inputs of the RHS, for the sweep pre-pass that forms MR itself (rin)
+struct RhsIn {
+  const double *Un, *Uq, *Fx, *Fy, *Fz, *vF;
+  double f_drive;
+};
 void launch_sweeps(hipStream_t st, double* Ui, const double* MR, double* Sbuf, double* NL, double* NR,
                    const double* prof, const Geo& g, double deltaT, int n_sweeps, const int* order, int* sync,
                    double* partial, double* er_dev, double DN, long spin_limit, const hipEvent_t* marks = nullptr,
-                   bool pipelined = true);
+                   bool pipelined = true, const RhsIn* rin = nullptr);
 // NL and NR of the last sweep back in the natural layout
 void launch_sweep_outputs(hipStream_t st, const double* Sbuf, double* NL, double* NR, const Geo& g);
 
