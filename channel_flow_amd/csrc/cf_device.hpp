@@ -53,6 +53,26 @@ __device__ __forceinline__ double prow(const double* __restrict__ prof, int Ym, 
   return prof[(long)row * Ym + j];
 }
 
+// XCD-aware block order.  Consecutive workgroups are dealt to the 8 XCDs in
+// turn, and each XCD has its own L2; this bijection of [0, n) gives XCD x
+// (the physical blocks b with b % 8 == x) one contiguous range of logical
+// blocks, so neighbouring logical blocks share an L2.
+constexpr int N_XCD = 8;
+__device__ __forceinline__ long xcd_block(long b, long n) {
+  const long x = b % N_XCD, i = b / N_XCD, per = n / N_XCD, rem = n % N_XCD;
+  return x * per + (x < rem ? x : rem) + i;
+}
+
+// the logical (x, y, z) block of a 3-D grid after xcd_block
+__device__ __forceinline__ void xcd_block3(int* bx, int* by, int* bz) {
+  const long gx = gridDim.x, gy = gridDim.y;
+  const long n = gx * gy * gridDim.z;
+  const long L = xcd_block((blockIdx.z * gy + blockIdx.y) * gx + blockIdx.x, n);
+  *bx = (int)(L % gx);
+  *by = (int)((L / gx) % gy);
+  *bz = (int)(L / (gx * gy));
+}
+
 // x/d for a divisor d fixed by the grid, with r = RN(1/d):
 //   q0 = x r,  q = q0 + fma(-q0, d, x) r  (both steps single roundings).
 // When |r d - 1| <= 2^-54, q0 is within one ulp of x/d for every x, and

@@ -972,7 +972,9 @@ __global__ void __launch_bounds__(256) k_sweep_prep(const double* __restrict__ U
                                                     double* __restrict__ MRs, double* __restrict__ NLs, Geo g, Sk q,
                                                     double deltaT) {
   __shared__ double L[5][64][PCP];
-  const int tile = blockIdx.y, s0 = blockIdx.x * PC;
+  int bx, by, bz;
+  xcd_block3(&bx, &by, &bz);
+  const int tile = by, s0 = bx * PC;
   prep_stage(L, Ui, g, q, tile, s0);
   __syncthreads();
   const double sv = __longlong_as_double((long long)SENTINEL);
@@ -1018,7 +1020,9 @@ __global__ void __launch_bounds__(256) k_sweep_prep_rhs(const double* __restrict
                                                         double* __restrict__ MRs, double* __restrict__ NLs, Geo g,
                                                         Sk q, double deltaT) {
   __shared__ double L[10][64][PRP];  // U_ 5, MR 5
-  const int tile = blockIdx.y, s0 = blockIdx.x * PR;
+  int bx, by, bz;
+  xcd_block3(&bx, &by, &bz);
+  const int tile = by, s0 = bx * PR;
   const int TX = tile % q.nTI, TY = tile / q.nTI;
   for (int idx = threadIdx.x; idx < 64 * PR; idx += blockDim.x) {
     const int col = idx / PR, t = idx - col * PR;
@@ -1887,7 +1891,9 @@ __global__ void __launch_bounds__(256) k_update_sk(double* __restrict__ Ui, cons
   __shared__ double L[5][64][PCP];
   __shared__ double red[5][256];
   const int tid = threadIdx.x;
-  const int tile = blockIdx.y, s0 = blockIdx.x * PC;
+  int bx, by, bz;
+  xcd_block3(&bx, &by, &bz);
+  const int tile = by, s0 = bx * PC;
   const int TX = tile % q.nTI, TY = tile / q.nTI;
   for (int idx = tid; idx < PC * 64; idx += blockDim.x) {
     const int t = idx >> 6, lane = idx & 63, s = s0 + t;
@@ -1938,7 +1944,7 @@ __global__ void __launch_bounds__(256) k_update_sk(double* __restrict__ Ui, cons
     __syncthreads();
   }
   if (tid == 0) {
-    const long blk = (long)blockIdx.y * gridDim.x + blockIdx.x;
+    const long blk = (long)by * gridDim.x + bx;
 #pragma unroll
     for (int v = 0; v < 5; v++) partial[blk * 5 + v] = red[v][0];
   }
