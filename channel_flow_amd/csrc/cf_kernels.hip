@@ -243,71 +243,85 @@ __device__ __forceinline__ bool in_box(const Geo& g, int i, int j, int k) {
   return i >= 3 && i <= g.gend && j >= 2 && j <= g.ny && k >= 2 && k <= g.nz;
 }
 
-template <int WIDE>
-__global__ void __launch_bounds__(256) k_filt_x(const double* __restrict__ src, const double* __restrict__ U,
-                                                double* __restrict__ Gx, Geo g) {
-  const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int i = 3 + blockIdx.z;
-  if (k > g.nz || j > g.ny) return;
-  const long p = at(g, i, j, k);
-#pragma unroll
-  for (int v = 0; v < 4; v++) {
-    const double* s = (i - 1 < 3 ? U : src) + v * g.nf;
-    const double* t = (i + 1 > g.gend ? U : src) + v * g.nf;
-    const double a = s[p - g.plane], b = src[v * g.nf + p], c = t[p + g.plane];
-    Gx[v * g.nf + p] = WIDE ? 0.25 * (a + 2 * b + c) : 0.125 * (a + 6 * b + c);
-  }
-}
+// One filter pass (x, then y, then z, Filter.cpp:218-775) in one kernel: a
+// block owns an 8 (j) x 32 (k) tile of plane i.  Gx of the tile and of a
+// one-cell rim, and Gy of the tile rows and rim columns, stay in LDS; a rim
+// position outside the filter box holds U_'s own value there, as the
+// reference's passes read it (their work arrays start as copies of U_ and no
+// pass writes outside the box).  Each 1-D filter keeps the reference's
+// operands and order, so the three fused passes give the same bits.
+constexpr int FJ = 8, FK = 32;
 
 template <int WIDE>
-__global__ void __launch_bounds__(256) k_filt_y(const double* __restrict__ Gx, const double* __restrict__ U,
-                                                const double* __restrict__ prof, const double* __restrict__ Yp,
-                                                double* __restrict__ Gy, Geo g) {
-  const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int i = 3 + blockIdx.z;
-  if (k > g.nz || j > g.ny) return;
-  const long p = at(g, i, j, k);
-  const double A = Yp[j + 1] - Yp[j - 1];  // H_L + H_R
-  const double B = Yp[j + 1] - Yp[j];      // H_R
-  const double C = Yp[j] - Yp[j - 1];      // H_L
-  const double Jm = pf(prof, g.Ym, SET_C, 0, j - 1), J0 = pf(prof, g.Ym, SET_C, 0, j), Jp = pf(prof, g.Ym, SET_C, 0, j + 1);
+__global__ void __launch_bounds__(256) k_filt_pass(const double* __restrict__ src, const double* __restrict__ U,
+                                                   const double* __restrict__ prof, const double* __restrict__ Yp,
+                                                   double* __restrict__ G, double* __restrict__ acc, int pass, Geo g) {
+  __shared__ double Lx[4][FJ + 2][FK + 2];  // rows j0-1 .. j0+FJ, columns k0-1 .. k0+FK
+  __shared__ double Ly[4][FJ][FK + 2];      // rows j0 .. j0+FJ-1
+  const int k0 = 2 + blockIdx.x * FK, j0 = 2 + blockIdx.y * FJ, i = 3 + blockIdx.z;
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < (FJ + 2) * (FK + 2); idx += 256) {
+    const int r = idx / (FK + 2), c = idx - r * (FK + 2);
+    const int j = j0 - 1 + r, k = k0 - 1 + c;
+    if (j > g.ny + 1 || k > g.nz + 1) continue;
+    const long p = at(g, i, j, k);
+    if (in_box(g, i, j, k)) {
 #pragma unroll
-  for (int v = 0; v < 4; v++) {
-    const double gm = (j - 1 < 2 ? U : Gx)[v * g.nf + p - g.Zp];
-    const double gp = (j + 1 > g.ny ? U : Gx)[v * g.nf + p + g.Zp];
-    const double gx = Gx[v * g.nf + p];
-    Gy[v * g.nf + p] = WIDE ? 0.5 * (B / A * gm * Jm + gx * J0 + C / A * gp * Jp) / J0
-                            : 0.125 * (2 * B / A * gm * Jm + 6 * gx * J0 + 2 * C / A * gp * Jp) / J0;
+      for (int v = 0; v < 4; v++) {
+        const double* sm = (i - 1 < 3 ? U : src) + v * g.nf;
+        const double* tp = (i + 1 > g.gend ? U : src) + v * g.nf;
+        const double a = sm[p - g.plane], b = src[v * g.nf + p], cc = tp[p + g.plane];
+        Lx[v][r][c] = WIDE ? 0.25 * (a + 2 * b + cc) : 0.125 * (a + 6 * b + cc);
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < 4; v++) Lx[v][r][c] = U[v * g.nf + p];
+    }
   }
-}
-
-// z pass, then the running sum 6 G - 15 G^2 + 20 G^3 - 15 G^4 + 6 G^5 - G^6
-// in the reference's left-to-right order (Filter.cpp:763-775)
-template <int WIDE>
-__global__ void __launch_bounds__(256) k_filt_z(const double* __restrict__ Gy, const double* __restrict__ U,
-                                                double* __restrict__ G, double* __restrict__ acc, int pass, Geo g) {
-  const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int i = 3 + blockIdx.z;
-  if (k > g.nz || j > g.ny) return;
-  const long p = at(g, i, j, k);
+  __syncthreads();
+  for (int idx = tid; idx < FJ * (FK + 2); idx += 256) {
+    const int r = idx / (FK + 2), c = idx - r * (FK + 2);
+    const int j = j0 + r, k = k0 - 1 + c;
+    if (j > g.ny + 1 || k > g.nz + 1) continue;
+    if (in_box(g, i, j, k)) {
+      const double A = Yp[j + 1] - Yp[j - 1];  // H_L + H_R
+      const double B = Yp[j + 1] - Yp[j];      // H_R
+      const double C = Yp[j] - Yp[j - 1];      // H_L
+      const double Jm = pf(prof, g.Ym, SET_C, 0, j - 1), J0 = pf(prof, g.Ym, SET_C, 0, j),
+                   Jp = pf(prof, g.Ym, SET_C, 0, j + 1);
 #pragma unroll
-  for (int v = 0; v < 4; v++) {
-    const double a = (k - 1 < 2 ? U : Gy)[v * g.nf + p - 1];
-    const double c = (k + 1 > g.nz ? U : Gy)[v * g.nf + p + 1];
-    const double y = Gy[v * g.nf + p];
-    const double t = WIDE ? 0.25 * (a + 2 * y + c) : 0.125 * (a + 6 * y + c);
-    G[v * g.nf + p] = t;
-    double& s = acc[v * g.nf + p];
-    switch (pass) {
-      case 0: s = 6 * t; break;
-      case 1: s = s - 15 * t; break;
-      case 2: s = s + 20 * t; break;
-      case 3: s = s - 15 * t; break;
-      case 4: s = s + 6 * t; break;
-      default: s = s - t; break;
+      for (int v = 0; v < 4; v++) {
+        const double gm = Lx[v][r][c], gx = Lx[v][r + 1][c], gp = Lx[v][r + 2][c];
+        Ly[v][r][c] = WIDE ? 0.5 * (B / A * gm * Jm + gx * J0 + C / A * gp * Jp) / J0
+                           : 0.125 * (2 * B / A * gm * Jm + 6 * gx * J0 + 2 * C / A * gp * Jp) / J0;
+      }
+    } else {
+      const long p = at(g, i, j, k);
+#pragma unroll
+      for (int v = 0; v < 4; v++) Ly[v][r][c] = U[v * g.nf + p];
+    }
+  }
+  __syncthreads();
+  {
+    const int r = tid / FK, c = tid - r * FK;  // 256 threads = the FJ x FK tile
+    const int j = j0 + r, k = k0 + c;
+    if (j <= g.ny && k <= g.nz) {
+      const long p = at(g, i, j, k);
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const double a = Ly[v][r][c], y = Ly[v][r][c + 1], cc = Ly[v][r][c + 2];
+        const double t = WIDE ? 0.25 * (a + 2 * y + cc) : 0.125 * (a + 6 * y + cc);
+        G[v * g.nf + p] = t;
+        double& s = acc[v * g.nf + p];
+        switch (pass) {
+          case 0: s = 6 * t; break;
+          case 1: s = s - 15 * t; break;
+          case 2: s = s + 20 * t; break;
+          case 3: s = s - 15 * t; break;
+          case 4: s = s + 6 * t; break;
+          default: s = s - t; break;
+        }
+      }
     }
   }
 }
@@ -2041,25 +2055,21 @@ void launch_flux_limiter(hipStream_t st, int dir, const double* U, const double*
 
 void launch_filter(hipStream_t st, const double* U, const double* prof, const double* Yp, double* W, double crit,
                    double E_high, double E_low, double* EpX, double* EpY, double* EpZ, const Geo& g) {
-  // W: 22 fields = Gx(4) Gy(4) G0(4) G1(4) acc(4) E_h E_2h
+  // W: 14 fields = G0(4) G1(4) acc(4) E_h E_2h (the passes' Gx, Gy stay in LDS)
   const long n = g.nf;
-  double *Gx = W, *Gy = W + 4 * n, *Gb[2] = {W + 8 * n, W + 12 * n}, *acc = W + 16 * n;
-  double *Eh = W + 20 * n, *E2h = W + 21 * n;
+  double *Gb[2] = {W, W + 4 * n}, *acc = W + 8 * n;
+  double *Eh = W + 12 * n, *E2h = W + 13 * n;
   dim3 blk(64, 4);
   dim3 gr = box_grid(g.nz - 1, g.ny - 1, g.gend - 2, blk);
+  const dim3 gp((g.nz - 1 + FK - 1) / FK, (g.ny - 1 + FJ - 1) / FJ, g.gend - 2);
   for (int wide = 0; wide < 2; wide++) {
     const double* src = U;
     for (int pass = 0; pass < 6; pass++) {
       double* out = Gb[pass & 1];
-      if (wide) {
-        hipLaunchKernelGGL(k_filt_x<1>, gr, blk, 0, st, src, U, Gx, g);
-        hipLaunchKernelGGL(k_filt_y<1>, gr, blk, 0, st, Gx, U, prof, Yp, Gy, g);
-        hipLaunchKernelGGL(k_filt_z<1>, gr, blk, 0, st, Gy, U, out, acc, pass, g);
-      } else {
-        hipLaunchKernelGGL(k_filt_x<0>, gr, blk, 0, st, src, U, Gx, g);
-        hipLaunchKernelGGL(k_filt_y<0>, gr, blk, 0, st, Gx, U, prof, Yp, Gy, g);
-        hipLaunchKernelGGL(k_filt_z<0>, gr, blk, 0, st, Gy, U, out, acc, pass, g);
-      }
+      if (wide)
+        hipLaunchKernelGGL(k_filt_pass<1>, gp, dim3(256), 0, st, src, U, prof, Yp, out, acc, pass, g);
+      else
+        hipLaunchKernelGGL(k_filt_pass<0>, gp, dim3(256), 0, st, src, U, prof, Yp, out, acc, pass, g);
       src = out;
     }
     hipLaunchKernelGGL(k_filt_energy, gr, blk, 0, st, U, acc, wide ? E2h : Eh, g);
