@@ -112,7 +112,7 @@ def main():
                     help="also run one plain 1 GiB device copy (the reference kernel for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
+        print("bench.py measures one GPU only: the solver has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr)
         sys.exit(2)
 

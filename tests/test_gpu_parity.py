@@ -5,8 +5,8 @@ The library is compiled with -ffp-contract=off, so every FP64 operation is
 rounded once in the reference's order; the per-function outputs are expected
 to match the reference bit for bit.  The residual norms are reduced in a
 different (tree) order on the GPU, so er and the states downstream of a
-physical step are compared with a relative tolerance of 1e-12 (north-star
-bar: relative L2 <= 1e-12 per kernel, state after 10 steps <= 1e-10).
+physical step are compared with a relative tolerance of 1e-12 (relative L2
+<= 1e-12 per kernel, state after 10 steps <= 1e-10).
 """
 import numpy as np
 import pytest
