@@ -43,8 +43,8 @@ def stage_fields(n_sweeps):
 
 # stages that are one kernel per launch, and that kernel's name in rocprofv3
 STAGE_KERNEL = {"flux_x": "cf::k_flux<0>", "flux_y": "cf::k_flux<1>", "flux_z": "cf::k_flux<2>", "rhs": "cf::k_rhs",
-                "lusgs_fwd": "cf::k_sweep_pipe<true, 3, 2>", "lusgs_bwd": "cf::k_sweep_pipe<false, 3, 2>",
-                "sweep_prep": "cf::k_sweep_prep_rhs"}
+                "lusgs_fwd": "cf::k_sweep_pipe<true, 3, 2, double>", "lusgs_bwd": "cf::k_sweep_pipe<false, 3, 2, double>",
+                "sweep_prep": "cf::k_sweep_prep_rhs<double>"}
 TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_256x192x256.md"
 
 
@@ -101,13 +101,88 @@ def cpu_baseline(cfg, seconds_budget=30.0):
                       f"{GRID[0]}x{GRID[1]}x{GRID[2]} (OpenMP, {rec['seconds']:.1f} s, {cpu_model})"}
 
 
+def physical_steps(s, Q, steps=2, cap=800):
+    """From the seeded state (Q at both levels): physical steps, each with
+    inner iterations until max_k er_k/er_k(1) <= 1e-8 (k = 1..4), then the
+    level shift.  The second step, the first with a BDF2 history, is the one
+    reported; the iterate after it is returned for comparisons."""
+    import torch
+    s.load_state(Q, Q)
+    recs = []
+    for step in range(steps):
+        if step:
+            s.advance_levels()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        first, its, ratio = None, 0, 1.0
+        while its < cap:
+            e = s.inner_iteration()
+            its += 1
+            if first is None:
+                first = e.copy()
+            ratio = float(np.max(e[:4] / first[:4]))
+            if ratio <= 1e-8:
+                break
+        torch.cuda.synchronize()
+        recs.append({"wall_s": round(time.perf_counter() - t1, 4), "inner_iterations": its, "residual_drop": ratio,
+                     "reached_1e-8": ratio <= 1e-8})
+    conv = dict(recs[-1], cap=cap, step=steps, earlier_steps=recs[:-1])
+    return conv, s.read_state()[2]
+
+
+def mean_profiles(q):
+    """Mean velocity and Reynolds-stress profiles over x and z of a state Q."""
+    rho = q[0]
+    u, v, w = q[1] / rho, q[2] / rho, q[3] / rho
+    m = lambda a: a.mean(axis=(0, 2))  # noqa: E731
+    U, V, W = m(u), m(v), m(w)
+    return {"U": U, "V": V, "W": W, "uu": m(u * u) - U * U, "vv": m(v * v) - V * V, "ww": m(w * w) - W * W,
+            "uv": m(u * v) - U * V}
+
+
+def fp32_sweeps_leg(cfg, Q, args, q64):
+    """The same measurements with CF_FP32_SWEEPS (LU-SGS sweeps and split
+    Jacobians in float inside the FP64 loop), and the converged state and
+    mean profiles against the FP64 run's."""
+    import torch
+    from channel_flow_amd.solver import CF_FP32_SWEEPS, Solver
+    s = Solver(cfg, precision=CF_FP32_SWEEPS)
+    s.Jacobian_all(cfg.gamma_metric, want_arrays=False)
+    s.load_state(Q, Q)
+    for _ in range(args.warmup):
+        s.inner_iteration()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.inner_iteration()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    sts = [s.profile_inner_iteration()[1] for _ in range(3)]
+    stage_ms = {k: round(float(np.median([x[k][0] for x in sts])), 4) for k in sts[0]}
+    out = {"value": round(cfg.cells * args.steps / dt / 1e6, 3), "unit": "Mcell-iter/s",
+           "ms_per_step": round(dt / args.steps * 1e3, 4), "stages_ms": stage_ms}
+    if q64 is not None:
+        conv, q32 = physical_steps(s, Q)
+        out["phys_step_to_1e-8"] = conv
+        change = float(np.linalg.norm(q64 - Q))
+        diff = float(np.linalg.norm(q32 - q64))
+        p64, p32 = mean_profiles(q64), mean_profiles(q32)
+        out["vs_fp64"] = {
+            "state_rel_l2": diff / float(np.linalg.norm(q64)),
+            "state_diff_over_change": diff / change if change > 0 else None,
+            "profile_max_rel_err": {k: float(np.abs(p32[k] - p64[k]).max() / np.abs(p64[k]).max()) for k in p64}}
+    s.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-converge", action="store_true", help="skip the physical step to 1e-8")
+    ap.add_argument("--no-converge", action="store_true", help="skip the physical steps to 1e-8")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-sweep leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain 1 GiB device copy (the reference kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -138,7 +213,6 @@ def main():
     s = Solver(cfg)
     s.Jacobian_all(cfg.gamma_metric, want_arrays=False)
     s.load_state(Q, Q)
-    del Q
     for _ in range(args.warmup):
         s.inner_iteration()
     torch.cuda.synchronize()
@@ -195,30 +269,17 @@ def main():
                 for k, v in vt.items()}
     log(f"variants: {json.dumps(variants)}")
 
-    conv = None
+    conv = q64 = None
     if not args.no_converge:
-        # one physical step with inner iterations until max_k er_k/er_k(1) <= 1e-8 (k = 1..4)
-        s.advance_levels()
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        first = None
-        its = 0
-        ratio = 1.0
-        cap = 400
-        while its < cap:
-            e = s.inner_iteration()
-            its += 1
-            if first is None:
-                first = e.copy()
-            ratio = float(np.max(e[:4] / first[:4]))
-            if ratio <= 1e-8:
-                break
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t1
-        conv = {"wall_s": round(wall, 4), "inner_iterations": its, "residual_drop": ratio,
-                "reached_1e-8": ratio <= 1e-8, "cap": cap}
-        log(f"physical step to 1e-8: {conv}")
+        conv, q64 = physical_steps(s, Q)
+        log(f"physical steps to 1e-8 (FP64): {conv}")
     s.close()
+
+    fp32 = None
+    if not args.no_fp32:
+        fp32 = fp32_sweeps_leg(cfg, Q, args, q64)
+        log(f"FP32 sweeps: {json.dumps(fp32)}")
+    del Q
 
     cpu = None if args.no_cpu else cpu_baseline(cfg)
     out = {
@@ -232,6 +293,7 @@ def main():
                    "grid": "x".join(map(str, GRID)), "cells": cfg.cells, "n_sweeps": cfg.n_sweeps,
                    "parallelism": "single GPU"},
         "phys_step_to_1e-8": conv,
+        "fp32_sweeps": fp32,
         "stages_ms": per_stage,
         "variants_ms": variants,
         "roofline": roofline,

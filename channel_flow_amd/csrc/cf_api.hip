@@ -51,6 +51,7 @@ struct cf_ctx {
   int schedule = 0;       // 0: persistent tiles, 1: one hyperplane per launch
   int solver = 0;         // implicit part: 0 LU-SGS (Dual_time_stepping), 1 DPLUSGS
   int visc_schedule = 0;  // 0: fused viscous kernel, 1: three kernels through Pr/VS (allocated on first use)
+  bool fp32 = false;      // CF_FP32_SWEEPS: LU-SGS sweeps and split Jacobians in float
   long spin_limit = 1L << 22;
   double* partial = nullptr;
   double* er_dev = nullptr;
@@ -183,9 +184,10 @@ extern "C" int cf_create(const cf_params* p, cf_ctx** out) {
     return CF_ERR_ARG;
   if (!(p->deltaXI > 0) || !(p->deltaET > 0) || !(p->deltaZT > 0) || !(p->deltaT > 0)) return CF_ERR_ARG;
   if (p->n_sweeps < 1 || p->iteration_end_step < 1) return CF_ERR_ARG;
-  if (p->precision != CF_FP64) return CF_ERR_ARG;
+  if (p->precision != CF_FP64 && p->precision != CF_FP32_SWEEPS) return CF_ERR_ARG;
   cf_ctx* c = new cf_ctx();
   c->p = *p;
+  c->fp32 = p->precision == CF_FP32_SWEEPS;
   Geo& g = c->g;
   g.Xnp = p->X_out + 6;
   g.Ym = p->Y_out + 4;
@@ -626,9 +628,17 @@ static void sweeps(cf_ctx* c, double deltaT, const hipEvent_t* marks = nullptr, 
   }
   double DN = 1. / ((double)c->p.X_out * c->p.Y_out * c->p.Z_out);
   const cf::RhsIn rin{c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->p.f_drive};
-  cf::launch_sweeps(c->stream, c->Ui, c->MR, c->Sm, c->NL, c->NR, c->prof, g, deltaT, c->p.n_sweeps, c->order,
-                    c->sync, c->partial, c->er_dev, DN, c->spin_limit, marks, c->schedule != 2,
-                    fuse_rhs ? &rin : nullptr);
+  cf::launch_sweeps(c->stream, c->Ui, c->MR, c->Sm, c->prof, g, deltaT, c->p.n_sweeps, c->order, c->sync,
+                    c->partial, c->er_dev, DN, c->spin_limit, marks, c->schedule != 2, fuse_rhs ? &rin : nullptr,
+                    c->fp32);
+}
+
+// FP32 sweeps exist for the pipelined tiled schedule, which needs diagonal
+// metric rows; any other combination is refused rather than run in FP64
+static int check_precision(cf_ctx* c) {
+  if (c->fp32 && (c->schedule != 0 || effective_schedule(c) != 0))
+    return fail(c, CF_ERR_ARG, "FP32 sweeps need diagonal metrics and the pipelined sweep schedule (0)");
+  return CF_OK;
 }
 
 // the RHS runs inside the tiled sweeps' pre-pass when the caller needs
@@ -678,13 +688,14 @@ static int dplusgs_device(cf_ctx* c, double deltaT, double er[5]) {
 static int dual_time_device(cf_ctx* c, double deltaT, double er[5], bool want_nl) {
   const Geo& g = c->g;
   if (c->solver == 1) return dplusgs_device(c, deltaT, er);
+  if (int r = check_precision(c)) return r;
   const bool fuse = rhs_fused(c, want_nl);
   if (!fuse)
     cf::launch_rhs(c->stream, c->Ui, c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->prof, c->MR,
                    want_nl ? c->NL : nullptr, g, deltaT, c->p.f_drive);
   sweeps(c, deltaT, nullptr, fuse);
   if (want_nl) {
-    if (effective_schedule(c) == 0) cf::launch_sweep_outputs(c->stream, c->Sm, c->NL, c->NR, g);
+    if (effective_schedule(c) == 0) cf::launch_sweep_outputs(c->stream, c->Sm, c->NL, c->NR, g, c->fp32);
     cf::launch_copy_interior(c->stream, c->NR, c->NL, g);
   }
   update(c);
@@ -879,6 +890,7 @@ extern "C" int cf_profile_inner_iteration(cf_ctx* c, double er[5], double stage_
                                           int launches[CF_PROFILE_STAGES]) {
   if (!c || !er || !stage_ms || !launches) return CF_ERR_ARG;
   if (!c->have_state) return fail(c, CF_ERR_STATE, "no state loaded");
+  if (int r = check_precision(c)) return r;
   const Geo& g = c->g;
   const int ns = c->p.n_sweeps;
   // ev[0..6]: stage ends up to the RHS; then the sweep marks; then the end
@@ -1149,6 +1161,7 @@ extern "C" int cf_set_viscous_schedule(cf_ctx* c, int schedule) {
 
 extern "C" int cf_set_sweep_schedule(cf_ctx* c, int schedule) {
   if (!c || schedule < 0 || schedule > 2) return fail(c, CF_ERR_ARG, "bad schedule");
+  if (c->fp32 && schedule != 0) return fail(c, CF_ERR_ARG, "FP32 sweeps run on the pipelined schedule (0) only");
   c->schedule = schedule;
   return CF_OK;
 }

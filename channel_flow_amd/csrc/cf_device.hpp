@@ -440,19 +440,23 @@ struct DNb {  // one neighbour as seen along one direction
 // metric n: forward A+ (c11 = p11, c22 = p22) or backward A- (m11, m22);
 // c1q, c5q, pd, p51 are shared.  They depend on the neighbour's state and
 // the metric at its j only.
-struct SplitCoef {
-  double c11, c1q, c22, c5q, pd, p51;
+// R is the sweep's arithmetic type: double (the reference's arithmetic, in its
+// order) or float (the FP32 inner solve)
+template <class R>
+struct SplitCoefT {
+  R c11, c1q, c22, c5q, pd, p51;
 };
+using SplitCoef = SplitCoefT<double>;
 
-template <int D, bool FWD>
-__device__ __forceinline__ SplitCoef diag_coef(double P, double T, double beta, double U, double lam, double pd,
-                                          double p51, double n) {
-  const double dd = fabs(lam);
-  SplitCoef c;
-  c.c11 = FWD ? 0.5 * (beta * U + dd) : 0.5 * (beta * U - dd);
-  c.c1q = 0.5 * P * KG * beta * n;
-  c.c22 = FWD ? 0.5 * (U + dd) : 0.5 * (U - dd);
-  c.c5q = D == 0 ? 0.5 * n * beta * T * (KG - 1) : 0.5 * beta * T * (KG - 1) * n;
+template <int D, bool FWD, class R>
+__device__ __forceinline__ SplitCoefT<R> diag_coef(R P, R T, R beta, R U, R lam, R pd, R p51, R n) {
+  const R h = R(0.5), kg = R(KG), kg1 = R(KG - 1);
+  const R dd = fabs(lam);
+  SplitCoefT<R> c;
+  c.c11 = FWD ? h * (beta * U + dd) : h * (beta * U - dd);
+  c.c1q = h * P * kg * beta * n;
+  c.c22 = FWD ? h * (U + dd) : h * (U - dd);
+  c.c5q = D == 0 ? h * n * beta * T * kg1 : h * beta * T * kg1 * n;
   c.pd = pd;
   c.p51 = p51;
   return c;
@@ -460,8 +464,8 @@ __device__ __forceinline__ SplitCoef diag_coef(double P, double T, double beta, 
 
 // the neighbour's contribution: forward -A+ N (Dual_time_stepping.cpp:751-811),
 // backward A- N (Dual_time_stepping.cpp:1116-1177), without the exact-zero terms
-template <int D, bool FWD>
-__device__ __forceinline__ void diag_apply(const SplitCoef& c, const double N[5], double o[5]) {
+template <int D, bool FWD, class R>
+__device__ __forceinline__ void diag_apply(const SplitCoefT<R>& c, const R N[5], R o[5]) {
   constexpr int q = D + 1;
   if (FWD) {
     o[0] = -c.c11 * N[0] - c.c1q * N[q];

@@ -8,6 +8,8 @@
 // j-profiles (they vary only with j, Jacobian*.cpp).
 #include "cf_kernels.hpp"
 
+#include <type_traits>
+
 namespace cf {
 
 // ---------------------------------------------------------------------------
@@ -904,6 +906,23 @@ __device__ __forceinline__ bool is_sentinel(double v) {
   return (unsigned long long)__double_as_longlong(v) == SENTINEL;
 }
 
+// the FP32 sweep's hand-off slots: a signalling-NaN payload that no
+// arithmetic produces (loads, stores and cross-lane moves keep the bits)
+constexpr unsigned SENTINEL32 = 0x7FA5EED1u;
+
+__device__ __forceinline__ bool is_sentinel(float v) { return __float_as_uint(v) == SENTINEL32; }
+
+template <class T>
+__device__ __forceinline__ T sentinel_value();
+template <>
+__device__ __forceinline__ double sentinel_value<double>() {
+  return __longlong_as_double((long long)SENTINEL);
+}
+template <>
+__device__ __forceinline__ float sentinel_value<float>() {
+  return __uint_as_float(SENTINEL32);
+}
+
 // Sweep-order layout: element (tile, s, lane) with lane = a + 8b and
 // s = (k-2) + a + b, i.e. the step at which the forward walk visits the cell;
 // the backward walk visits it at step nsteps-1-s.  At every step the 64 lanes
@@ -981,9 +1000,10 @@ __device__ __forceinline__ void prep_stage(double (*L)[64][PCP], const double* _
 
 // the summary and MR in sweep order, and the NL hand-off slots seeded with the
 // sentinel; each wave writes whole 512-byte rows
+template <class T>
 __global__ void __launch_bounds__(256) k_sweep_prep(const double* __restrict__ Ui, const double* __restrict__ MR,
-                                                    const double* __restrict__ prof, double* __restrict__ S,
-                                                    double* __restrict__ MRs, double* __restrict__ NLs, Geo g, Sk q,
+                                                    const double* __restrict__ prof, T* __restrict__ S,
+                                                    T* __restrict__ MRs, T* __restrict__ NLs, Geo g, Sk q,
                                                     double deltaT) {
   __shared__ double L[5][64][PCP];
   int bx, by, bz;
@@ -991,7 +1011,7 @@ __global__ void __launch_bounds__(256) k_sweep_prep(const double* __restrict__ U
   const int tile = by, s0 = bx * PC;
   prep_stage(L, Ui, g, q, tile, s0);
   __syncthreads();
-  const double sv = __longlong_as_double((long long)SENTINEL);
+  const T sv = sentinel_value<T>();
   for (int idx = threadIdx.x; idx < PC * 64; idx += blockDim.x) {
     const int t = idx >> 6, lane = idx & 63, s = s0 + t;
     if (s >= q.nsteps) continue;
@@ -1008,7 +1028,7 @@ __global__ void __launch_bounds__(256) k_sweep_prep(const double* __restrict__ U
     for (int f = 0; f < 5; f++) u5[f] = L[f][lane][t];
     prep_summary(u5, prof, g, j, deltaT, v);
 #pragma unroll
-    for (int r = 0; r < G_N; r++) S[r * q.nsk + o] = v[r];
+    for (int r = 0; r < G_N; r++) S[r * q.nsk + o] = T(v[r]);
   }
   __syncthreads();
   prep_stage(L, MR, g, q, tile, s0);
@@ -1019,7 +1039,7 @@ __global__ void __launch_bounds__(256) k_sweep_prep(const double* __restrict__ U
     if (!sk_cell(q, g, tile, s, lane, &i, &j, &k)) continue;
     const long o = sk_at(q, tile, s, lane);
 #pragma unroll
-    for (int r = 0; r < 5; r++) MRs[r * q.nsk + o] = L[r][lane][t];
+    for (int r = 0; r < 5; r++) MRs[r * q.nsk + o] = T(L[r][lane][t]);
   }
 }
 
@@ -1029,9 +1049,10 @@ __global__ void __launch_bounds__(256) k_sweep_prep(const double* __restrict__ U
 // block keep the ten staged fields within 66 KB of LDS.
 constexpr int PR = 12, PRP = PR + 1;
 
+template <class T>
 __global__ void __launch_bounds__(256) k_sweep_prep_rhs(const double* __restrict__ Ui, RhsIn rin,
-                                                        const double* __restrict__ prof, double* __restrict__ S,
-                                                        double* __restrict__ MRs, double* __restrict__ NLs, Geo g,
+                                                        const double* __restrict__ prof, T* __restrict__ S,
+                                                        T* __restrict__ MRs, T* __restrict__ NLs, Geo g,
                                                         Sk q, double deltaT) {
   __shared__ double L[10][64][PRP];  // U_ 5, MR 5
   int bx, by, bz;
@@ -1054,7 +1075,7 @@ __global__ void __launch_bounds__(256) k_sweep_prep_rhs(const double* __restrict
     }
   }
   __syncthreads();
-  const double sv = __longlong_as_double((long long)SENTINEL);
+  const T sv = sentinel_value<T>();
   for (int idx = threadIdx.x; idx < PR * 64; idx += blockDim.x) {
     const int t = idx >> 6, lane = idx & 63, s = s0 + t;
     if (s >= q.nsteps) continue;
@@ -1070,14 +1091,15 @@ __global__ void __launch_bounds__(256) k_sweep_prep_rhs(const double* __restrict
     for (int f = 0; f < 5; f++) u5[f] = L[f][lane][t];
     prep_summary(u5, prof, g, j, deltaT, v);
 #pragma unroll
-    for (int r = 0; r < G_N; r++) S[r * q.nsk + o] = v[r];
+    for (int r = 0; r < G_N; r++) S[r * q.nsk + o] = T(v[r]);
 #pragma unroll
-    for (int r = 0; r < 5; r++) MRs[r * q.nsk + o] = L[5 + r][lane][t];
+    for (int r = 0; r < 5; r++) MRs[r * q.nsk + o] = T(L[5 + r][lane][t]);
   }
 }
 
 // sweep-order field(s) back to the natural layout
-__global__ void __launch_bounds__(1024) k_sk_to_natural(const double* __restrict__ src, double* __restrict__ dst,
+template <class T>
+__global__ void __launch_bounds__(1024) k_sk_to_natural(const T* __restrict__ src, double* __restrict__ dst,
                                                         Geo g, Sk q, int nfields) {
   const int lane = threadIdx.x & 63;
   const int s = blockIdx.x * 16 + (threadIdx.x >> 6);
@@ -1085,7 +1107,7 @@ __global__ void __launch_bounds__(1024) k_sk_to_natural(const double* __restrict
   int i, j, k;
   if (!sk_cell(q, g, tile, s, lane, &i, &j, &k)) return;
   const long c = at(g, i, j, k), o = sk_at(q, tile, s, lane);
-  for (int v = 0; v < nfields; v++) dst[v * g.nf + c] = src[v * q.nsk + o];
+  for (int v = 0; v < nfields; v++) dst[v * g.nf + c] = double(src[v * q.nsk + o]);
 }
 
 __global__ void k_sweep_reset(int* __restrict__ sync, int n) {
@@ -1098,6 +1120,14 @@ __device__ __forceinline__ double ld_coherent(const double* p) {
 }
 
 __device__ __forceinline__ void st_coherent(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ float ld_coherent(const float* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void st_coherent(float* p, float v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -1373,11 +1403,14 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// T: the sweep's arithmetic and storage type (double, or float for the FP32
+// inner solve)
+template <class T>
 struct PipeData {
-  double own[3][PO_N][64];  // D11 D22 D51, MR or NL (5), z SplitCoef (6) of the lane cells
-  double xc[3][6][72];      // x SplitCoef: lanes, then the x halo by b
-  double yc[3][6][72];      // y SplitCoef: lanes, then the y halo by a
-  double N[2][5][80];       // outputs of the previous step: lanes, x halo by b (64+), y halo by a (72+)
+  T own[3][PO_N][64];  // D11 D22 D51, MR or NL (5), z SplitCoef (6) of the lane cells
+  T xc[3][6][72];      // x SplitCoef: lanes, then the x halo by b
+  T yc[3][6][72];      // y SplitCoef: lanes, then the y halo by a
+  T N[2][5][80];       // outputs of the previous step: lanes, x halo by b (64+), y halo by a (72+)
   int ticket;
 };
 
@@ -1387,12 +1420,14 @@ struct PipeData {
 // extra tiles in flight gain (256x192x256, one forward sweep: 3.46 ms with
 // three tiles per CU, 2.79 ms with two, 2.24 ms with one).
 constexpr int PIPE_LDS_BYTES = 82 * 1024;
-struct PipeLds : PipeData {
-  char reserve[PIPE_LDS_BYTES - sizeof(PipeData)];
+template <class T>
+struct PipeLds : PipeData<T> {
+  char reserve[PIPE_LDS_BYTES - sizeof(PipeData<T>)];
 };
-static_assert(sizeof(PipeLds) > 80 * 1024, "one sweep block per CU");
+static_assert(sizeof(PipeLds<double>) > 80 * 1024 && sizeof(PipeLds<float>) > 80 * 1024, "one sweep block per CU");
 
-__device__ __forceinline__ void coef_store(double (&row)[6][72], int col, const SplitCoef& c) {
+template <class T>
+__device__ __forceinline__ void coef_store(T (&row)[6][72], int col, const SplitCoefT<T>& c) {
   row[0][col] = c.c11;
   row[1][col] = c.c1q;
   row[2][col] = c.c22;
@@ -1401,8 +1436,9 @@ __device__ __forceinline__ void coef_store(double (&row)[6][72], int col, const 
   row[5][col] = c.p51;
 }
 
-__device__ __forceinline__ SplitCoef coef_load(const double (&row)[6][72], int col) {
-  return SplitCoef{row[0][col], row[1][col], row[2][col], row[3][col], row[4][col], row[5][col]};
+template <class T>
+__device__ __forceinline__ SplitCoefT<T> coef_load(const T (&row)[6][72], int col) {
+  return SplitCoefT<T>{row[0][col], row[1][col], row[2][col], row[3][col], row[4][col], row[5][col]};
 }
 
 // x/d by a spacing: FD when all three spacings have checked reciprocals
@@ -1412,10 +1448,12 @@ __device__ __forceinline__ double div_sp(double x, double d, double r) {
   return x / d;
 }
 
-// wave 0: one cell per lane and step, inputs from LDS only
-template <bool FWD, bool FD>
-__device__ void pipe_compute(PipeLds& L, double* __restrict__ SRC, double* __restrict__ DST,
-                             double* __restrict__ AUX, const Geo& g, const Sk& q, int tile, int TX, int TY) {
+// wave 0: one cell per lane and step, inputs from LDS only.  In FP32 the
+// divisions by the spacings become products with their float reciprocals.
+template <bool FWD, bool FD, class T>
+__device__ void pipe_compute(PipeLds<T>& L, T* __restrict__ SRC, T* __restrict__ DST, T* __restrict__ AUX,
+                             const Geo& g, const Sk& q, int tile, int TX, int TY) {
+  constexpr bool F64 = std::is_same<T, double>::value;
   const int lane = threadIdx.x & 63;
   const int a = lane & 7, b = lane >> 3;
   const int i = 3 + TX * TI + a, j = 2 + TY * TJ + b;
@@ -1433,50 +1471,55 @@ __device__ void pipe_compute(PipeLds& L, double* __restrict__ SRC, double* __res
   const long nf = q.nsk;
   const bool edge_fwd = a == TI - 1 || b == TJ - 1;
   const bool edge_bwd = a == 0 || b == 0;
-  const double sv = __longlong_as_double((long long)SENTINEL);
-  double zN[5] = {0, 0, 0, 0, 0};
-  SplitCoef zc = {0, 0, 0, 0, 0, 0};
+  const T sv = sentinel_value<T>();
+  const float fXI = float(1.0 / g.dXI), fET = float(1.0 / g.dET), fZT = float(1.0 / g.dZT);
+  T zN[5] = {0, 0, 0, 0, 0};
+  SplitCoefT<T> zc = {0, 0, 0, 0, 0, 0};
   int cs = 0, ps = 2;
   lds_barrier();  // step 0 inputs are in LDS
   for (int tau = 0; tau < nsteps; tau++) {
     const int kk = tau - off;
     const int nr = (tau & 1) ^ 1, nw = tau & 1;
     if (col && kk >= 0 && kk < Nz) {
-      const double d11 = L.own[cs][PO_D11][lane], d22 = L.own[cs][PO_D22][lane], d51 = L.own[cs][PO_D51][lane];
-      double M[5];
+      const T d11 = L.own[cs][PO_D11][lane], d22 = L.own[cs][PO_D22][lane], d51 = L.own[cs][PO_D51][lane];
+      T M[5];
 #pragma unroll
       for (int v = 0; v < 5; v++) M[v] = L.own[cs][PO_M + v][lane];
-      double acc[3][5];
+      T acc[3][5];
       if (x_dom) {
-        double Nb[5];
+        T Nb[5];
 #pragma unroll
         for (int v = 0; v < 5; v++) Nb[v] = L.N[nr][v][xi];
         diag_apply<0, FWD>(coef_load(L.xc[ps], xi), Nb, acc[0]);
       } else {
 #pragma unroll
-        for (int v = 0; v < 5; v++) acc[0][v] = 0.0;
+        for (int v = 0; v < 5; v++) acc[0][v] = 0;
       }
       if (y_dom) {
-        double Nb[5];
+        T Nb[5];
 #pragma unroll
         for (int v = 0; v < 5; v++) Nb[v] = L.N[nr][v][yiN];
         diag_apply<1, FWD>(coef_load(L.yc[ps], yi), Nb, acc[1]);
       } else {
 #pragma unroll
-        for (int v = 0; v < 5; v++) acc[1][v] = 0.0;
+        for (int v = 0; v < 5; v++) acc[1][v] = 0;
       }
       if (kk > 0) {
         diag_apply<2, FWD>(zc, zN, acc[2]);
       } else {
 #pragma unroll
-        for (int v = 0; v < 5; v++) acc[2][v] = 0.0;
+        for (int v = 0; v < 5; v++) acc[2][v] = 0;
       }
-      double Lv[5];
+      T Lv[5];
 #pragma unroll
-      for (int v = 0; v < 5; v++)
-        Lv[v] = div_sp<FD>(acc[0][v], g.dXI, g.rXI) + div_sp<FD>(acc[1][v], g.dET, g.rET) +
-                div_sp<FD>(acc[2][v], g.dZT, g.rZT);
-      double out[5];
+      for (int v = 0; v < 5; v++) {
+        if constexpr (F64)
+          Lv[v] = div_sp<FD>(acc[0][v], g.dXI, g.rXI) + div_sp<FD>(acc[1][v], g.dET, g.rET) +
+                  div_sp<FD>(acc[2][v], g.dZT, g.rZT);
+        else
+          Lv[v] = acc[0][v] * fXI + acc[1][v] * fET + acc[2][v] * fZT;
+      }
+      T out[5];
       if (FWD) {
         out[0] = d11 * (M[0] - Lv[0]);
         out[1] = d22 * (M[1] - Lv[1]);
@@ -1508,8 +1551,8 @@ __device__ void pipe_compute(PipeLds& L, double* __restrict__ SRC, double* __res
         L.N[nw][v][lane] = out[v];
         zN[v] = out[v];
       }
-      zc = SplitCoef{L.own[cs][PO_Z + 0][lane], L.own[cs][PO_Z + 1][lane], L.own[cs][PO_Z + 2][lane],
-                L.own[cs][PO_Z + 3][lane], L.own[cs][PO_Z + 4][lane], L.own[cs][PO_Z + 5][lane]};
+      zc = SplitCoefT<T>{L.own[cs][PO_Z + 0][lane], L.own[cs][PO_Z + 1][lane], L.own[cs][PO_Z + 2][lane],
+                         L.own[cs][PO_Z + 3][lane], L.own[cs][PO_Z + 4][lane], L.own[cs][PO_Z + 5][lane]};
     }
     ps = cs;
     cs = cs == 2 ? 0 : cs + 1;
@@ -1544,10 +1587,10 @@ __device__ __forceinline__ void pipe_loader_loop(int nsteps, Consume consume, Is
   }
 }
 
-template <int Q, int NF>
+template <int Q, int NF, class T>
 struct Slots {
-  double v[Q][NF];
-  __device__ __forceinline__ void put(int sl, const double (&d)[NF]) {
+  T v[Q][NF];
+  __device__ __forceinline__ void put(int sl, const T (&d)[NF]) {
 #pragma unroll
     for (int k = 0; k < Q; k++)
       if (k == sl) {
@@ -1555,7 +1598,7 @@ struct Slots {
         for (int f = 0; f < NF; f++) v[k][f] = d[f];
       }
   }
-  __device__ __forceinline__ void get(int sl, double (&d)[NF]) const {
+  __device__ __forceinline__ void get(int sl, T (&d)[NF]) const {
 #pragma unroll
     for (int f = 0; f < NF; f++) d[f] = v[0][f];
 #pragma unroll
@@ -1570,8 +1613,8 @@ struct Slots {
 // waves 1 and 2: the lane cells' fields QS steps ahead.  XY (wave 1): P, T,
 // beta and the x, y summaries -> x, y SplitCoef.  Otherwise (wave 2): D^-1, SRC and
 // the z summary -> own rows.
-template <bool FWD, bool XY, int QS>
-__device__ void pipe_own_loader(PipeLds& L, const double* __restrict__ S, const double* __restrict__ SRC,
+template <bool FWD, bool XY, int QS, class T>
+__device__ void pipe_own_loader(PipeLds<T>& L, const T* __restrict__ S, const T* __restrict__ SRC,
                                 const double* __restrict__ prof, const Geo& g, const Sk& q, int tile, int TY) {
   constexpr int NF = XY ? 11 : 15;
   const int lane = threadIdx.x & 63;
@@ -1580,36 +1623,37 @@ __device__ void pipe_own_loader(PipeLds& L, const double* __restrict__ S, const 
   const long row0 = (long)tile * nsteps * 64 + lane;
   // metric of the lane's j (clamped for columns outside the domain)
   const int j = min(2 + TY * TJ + (lane >> 3), g.ny);
-  const double nx = pf(prof, g.Ym, SET_C, 1, j), ny_ = pf(prof, g.Ym, SET_C, 5, j), nz_ = pf(prof, g.Ym, SET_C, 9, j);
+  const T nx = T(pf(prof, g.Ym, SET_C, 1, j)), ny_ = T(pf(prof, g.Ym, SET_C, 5, j)),
+          nz_ = T(pf(prof, g.Ym, SET_C, 9, j));
   // fields: XY: P T beta UX LX PDX P51X UY LY PDY P51Y;
   //         z:  D11 D22 D51 SRC0..4 P T beta UZ LZ PDZ P51Z
-  auto field = [&](int f) -> const double* {
+  auto field = [&](int f) -> const T* {
     if (XY) return S + (long)(f < 3 ? G_P + f : G_UX + (f - 3)) * nf;
     if (f < 3) return S + (long)(G_D11 + f) * nf;
     if (f < 8) return SRC + (long)(f - 3) * nf;
     return S + (long)(f < 11 ? G_P + (f - 8) : G_UZ + (f - 11)) * nf;
   };
-  Slots<QS, NF> buf;
+  Slots<QS, NF, T> buf;
   auto issue = [&](int sig, int sl) {
     const int sc = min(sig, nsteps - 1);
     const long c = row0 + (long)(FWD ? sc : nsteps - 1 - sc) * 64;
-    double d[NF];
+    T d[NF];
 #pragma unroll
     for (int f = 0; f < NF; f++) d[f] = field(f)[c];
     buf.put(sl, d);
   };
   auto consume = [&](int sig, int sl) {
     if (sig >= nsteps) return;
-    double d[NF];
+    T d[NF];
     buf.get(sl, d);
     const int s3 = sig % 3;
-    if (XY) {
+    if constexpr (XY) {
       coef_store(L.xc[s3], lane, diag_coef<0, FWD>(d[0], d[1], d[2], d[3], d[4], d[5], d[6], nx));
       coef_store(L.yc[s3], lane, diag_coef<1, FWD>(d[0], d[1], d[2], d[7], d[8], d[9], d[10], ny_));
     } else {
 #pragma unroll
       for (int f = 0; f < 8; f++) L.own[s3][f][lane] = d[f];
-      const SplitCoef z = diag_coef<2, FWD>(d[8], d[9], d[10], d[11], d[12], d[13], d[14], nz_);
+      const SplitCoefT<T> z = diag_coef<2, FWD>(d[8], d[9], d[10], d[11], d[12], d[13], d[14], nz_);
       L.own[s3][PO_Z + 0][lane] = z.c11;
       L.own[s3][PO_Z + 1][lane] = z.c1q;
       L.own[s3][PO_Z + 2][lane] = z.c22;
@@ -1633,8 +1677,8 @@ __device__ void pipe_own_loader(PipeLds& L, const double* __restrict__ S, const 
 // values 3g..3g+2 (g = l >> 4) for consumer l & 15, so all 64 lanes carry
 // three values per step; after the sentinel poll, lanes 0..15 gather their
 // consumer's twelve values with cross-lane shuffles.
-template <bool FWD, int QD>
-__device__ void pipe_halo_loader(PipeLds& L, const double* __restrict__ S, const double* __restrict__ DST,
+template <bool FWD, int QD, class T>
+__device__ void pipe_halo_loader(PipeLds<T>& L, const T* __restrict__ S, const T* __restrict__ DST,
                                  const double* __restrict__ prof, const Geo& g, const Sk& q, int tile, int TX,
                                  int TY, int* err, long spin_limit) {
   const int lane = threadIdx.x & 63;
@@ -1655,7 +1699,7 @@ __device__ void pipe_halo_loader(PipeLds& L, const double* __restrict__ S, const
   const long rem0 = own0 + (dir == 0 ? sx : sy);
   // metric at the remote cell's j
   const int jr = dir == 0 ? j : (FWD ? j - 1 : j + 1);
-  const double n = pf(prof, g.Ym, SET_C, dir == 0 ? 1 : 5, min(max(jr, 0), g.Ym - 1));
+  const T n = T(pf(prof, g.Ym, SET_C, dir == 0 ? 1 : 5, min(max(jr, 0), g.Ym - 1)));
   const int dU = G_UX + 4 * dir;
   auto live = [=](int sig) -> bool {
     const int kk = sig - off;
@@ -1667,24 +1711,24 @@ __device__ void pipe_halo_loader(PipeLds& L, const double* __restrict__ S, const
     return own0 + (live(sig) ? rdiff : 0L) + (long)(FWD ? sc : nsteps - 1 - sc) * 64;
   };
   // this lane's three values f = 3 grp + r: P T beta U lam pd p51 N0..N4
-  const double* fld[3];
+  const T* fld[3];
 #pragma unroll
   for (int r = 0; r < 3; r++) {
     const int f = 3 * grp + r;
     fld[r] = f < 3 ? S + (long)(G_P + f) * nf : (f < 7 ? S + (long)(dU + f - 3) * nf : DST + (long)(f - 7) * nf);
   }
   const bool dyn[3] = {3 * grp >= 7, 3 * grp + 1 >= 7, 3 * grp + 2 >= 7};
-  Slots<QD, 3> buf;
+  Slots<QD, 3, T> buf;
   auto issue = [&](int sig, int sl) {
     const long c = cell(sig);
-    double d[3];
+    T d[3];
 #pragma unroll
     for (int r = 0; r < 3; r++) d[r] = ld_coherent(fld[r] + c);
     buf.put(sl, d);
   };
   auto consume = [&](int sig, int sl) {
     if (sig >= nsteps) return;
-    double d[3];
+    T d[3];
     buf.get(sl, d);
     const bool lv = live(sig);
     bool pend = false;
@@ -1709,7 +1753,7 @@ __device__ void pipe_halo_loader(PipeLds& L, const double* __restrict__ S, const
           }
       }
     }
-    double v[12];
+    T v[12];
 #pragma unroll
     for (int gq = 0; gq < 4; gq++)
 #pragma unroll
@@ -1735,16 +1779,16 @@ __device__ void pipe_halo_loader(PipeLds& L, const double* __restrict__ S, const
   pipe_loader_loop<QD>(nsteps, consume, issue);
 }
 
-template <bool FWD, int QS, int QD>
-__global__ void __launch_bounds__(256) k_sweep_pipe(const double* __restrict__ S, double* __restrict__ SRC,
-                                                    double* __restrict__ DST, double* __restrict__ AUX,
+template <bool FWD, int QS, int QD, class T>
+__global__ void __launch_bounds__(256) k_sweep_pipe(const T* __restrict__ S, T* __restrict__ SRC,
+                                                    T* __restrict__ DST, T* __restrict__ AUX,
                                                     const double* __restrict__ prof, Geo g, Sk q,
                                                     const int* __restrict__ order, int* __restrict__ sync,
                                                     long spin_limit) {
   // arrays in sweep order (Sk); S holds the DiagField summary
   // FWD: SRC = MR, DST = NL   (NL = D^-1 (MR - L)); seeds the NR (AUX) sentinels
   // BWD: SRC = NL, DST = NR   (NR = NL - D^-1 U); re-seeds the NL sentinels
-  __shared__ PipeLds L;
+  __shared__ PipeLds<T> L;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (threadIdx.x == 0) L.ticket = atomicAdd(sync, 1);
   __syncthreads();
@@ -1754,10 +1798,14 @@ __global__ void __launch_bounds__(256) k_sweep_pipe(const double* __restrict__ S
   const int TX = tile % q.nTI, TY = tile / q.nTI;
   if (wave == 0) {
     // the kernel arguments make this choice uniform: one scalar branch
-    if (g.rXI != 0.0 && g.rET != 0.0 && g.rZT != 0.0)
-      pipe_compute<FWD, true>(L, SRC, DST, AUX, g, q, tile, TX, TY);
-    else
+    if constexpr (std::is_same<T, double>::value) {
+      if (g.rXI != 0.0 && g.rET != 0.0 && g.rZT != 0.0)
+        pipe_compute<FWD, true>(L, SRC, DST, AUX, g, q, tile, TX, TY);
+      else
+        pipe_compute<FWD, false>(L, SRC, DST, AUX, g, q, tile, TX, TY);
+    } else {
       pipe_compute<FWD, false>(L, SRC, DST, AUX, g, q, tile, TX, TY);
+    }
   } else if (wave == 1) {
     pipe_own_loader<FWD, true, QS>(L, S, SRC, prof, g, q, tile, TY);
   } else if (wave == 2) {
@@ -1899,7 +1947,8 @@ __global__ void __launch_bounds__(256) k_update(double* __restrict__ Ui, const d
 // covers PC steps of one tile (as k_sweep_prep does), loads those NR rows
 // coalesced, then walks each column's run of PC consecutive k in the natural
 // layout.  Per-block partial sums of the residuals, in a fixed order.
-__global__ void __launch_bounds__(256) k_update_sk(double* __restrict__ Ui, const double* __restrict__ NR,
+template <class R>
+__global__ void __launch_bounds__(256) k_update_sk(double* __restrict__ Ui, const R* __restrict__ NR,
                                                    const double* __restrict__ prof, double* __restrict__ partial,
                                                    Geo g, Sk q) {
   __shared__ double L[5][64][PCP];
@@ -1914,7 +1963,7 @@ __global__ void __launch_bounds__(256) k_update_sk(double* __restrict__ Ui, cons
     if (s >= q.nsteps) continue;
     const long o = sk_at(q, tile, s, lane);
 #pragma unroll
-    for (int f = 0; f < 5; f++) L[f][lane][t] = NR[f * q.nsk + o];
+    for (int f = 0; f < 5; f++) L[f][lane][t] = double(NR[f * q.nsk + o]);
   }
   __syncthreads();
   double r[5] = {0, 0, 0, 0, 0};
@@ -2252,56 +2301,78 @@ long sweep_update_blocks(const Geo& g) {
   return (long)((q.nsteps + 3) / 4) * q.ntiles;
 }
 
-void launch_sweeps(hipStream_t st, double* Ui, const double* MR, double* Sbuf, double* NL, double* NR,
-                   const double* prof, const Geo& g, double deltaT, int n_sweeps, const int* order, int* sync,
-                   double* partial, double* er_dev, double DN, long spin_limit, const hipEvent_t* marks,
-                   bool pipelined, const RhsIn* rin) {
-  // Sbuf holds G_N summary fields, then MR, NL, NR (5 each) in sweep order
+// Sbuf holds G_N summary fields, then MR, NL, NR (5 each) in sweep order, as
+// T; a float layout uses the first half of the buffer
+template <class T>
+static void sweeps_t(hipStream_t st, double* Ui, const double* MR, double* Sbuf, const double* prof, const Geo& g,
+                     double deltaT, int n_sweeps, const int* order, int* sync, double* partial, double* er_dev,
+                     double DN, long spin_limit, const hipEvent_t* marks, bool pipelined, const RhsIn* rin) {
   Sk q = sweep_layout(g);
-  double* S = Sbuf;
-  double* MRs = Sbuf + (long)G_N * q.nsk;
-  double* NLs = MRs + 5 * q.nsk;
-  double* NRs = NLs + 5 * q.nsk;
+  T* S = reinterpret_cast<T*>(Sbuf);
+  T* MRs = S + (long)G_N * q.nsk;
+  T* NLs = MRs + 5 * q.nsk;
+  T* NRs = NLs + 5 * q.nsk;
   if (rin)
-    hipLaunchKernelGGL(k_sweep_prep_rhs, dim3((q.nsteps + PR - 1) / PR, q.ntiles), dim3(256), 0, st, Ui, *rin, prof,
-                       S, MRs, NLs, g, q, deltaT);
+    hipLaunchKernelGGL(k_sweep_prep_rhs<T>, dim3((q.nsteps + PR - 1) / PR, q.ntiles), dim3(256), 0, st, Ui, *rin,
+                       prof, S, MRs, NLs, g, q, deltaT);
   else
-    hipLaunchKernelGGL(k_sweep_prep, dim3((q.nsteps + PC - 1) / PC, q.ntiles), dim3(256), 0, st, Ui, MR, prof, S,
+    hipLaunchKernelGGL(k_sweep_prep<T>, dim3((q.nsteps + PC - 1) / PC, q.ntiles), dim3(256), 0, st, Ui, MR, prof, S,
                        MRs, NLs, g, q, deltaT);
   if (marks) (void)hipEventRecord(marks[0], st);
   const int nt = q.ntiles;
   for (int sweep = 0; sweep < n_sweeps; sweep++) {
     hipLaunchKernelGGL(k_sweep_reset, dim3((2 * nt + 4 + 255) / 256), dim3(256), 0, st, sync, 2 * nt + 4);
     if (marks) (void)hipEventRecord(marks[1 + 3 * sweep], st);
+    if constexpr (std::is_same<T, double>::value) {
+      if (!pipelined)
+        hipLaunchKernelGGL(k_sweep_tiles<true>, dim3(nt), dim3(64), 0, st, S, MRs, NLs, NRs, prof, g, q, order, sync,
+                           spin_limit);
+    }
     if (pipelined)
-      hipLaunchKernelGGL((k_sweep_pipe<true, PIPE_QS, PIPE_QD>), dim3(nt), dim3(256), 0, st, S, MRs, NLs, NRs, prof,
-                         g, q, order, sync, spin_limit);
-    else
-      hipLaunchKernelGGL(k_sweep_tiles<true>, dim3(nt), dim3(64), 0, st, S, MRs, NLs, NRs, prof, g, q, order, sync,
-                         spin_limit);
+      hipLaunchKernelGGL((k_sweep_pipe<true, PIPE_QS, PIPE_QD, T>), dim3(nt), dim3(256), 0, st, S, MRs, NLs, NRs,
+                         prof, g, q, order, sync, spin_limit);
     if (marks) (void)hipEventRecord(marks[2 + 3 * sweep], st);
+    if constexpr (std::is_same<T, double>::value) {
+      if (!pipelined)
+        hipLaunchKernelGGL(k_sweep_tiles<false>, dim3(nt), dim3(64), 0, st, S, NLs, NRs, NRs, prof, g, q, order,
+                           sync + nt + 2, spin_limit);
+    }
     if (pipelined)
-      hipLaunchKernelGGL((k_sweep_pipe<false, PIPE_QS, PIPE_QD>), dim3(nt), dim3(256), 0, st, S, NLs, NRs, NRs, prof,
-                         g, q, order, sync + nt + 2, spin_limit);
-    else
-      hipLaunchKernelGGL(k_sweep_tiles<false>, dim3(nt), dim3(64), 0, st, S, NLs, NRs, NRs, prof, g, q, order,
-                         sync + nt + 2, spin_limit);
+      hipLaunchKernelGGL((k_sweep_pipe<false, PIPE_QS, PIPE_QD, T>), dim3(nt), dim3(256), 0, st, S, NLs, NRs, NRs,
+                         prof, g, q, order, sync + nt + 2, spin_limit);
     if (marks) (void)hipEventRecord(marks[3 + 3 * sweep], st);
   }
   // primitive update reading NR from the sweep order
   dim3 gr((q.nsteps + PC - 1) / PC, q.ntiles);
-  hipLaunchKernelGGL(k_update_sk, gr, dim3(256), 0, st, Ui, NRs, prof, partial, g, q);
+  hipLaunchKernelGGL(k_update_sk<T>, gr, dim3(256), 0, st, Ui, NRs, prof, partial, g, q);
   hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, st, partial, (long)gr.x * gr.y, er_dev, DN);
-  (void)NL;
-  (void)NR;
 }
 
-void launch_sweep_outputs(hipStream_t st, const double* Sbuf, double* NL, double* NR, const Geo& g) {
+void launch_sweeps(hipStream_t st, double* Ui, const double* MR, double* Sbuf, const double* prof, const Geo& g,
+                   double deltaT, int n_sweeps, const int* order, int* sync, double* partial, double* er_dev,
+                   double DN, long spin_limit, const hipEvent_t* marks, bool pipelined, const RhsIn* rin, bool fp32) {
+  if (fp32)
+    sweeps_t<float>(st, Ui, MR, Sbuf, prof, g, deltaT, n_sweeps, order, sync, partial, er_dev, DN, spin_limit, marks,
+                    true, rin);
+  else
+    sweeps_t<double>(st, Ui, MR, Sbuf, prof, g, deltaT, n_sweeps, order, sync, partial, er_dev, DN, spin_limit,
+                     marks, pipelined, rin);
+}
+
+template <class T>
+static void sweep_outputs_t(hipStream_t st, const double* Sbuf, double* NL, double* NR, const Geo& g) {
   Sk q = sweep_layout(g);
-  const double* NLs = Sbuf + (long)(G_N + 5) * q.nsk;
-  const double* NRs = NLs + 5 * q.nsk;
-  hipLaunchKernelGGL(k_sk_to_natural, dim3((q.nsteps + 15) / 16, q.ntiles), dim3(1024), 0, st, NLs, NL, g, q, 5);
-  hipLaunchKernelGGL(k_sk_to_natural, dim3((q.nsteps + 15) / 16, q.ntiles), dim3(1024), 0, st, NRs, NR, g, q, 5);
+  const T* NLs = reinterpret_cast<const T*>(Sbuf) + (long)(G_N + 5) * q.nsk;
+  const T* NRs = NLs + 5 * q.nsk;
+  hipLaunchKernelGGL(k_sk_to_natural<T>, dim3((q.nsteps + 15) / 16, q.ntiles), dim3(1024), 0, st, NLs, NL, g, q, 5);
+  hipLaunchKernelGGL(k_sk_to_natural<T>, dim3((q.nsteps + 15) / 16, q.ntiles), dim3(1024), 0, st, NRs, NR, g, q, 5);
+}
+
+void launch_sweep_outputs(hipStream_t st, const double* Sbuf, double* NL, double* NR, const Geo& g, bool fp32) {
+  if (fp32)
+    sweep_outputs_t<float>(st, Sbuf, NL, NR, g);
+  else
+    sweep_outputs_t<double>(st, Sbuf, NL, NR, g);
 }
 
 

@@ -46,12 +46,14 @@ struct RhsIn {
   const double *Un, *Uq, *Fx, *Fy, *Fz, *vF;
   double f_drive;
 };
-void launch_sweeps(hipStream_t st, double* Ui, const double* MR, double* Sbuf, double* NL, double* NR,
-                   const double* prof, const Geo& g, double deltaT, int n_sweeps, const int* order, int* sync,
-                   double* partial, double* er_dev, double DN, long spin_limit, const hipEvent_t* marks = nullptr,
-                   bool pipelined = true, const RhsIn* rin = nullptr);
+// fp32: the sweep-order fields, the split Jacobians and the sweeps in float
+// (pipelined schedule only); the RHS, D^-1's inputs and the update stay FP64
+void launch_sweeps(hipStream_t st, double* Ui, const double* MR, double* Sbuf, const double* prof, const Geo& g,
+                   double deltaT, int n_sweeps, const int* order, int* sync, double* partial, double* er_dev,
+                   double DN, long spin_limit, const hipEvent_t* marks = nullptr, bool pipelined = true,
+                   const RhsIn* rin = nullptr, bool fp32 = false);
 // NL and NR of the last sweep back in the natural layout
-void launch_sweep_outputs(hipStream_t st, const double* Sbuf, double* NL, double* NR, const Geo& g);
+void launch_sweep_outputs(hipStream_t st, const double* Sbuf, double* NL, double* NR, const Geo& g, bool fp32 = false);
 
 void launch_sweeps_planes(hipStream_t st, const double* Ui, const double* MR, double* NL, double* NR,
                           const double* prof, const Geo& g, double deltaT, int n_sweeps,

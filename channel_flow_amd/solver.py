@@ -20,6 +20,9 @@ from .config import ChannelConfig
 LIB_PATH = Path(__file__).resolve().parent / "libchannel_flow.so"
 
 CF_OK = 0
+# enum cf_precision
+CF_FP64 = 0          # everything FP64, bit-identical to the reference
+CF_FP32_SWEEPS = 1   # LU-SGS sweeps and split Jacobians in float (pipelined sweep schedule)
 ERRORS = {-1: "CF_ERR_ARG", -2: "CF_ERR_HIP", -3: "CF_ERR_NONFINITE", -4: "CF_ERR_METRICS",
           -5: "CF_ERR_IO", -6: "CF_ERR_STATE", -7: "CF_ERR_TIMEOUT"}
 
@@ -122,9 +125,14 @@ def _pp(arrs):
 
 
 class Solver:
-    """One GPU context for one grid (replaces main.cpp's driver state)."""
+    """One GPU context for one grid (replaces main.cpp's driver state).
 
-    def __init__(self, cfg: ChannelConfig, precision: int = 0):
+    precision: CF_FP64 (default) or CF_FP32_SWEEPS.  The latter runs the
+    LU-SGS sweeps in float inside the FP64 defect-correction loop; the
+    converged state is the FP64 one, the per-iteration increments are not.
+    """
+
+    def __init__(self, cfg: ChannelConfig, precision: int = CF_FP64):
         self.cfg = cfg
         self.lib = load_library()
         p = CfParams(cfg.X_out, cfg.Y_out, cfg.Z_out, cfg.deltaXI, cfg.deltaET, cfg.deltaZT, cfg.high,

@@ -46,7 +46,13 @@ enum cf_status {
 
 enum cf_precision {
   CF_FP64 = 0,            /* everything FP64, as the reference */
-  CF_FP32_SWEEPS = 1      /* LU-SGS sweeps and split Jacobians in FP32 */
+  CF_FP32_SWEEPS = 1      /* LU-SGS sweeps and split Jacobians in FP32, inside
+                             the FP64 loop: the RHS, D^-1's inputs and the
+                             update stay FP64, so the converged state is the
+                             FP64 one.  Needs diagonal metric rows and sweep
+                             schedule 0; other schedules are refused
+                             (CF_ERR_ARG).  DPLUSGS stays FP64.  No reference
+                             counterpart (the reference is FP64 only). */
 };
 
 /* Runtime replacement of Resolution.h / prm.h / main.cpp:47-57 constants. */
@@ -208,7 +214,8 @@ int cf_profile_variants(cf_ctx* ctx, double Roe_criterion, double ms[CF_VARIANT_
  * dependency order, each tile one compute wave fed by three loader waves;
  * schedule 1: one launch per hyperplane i+j+k = m (the reference order,
  * Hyperplane.cpp:50-78); schedule 2: the tiles of schedule 0 as single waves
- * that load their own inputs. */
+ * that load their own inputs.  A CF_FP32_SWEEPS context accepts schedule 0
+ * only. */
 int cf_set_sweep_schedule(cf_ctx* ctx, int schedule);
 /* RK3NODT.cpp:115-183 (main.h RK3NODT; the explicit path of the shipped
  * main.cpp): stage RK (1, 2, 3) of the SSP-RK3 update
