@@ -421,16 +421,23 @@ __device__ __forceinline__ Split split_nb(const Nb& c, double n0, double n1, dou
 
 // Sweep summary for metrics whose rows are diagonal (xi = (xix,0,0),
 // eta = (0,ety,0), zeta = (0,0,ztz): every grid Jacobian*.cpp produces).
-// The off-diagonal split-Jacobian entries are then exact zeros and the
-// divisions are taken once per cell: PD = 0.5*n_dd/rho (A+21, B+31, C+41) and
-// P51 = 0.5*U*(K-1)*(beta-1)/rho/K for each direction.
+// The off-diagonal split-Jacobian entries are then exact zeros.  Per cell:
+// P, T, beta, rho, D^-1 and, per direction d, the contravariant velocity
+// (G_UX + 2d) and the spectral radius (G_LX + 2d); the two entries that
+// divide by rho are formed from them where they are used (split_pd_p51).
 enum DiagField : int {
-  G_P = 0, G_T, G_BETA, G_D11, G_D22, G_D51,
-  G_UX, G_LX, G_PDX, G_P51X,
-  G_UY, G_LY, G_PDY, G_P51Y,
-  G_UZ, G_LZ, G_PDZ, G_P51Z,
+  G_P = 0, G_T, G_BETA, G_RHO, G_D11, G_D22, G_D51,
+  G_UX, G_LX, G_UY, G_LY, G_UZ, G_LZ,
   G_N
 };
+
+// PD = 0.5*n_dd/rho (A+21, B+31, C+41) and P51 = 0.5*U*(K-1)*(beta-1)/rho/K
+// of one direction (Dual_time_stepping.cpp:751-811), in the reference's order
+template <class R>
+__device__ __forceinline__ void split_pd_p51(R n, R U, R beta, R rho, R* pd, R* p51) {
+  *pd = R(0.5) * n / rho;
+  *p51 = R(0.5) * U * R(KG - 1) * (beta - R(1)) / rho / R(KG);
+}
 
 struct DNb {  // one neighbour as seen along one direction
   double P, T, beta, U, lam, pd, p51;

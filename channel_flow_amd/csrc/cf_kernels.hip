@@ -965,16 +965,14 @@ __device__ __forceinline__ void prep_summary(const double u5[5], const double* _
   v[G_P] = st.P;
   v[G_T] = st.T;
   v[G_BETA] = st.beta;
+  v[G_RHO] = st.rho;
   v[G_D11] = d11;
   v[G_D22] = d22;
   v[G_D51] = d51;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    const double n = m9[4 * d];  // xix, ety, ztz
-    v[G_UX + 4 * d] = U3[d];
-    v[G_LX + 4 * d] = L3[d];
-    v[G_PDX + 4 * d] = 0.5 * n / st.rho;
-    v[G_P51X + 4 * d] = 0.5 * U3[d] * (KG - 1) * (st.beta - 1) / st.rho / KG;
+    v[G_UX + 2 * d] = U3[d];
+    v[G_LX + 2 * d] = L3[d];
   }
 }
 
@@ -1142,15 +1140,15 @@ struct StepIn {
 // LDS slot of one lane: N(5), P, T, beta, x: U, lam, pd, p51, y: U, lam, pd, p51
 constexpr int LDS_F = 16;
 
-__device__ __forceinline__ DNb dnb_at(const double* __restrict__ S, long nsk, long p, int d) {
+// neighbour p along direction d; n: the diagonal metric entry at its j
+__device__ __forceinline__ DNb dnb_at(const double* __restrict__ S, long nsk, long p, int d, double n) {
   DNb c;
   c.P = S[G_P * nsk + p];
   c.T = S[G_T * nsk + p];
   c.beta = S[G_BETA * nsk + p];
-  c.U = S[(G_UX + 4 * d) * nsk + p];
-  c.lam = S[(G_LX + 4 * d) * nsk + p];
-  c.pd = S[(G_PDX + 4 * d) * nsk + p];
-  c.p51 = S[(G_P51X + 4 * d) * nsk + p];
+  c.U = S[(G_UX + 2 * d) * nsk + p];
+  c.lam = S[(G_LX + 2 * d) * nsk + p];
+  split_pd_p51(n, c.U, c.beta, S[G_RHO * nsk + p], &c.pd, &c.p51);
   return c;
 }
 
@@ -1198,6 +1196,7 @@ __global__ void __launch_bounds__(64) k_sweep_tiles(const double* __restrict__ S
   const long sy = FWD ? (-(long)nTI * q.nsteps * 64 + 7 * 64 + 56) : ((long)nTI * q.nsteps * 64 - 7 * 64 - 56);
   // diagonal metric entries of the neighbours (functions of j)
   const double nx = pf(prof, g.Ym, SET_C, 1, j), ny_ = pf(prof, g.Ym, SET_C, 5, jy), nz_ = pf(prof, g.Ym, SET_C, 9, j);
+  const double nyo = pf(prof, g.Ym, SET_C, 5, j);  // at the lane's own j, for the entries it hands on
 
   auto cell_of = [&](int tau) -> long { return sk_at(q, tile, FWD ? tau : q.nsteps - 1 - tau, lane); };
   auto active = [&](int tau) -> bool {
@@ -1215,13 +1214,13 @@ __global__ void __launch_bounds__(64) k_sweep_tiles(const double* __restrict__ S
       const long p = c + sx;
 #pragma unroll
       for (int v = 0; v < 5; v++) in.xN[v] = ld_coherent(DST + v * nf + p);
-      in.xc = dnb_at(S, nf, p, 0);
+      in.xc = dnb_at(S, nf, p, 0, nx);
     }
     if (y_remote) {
       const long p = c + sy;
 #pragma unroll
       for (int v = 0; v < 5; v++) in.yN[v] = ld_coherent(DST + v * nf + p);
-      in.yc = dnb_at(S, nf, p, 1);
+      in.yc = dnb_at(S, nf, p, 1, ny_);
     }
   };
   // re-reads remote values that were not ready when first loaded
@@ -1349,18 +1348,22 @@ __global__ void __launch_bounds__(64) k_sweep_tiles(const double* __restrict__ S
       lds[cb][5][lane] = cur.own[G_P];
       lds[cb][6][lane] = cur.own[G_T];
       lds[cb][7][lane] = cur.own[G_BETA];
+      const double dn[3] = {nx, nyo, nz_};
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        lds[cb][8 + r][lane] = cur.own[G_UX + r];
-        lds[cb][12 + r][lane] = cur.own[G_UY + r];
+      for (int d = 0; d < 2; d++) {
+        double pd, p51;
+        split_pd_p51(dn[d], cur.own[G_UX + 2 * d], cur.own[G_BETA], cur.own[G_RHO], &pd, &p51);
+        lds[cb][8 + 4 * d][lane] = cur.own[G_UX + 2 * d];
+        lds[cb][9 + 4 * d][lane] = cur.own[G_LX + 2 * d];
+        lds[cb][10 + 4 * d][lane] = pd;
+        lds[cb][11 + 4 * d][lane] = p51;
       }
       zc.P = cur.own[G_P];
       zc.T = cur.own[G_T];
       zc.beta = cur.own[G_BETA];
       zc.U = cur.own[G_UZ];
       zc.lam = cur.own[G_LZ];
-      zc.pd = cur.own[G_PDZ];
-      zc.p51 = cur.own[G_P51Z];
+      split_pd_p51(dn[2], zc.U, zc.beta, cur.own[G_RHO], &zc.pd, &zc.p51);
     }
     // one wave per block: LDS writes of this step are complete and ordered
     // before the next step's reads
@@ -1616,7 +1619,7 @@ struct Slots {
 template <bool FWD, bool XY, int QS, class T>
 __device__ void pipe_own_loader(PipeLds<T>& L, const T* __restrict__ S, const T* __restrict__ SRC,
                                 const double* __restrict__ prof, const Geo& g, const Sk& q, int tile, int TY) {
-  constexpr int NF = XY ? 11 : 15;
+  constexpr int NF = XY ? 8 : 14;
   const int lane = threadIdx.x & 63;
   const int nsteps = q.nsteps;
   const long nf = q.nsk;
@@ -1625,13 +1628,13 @@ __device__ void pipe_own_loader(PipeLds<T>& L, const T* __restrict__ S, const T*
   const int j = min(2 + TY * TJ + (lane >> 3), g.ny);
   const T nx = T(pf(prof, g.Ym, SET_C, 1, j)), ny_ = T(pf(prof, g.Ym, SET_C, 5, j)),
           nz_ = T(pf(prof, g.Ym, SET_C, 9, j));
-  // fields: XY: P T beta UX LX PDX P51X UY LY PDY P51Y;
-  //         z:  D11 D22 D51 SRC0..4 P T beta UZ LZ PDZ P51Z
+  // fields: XY: P T beta rho UX LX UY LY;
+  //         z:  D11 D22 D51 SRC0..4 P T beta rho UZ LZ
   auto field = [&](int f) -> const T* {
-    if (XY) return S + (long)(f < 3 ? G_P + f : G_UX + (f - 3)) * nf;
+    if (XY) return S + (long)(f < 4 ? G_P + f : G_UX + (f - 4)) * nf;
     if (f < 3) return S + (long)(G_D11 + f) * nf;
     if (f < 8) return SRC + (long)(f - 3) * nf;
-    return S + (long)(f < 11 ? G_P + (f - 8) : G_UZ + (f - 11)) * nf;
+    return S + (long)(f < 12 ? G_P + (f - 8) : G_UZ + (f - 12)) * nf;
   };
   Slots<QS, NF, T> buf;
   auto issue = [&](int sig, int sl) {
@@ -1648,12 +1651,17 @@ __device__ void pipe_own_loader(PipeLds<T>& L, const T* __restrict__ S, const T*
     buf.get(sl, d);
     const int s3 = sig % 3;
     if constexpr (XY) {
-      coef_store(L.xc[s3], lane, diag_coef<0, FWD>(d[0], d[1], d[2], d[3], d[4], d[5], d[6], nx));
-      coef_store(L.yc[s3], lane, diag_coef<1, FWD>(d[0], d[1], d[2], d[7], d[8], d[9], d[10], ny_));
+      T pdx, p51x, pdy, p51y;
+      split_pd_p51(nx, d[4], d[2], d[3], &pdx, &p51x);
+      split_pd_p51(ny_, d[6], d[2], d[3], &pdy, &p51y);
+      coef_store(L.xc[s3], lane, diag_coef<0, FWD>(d[0], d[1], d[2], d[4], d[5], pdx, p51x, nx));
+      coef_store(L.yc[s3], lane, diag_coef<1, FWD>(d[0], d[1], d[2], d[6], d[7], pdy, p51y, ny_));
     } else {
 #pragma unroll
       for (int f = 0; f < 8; f++) L.own[s3][f][lane] = d[f];
-      const SplitCoefT<T> z = diag_coef<2, FWD>(d[8], d[9], d[10], d[11], d[12], d[13], d[14], nz_);
+      T pdz, p51z;
+      split_pd_p51(nz_, d[12], d[10], d[11], &pdz, &p51z);
+      const SplitCoefT<T> z = diag_coef<2, FWD>(d[8], d[9], d[10], d[12], d[13], pdz, p51z, nz_);
       L.own[s3][PO_Z + 0][lane] = z.c11;
       L.own[s3][PO_Z + 1][lane] = z.c1q;
       L.own[s3][PO_Z + 2][lane] = z.c22;
@@ -1673,10 +1681,10 @@ __device__ void pipe_own_loader(PipeLds<T>& L, const T* __restrict__ S, const T*
 // wave 3: the halo of the next step QD steps ahead.  There are 16 consumer
 // lanes of the compute wave: 0..7 on the x edge (b = c) and 8..15 on the y
 // edge (a = c - 8).  Each needs its remote cell's summary along the
-// direction (P T beta U lam pd p51) and NL/NR (5 values).  Lane l loads
-// values 3g..3g+2 (g = l >> 4) for consumer l & 15, so all 64 lanes carry
-// three values per step; after the sentinel poll, lanes 0..15 gather their
-// consumer's twelve values with cross-lane shuffles.
+// direction (P T beta rho U lam) and NL/NR (5 values).  Lane l loads
+// values 3g..3g+2 (g = l >> 4) for consumer l & 15 (slot 11 is spare), so
+// all 64 lanes carry three values per step; after the sentinel poll, lanes
+// 0..15 gather their consumer's eleven values with cross-lane shuffles.
 template <bool FWD, int QD, class T>
 __device__ void pipe_halo_loader(PipeLds<T>& L, const T* __restrict__ S, const T* __restrict__ DST,
                                  const double* __restrict__ prof, const Geo& g, const Sk& q, int tile, int TX,
@@ -1700,7 +1708,7 @@ __device__ void pipe_halo_loader(PipeLds<T>& L, const T* __restrict__ S, const T
   // metric at the remote cell's j
   const int jr = dir == 0 ? j : (FWD ? j - 1 : j + 1);
   const T n = T(pf(prof, g.Ym, SET_C, dir == 0 ? 1 : 5, min(max(jr, 0), g.Ym - 1)));
-  const int dU = G_UX + 4 * dir;
+  const int dU = G_UX + 2 * dir;
   auto live = [=](int sig) -> bool {
     const int kk = sig - off;
     return ok && kk >= 0 && kk < Nz && sig < nsteps;
@@ -1710,14 +1718,17 @@ __device__ void pipe_halo_loader(PipeLds<T>& L, const T* __restrict__ S, const T
     const int sc = min(sig, nsteps - 1);
     return own0 + (live(sig) ? rdiff : 0L) + (long)(FWD ? sc : nsteps - 1 - sc) * 64;
   };
-  // this lane's three values f = 3 grp + r: P T beta U lam pd p51 N0..N4
+  // this lane's three values f = 3 grp + r: P T beta rho U lam N0..N4 (f = 11
+  // spare: it reads P again)
   const T* fld[3];
+  bool dyn[3];
 #pragma unroll
   for (int r = 0; r < 3; r++) {
     const int f = 3 * grp + r;
-    fld[r] = f < 3 ? S + (long)(G_P + f) * nf : (f < 7 ? S + (long)(dU + f - 3) * nf : DST + (long)(f - 7) * nf);
+    fld[r] = f < 4 ? S + (long)(G_P + f) * nf
+                   : (f < 6 ? S + (long)(dU + f - 4) * nf : (f < 11 ? DST + (long)(f - 6) * nf : S + (long)G_P * nf));
+    dyn[r] = f >= 6 && f < 11;
   }
-  const bool dyn[3] = {3 * grp >= 7, 3 * grp + 1 >= 7, 3 * grp + 2 >= 7};
   Slots<QD, 3, T> buf;
   auto issue = [&](int sig, int sl) {
     const long c = cell(sig);
@@ -1760,14 +1771,16 @@ __device__ void pipe_halo_loader(PipeLds<T>& L, const T* __restrict__ S, const T
       for (int r = 0; r < 3; r++) v[3 * gq + r] = __shfl(d[r], cid + 16 * gq, 64);
     if (lane < 16 && lv) {
       const int sp = (sig + 2) % 3, np = (sig & 1) ^ 1;
+      T pd, p51;
+      split_pd_p51(n, v[4], v[2], v[3], &pd, &p51);
       if (dir == 0) {
-        coef_store(L.xc[sp], 64 + idx, diag_coef<0, FWD>(v[0], v[1], v[2], v[3], v[4], v[5], v[6], n));
+        coef_store(L.xc[sp], 64 + idx, diag_coef<0, FWD>(v[0], v[1], v[2], v[4], v[5], pd, p51, n));
 #pragma unroll
-        for (int w = 0; w < 5; w++) L.N[np][w][64 + idx] = v[7 + w];
+        for (int w = 0; w < 5; w++) L.N[np][w][64 + idx] = v[6 + w];
       } else {
-        coef_store(L.yc[sp], 64 + idx, diag_coef<1, FWD>(v[0], v[1], v[2], v[3], v[4], v[5], v[6], n));
+        coef_store(L.yc[sp], 64 + idx, diag_coef<1, FWD>(v[0], v[1], v[2], v[4], v[5], pd, p51, n));
 #pragma unroll
-        for (int w = 0; w < 5; w++) L.N[np][w][72 + idx] = v[7 + w];
+        for (int w = 0; w < 5; w++) L.N[np][w][72 + idx] = v[6 + w];
       }
     }
   };
