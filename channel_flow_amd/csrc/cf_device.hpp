@@ -422,11 +422,12 @@ __device__ __forceinline__ Split split_nb(const Nb& c, double n0, double n1, dou
 // Sweep summary for metrics whose rows are diagonal (xi = (xix,0,0),
 // eta = (0,ety,0), zeta = (0,0,ztz): every grid Jacobian*.cpp produces).
 // The off-diagonal split-Jacobian entries are then exact zeros.  Per cell:
-// P, T, beta, rho, D^-1 and, per direction d, the contravariant velocity
-// (G_UX + 2d) and the spectral radius (G_LX + 2d); the two entries that
-// divide by rho are formed from them where they are used (split_pd_p51).
+// P, beta, rho, D^-1 and, per direction d, the contravariant velocity
+// (G_UX + 2d) and the spectral radius (G_LX + 2d).  T = P/rho (prim_dt's
+// own expression) and the two entries that divide by rho are formed from
+// them where they are used (split_pd_p51).
 enum DiagField : int {
-  G_P = 0, G_T, G_BETA, G_RHO, G_D11, G_D22, G_D51,
+  G_P = 0, G_BETA, G_RHO, G_D11, G_D22, G_D51,
   G_UX, G_LX, G_UY, G_LY, G_UZ, G_LZ,
   G_N
 };

@@ -963,7 +963,6 @@ __device__ __forceinline__ void prep_summary(const double u5[5], const double* _
   double d11, d22, d51;
   dinv_lam(st, L3[0], L3[1], L3[2], g.dXI, g.dET, g.dZT, deltaT, &d11, &d22, &d51);
   v[G_P] = st.P;
-  v[G_T] = st.T;
   v[G_BETA] = st.beta;
   v[G_RHO] = st.rho;
   v[G_D11] = d11;
@@ -1143,12 +1142,13 @@ constexpr int LDS_F = 16;
 // neighbour p along direction d; n: the diagonal metric entry at its j
 __device__ __forceinline__ DNb dnb_at(const double* __restrict__ S, long nsk, long p, int d, double n) {
   DNb c;
+  const double rho = S[G_RHO * nsk + p];
   c.P = S[G_P * nsk + p];
-  c.T = S[G_T * nsk + p];
+  c.T = c.P / rho;
   c.beta = S[G_BETA * nsk + p];
   c.U = S[(G_UX + 2 * d) * nsk + p];
   c.lam = S[(G_LX + 2 * d) * nsk + p];
-  split_pd_p51(n, c.U, c.beta, S[G_RHO * nsk + p], &c.pd, &c.p51);
+  split_pd_p51(n, c.U, c.beta, rho, &c.pd, &c.p51);
   return c;
 }
 
@@ -1345,8 +1345,9 @@ __global__ void __launch_bounds__(64) k_sweep_tiles(const double* __restrict__ S
         lds[cb][v][lane] = out[v];
         zN[v] = out[v];
       }
+      const double Tc = cur.own[G_P] / cur.own[G_RHO];
       lds[cb][5][lane] = cur.own[G_P];
-      lds[cb][6][lane] = cur.own[G_T];
+      lds[cb][6][lane] = Tc;
       lds[cb][7][lane] = cur.own[G_BETA];
       const double dn[3] = {nx, nyo, nz_};
 #pragma unroll
@@ -1359,7 +1360,7 @@ __global__ void __launch_bounds__(64) k_sweep_tiles(const double* __restrict__ S
         lds[cb][11 + 4 * d][lane] = p51;
       }
       zc.P = cur.own[G_P];
-      zc.T = cur.own[G_T];
+      zc.T = Tc;
       zc.beta = cur.own[G_BETA];
       zc.U = cur.own[G_UZ];
       zc.lam = cur.own[G_LZ];
@@ -1619,7 +1620,7 @@ struct Slots {
 template <bool FWD, bool XY, int QS, class T>
 __device__ void pipe_own_loader(PipeLds<T>& L, const T* __restrict__ S, const T* __restrict__ SRC,
                                 const double* __restrict__ prof, const Geo& g, const Sk& q, int tile, int TY) {
-  constexpr int NF = XY ? 8 : 14;
+  constexpr int NF = XY ? 7 : 13;
   const int lane = threadIdx.x & 63;
   const int nsteps = q.nsteps;
   const long nf = q.nsk;
@@ -1628,13 +1629,13 @@ __device__ void pipe_own_loader(PipeLds<T>& L, const T* __restrict__ S, const T*
   const int j = min(2 + TY * TJ + (lane >> 3), g.ny);
   const T nx = T(pf(prof, g.Ym, SET_C, 1, j)), ny_ = T(pf(prof, g.Ym, SET_C, 5, j)),
           nz_ = T(pf(prof, g.Ym, SET_C, 9, j));
-  // fields: XY: P T beta rho UX LX UY LY;
-  //         z:  D11 D22 D51 SRC0..4 P T beta rho UZ LZ
+  // fields: XY: P beta rho UX LX UY LY;
+  //         z:  D11 D22 D51 SRC0..4 P beta rho UZ LZ
   auto field = [&](int f) -> const T* {
-    if (XY) return S + (long)(f < 4 ? G_P + f : G_UX + (f - 4)) * nf;
+    if (XY) return S + (long)(f < 3 ? G_P + f : G_UX + (f - 3)) * nf;
     if (f < 3) return S + (long)(G_D11 + f) * nf;
     if (f < 8) return SRC + (long)(f - 3) * nf;
-    return S + (long)(f < 12 ? G_P + (f - 8) : G_UZ + (f - 12)) * nf;
+    return S + (long)(f < 11 ? G_P + (f - 8) : G_UZ + (f - 11)) * nf;
   };
   Slots<QS, NF, T> buf;
   auto issue = [&](int sig, int sl) {
@@ -1651,17 +1652,19 @@ __device__ void pipe_own_loader(PipeLds<T>& L, const T* __restrict__ S, const T*
     buf.get(sl, d);
     const int s3 = sig % 3;
     if constexpr (XY) {
+      const T Tc = d[0] / d[2];
       T pdx, p51x, pdy, p51y;
-      split_pd_p51(nx, d[4], d[2], d[3], &pdx, &p51x);
-      split_pd_p51(ny_, d[6], d[2], d[3], &pdy, &p51y);
-      coef_store(L.xc[s3], lane, diag_coef<0, FWD>(d[0], d[1], d[2], d[4], d[5], pdx, p51x, nx));
-      coef_store(L.yc[s3], lane, diag_coef<1, FWD>(d[0], d[1], d[2], d[6], d[7], pdy, p51y, ny_));
+      split_pd_p51(nx, d[3], d[1], d[2], &pdx, &p51x);
+      split_pd_p51(ny_, d[5], d[1], d[2], &pdy, &p51y);
+      coef_store(L.xc[s3], lane, diag_coef<0, FWD>(d[0], Tc, d[1], d[3], d[4], pdx, p51x, nx));
+      coef_store(L.yc[s3], lane, diag_coef<1, FWD>(d[0], Tc, d[1], d[5], d[6], pdy, p51y, ny_));
     } else {
 #pragma unroll
       for (int f = 0; f < 8; f++) L.own[s3][f][lane] = d[f];
+      const T Tc = d[8] / d[10];
       T pdz, p51z;
-      split_pd_p51(nz_, d[12], d[10], d[11], &pdz, &p51z);
-      const SplitCoefT<T> z = diag_coef<2, FWD>(d[8], d[9], d[10], d[12], d[13], pdz, p51z, nz_);
+      split_pd_p51(nz_, d[11], d[9], d[10], &pdz, &p51z);
+      const SplitCoefT<T> z = diag_coef<2, FWD>(d[8], Tc, d[9], d[11], d[12], pdz, p51z, nz_);
       L.own[s3][PO_Z + 0][lane] = z.c11;
       L.own[s3][PO_Z + 1][lane] = z.c1q;
       L.own[s3][PO_Z + 2][lane] = z.c22;
@@ -1681,10 +1684,10 @@ __device__ void pipe_own_loader(PipeLds<T>& L, const T* __restrict__ S, const T*
 // wave 3: the halo of the next step QD steps ahead.  There are 16 consumer
 // lanes of the compute wave: 0..7 on the x edge (b = c) and 8..15 on the y
 // edge (a = c - 8).  Each needs its remote cell's summary along the
-// direction (P T beta rho U lam) and NL/NR (5 values).  Lane l loads
-// values 3g..3g+2 (g = l >> 4) for consumer l & 15 (slot 11 is spare), so
+// direction (P beta rho U lam) and NL/NR (5 values).  Lane l loads values
+// 3g..3g+2 (g = l >> 4) for consumer l & 15 (slots 10 and 11 are spare), so
 // all 64 lanes carry three values per step; after the sentinel poll, lanes
-// 0..15 gather their consumer's eleven values with cross-lane shuffles.
+// 0..15 gather their consumer's ten values with cross-lane shuffles.
 template <bool FWD, int QD, class T>
 __device__ void pipe_halo_loader(PipeLds<T>& L, const T* __restrict__ S, const T* __restrict__ DST,
                                  const double* __restrict__ prof, const Geo& g, const Sk& q, int tile, int TX,
@@ -1718,16 +1721,16 @@ __device__ void pipe_halo_loader(PipeLds<T>& L, const T* __restrict__ S, const T
     const int sc = min(sig, nsteps - 1);
     return own0 + (live(sig) ? rdiff : 0L) + (long)(FWD ? sc : nsteps - 1 - sc) * 64;
   };
-  // this lane's three values f = 3 grp + r: P T beta rho U lam N0..N4 (f = 11
-  // spare: it reads P again)
+  // this lane's three values f = 3 grp + r: P beta rho U lam N0..N4 (f = 10,
+  // 11 spare: they read P again)
   const T* fld[3];
   bool dyn[3];
 #pragma unroll
   for (int r = 0; r < 3; r++) {
     const int f = 3 * grp + r;
-    fld[r] = f < 4 ? S + (long)(G_P + f) * nf
-                   : (f < 6 ? S + (long)(dU + f - 4) * nf : (f < 11 ? DST + (long)(f - 6) * nf : S + (long)G_P * nf));
-    dyn[r] = f >= 6 && f < 11;
+    fld[r] = f < 3 ? S + (long)(G_P + f) * nf
+                   : (f < 5 ? S + (long)(dU + f - 3) * nf : (f < 10 ? DST + (long)(f - 5) * nf : S + (long)G_P * nf));
+    dyn[r] = f >= 5 && f < 10;
   }
   Slots<QD, 3, T> buf;
   auto issue = [&](int sig, int sl) {
@@ -1771,16 +1774,17 @@ __device__ void pipe_halo_loader(PipeLds<T>& L, const T* __restrict__ S, const T
       for (int r = 0; r < 3; r++) v[3 * gq + r] = __shfl(d[r], cid + 16 * gq, 64);
     if (lane < 16 && lv) {
       const int sp = (sig + 2) % 3, np = (sig & 1) ^ 1;
+      const T Tc = v[0] / v[2];
       T pd, p51;
-      split_pd_p51(n, v[4], v[2], v[3], &pd, &p51);
+      split_pd_p51(n, v[3], v[1], v[2], &pd, &p51);
       if (dir == 0) {
-        coef_store(L.xc[sp], 64 + idx, diag_coef<0, FWD>(v[0], v[1], v[2], v[4], v[5], pd, p51, n));
+        coef_store(L.xc[sp], 64 + idx, diag_coef<0, FWD>(v[0], Tc, v[1], v[3], v[4], pd, p51, n));
 #pragma unroll
-        for (int w = 0; w < 5; w++) L.N[np][w][64 + idx] = v[6 + w];
+        for (int w = 0; w < 5; w++) L.N[np][w][64 + idx] = v[5 + w];
       } else {
-        coef_store(L.yc[sp], 64 + idx, diag_coef<1, FWD>(v[0], v[1], v[2], v[4], v[5], pd, p51, n));
+        coef_store(L.yc[sp], 64 + idx, diag_coef<1, FWD>(v[0], Tc, v[1], v[3], v[4], pd, p51, n));
 #pragma unroll
-        for (int w = 0; w < 5; w++) L.N[np][w][72 + idx] = v[6 + w];
+        for (int w = 0; w < 5; w++) L.N[np][w][72 + idx] = v[5 + w];
       }
     }
   };
