@@ -432,12 +432,20 @@ enum DiagField : int {
   G_N
 };
 
+// x / rho and x / K: correctly rounded in FP64; in the FP32 sweeps, products
+// with the hardware reciprocal (1 ulp) and with 1/K, which the inexact inner
+// solve tolerates
+__device__ __forceinline__ double over_rho(double x, double rho) { return x / rho; }
+__device__ __forceinline__ float over_rho(float x, float rho) { return x * __builtin_amdgcn_rcpf(rho); }
+__device__ __forceinline__ double over_k(double x) { return x / KG; }
+__device__ __forceinline__ float over_k(float x) { return x * float(1.0 / KG); }
+
 // PD = 0.5*n_dd/rho (A+21, B+31, C+41) and P51 = 0.5*U*(K-1)*(beta-1)/rho/K
 // of one direction (Dual_time_stepping.cpp:751-811), in the reference's order
 template <class R>
 __device__ __forceinline__ void split_pd_p51(R n, R U, R beta, R rho, R* pd, R* p51) {
-  *pd = R(0.5) * n / rho;
-  *p51 = R(0.5) * U * R(KG - 1) * (beta - R(1)) / rho / R(KG);
+  *pd = over_rho(R(0.5) * n, rho);
+  *p51 = over_k(over_rho(R(0.5) * U * R(KG - 1) * (beta - R(1)), rho));
 }
 
 struct DNb {  // one neighbour as seen along one direction

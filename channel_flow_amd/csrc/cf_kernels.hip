@@ -1652,7 +1652,7 @@ __device__ void pipe_own_loader(PipeLds<T>& L, const T* __restrict__ S, const T*
     buf.get(sl, d);
     const int s3 = sig % 3;
     if constexpr (XY) {
-      const T Tc = d[0] / d[2];
+      const T Tc = over_rho(d[0], d[2]);
       T pdx, p51x, pdy, p51y;
       split_pd_p51(nx, d[3], d[1], d[2], &pdx, &p51x);
       split_pd_p51(ny_, d[5], d[1], d[2], &pdy, &p51y);
@@ -1661,7 +1661,7 @@ __device__ void pipe_own_loader(PipeLds<T>& L, const T* __restrict__ S, const T*
     } else {
 #pragma unroll
       for (int f = 0; f < 8; f++) L.own[s3][f][lane] = d[f];
-      const T Tc = d[8] / d[10];
+      const T Tc = over_rho(d[8], d[10]);
       T pdz, p51z;
       split_pd_p51(nz_, d[11], d[9], d[10], &pdz, &p51z);
       const SplitCoefT<T> z = diag_coef<2, FWD>(d[8], Tc, d[9], d[11], d[12], pdz, p51z, nz_);
@@ -1774,7 +1774,7 @@ __device__ void pipe_halo_loader(PipeLds<T>& L, const T* __restrict__ S, const T
       for (int r = 0; r < 3; r++) v[3 * gq + r] = __shfl(d[r], cid + 16 * gq, 64);
     if (lane < 16 && lv) {
       const int sp = (sig + 2) % 3, np = (sig & 1) ^ 1;
-      const T Tc = v[0] / v[2];
+      const T Tc = over_rho(v[0], v[2]);
       T pd, p51;
       split_pd_p51(n, v[3], v[1], v[2], &pd, &p51);
       if (dir == 0) {
