@@ -422,12 +422,12 @@ __device__ __forceinline__ Split split_nb(const Nb& c, double n0, double n1, dou
 // Sweep summary for metrics whose rows are diagonal (xi = (xix,0,0),
 // eta = (0,ety,0), zeta = (0,0,ztz): every grid Jacobian*.cpp produces).
 // The off-diagonal split-Jacobian entries are then exact zeros.  Per cell:
-// P, beta, rho, D^-1 and, per direction d, the contravariant velocity
-// (G_UX + 2d) and the spectral radius (G_LX + 2d).  T = P/rho (prim_dt's
-// own expression) and the two entries that divide by rho are formed from
-// them where they are used (split_pd_p51).
+// P, beta, rho and, per direction d, the contravariant velocity (G_UX + 2d)
+// and the spectral radius (G_LX + 2d).  T = P/rho (prim_dt's own
+// expression), D^-1 (diag_dinv) and the two split-Jacobian entries that
+// divide by rho (split_pd_p51) are formed from them where they are used.
 enum DiagField : int {
-  G_P = 0, G_BETA, G_RHO, G_D11, G_D22, G_D51,
+  G_P = 0, G_BETA, G_RHO,
   G_UX, G_LX, G_UY, G_LY, G_UZ, G_LZ,
   G_N
 };
@@ -439,6 +439,30 @@ __device__ __forceinline__ double over_rho(double x, double rho) { return x / rh
 __device__ __forceinline__ float over_rho(float x, float rho) { return x * __builtin_amdgcn_rcpf(rho); }
 __device__ __forceinline__ double over_k(double x) { return x / KG; }
 __device__ __forceinline__ float over_k(float x) { return x * float(1.0 / KG); }
+
+// x / y in the sweep's type: correctly rounded in FP64, reciprocal product in
+// the FP32 sweeps
+__device__ __forceinline__ double sweep_div(double x, double y) { return x / y; }
+__device__ __forceinline__ float sweep_div(float x, float y) { return x * __builtin_amdgcn_rcpf(y); }
+
+// D^-1 = {d11, d22 (= d33 = d44), d51} of one cell from its spectral radii
+// (Dual_time_stepping.cpp:464-489), dinv_lam's expressions in its order;
+// rdXI.. are 1/dXI.. for the FP32 sweeps (unused in FP64)
+template <class R>
+__device__ __forceinline__ void diag_dinv(R beta, R rho, R lx, R ly, R lz, double dXI, double dET, double dZT,
+                                          double deltaT, R* d11, R* d22, R* d51) {
+  R lambda;
+  if constexpr (sizeof(R) == sizeof(double))
+    lambda = lx / dXI + ly / dET + lz / dZT;
+  else
+    lambda = lx * R(1.0 / dXI) + ly * R(1.0 / dET) + lz * R(1.0 / dZT);
+  const R dt = R(deltaT);
+  *d11 = sweep_div(R(2) * dt, R(3) * beta + R(2) * lambda * dt);
+  *d22 = sweep_div(R(2) * dt, R(3) + R(2) * lambda * dt);
+  const R temp1 = R(6 * (KG - 1)) * (beta - R(1)) * dt;
+  const R temp2 = R(KG) * (R(3) + R(2) * lambda * dt) * (R(3) * beta + R(2) * lambda * dt) * rho;
+  *d51 = -sweep_div(temp1, temp2);
+}
 
 // PD = 0.5*n_dd/rho (A+21, B+31, C+41) and P51 = 0.5*U*(K-1)*(beta-1)/rho/K
 // of one direction (Dual_time_stepping.cpp:751-811), in the reference's order

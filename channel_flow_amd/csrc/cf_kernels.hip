@@ -949,9 +949,9 @@ __device__ __forceinline__ bool sk_cell(const Sk& q, const Geo& g, int tile, int
 }
 
 // diagonal-metric summary (DiagField) of one cell (Dual_time_stepping.cpp:
-// 343-372 primitives, 464-489 D^-1, 751-811 split-Jacobian ingredients)
+// 343-372 primitives, 751-811 split-Jacobian ingredients)
 __device__ __forceinline__ void prep_summary(const double u5[5], const double* __restrict__ prof, const Geo& g, int j,
-                                             double deltaT, double v[G_N]) {
+                                             double v[G_N]) {
   PState st = prim_dt(u5[0], u5[1], u5[2], u5[3], u5[4], pf(prof, g.Ym, SET_C, 0, j));
   double m9[9];
 #pragma unroll
@@ -960,14 +960,9 @@ __device__ __forceinline__ void prep_summary(const double u5[5], const double* _
   L3[0] = lambda_dir(st, m9[0], m9[1], m9[2], &U3[0]);
   L3[1] = lambda_dir(st, m9[3], m9[4], m9[5], &U3[1]);
   L3[2] = lambda_dir(st, m9[6], m9[7], m9[8], &U3[2]);
-  double d11, d22, d51;
-  dinv_lam(st, L3[0], L3[1], L3[2], g.dXI, g.dET, g.dZT, deltaT, &d11, &d22, &d51);
   v[G_P] = st.P;
   v[G_BETA] = st.beta;
   v[G_RHO] = st.rho;
-  v[G_D11] = d11;
-  v[G_D22] = d22;
-  v[G_D51] = d51;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     v[G_UX + 2 * d] = U3[d];
@@ -1000,8 +995,7 @@ __device__ __forceinline__ void prep_stage(double (*L)[64][PCP], const double* _
 template <class T>
 __global__ void __launch_bounds__(256) k_sweep_prep(const double* __restrict__ Ui, const double* __restrict__ MR,
                                                     const double* __restrict__ prof, T* __restrict__ S,
-                                                    T* __restrict__ MRs, T* __restrict__ NLs, Geo g, Sk q,
-                                                    double deltaT) {
+                                                    T* __restrict__ MRs, T* __restrict__ NLs, Geo g, Sk q) {
   __shared__ double L[5][64][PCP];
   int bx, by, bz;
   xcd_block3(&bx, &by, &bz);
@@ -1023,7 +1017,7 @@ __global__ void __launch_bounds__(256) k_sweep_prep(const double* __restrict__ U
     double u5[5], v[G_N];
 #pragma unroll
     for (int f = 0; f < 5; f++) u5[f] = L[f][lane][t];
-    prep_summary(u5, prof, g, j, deltaT, v);
+    prep_summary(u5, prof, g, j, v);
 #pragma unroll
     for (int r = 0; r < G_N; r++) S[r * q.nsk + o] = T(v[r]);
   }
@@ -1086,7 +1080,7 @@ __global__ void __launch_bounds__(256) k_sweep_prep_rhs(const double* __restrict
     double u5[5], v[G_N];
 #pragma unroll
     for (int f = 0; f < 5; f++) u5[f] = L[f][lane][t];
-    prep_summary(u5, prof, g, j, deltaT, v);
+    prep_summary(u5, prof, g, j, v);
 #pragma unroll
     for (int r = 0; r < G_N; r++) S[r * q.nsk + o] = T(v[r]);
 #pragma unroll
@@ -1157,7 +1151,7 @@ __global__ void __launch_bounds__(64) k_sweep_tiles(const double* __restrict__ S
                                                     double* __restrict__ DST, double* __restrict__ AUX,
                                                     const double* __restrict__ prof, Geo g, Sk q,
                                                     const int* __restrict__ order, int* __restrict__ sync,
-                                                    long spin_limit) {
+                                                    long spin_limit, double deltaT) {
   // all arrays in sweep order (Sk); S holds the DiagField summary
   // FWD: SRC = MR, DST = NL   (NL = D^-1 (MR - L)); seeds the NR (AUX) sentinels
   // BWD: SRC = NL, DST = NR   (NR = NL - D^-1 U); re-seeds the NL sentinels
@@ -1312,7 +1306,9 @@ __global__ void __launch_bounds__(64) k_sweep_tiles(const double* __restrict__ S
       for (int v = 0; v < 5; v++)
         L[v] = div_prof(acc[0][v], g.dXI, g.rXI) + div_prof(acc[1][v], g.dET, g.rET) +
                div_prof(acc[2][v], g.dZT, g.rZT);
-      const double d11 = cur.own[G_D11], d22 = cur.own[G_D22], d51 = cur.own[G_D51];
+      double d11, d22, d51;
+      diag_dinv(cur.own[G_BETA], cur.own[G_RHO], cur.own[G_LX], cur.own[G_LY], cur.own[G_LZ], g.dXI, g.dET, g.dZT,
+                deltaT, &d11, &d22, &d51);
       const double* M = cur.src;
       double out[5];
       if (FWD) {
@@ -1619,8 +1615,9 @@ struct Slots {
 // the z summary -> own rows.
 template <bool FWD, bool XY, int QS, class T>
 __device__ void pipe_own_loader(PipeLds<T>& L, const T* __restrict__ S, const T* __restrict__ SRC,
-                                const double* __restrict__ prof, const Geo& g, const Sk& q, int tile, int TY) {
-  constexpr int NF = XY ? 7 : 13;
+                                const double* __restrict__ prof, const Geo& g, const Sk& q, int tile, int TY,
+                                double deltaT) {
+  constexpr int NF = XY ? 7 : 12;
   const int lane = threadIdx.x & 63;
   const int nsteps = q.nsteps;
   const long nf = q.nsk;
@@ -1630,12 +1627,13 @@ __device__ void pipe_own_loader(PipeLds<T>& L, const T* __restrict__ S, const T*
   const T nx = T(pf(prof, g.Ym, SET_C, 1, j)), ny_ = T(pf(prof, g.Ym, SET_C, 5, j)),
           nz_ = T(pf(prof, g.Ym, SET_C, 9, j));
   // fields: XY: P beta rho UX LX UY LY;
-  //         z:  D11 D22 D51 SRC0..4 P beta rho UZ LZ
+  //         z:  SRC0..4 P beta rho LX LY UZ LZ
   auto field = [&](int f) -> const T* {
     if (XY) return S + (long)(f < 3 ? G_P + f : G_UX + (f - 3)) * nf;
-    if (f < 3) return S + (long)(G_D11 + f) * nf;
-    if (f < 8) return SRC + (long)(f - 3) * nf;
-    return S + (long)(f < 11 ? G_P + (f - 8) : G_UZ + (f - 11)) * nf;
+    if (f < 5) return SRC + (long)f * nf;
+    if (f < 8) return S + (long)(G_P + (f - 5)) * nf;
+    constexpr int zf[4] = {G_LX, G_LY, G_UZ, G_LZ};
+    return S + (long)zf[f - 8] * nf;
   };
   Slots<QS, NF, T> buf;
   auto issue = [&](int sig, int sl) {
@@ -1659,12 +1657,18 @@ __device__ void pipe_own_loader(PipeLds<T>& L, const T* __restrict__ S, const T*
       coef_store(L.xc[s3], lane, diag_coef<0, FWD>(d[0], Tc, d[1], d[3], d[4], pdx, p51x, nx));
       coef_store(L.yc[s3], lane, diag_coef<1, FWD>(d[0], Tc, d[1], d[5], d[6], pdy, p51y, ny_));
     } else {
+      // d: SRC0..4, P, beta, rho, LX, LY, UZ, LZ
+      T d11, d22, d51;
+      diag_dinv(d[6], d[7], d[8], d[9], d[11], g.dXI, g.dET, g.dZT, deltaT, &d11, &d22, &d51);
+      L.own[s3][PO_D11][lane] = d11;
+      L.own[s3][PO_D22][lane] = d22;
+      L.own[s3][PO_D51][lane] = d51;
 #pragma unroll
-      for (int f = 0; f < 8; f++) L.own[s3][f][lane] = d[f];
-      const T Tc = over_rho(d[8], d[10]);
+      for (int f = 0; f < 5; f++) L.own[s3][PO_M + f][lane] = d[f];
+      const T Tc = over_rho(d[5], d[7]);
       T pdz, p51z;
-      split_pd_p51(nz_, d[11], d[9], d[10], &pdz, &p51z);
-      const SplitCoefT<T> z = diag_coef<2, FWD>(d[8], Tc, d[9], d[11], d[12], pdz, p51z, nz_);
+      split_pd_p51(nz_, d[10], d[6], d[7], &pdz, &p51z);
+      const SplitCoefT<T> z = diag_coef<2, FWD>(d[5], Tc, d[6], d[10], d[11], pdz, p51z, nz_);
       L.own[s3][PO_Z + 0][lane] = z.c11;
       L.own[s3][PO_Z + 1][lane] = z.c1q;
       L.own[s3][PO_Z + 2][lane] = z.c22;
@@ -1801,7 +1805,7 @@ __global__ void __launch_bounds__(256) k_sweep_pipe(const T* __restrict__ S, T* 
                                                     T* __restrict__ DST, T* __restrict__ AUX,
                                                     const double* __restrict__ prof, Geo g, Sk q,
                                                     const int* __restrict__ order, int* __restrict__ sync,
-                                                    long spin_limit) {
+                                                    long spin_limit, double deltaT) {
   // arrays in sweep order (Sk); S holds the DiagField summary
   // FWD: SRC = MR, DST = NL   (NL = D^-1 (MR - L)); seeds the NR (AUX) sentinels
   // BWD: SRC = NL, DST = NR   (NR = NL - D^-1 U); re-seeds the NL sentinels
@@ -1824,9 +1828,9 @@ __global__ void __launch_bounds__(256) k_sweep_pipe(const T* __restrict__ S, T* 
       pipe_compute<FWD, false>(L, SRC, DST, AUX, g, q, tile, TX, TY);
     }
   } else if (wave == 1) {
-    pipe_own_loader<FWD, true, QS>(L, S, SRC, prof, g, q, tile, TY);
+    pipe_own_loader<FWD, true, QS>(L, S, SRC, prof, g, q, tile, TY, deltaT);
   } else if (wave == 2) {
-    pipe_own_loader<FWD, false, QS>(L, S, SRC, prof, g, q, tile, TY);
+    pipe_own_loader<FWD, false, QS>(L, S, SRC, prof, g, q, tile, TY, deltaT);
   } else {
     pipe_halo_loader<FWD, QD>(L, S, DST, prof, g, q, tile, TX, TY, sync + 1, spin_limit);
   }
@@ -2334,7 +2338,7 @@ static void sweeps_t(hipStream_t st, double* Ui, const double* MR, double* Sbuf,
                        prof, S, MRs, NLs, g, q, deltaT);
   else
     hipLaunchKernelGGL(k_sweep_prep<T>, dim3((q.nsteps + PC - 1) / PC, q.ntiles), dim3(256), 0, st, Ui, MR, prof, S,
-                       MRs, NLs, g, q, deltaT);
+                       MRs, NLs, g, q);
   if (marks) (void)hipEventRecord(marks[0], st);
   const int nt = q.ntiles;
   for (int sweep = 0; sweep < n_sweeps; sweep++) {
@@ -2343,20 +2347,20 @@ static void sweeps_t(hipStream_t st, double* Ui, const double* MR, double* Sbuf,
     if constexpr (std::is_same<T, double>::value) {
       if (!pipelined)
         hipLaunchKernelGGL(k_sweep_tiles<true>, dim3(nt), dim3(64), 0, st, S, MRs, NLs, NRs, prof, g, q, order, sync,
-                           spin_limit);
+                           spin_limit, deltaT);
     }
     if (pipelined)
       hipLaunchKernelGGL((k_sweep_pipe<true, PIPE_QS, PIPE_QD, T>), dim3(nt), dim3(256), 0, st, S, MRs, NLs, NRs,
-                         prof, g, q, order, sync, spin_limit);
+                         prof, g, q, order, sync, spin_limit, deltaT);
     if (marks) (void)hipEventRecord(marks[2 + 3 * sweep], st);
     if constexpr (std::is_same<T, double>::value) {
       if (!pipelined)
         hipLaunchKernelGGL(k_sweep_tiles<false>, dim3(nt), dim3(64), 0, st, S, NLs, NRs, NRs, prof, g, q, order,
-                           sync + nt + 2, spin_limit);
+                           sync + nt + 2, spin_limit, deltaT);
     }
     if (pipelined)
       hipLaunchKernelGGL((k_sweep_pipe<false, PIPE_QS, PIPE_QD, T>), dim3(nt), dim3(256), 0, st, S, NLs, NRs, NRs,
-                         prof, g, q, order, sync + nt + 2, spin_limit);
+                         prof, g, q, order, sync + nt + 2, spin_limit, deltaT);
     if (marks) (void)hipEventRecord(marks[3 + 3 * sweep], st);
   }
   // primitive update reading NR from the sweep order
