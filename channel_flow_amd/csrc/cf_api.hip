@@ -52,6 +52,9 @@ struct cf_ctx {
   int solver = 0;         // implicit part: 0 LU-SGS (Dual_time_stepping), 1 DPLUSGS
   int visc_schedule = 0;  // 0: fused viscous kernel, 1: three kernels through Pr/VS (allocated on first use)
   bool fp32 = false;      // CF_FP32_SWEEPS: LU-SGS sweeps and split Jacobians in float
+  int guess_order = 0;    // initial iterate of cf_physical_step: 0 U^n, 1 linear, 2 quadratic extrapolation
+  double* Uqq = nullptr;  // U^{n-2} (5 fields), kept while guess_order is 2 (allocated on first use)
+  bool uqq_valid = false; // Uqq holds the level before Uq
   long spin_limit = 1L << 22;
   double* partial = nullptr;
   double* er_dev = nullptr;
@@ -260,8 +263,8 @@ extern "C" int cf_create(const cf_params* p, cf_ctx** out) {
 extern "C" void cf_destroy(cf_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  double* ptrs[] = {c->Ui, c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->MR,  c->NL,     c->NR,
-                    c->Pr, c->VS, c->Ep, c->Fw, c->ypt, c->Sm, c->partial, c->er_dev, c->prof, c->stat};
+  double* ptrs[] = {c->Ui, c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->MR,  c->NL,     c->NR,    c->Pr,
+                    c->VS, c->Ep, c->Fw, c->ypt, c->Sm, c->partial, c->er_dev, c->prof, c->stat, c->Uqq};
   for (double* q : ptrs)
     if (q) (void)hipFree(q);
   if (c->order) (void)hipFree(c->order);
@@ -825,6 +828,7 @@ extern "C" int cf_load_state(cf_ctx* c, const double* const Qn[5], const double*
   if ((r = upload_Q(c, c->Un, Qn)) || (r = upload_Q(c, c->Uq, Qnm1)) || (r = upload_Q(c, c->Ui, Qn))) return r;
   for (int v = 0; v < 5; v++) c->E[v] = 0;
   c->have_state = true;
+  c->uqq_valid = false;
   return CF_OK;
 }
 
@@ -880,7 +884,12 @@ extern "C" int cf_inner_iteration(cf_ctx* c, double er[5]) {
 extern "C" int cf_advance_levels(cf_ctx* c) {
   if (!c) return CF_ERR_ARG;
   if (!c->have_state) return fail(c, CF_ERR_STATE, "no state loaded");
-  // U^{n-1} <- U^n, U^n <- U_ on the interior (main.cpp:444-464)
+  // U^{n-1} <- U^n, U^n <- U_ on the interior (main.cpp:444-464); U^{n-2}
+  // <- U^{n-1} first while the quadratic initial guess keeps it
+  if (c->Uqq) {
+    cf::launch_copy_interior(c->stream, c->Uq, c->Uqq, c->g);
+    c->uqq_valid = true;
+  }
   cf::launch_copy_interior(c->stream, c->Un, c->Uq, c->g);
   cf::launch_copy_interior(c->stream, c->Ui, c->Un, c->g);
   return sync(c);
@@ -1034,18 +1043,21 @@ extern "C" int cf_save_checkpoint(cf_ctx* c, const char* path) {
   if ((r = sync(c))) return r;
   FILE* f = fopen(path, "wb");
   if (!f) return fail(c, CF_ERR_IO, "cannot open %s", path);
-  const int32_t dims[4] = {c->p.X_out, c->p.Y_out, c->p.Z_out, 0};
+  // dims[3]: fields after the 15 levels: 5 when U^{n-2} is kept and valid
+  const int nextra = c->Uqq && c->uqq_valid ? 5 : 0;
+  const int32_t dims[4] = {c->p.X_out, c->p.Y_out, c->p.Z_out, nextra};
   std::vector<double> st((size_t)7 * c->g.Ym);
   HIPCHK(c, hipMemcpy(st.data(), c->stat, st.size() * 8, hipMemcpyDeviceToHost));
   bool ok = fwrite(CKPT_MAGIC, 1, 8, f) == 8 && fwrite(dims, 4, 4, f) == 4 && fwrite(c->E, 8, 5, f) == 5 &&
             fwrite(st.data(), 8, st.size(), f) == st.size();
-  double* fields[15];
+  double* fields[20];
   for (int v = 0; v < 5; v++) {
     fields[v] = c->Ui + v * c->g.nf;
     fields[5 + v] = c->Un + v * c->g.nf;
     fields[10 + v] = c->Uq + v * c->g.nf;
+    fields[15 + v] = c->Uqq ? c->Uqq + v * c->g.nf : nullptr;
   }
-  for (int q = 0; ok && q < 15; q++)
+  for (int q = 0; ok && q < 15 + nextra; q++)
     if ((r = ckpt_field(c, f, fields[q], true))) {
       fclose(f);
       return r;
@@ -1065,7 +1077,8 @@ extern "C" int cf_load_checkpoint(cf_ctx* c, const char* path) {
   double E[5];
   std::vector<double> st((size_t)7 * c->g.Ym);
   bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, CKPT_MAGIC, 8) == 0 && fread(dims, 4, 4, f) == 4;
-  if (!ok || dims[0] != c->p.X_out || dims[1] != c->p.Y_out || dims[2] != c->p.Z_out) {
+  if (!ok || dims[0] != c->p.X_out || dims[1] != c->p.Y_out || dims[2] != c->p.Z_out ||
+      (dims[3] != 0 && dims[3] != 5)) {
     fclose(f);
     return fail(c, CF_ERR_IO, "%s is not a checkpoint of this grid", path);
   }
@@ -1074,13 +1087,28 @@ extern "C" int cf_load_checkpoint(cf_ctx* c, const char* path) {
     return fail(c, CF_ERR_IO, "short checkpoint");
   }
   int r;
-  double* fields[15];
+  const int nextra = dims[3];
+  if (nextra && !c->Uqq) {
+    // the file keeps U^{n-2}: hold it even if the quadratic guess is not selected yet
+    const size_t bytes = (size_t)c->g.nf * 5 * sizeof(double);
+    if (hipMalloc(&c->Uqq, bytes) != hipSuccess) {
+      c->Uqq = nullptr;
+      fclose(f);
+      return fail(c, CF_ERR_HIP, "hipMalloc of the U^{n-2} level failed");
+    }
+    if (hipMemset(c->Uqq, 0, bytes) != hipSuccess) {
+      fclose(f);
+      return fail(c, CF_ERR_HIP, "hipMemset of the U^{n-2} level failed");
+    }
+  }
+  double* fields[20];
   for (int v = 0; v < 5; v++) {
     fields[v] = c->Ui + v * c->g.nf;
     fields[5 + v] = c->Un + v * c->g.nf;
     fields[10 + v] = c->Uq + v * c->g.nf;
+    fields[15 + v] = c->Uqq ? c->Uqq + v * c->g.nf : nullptr;
   }
-  for (int q = 0; q < 15; q++)
+  for (int q = 0; q < 15 + nextra; q++)
     if ((r = ckpt_field(c, f, fields[q], false))) {
       fclose(f);
       return r;
@@ -1089,6 +1117,7 @@ extern "C" int cf_load_checkpoint(cf_ctx* c, const char* path) {
   HIPCHK(c, hipMemcpy(c->stat, st.data(), st.size() * 8, hipMemcpyHostToDevice));
   for (int v = 0; v < 5; v++) c->E[v] = E[v];
   c->have_state = true;
+  c->uqq_valid = nextra == 5;
   return CF_OK;
 }
 
@@ -1166,9 +1195,30 @@ extern "C" int cf_set_sweep_schedule(cf_ctx* c, int schedule) {
   return CF_OK;
 }
 
+extern "C" int cf_set_initial_guess(cf_ctx* c, int order) {
+  if (!c || order < 0 || order > 2) return fail(c, CF_ERR_ARG, "initial-guess order must be 0, 1 or 2");
+  if (order == 2 && !c->Uqq) {
+    const size_t bytes = (size_t)c->g.nf * 5 * sizeof(double);
+    if (hipMalloc(&c->Uqq, bytes) != hipSuccess) {
+      c->Uqq = nullptr;
+      return fail(c, CF_ERR_HIP, "hipMalloc of the U^{n-2} level failed");
+    }
+    HIPCHK(c, hipMemsetAsync(c->Uqq, 0, bytes, c->stream));
+    c->uqq_valid = false;
+  }
+  c->guess_order = order;
+  return sync(c);
+}
+
 extern "C" int cf_physical_step(cf_ctx* c, int* iterations, double er[5]) {
   if (!c || !iterations || !er) return CF_ERR_ARG;
   if (!c->have_state) return fail(c, CF_ERR_STATE, "no state loaded");
+  if (c->guess_order > 0) {
+    // quadratic once three levels are known, linear before
+    const int order = c->guess_order == 2 && c->uqq_valid ? 2 : 1;
+    cf::launch_extrapolate(c->stream, c->Ui, c->Un, c->Uq, c->Uqq, c->g, order);
+    HIPCHK(c, hipGetLastError());
+  }
   for (int it = 1;; it++) {
     int r = cf_inner_iteration(c, er);
     if (r) return r;

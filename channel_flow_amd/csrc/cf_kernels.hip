@@ -1898,6 +1898,23 @@ __global__ void __launch_bounds__(256) k_jacobi(const double* __restrict__ Ui, c
   NRout[4 * nf + c] = d51 * (M[0] - LL[0] - UU[0]) + d22 * (M[4] - LL[4] - UU[4]);
 }
 
+// Initial iterate of a physical step from the time levels, on the interior:
+// order 1: 2 U^n - U^{n-1}; order 2: 3 U^n - 3 U^{n-1} + U^{n-2}.  The
+// reference starts every physical step at U^n (main.cpp:444-464).
+__global__ void k_extrapolate(double* __restrict__ Ui, const double* __restrict__ Un, const double* __restrict__ Uq,
+                              const double* __restrict__ Uqq, Geo g, int order) {
+  const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int i = 3 + blockIdx.z;
+  if (k > g.nz || j > g.ny) return;
+  const long c = at(g, i, j, k);
+#pragma unroll
+  for (int v = 0; v < 5; v++) {
+    const long o = v * g.nf + c;
+    Ui[o] = order == 2 ? 3 * Un[o] - 3 * Uq[o] + Uqq[o] : 2 * Un[o] - Uq[o];
+  }
+}
+
 __global__ void k_copy_interior(const double* __restrict__ src, double* __restrict__ dst, Geo g) {
   int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
   int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
@@ -2418,6 +2435,13 @@ void launch_sweeps_planes(hipStream_t st, const double* Ui, const double* MR, do
     }
     if (marks) (void)hipEventRecord(marks[3 + 3 * sweep], st);
   }
+}
+
+void launch_extrapolate(hipStream_t st, double* Ui, const double* Un, const double* Uq, const double* Uqq, const Geo& g,
+                        int order) {
+  dim3 blk(64, 4);
+  hipLaunchKernelGGL(k_extrapolate, box_grid(g.nz - 1, g.ny - 1, g.gend - 2, blk), blk, 0, st, Ui, Un, Uq, Uqq, g,
+                     order);
 }
 
 void launch_copy_interior(hipStream_t st, const double* src, double* dst, const Geo& g) {

@@ -34,7 +34,7 @@ EXPORTS = [
     "cf_profile_inner_iteration", "cf_set_sweep_schedule", "cf_flux_x_limiter", "cf_flux_y_limiter",
     "cf_flux_z_limiter", "cf_grid", "cf_filter", "cf_profile_variants", "cf_dplusgs", "cf_set_implicit_solver",
     "cf_set_viscous_schedule", "cf_rk3nodt", "cf_rk3_step",
-    "cf_statistic", "cf_driven_force", "cf_save_checkpoint", "cf_load_checkpoint",
+    "cf_statistic", "cf_driven_force", "cf_save_checkpoint", "cf_load_checkpoint", "cf_set_initial_guess",
 ]
 
 
@@ -98,6 +98,7 @@ def load_library() -> C.CDLL:
     lib.cf_profile_inner_iteration.argtypes = [vp, P, P, C.POINTER(C.c_int)]
     lib.cf_set_sweep_schedule.argtypes = [vp, C.c_int]
     lib.cf_set_viscous_schedule.argtypes = [vp, C.c_int]
+    lib.cf_set_initial_guess.argtypes = [vp, C.c_int]
     lib.cf_rk3nodt.argtypes = [vp, C.c_int, C.c_double, P, PP, PP, PP, P, PP, PP, PP, PP, PP, PP]
     lib.cf_rk3_step.argtypes = [vp, C.c_int]
     lib.cf_statistic.argtypes = [vp, C.c_int, C.c_int, PP, C.c_char_p]
@@ -380,6 +381,11 @@ class Solver:
         """0: one fused viscous kernel (default); 1: three kernels through
         staging fields.  Both give the same bits."""
         self._chk(self.lib.cf_set_viscous_schedule(self._h, int(schedule)))
+
+    def set_initial_guess(self, order: int = 0):
+        """Initial iterate of each physical_step(): 0 U^n (default, the
+        reference's); 1 2U^n - U^{n-1}; 2 3U^n - 3U^{n-1} + U^{n-2}."""
+        self._chk(self.lib.cf_set_initial_guess(self._h, int(order)))
 
     def synchronize(self):
         self._chk(self.lib.cf_synchronize(self._h))

@@ -177,16 +177,26 @@ int cf_inner_iteration(cf_ctx* ctx, double er[5]);
  * iteration count and the last normalised er[5]. */
 int cf_physical_step(cf_ctx* ctx, int* iterations, double er[5]);
 /* The time-level shift alone: U^{n-1} <- U^n, U^n <- U_ on the interior
- * (main.cpp:444-464); cf_physical_step calls it on convergence. */
+ * (main.cpp:444-464); cf_physical_step calls it on convergence.  While the
+ * quadratic initial guess is selected, U^{n-2} <- U^{n-1} first. */
 int cf_advance_levels(cf_ctx* ctx);
+/* Initial iterate of each cf_physical_step, no counterpart in the reference
+ * (which starts every step at U^n, the level shift of main.cpp:444-464):
+ * 0 (default) U^n, as the reference; 1 the linear extrapolation
+ * 2 U^n - U^{n-1}; 2 the quadratic 3 U^n - 3 U^{n-1} + U^{n-2}, linear until
+ * three levels are known (it keeps a U^{n-2} level, 5 more fields).  The
+ * converged state is the same to the convergence tolerance; the number of
+ * inner iterations under the running-max test (main.cpp:387-406) drops. */
+int cf_set_initial_guess(cf_ctx* ctx, int order);
 /* Output (main.h:872-896, Output.cpp:56-190): writes <dir>/LES%05d.bin. */
 int cf_write_output(cf_ctx* ctx, int step, const char* dir);
 /* Exact restart, no counterpart in the reference (whose restart reads
  * LES%05d.bin, Q = U J, and starts E_k and the statistics afresh): the
  * interior of the iterate and of levels n and n-1 as stored (U = Q/J), the
  * running maxima E_k of cf_physical_step and the statistic sums.  Format:
- * "CFCKPT01", int32 X_out Y_out Z_out 0, double E[5], double sums[7][Y_m],
- * then 15 fields (U_, U, Uq) of X_out*Y_out*Z_out doubles, [x][y][z]. */
+ * "CFCKPT01", int32 X_out Y_out Z_out M, double E[5], double sums[7][Y_m],
+ * then 15 fields (U_, U, Uq) of X_out*Y_out*Z_out doubles, [x][y][z], then M
+ * more: 5 (level n-2) when the quadratic initial guess holds one, else 0. */
 int cf_save_checkpoint(cf_ctx* ctx, const char* path);
 int cf_load_checkpoint(cf_ctx* ctx, const char* path);
 /* Runs one inner iteration like cf_inner_iteration and times its stages
