@@ -130,6 +130,30 @@ def physical_steps(s, Q, steps=2, cap=800):
     return conv, s.read_state()[2]
 
 
+def reference_test_steps(cfg, Q, precision, order, steps=6):
+    """Physical steps under the reference's own convergence test
+    (cf_physical_step: er1..er4 over their running maxima below 1e-3, at most
+    200 inner iterations) from the seeded state, with the given sweep
+    precision and initial-guess order; per-step iterations and wall times."""
+    import torch
+    from channel_flow_amd.solver import Solver
+    s = Solver(cfg, precision=precision)
+    s.Jacobian_all(cfg.gamma_metric, want_arrays=False)
+    s.set_initial_guess(order)
+    s.load_state(Q, Q)
+    its, wall = [], []
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        it, _ = s.physical_step()
+        torch.cuda.synchronize()
+        its.append(it)
+        wall.append(time.perf_counter() - t0)
+    qn = s.read_state()[0]
+    s.close()
+    return its, wall, qn
+
+
 def mean_profiles(q):
     """Mean velocity and Reynolds-stress profiles over x and z of a state Q."""
     rho = q[0]
@@ -279,6 +303,25 @@ def main():
     if not args.no_fp32:
         fp32 = fp32_sweeps_leg(cfg, Q, args, q64)
         log(f"FP32 sweeps: {json.dumps(fp32)}")
+
+    ref_test = None
+    if not args.no_converge:
+        # the reference's convergence test over six physical steps: its own
+        # configuration (FP64, steps start at U^n), the FP32 sweeps, and the
+        # linearly extrapolated start
+        from channel_flow_amd.solver import CF_FP32_SWEEPS, CF_FP64
+        ref_test, base = {}, None
+        for name, prec, order in (("fp64_start_Un", CF_FP64, 0), ("fp32_sweeps_start_Un", CF_FP32_SWEEPS, 0),
+                                  ("fp64_start_linear", CF_FP64, 1)):
+            its, wall, qn = reference_test_steps(cfg, Q, prec, order)
+            rec = {"inner_iterations": its, "wall_s": [round(w, 4) for w in wall],
+                   "wall_s_per_step_3_to_6": round(float(np.mean(wall[2:])), 4)}
+            if base is None:
+                base = qn
+            else:
+                rec["state_rel_l2_vs_fp64_start_Un"] = float(np.linalg.norm(qn - base) / np.linalg.norm(base))
+            ref_test[name] = rec
+            log(f"reference convergence test, {name}: {rec}")
     del Q
 
     cpu = None if args.no_cpu else cpu_baseline(cfg)
@@ -294,6 +337,7 @@ def main():
                    "parallelism": "single GPU"},
         "phys_step_to_1e-8": conv,
         "fp32_sweeps": fp32,
+        "phys_steps_reference_test": ref_test,
         "stages_ms": per_stage,
         "variants_ms": variants,
         "roofline": roofline,
