@@ -44,7 +44,7 @@ struct cf_ctx {
   double* Ep = nullptr;   // limiter sensors EpX, EpY, EpZ on the face layouts (3)
   double* Fw = nullptr;   // Filter workspace (FILTER_WORK_FIELDS), allocated on first use
   double* ypt = nullptr;  // Y_point j-profile (Y_m)
-  double* Sm = nullptr;   // per-cell sweep summary (S_N)
+  double* Sm = nullptr;   // sweep-order fields: the per-cell summary (G_N) and MR, NL, NR
   int* order = nullptr;   // sweep tiles in dependency order
   int* sync = nullptr;    // ticket, error flag, per-tile progress (x2 directions)
   int ntiles = 0;

@@ -380,45 +380,6 @@ __device__ __forceinline__ Split split_jac(const PState& s, double n0, double n1
   return a;
 }
 
-// Per-cell summary used by the sweeps: the cell quantities the split
-// Jacobians need (rho, P, T, beta and, per direction, the contravariant
-// velocity and spectral radius) plus D^-1.  Each is the value the reference
-// recomputes at every use, computed once.
-enum SumField : int { S_RHO = 0, S_P, S_T, S_BETA, S_UX, S_LX, S_UY, S_LY, S_UZ, S_LZ, S_D11, S_D22, S_D51, S_N };
-
-struct Nb {
-  double rho, P, T, beta, U, lam;
-};
-
-// split_jac with the cell quantities taken from the summary
-__device__ __forceinline__ Split split_nb(const Nb& c, double n0, double n1, double n2, int dir) {
-  Split a;
-  double U = c.U;
-  double dd = fabs(c.lam);
-  double beta = c.beta, P = c.P, rho = c.rho, T = c.T;
-  a.p11 = 0.5 * (beta * U + dd);
-  a.p12 = 0.5 * P * KG * beta * n0;
-  a.p13 = 0.5 * P * KG * beta * n1;
-  a.p14 = 0.5 * P * KG * beta * n2;
-  a.p21 = 0.5 * n0 / rho;
-  a.p22 = 0.5 * (U + dd);
-  a.p31 = 0.5 * n1 / rho;
-  a.p41 = 0.5 * n2 / rho;
-  a.p51 = 0.5 * U * (KG - 1) * (beta - 1) / rho / KG;
-  if (dir == 0) {
-    a.p52 = 0.5 * n0 * beta * T * (KG - 1);
-    a.p53 = 0.5 * n1 * beta * T * (KG - 1);
-    a.p54 = 0.5 * n2 * beta * T * (KG - 1);
-  } else {
-    a.p52 = 0.5 * beta * T * (KG - 1) * n0;
-    a.p53 = 0.5 * beta * T * (KG - 1) * n1;
-    a.p54 = 0.5 * beta * T * (KG - 1) * n2;
-  }
-  a.m11 = 0.5 * (beta * U - dd);
-  a.m22 = 0.5 * (U - dd);
-  return a;
-}
-
 // Sweep summary for metrics whose rows are diagonal (xi = (xix,0,0),
 // eta = (0,ety,0), zeta = (0,0,ztz): every grid Jacobian*.cpp produces).
 // The off-diagonal split-Jacobian entries are then exact zeros.  Per cell:

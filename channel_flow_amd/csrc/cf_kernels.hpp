@@ -11,8 +11,8 @@ void launch_flux_tables(hipStream_t st, double* prof, const Geo& g);
 // limiter variants; Ep is the sensor on the face layout of F
 void launch_flux_limiter(hipStream_t st, int dir, const double* U, const double* Ep, const double* prof, double* F,
                          const Geo& g);
-// ADM sensor: Yp = Y_point j-profile, W = 22-field workspace; the sensors are
-// written on the face layouts
+// ADM sensor: Yp = Y_point j-profile, W = FILTER_WORK_FIELDS-field workspace;
+// the sensors are written on the face layouts
 constexpr int FILTER_WORK_FIELDS = 14;
 void launch_filter(hipStream_t st, const double* U, const double* prof, const double* Yp, double* W, double crit,
                    double E_high, double E_low, double* EpX, double* EpY, double* EpZ, const Geo& g);
@@ -36,18 +36,19 @@ void launch_rhs(hipStream_t st, const double* Ui, const double* Un, const double
 int sweep_tiles(const Geo& g, int* nTI, int* nTJ);
 long sweep_field_elems(const Geo& g);
 long sweep_update_blocks(const Geo& g);
-// marks (optional, 1 + 3 * n_sweeps events) are recorded after the summary
-// pre-pass, then per sweep after the sync reset, the forward and the backward
-// sweep.
-// persistent tiled sweeps (diagonal metrics); Sbuf: G_N + 15 fields of
-// sweep_field_elems doubles; ends with the primitive update of Ui and er
 // inputs of the RHS, for the sweep pre-pass that forms MR itself (rin)
 struct RhsIn {
   const double *Un, *Uq, *Fx, *Fy, *Fz, *vF;
   double f_drive;
 };
-// fp32: the sweep-order fields, the split Jacobians and the sweeps in float
-// (pipelined schedule only); the RHS, D^-1's inputs and the update stay FP64
+// Persistent tiled sweeps (diagonal metrics): the summary pre-pass (forming MR
+// with the RHS when rin is given, else reading MR), n_sweeps forward/backward
+// pairs, then the primitive update of Ui and er.  Sbuf: G_N + 15 fields of
+// sweep_field_elems doubles.  marks (optional, 1 + 3 * n_sweeps events) are
+// recorded after the pre-pass, then per sweep after the sync reset, the
+// forward and the backward sweep.  fp32: the sweep-order fields, D^-1, the
+// split Jacobians and the sweeps in float (pipelined schedule only); the RHS
+// and the update stay FP64.
 void launch_sweeps(hipStream_t st, double* Ui, const double* MR, double* Sbuf, const double* prof, const Geo& g,
                    double deltaT, int n_sweeps, const int* order, int* sync, double* partial, double* er_dev,
                    double DN, long spin_limit, const hipEvent_t* marks = nullptr, bool pipelined = true,
