@@ -1376,10 +1376,11 @@ __global__ void __launch_bounds__(64) k_sweep_tiles(const double* __restrict__ S
 // compute wave.  A block is four waves meeting at one LDS-only barrier per
 // step (loads stay in flight across steps):
 //   wave 0  computes one cell per lane and step from LDS only;
-//   wave 1  loads the lane cells' x and y summaries QS steps ahead and turns
-//           them into the split-Jacobian entries their downstream neighbours
-//           use (SplitCoef, diag_coef);
-//   wave 2  loads D^-1, MR (forward) or NL (backward) and the z summary;
+//   wave 1  loads the lane cells' summaries QS steps ahead and turns them
+//           into the x and y split-Jacobian entries their downstream
+//           neighbours use (SplitCoef, diag_coef);
+//   wave 2  loads MR (forward) or NL (backward) and the summaries, and forms
+//           D^-1 and the z entries;
 //   wave 3  fetches the cross-tile halo: the neighbour tile's summary (as
 //           SplitCoef) and its NL/NR, polled until the sentinel is overwritten,
 //           QD steps ahead.
@@ -1610,9 +1611,10 @@ struct Slots {
   }
 };
 
-// waves 1 and 2: the lane cells' fields QS steps ahead.  XY (wave 1): P, T,
-// beta and the x, y summaries -> x, y SplitCoef.  Otherwise (wave 2): D^-1, SRC and
-// the z summary -> own rows.
+// waves 1 and 2: the lane cells' fields QS steps ahead.  XY (wave 1): P,
+// beta, rho and the x, y summaries -> x, y SplitCoef.  Otherwise (wave 2): SRC
+// and P, beta, rho, the z summary and lambda -> D^-1, SRC and the z SplitCoef
+// in the own rows.
 template <bool FWD, bool XY, int QS, class T>
 __device__ void pipe_own_loader(PipeLds<T>& L, const T* __restrict__ S, const T* __restrict__ SRC,
                                 const double* __restrict__ prof, const Geo& g, const Sk& q, int tile, int TY,
