@@ -252,12 +252,21 @@ __device__ __forceinline__ bool in_box(const Geo& g, int i, int j, int k) {
 // reference's passes read it (their work arrays start as copies of U_ and no
 // pass writes outside the box).  Each 1-D filter keeps the reference's
 // operands and order, so the three fused passes give the same bits.
+// Passes 1-5 store their result G_p.  The sixth forms the filtered state
+// 6 G1 - 15 G2 + 20 G3 - 15 G4 + 6 G5 - G6 from the stored ones, accumulated
+// in the reference's order (Filter.cpp:222-775), and from it the energy
+// E = |u - u_f|^2 (Filter.cpp:790-800, 1465-1475).
 constexpr int FJ = 8, FK = 32;
 
-template <int WIDE>
+struct FiltHist {
+  const double* G[5];  // G1..G5, 4 fields each
+};
+
+template <int WIDE, bool LAST>
 __global__ void __launch_bounds__(256) k_filt_pass(const double* __restrict__ src, const double* __restrict__ U,
                                                    const double* __restrict__ prof, const double* __restrict__ Yp,
-                                                   double* __restrict__ G, double* __restrict__ acc, int pass, Geo g) {
+                                                   double* __restrict__ G, FiltHist hist, double* __restrict__ E,
+                                                   Geo g) {
   __shared__ double Lx[4][FJ + 2][FK + 2];  // rows j0-1 .. j0+FJ, columns k0-1 .. k0+FK
   __shared__ double Ly[4][FJ][FK + 2];      // rows j0 .. j0+FJ-1
   const int k0 = 2 + blockIdx.x * FK, j0 = 2 + blockIdx.y * FJ, i = 3 + blockIdx.z;
@@ -309,37 +318,32 @@ __global__ void __launch_bounds__(256) k_filt_pass(const double* __restrict__ sr
     const int j = j0 + r, k = k0 + c;
     if (j <= g.ny && k <= g.nz) {
       const long p = at(g, i, j, k);
+      double f[4];
 #pragma unroll
       for (int v = 0; v < 4; v++) {
         const double a = Ly[v][r][c], y = Ly[v][r][c + 1], cc = Ly[v][r][c + 2];
         const double t = WIDE ? 0.25 * (a + 2 * y + cc) : 0.125 * (a + 6 * y + cc);
-        G[v * g.nf + p] = t;
-        double& s = acc[v * g.nf + p];
-        switch (pass) {
-          case 0: s = 6 * t; break;
-          case 1: s = s - 15 * t; break;
-          case 2: s = s + 20 * t; break;
-          case 3: s = s - 15 * t; break;
-          case 4: s = s + 6 * t; break;
-          default: s = s - t; break;
+        if (LAST) {
+          const long o = v * g.nf + p;
+          double s = 6 * hist.G[0][o];
+          s = s - 15 * hist.G[1][o];
+          s = s + 20 * hist.G[2][o];
+          s = s - 15 * hist.G[3][o];
+          s = s + 6 * hist.G[4][o];
+          f[v] = s - t;
+        } else {
+          G[v * g.nf + p] = t;
         }
+      }
+      if (LAST) {
+        const long n = g.nf;
+        const double Ue = U[n + p] / U[p] - f[1] / f[0];
+        const double Ve = U[2 * n + p] / U[p] - f[2] / f[0];
+        const double We = U[3 * n + p] / U[p] - f[3] / f[0];
+        E[p] = Ue * Ue + Ve * Ve + We * We;
       }
     }
   }
-}
-
-// E = |u - u_f|^2 (Filter.cpp:790-800, 1465-1475)
-__global__ void __launch_bounds__(256) k_filt_energy(const double* __restrict__ U, const double* __restrict__ acc,
-                                                     double* __restrict__ E, Geo g) {
-  const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int i = 3 + blockIdx.z;
-  if (k > g.nz || j > g.ny) return;
-  const long p = at(g, i, j, k), n = g.nf;
-  const double Ue = U[n + p] / U[p] - acc[n + p] / acc[p];
-  const double Ve = U[2 * n + p] / U[p] - acc[2 * n + p] / acc[p];
-  const double We = U[3 * n + p] / U[p] - acc[3 * n + p] / acc[p];
-  E[p] = Ue * Ue + Ve * Ve + We * We;
 }
 
 // E at a cell of the sensor stencils: the planes next to the box copy the
@@ -2144,24 +2148,31 @@ void launch_flux_limiter(hipStream_t st, int dir, const double* U, const double*
 
 void launch_filter(hipStream_t st, const double* U, const double* prof, const double* Yp, double* W, double crit,
                    double E_high, double E_low, double* EpX, double* EpY, double* EpZ, const Geo& g) {
-  // W: 14 fields = G0(4) G1(4) acc(4) E_h E_2h (the passes' Gx, Gy stay in LDS)
+  // W: 22 fields = G1..G5 (4 each) E_h E_2h (the passes' Gx, Gy stay in LDS)
   const long n = g.nf;
-  double *Gb[2] = {W, W + 4 * n}, *acc = W + 8 * n;
-  double *Eh = W + 12 * n, *E2h = W + 13 * n;
+  FiltHist hist;
+  double* Gp[5];
+  for (int q = 0; q < 5; q++) hist.G[q] = Gp[q] = W + 4 * q * n;
+  double *Eh = W + 20 * n, *E2h = W + 21 * n;
   dim3 blk(64, 4);
-  dim3 gr = box_grid(g.nz - 1, g.ny - 1, g.gend - 2, blk);
   const dim3 gp((g.nz - 1 + FK - 1) / FK, (g.ny - 1 + FJ - 1) / FJ, g.gend - 2);
   for (int wide = 0; wide < 2; wide++) {
-    const double* src = U;
+    double* E = wide ? E2h : Eh;
     for (int pass = 0; pass < 6; pass++) {
-      double* out = Gb[pass & 1];
-      if (wide)
-        hipLaunchKernelGGL(k_filt_pass<1>, gp, dim3(256), 0, st, src, U, prof, Yp, out, acc, pass, g);
-      else
-        hipLaunchKernelGGL(k_filt_pass<0>, gp, dim3(256), 0, st, src, U, prof, Yp, out, acc, pass, g);
-      src = out;
+      const double* src = pass == 0 ? U : Gp[pass - 1];
+      double* out = pass < 5 ? Gp[pass] : nullptr;
+      if (wide) {
+        if (pass < 5)
+          hipLaunchKernelGGL((k_filt_pass<1, false>), gp, dim3(256), 0, st, src, U, prof, Yp, out, hist, E, g);
+        else
+          hipLaunchKernelGGL((k_filt_pass<1, true>), gp, dim3(256), 0, st, src, U, prof, Yp, out, hist, E, g);
+      } else {
+        if (pass < 5)
+          hipLaunchKernelGGL((k_filt_pass<0, false>), gp, dim3(256), 0, st, src, U, prof, Yp, out, hist, E, g);
+        else
+          hipLaunchKernelGGL((k_filt_pass<0, true>), gp, dim3(256), 0, st, src, U, prof, Yp, out, hist, E, g);
+      }
     }
-    hipLaunchKernelGGL(k_filt_energy, gr, blk, 0, st, U, acc, wide ? E2h : Eh, g);
   }
   hipLaunchKernelGGL(k_filt_sensor<0>, box_grid(g.nz - 1, g.ny - 1, g.gend - 1, blk), blk, 0, st, Eh, E2h, crit,
                      E_high, E_low, EpX, g);

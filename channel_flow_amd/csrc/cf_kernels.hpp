@@ -13,7 +13,7 @@ void launch_flux_limiter(hipStream_t st, int dir, const double* U, const double*
                          const Geo& g);
 // ADM sensor: Yp = Y_point j-profile, W = FILTER_WORK_FIELDS-field workspace;
 // the sensors are written on the face layouts
-constexpr int FILTER_WORK_FIELDS = 14;
+constexpr int FILTER_WORK_FIELDS = 22;
 void launch_filter(hipStream_t st, const double* U, const double* prof, const double* Yp, double* W, double crit,
                    double E_high, double E_low, double* EpX, double* EpY, double* EpZ, const Geo& g);
 // fused: one kernel with the primitives and face fluxes in LDS (default);
