@@ -225,7 +225,7 @@ int main(int argc, char** argv) {
   MPI_Init(&argc, &argv);
   int myid = 0;
   MPI_Comm_rank(MPI_COMM_WORLD, &myid);
-  if (argc < 4) { fprintf(stderr, "usage: ref_driver unit|traj|time <dir> <deltaT> ...\n"); return 2; }
+  if (argc < 4) { fprintf(stderr, "usage: ref_driver unit|traj|filter|limiter|stat|force|time <dir> <deltaT> ...\n"); return 2; }
   std::string mode = argv[1], dir = argv[2];
   double deltaT = atof(argv[3]);
   X_np = X_out + 6;
@@ -361,6 +361,20 @@ int main(int argc, char** argv) {
       r++;
     }
     fclose(cf);
+  } else if (mode == "limiter") {
+    // the limiter fluxes on the post-BC iterate with the smooth sensor field
+    // of the unit mode; the sensors are dumped as well (EpX_s ..)
+    load5("iterate.bin", s.U_, s.J);
+    halo_bc(s, myid);
+    for (int i = 0; i < X_np; i++)
+      for (int j = 0; j < Y_m; j++)
+        for (int k = 0; k < Z_m; k++) {
+          double t = 0.5 + 0.5 * sin(0.7 * i + 1.3 * j + 0.4 * k);
+          s.EpX[i][j][k] = t; s.EpY[i][j][k] = 1.0 - 0.5 * t; s.EpZ[i][j][k] = 0.25 + 0.5 * t;
+        }
+    dump(dir, "EpX_s", s.EpX); dump(dir, "EpY_s", s.EpY); dump(dir, "EpZ_s", s.EpZ);
+    fluxes_limiter(s, myid);
+    dump5(dir, "inFxS", s.inFx); dump5(dir, "inFyS", s.inFy); dump5(dir, "inFzS", s.inFz);
   } else if (mode == "stat") {
     // Statistic.cpp:1043-1154, the mean profiles: step 1 accumulates the
     // post-BC iterate and step 2 the level-n state with statistic_step = 2

@@ -8,7 +8,9 @@ inner iterations must leave the same state bit for bit; the raw residual
 norms, summed in another order on the GPU, agree to 1e-12.  The sweep and
 viscous schedules must also agree bit for bit at that size, and at
 512x384x512, where no reference run fits in a test, the schedules agree and
-the norms are finite and positive.
+the norms are finite and positive.  The variant kernels (Filter, the
+limiter fluxes) are compared bit for bit with the reference's at 256x192x256
+as well, on a perturbed state.
 """
 import os
 import subprocess
@@ -66,6 +68,100 @@ def test_gpu_benchmark_grid_matches_reference(tmp_path):
     np.testing.assert_allclose(np.array(raw), ref_er[:, 2:7], rtol=1e-12, atol=0)
     np.testing.assert_array_equal(qn, ref_n)
     np.testing.assert_array_equal(qm, ref_m)
+
+
+def _perturbed(cfg, seed):
+    """Poiseuille state with density and energy perturbed by +-2 % (momenta
+    with the density), so that no two neighbouring cells are equal and the
+    limiters' slope ratios are finite."""
+    from channel_flow_amd.config import FlowParams, poiseuille_state
+    Q = np.array(poiseuille_state(cfg, FlowParams(), seed=seed, stream=2))
+    rng = np.random.default_rng(seed)
+    f_rho = 1.0 + 0.02 * rng.uniform(-1.0, 1.0, Q[0].shape)
+    f_e = 1.0 + 0.02 * rng.uniform(-1.0, 1.0, Q[0].shape)
+    for v in range(4):
+        Q[v] *= f_rho
+    Q[4] *= f_e
+    return Q
+
+
+def _reference_variant(tmp_path, cfg, mode, Q, extra=()):
+    """Runs the reference driver in `mode` on the iterate Q at the benchmark
+    grid (initial.bin holds Q at both levels; the driver overwrites the
+    iterate with iterate.bin)."""
+    from channel_flow_amd.config import write_initial_bin
+    exe = ROOT / "oracle" / "_ref" / "g{}x{}x{}".format(*BENCH_GRID) / "ref_driver"
+    if not exe.exists():
+        pytest.skip(f"{exe} is not built (it needs the reference sources: __graft_entry__.build())")
+    write_initial_bin(tmp_path / "initial.bin", Q, Q)
+    np.ascontiguousarray(Q).tofile(tmp_path / "iterate.bin")
+    threads = min(16, os.cpu_count() or 1)
+    _log(f"reference {mode} at {BENCH_GRID}, {threads} threads")
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
+    subprocess.run([str(exe), mode, str(tmp_path), "1e-8", *extra], check=True, env=env,
+                   stdout=subprocess.DEVNULL, timeout=600)
+    (tmp_path / "initial.bin").unlink()
+    (tmp_path / "iterate.bin").unlink()
+
+
+def _gpu_post_bc(s, cfg, Q):
+    """The iterate Q as the reference loads it (U = Q/J on the interior), with
+    the GPU's ghost cells, and the metric arrays."""
+    met = s.Jacobian_all(1.5)
+    g, ny, nz = cfg.gend, cfg.ny, cfg.nz
+    U_ = s.zeros(5)
+    for v in range(5):
+        U_[v][3:g + 1, 2:ny + 1, 2:nz + 1] = Q[v] / met["J"][3:g + 1, 2:ny + 1, 2:nz + 1]
+    s.boundary_conditions(U_, met["J"])
+    return U_, met
+
+
+def test_gpu_benchmark_grid_filter_matches_reference(tmp_path):
+    """Filter(0.007, 1, 0) at 256x192x256 on a perturbed state: EpX, EpY and
+    EpZ bit for bit against the reference's Filter on the same input."""
+    from channel_flow_amd.config import ChannelConfig
+    from channel_flow_amd.solver import Solver
+    from conftest import masks
+    cfg = ChannelConfig(*BENCH_GRID, gamma_metric=1.5)
+    Q = _perturbed(cfg, seed=6)
+    (tmp_path / "crit.txt").write_text(float(0.007).hex() + "\n")
+    _reference_variant(tmp_path, cfg, "filter", Q, extra=(str(tmp_path / "crit.txt"),))
+    _log("GPU Filter")
+    with Solver(cfg) as s:
+        U_, met = _gpu_post_bc(s, cfg, Q)
+        _, Yp = s.Grid(2.8)
+        Ep = s.Filter(0.007, 1.0, 0.0, U_, Yp, met["J"])
+    for a, name in enumerate("XYZ"):
+        ref = np.fromfile(tmp_path / f"Ep{name}_c0.bin").reshape(cfg.shape)
+        m = masks(cfg)["xyz"[a]]
+        np.testing.assert_array_equal(Ep[a][m], ref[m], err_msg=f"Ep{name}")
+        _log(f"Ep{name}: {np.count_nonzero(ref[m] == 1.0)} of {np.count_nonzero(m)} faces at E_high")
+
+
+def test_gpu_benchmark_grid_limiter_fluxes_match_reference(tmp_path):
+    """Flux_X/Y/Z_limiter at 256x192x256 (where Flux_Z_limiter's boundary
+    branch is live, Z_out >= Y_out + 2) on a perturbed state with a smooth
+    sensor field: bit for bit against the reference, except the y near-wall
+    faces where the reference reads unassigned values."""
+    from channel_flow_amd.config import ChannelConfig
+    from channel_flow_amd.solver import Solver
+    from conftest import limiter_mask
+    cfg = ChannelConfig(*BENCH_GRID, gamma_metric=1.5)
+    Q = _perturbed(cfg, seed=7)
+    _reference_variant(tmp_path, cfg, "limiter", Q)
+    _log("GPU limiter fluxes")
+    Ep = [np.fromfile(tmp_path / f"Ep{n}_s.bin").reshape(cfg.shape) for n in "XYZ"]
+    with Solver(cfg) as s:
+        U_, met = _gpu_post_bc(s, cfg, Q)
+        out = [s.Flux_X_limiter(U_, met["J"], met["J_u"], met["m_u"], Ep[0]),
+               s.Flux_Y_limiter(U_, met["J"], met["J_v"], met["m_v"], Ep[1]),
+               s.Flux_Z_limiter(U_, met["J"], met["J_w"], met["m_w"], Ep[2])]
+    for a, name in enumerate("xyz"):
+        m = limiter_mask(cfg, a)
+        for v in range(5):
+            ref = np.fromfile(tmp_path / f"inF{name}S{v + 1}.bin").reshape(cfg.shape)
+            assert np.isfinite(ref[m]).all()
+            np.testing.assert_array_equal(out[a][v][m], ref[m], err_msg=f"inF{name}{v + 1}")
 
 
 def _schedule_runs(cfg, Qn, Qm, runs, iterations):
