@@ -1760,12 +1760,9 @@ __device__ void pipe_halo_loader(PipeLds<T>& L, const T* __restrict__ S, const T
     for (int r = 0; r < 3; r++) pend |= lv && dyn[r] && is_sentinel(d[r]);
     long cnt = 0;
     while (__any(pend)) {
-      const int e = __builtin_amdgcn_readfirstlane(__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-      if (e != 0 || ++cnt > spin_limit) {
-        if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
+      // one memory round trip per poll: the pending values and the error
+      // flag (set by a tile that gave up) are loaded together
+      const int e = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (pend) {
         const long c = cell(sig);
         pend = false;
@@ -1775,6 +1772,10 @@ __device__ void pipe_halo_loader(PipeLds<T>& L, const T* __restrict__ S, const T
             d[r] = ld_coherent(fld[r] + c);
             pend |= is_sentinel(d[r]);
           }
+      }
+      if (__builtin_amdgcn_readfirstlane(e) != 0 || ++cnt > spin_limit) {
+        if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
       }
     }
     T v[12];
