@@ -110,3 +110,35 @@ def test_gpu_fp32_sweeps_refuse_other_schedules():
             with pytest.raises(RuntimeError, match="pipelined"):
                 s.set_sweep_schedule(schedule)
         s.set_sweep_schedule(0)
+
+
+def test_gpu_fp32_sweeps_benchmark_grid():
+    """256x192x256, stretching 2.0, dt 1e-8: one physical step from the
+    seeded state to a 1e-8 drop of er in both precisions.  Same iteration
+    count within one, and state and mean profiles of FP64 to 1e-9 of the
+    step's change (measured in the benchmark: about 1e-11)."""
+    from channel_flow_amd.config import ChannelConfig, FlowParams, poiseuille_state
+    from channel_flow_amd.solver import Solver
+    fl = FlowParams()
+    cfg = ChannelConfig(256, 192, 256, gamma_metric=2.0, deltaT=1e-8, f_drive=fl.driving_force(0.005))
+    Q = poiseuille_state(cfg, fl, seed=1)
+    res = {}
+    for prec in (FP64, FP32_SWEEPS):
+        with Solver(cfg, precision=prec) as s:
+            s.Jacobian_all(cfg.gamma_metric, want_arrays=False)
+            s.load_state(Q, Q)
+            first = None
+            for it in range(1, 801):
+                e = s.inner_iteration()
+                first = e.copy() if first is None else first
+                if np.max(e[:4] / first[:4]) <= 1e-8:
+                    break
+            res[prec] = (it, s.read_state()[2])
+            print(f"precision {prec}: {it} inner iterations", flush=True)
+    (i64, q64), (i32, q32) = res[FP64], res[FP32_SWEEPS]
+    assert abs(i32 - i64) <= 1
+    change = np.linalg.norm(q64 - Q)
+    assert np.linalg.norm(q32 - q64) <= 1e-9 * change
+    p64, p32 = _profiles(q64), _profiles(q32)
+    for k in p64:
+        assert np.abs(p32[k] - p64[k]).max() <= 1e-9 * np.abs(p64[k]).max(), k
