@@ -1490,24 +1490,23 @@ __device__ void pipe_compute(PipeLds<T>& L, T* __restrict__ SRC, T* __restrict__
       T M[5];
 #pragma unroll
       for (int v = 0; v < 5; v++) M[v] = L.own[cs][PO_M + v][lane];
-      T acc[3][5];
-      if (x_dom) {
-        T Nb[5];
+      // both neighbours' inputs are read unconditionally (the indices are
+      // inside the LDS rows for every lane), so the x and y reads share one
+      // LDS round trip; a neighbour outside the domain contributes zero
+      T Nx[5], Ny[5];
 #pragma unroll
-        for (int v = 0; v < 5; v++) Nb[v] = L.N[nr][v][xi];
-        diag_apply<0, FWD>(coef_load(L.xc[ps], xi), Nb, acc[0]);
-      } else {
-#pragma unroll
-        for (int v = 0; v < 5; v++) acc[0][v] = 0;
+      for (int v = 0; v < 5; v++) {
+        Nx[v] = L.N[nr][v][xi];
+        Ny[v] = L.N[nr][v][yiN];
       }
-      if (y_dom) {
-        T Nb[5];
+      const SplitCoefT<T> cx = coef_load(L.xc[ps], xi), cy = coef_load(L.yc[ps], yi);
+      T acc[3][5];
+      diag_apply<0, FWD>(cx, Nx, acc[0]);
+      diag_apply<1, FWD>(cy, Ny, acc[1]);
 #pragma unroll
-        for (int v = 0; v < 5; v++) Nb[v] = L.N[nr][v][yiN];
-        diag_apply<1, FWD>(coef_load(L.yc[ps], yi), Nb, acc[1]);
-      } else {
-#pragma unroll
-        for (int v = 0; v < 5; v++) acc[1][v] = 0;
+      for (int v = 0; v < 5; v++) {
+        acc[0][v] = x_dom ? acc[0][v] : T(0);
+        acc[1][v] = y_dom ? acc[1][v] : T(0);
       }
       if (kk > 0) {
         diag_apply<2, FWD>(zc, zN, acc[2]);
@@ -1694,10 +1693,12 @@ __device__ void pipe_own_loader(PipeLds<T>& L, const T* __restrict__ S, const T*
 // wave 3: the halo of the next step QD steps ahead.  There are 16 consumer
 // lanes of the compute wave: 0..7 on the x edge (b = c) and 8..15 on the y
 // edge (a = c - 8).  Each needs its remote cell's summary along the
-// direction (P beta rho U lam) and NL/NR (5 values).  Lane l loads values
-// 3g..3g+2 (g = l >> 4) for consumer l & 15 (slots 10 and 11 are spare), so
-// all 64 lanes carry three values per step; after the sentinel poll, lanes
-// 0..15 gather their consumer's ten values with cross-lane shuffles.
+// direction (P beta rho U lam), which the prep pass wrote, and the remote
+// tile's NL/NR output (5 values), which may still be in flight.  Lane c of
+// the first 16 loads consumer c's five outputs; lane 16 + c loads its five
+// summary values, which reach lane c by cross-lane shuffles and become the
+// split-Jacobian entries before the outputs are polled, so that once they
+// arrive only their LDS stores stand between them and the barrier.
 template <bool FWD, int QD, class T>
 __device__ void pipe_halo_loader(PipeLds<T>& L, const T* __restrict__ S, const T* __restrict__ DST,
                                  const double* __restrict__ prof, const Geo& g, const Sk& q, int tile, int TX,
@@ -1731,72 +1732,67 @@ __device__ void pipe_halo_loader(PipeLds<T>& L, const T* __restrict__ S, const T
     const int sc = min(sig, nsteps - 1);
     return own0 + (live(sig) ? rdiff : 0L) + (long)(FWD ? sc : nsteps - 1 - sc) * 64;
   };
-  // this lane's three values f = 3 grp + r: P beta rho U lam N0..N4 (f = 10,
-  // 11 spare: they read P again)
-  const T* fld[3];
-  bool dyn[3];
+  // this lane's five fields: group 0 the outputs N0..N4, group 1 P beta rho
+  // U lam, groups 2 and 3 re-read group 1's (a uniform load count per step)
+  const bool outs = grp == 0;
+  const T* fld[5];
 #pragma unroll
-  for (int r = 0; r < 3; r++) {
-    const int f = 3 * grp + r;
-    fld[r] = f < 3 ? S + (long)(G_P + f) * nf
-                   : (f < 5 ? S + (long)(dU + f - 3) * nf : (f < 10 ? DST + (long)(f - 5) * nf : S + (long)G_P * nf));
-    dyn[r] = f >= 5 && f < 10;
-  }
-  Slots<QD, 3, T> buf;
+  for (int r = 0; r < 5; r++)
+    fld[r] = outs ? DST + (long)r * nf : (r < 3 ? S + (long)(G_P + r) * nf : S + (long)(dU + r - 3) * nf);
+  Slots<QD, 5, T> buf;
   auto issue = [&](int sig, int sl) {
     const long c = cell(sig);
-    T d[3];
+    T d[5];
 #pragma unroll
-    for (int r = 0; r < 3; r++) d[r] = ld_coherent(fld[r] + c);
+    for (int r = 0; r < 5; r++) d[r] = ld_coherent(fld[r] + c);
     buf.put(sl, d);
   };
   auto consume = [&](int sig, int sl) {
     if (sig >= nsteps) return;
-    T d[3];
+    T d[5];
     buf.get(sl, d);
     const bool lv = live(sig);
-    bool pend = false;
+    const int sp = (sig + 2) % 3, np = (sig & 1) ^ 1;
+    // the remote summary to consumer lanes, and its split Jacobian
+    T v[5];
 #pragma unroll
-    for (int r = 0; r < 3; r++) pend |= lv && dyn[r] && is_sentinel(d[r]);
+    for (int r = 0; r < 5; r++) v[r] = __shfl(d[r], cid + 16, 64);
+    if (outs && lv) {
+      const T Tc = over_rho(v[0], v[2]);
+      T pd, p51;
+      split_pd_p51(n, v[3], v[1], v[2], &pd, &p51);
+      if (dir == 0)
+        coef_store(L.xc[sp], 64 + idx, diag_coef<0, FWD>(v[0], Tc, v[1], v[3], v[4], pd, p51, n));
+      else
+        coef_store(L.yc[sp], 64 + idx, diag_coef<1, FWD>(v[0], Tc, v[1], v[3], v[4], pd, p51, n));
+    }
+    bool pend = false;
+    if (outs && lv) {
+#pragma unroll
+      for (int r = 0; r < 5; r++) pend |= is_sentinel(d[r]);
+    }
     long cnt = 0;
     while (__any(pend)) {
-      // one memory round trip per poll: the pending values and the error
+      // one memory round trip per poll: the pending outputs and the error
       // flag (set by a tile that gave up) are loaded together
       const int e = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (pend) {
         const long c = cell(sig);
         pend = false;
 #pragma unroll
-        for (int r = 0; r < 3; r++)
-          if (dyn[r]) {
-            d[r] = ld_coherent(fld[r] + c);
-            pend |= is_sentinel(d[r]);
-          }
+        for (int r = 0; r < 5; r++) {
+          d[r] = ld_coherent(fld[r] + c);
+          pend |= is_sentinel(d[r]);
+        }
       }
       if (__builtin_amdgcn_readfirstlane(e) != 0 || ++cnt > spin_limit) {
         if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         break;
       }
     }
-    T v[12];
+    if (outs && lv) {
 #pragma unroll
-    for (int gq = 0; gq < 4; gq++)
-#pragma unroll
-      for (int r = 0; r < 3; r++) v[3 * gq + r] = __shfl(d[r], cid + 16 * gq, 64);
-    if (lane < 16 && lv) {
-      const int sp = (sig + 2) % 3, np = (sig & 1) ^ 1;
-      const T Tc = over_rho(v[0], v[2]);
-      T pd, p51;
-      split_pd_p51(n, v[3], v[1], v[2], &pd, &p51);
-      if (dir == 0) {
-        coef_store(L.xc[sp], 64 + idx, diag_coef<0, FWD>(v[0], Tc, v[1], v[3], v[4], pd, p51, n));
-#pragma unroll
-        for (int w = 0; w < 5; w++) L.N[np][w][64 + idx] = v[5 + w];
-      } else {
-        coef_store(L.yc[sp], 64 + idx, diag_coef<1, FWD>(v[0], Tc, v[1], v[3], v[4], pd, p51, n));
-#pragma unroll
-        for (int w = 0; w < 5; w++) L.N[np][w][72 + idx] = v[5 + w];
-      }
+      for (int w = 0; w < 5; w++) L.N[np][w][(dir == 0 ? 64 : 72) + idx] = d[w];
     }
   };
 #pragma unroll
