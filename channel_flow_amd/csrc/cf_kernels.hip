@@ -246,21 +246,38 @@ __device__ __forceinline__ bool in_box(const Geo& g, int i, int j, int k) {
 }
 
 // One filter pass (x, then y, then z, Filter.cpp:218-775) in one kernel: a
-// block owns an 8 (j) x 32 (k) tile of plane i.  Gx of the tile and of a
-// one-cell rim, and Gy of the tile rows and rim columns, stay in LDS; a rim
-// position outside the filter box holds U_'s own value there, as the
-// reference's passes read it (their work arrays start as copies of U_ and no
-// pass writes outside the box).  Each 1-D filter keeps the reference's
-// operands and order, so the three fused passes give the same bits.
-// Passes 1-5 store their result G_p.  The sixth forms the filtered state
-// 6 G1 - 15 G2 + 20 G3 - 15 G4 + 6 G5 - G6 from the stored ones, accumulated
-// in the reference's order (Filter.cpp:222-775), and from it the energy
-// E = |u - u_f|^2 (Filter.cpp:790-800, 1465-1475).
-constexpr int FJ = 8, FK = 32;
+// block owns an 8 (j) x 32 (k) tile and marches through FI planes i.  Each
+// thread keeps its (at most two) tile-and-rim positions' values at planes
+// i-1, i, i+1 in registers and loads plane i+2 while plane i is filtered,
+// so every plane is read once per block.  Gx of the tile and of a one-cell
+// rim, and Gy of the tile rows and rim columns, stay in LDS; a rim position
+// outside the filter box holds U_'s own value there, and the x filter reads
+// U_'s planes next to the box, as the reference's passes read them (their
+// work arrays start as copies of U_ and no pass writes outside the box).
+// Each 1-D filter keeps the reference's operands and order, so the three
+// fused passes give the same bits.  Passes 1-5 store their result G_p.  The
+// sixth forms the filtered state 6 G1 - 15 G2 + 20 G3 - 15 G4 + 6 G5 - G6
+// from the stored ones, accumulated in the reference's order
+// (Filter.cpp:222-775), and from it the energy E = |u - u_f|^2
+// (Filter.cpp:790-800, 1465-1475).
+#ifndef CF_FILT_FI
+#define CF_FILT_FI 8
+#endif
+constexpr int FJ = 8, FK = 32, FI = CF_FILT_FI;  // FI planes per block
+constexpr int FNP = (FJ + 2) * (FK + 2);  // tile and rim positions of a plane
+static_assert(FNP <= 512 && FJ * FK == 256, "two rim positions per thread, one tile cell per thread");
 
 struct FiltHist {
   const double* G[5];  // G1..G5, 4 fields each
 };
+
+// field v at plane ii of a rim position (offset pp within a plane) as the x
+// filter reads it: the pass input inside the box, U_ outside it
+__device__ __forceinline__ double filt_x_in(const double* __restrict__ src, const double* __restrict__ U, bool box,
+                                            long pp, int ii, int v, const Geo& g) {
+  const double* base = (box && ii >= 3 && ii <= g.gend) ? src : U;
+  return base[v * g.nf + (long)ii * g.plane + pp];
+}
 
 template <int WIDE, bool LAST>
 __global__ void __launch_bounds__(256) k_filt_pass(const double* __restrict__ src, const double* __restrict__ U,
@@ -269,80 +286,113 @@ __global__ void __launch_bounds__(256) k_filt_pass(const double* __restrict__ sr
                                                    Geo g) {
   __shared__ double Lx[4][FJ + 2][FK + 2];  // rows j0-1 .. j0+FJ, columns k0-1 .. k0+FK
   __shared__ double Ly[4][FJ][FK + 2];      // rows j0 .. j0+FJ-1
-  const int k0 = 2 + blockIdx.x * FK, j0 = 2 + blockIdx.y * FJ, i = 3 + blockIdx.z;
+  const int k0 = 2 + blockIdx.x * FK, j0 = 2 + blockIdx.y * FJ;
+  const int i0 = 3 + blockIdx.z * FI, i1 = min(i0 + FI, g.gend + 1);  // planes i0 .. i1-1
   const int tid = threadIdx.x;
-  for (int idx = tid; idx < (FJ + 2) * (FK + 2); idx += 256) {
-    const int r = idx / (FK + 2), c = idx - r * (FK + 2);
-    const int j = j0 - 1 + r, k = k0 - 1 + c;
-    if (j > g.ny + 1 || k > g.nz + 1) continue;
-    const long p = at(g, i, j, k);
-    if (in_box(g, i, j, k)) {
+  const long nf = g.nf;
+  // this thread's rim positions: row, column, offset within a plane, whether
+  // the position exists and whether it is inside the box (in j and k; every
+  // plane of the block is inside in i)
+  int pr[2], pc[2];
+  long pp[2];
+  bool pv[2], pb[2];
 #pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const double* sm = (i - 1 < 3 ? U : src) + v * g.nf;
-        const double* tp = (i + 1 > g.gend ? U : src) + v * g.nf;
-        const double a = sm[p - g.plane], b = src[v * g.nf + p], cc = tp[p + g.plane];
-        Lx[v][r][c] = WIDE ? 0.25 * (a + 2 * b + cc) : 0.125 * (a + 6 * b + cc);
-      }
-    } else {
-#pragma unroll
-      for (int v = 0; v < 4; v++) Lx[v][r][c] = U[v * g.nf + p];
-    }
+  for (int q = 0; q < 2; q++) {
+    const int idx = tid + 256 * q;
+    pr[q] = idx / (FK + 2);
+    pc[q] = idx - pr[q] * (FK + 2);
+    const int j = j0 - 1 + pr[q], k = k0 - 1 + pc[q];
+    pv[q] = idx < FNP && j <= g.ny + 1 && k <= g.nz + 1;
+    pb[q] = pv[q] && j >= 2 && j <= g.ny && k >= 2 && k <= g.nz;
+    pp[q] = pv[q] ? at(g, 0, j, k) : 0;
   }
-  __syncthreads();
-  for (int idx = tid; idx < FJ * (FK + 2); idx += 256) {
-    const int r = idx / (FK + 2), c = idx - r * (FK + 2);
-    const int j = j0 + r, k = k0 - 1 + c;
-    if (j > g.ny + 1 || k > g.nz + 1) continue;
-    if (in_box(g, i, j, k)) {
-      const double A = Yp[j + 1] - Yp[j - 1];  // H_L + H_R
-      const double B = Yp[j + 1] - Yp[j];      // H_R
-      const double C = Yp[j] - Yp[j - 1];      // H_L
-      const double Jm = pf(prof, g.Ym, SET_C, 0, j - 1), J0 = pf(prof, g.Ym, SET_C, 0, j),
-                   Jp = pf(prof, g.Ym, SET_C, 0, j + 1);
+  double w[2][3][4];  // planes i-1, i, i+1
+#pragma unroll
+  for (int q = 0; q < 2; q++)
+    if (pv[q])
+#pragma unroll
+      for (int s = 0; s < 3; s++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) w[q][s][v] = filt_x_in(src, U, pb[q], pp[q], i0 - 1 + s, v, g);
+  for (int i = i0; i < i1; i++) {
+    // plane i+2 for the next plane's x filter, in flight during this one
+    double nx[2][4];
+    const bool more = i + 1 < i1;
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) nx[q][v] = (pv[q] && more) ? filt_x_in(src, U, pb[q], pp[q], i + 2, v, g) : 0.0;
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      if (!pv[q]) continue;
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const double gm = Lx[v][r][c], gx = Lx[v][r + 1][c], gp = Lx[v][r + 2][c];
-        Ly[v][r][c] = WIDE ? 0.5 * (B / A * gm * Jm + gx * J0 + C / A * gp * Jp) / J0
-                           : 0.125 * (2 * B / A * gm * Jm + 6 * gx * J0 + 2 * C / A * gp * Jp) / J0;
+        const double a = w[q][0][v], b = w[q][1][v], cc = w[q][2][v];
+        Lx[v][pr[q]][pc[q]] = !pb[q] ? b : (WIDE ? 0.25 * (a + 2 * b + cc) : 0.125 * (a + 6 * b + cc));
       }
-    } else {
-      const long p = at(g, i, j, k);
-#pragma unroll
-      for (int v = 0; v < 4; v++) Ly[v][r][c] = U[v * g.nf + p];
     }
-  }
-  __syncthreads();
-  {
-    const int r = tid / FK, c = tid - r * FK;  // 256 threads = the FJ x FK tile
-    const int j = j0 + r, k = k0 + c;
-    if (j <= g.ny && k <= g.nz) {
-      const long p = at(g, i, j, k);
-      double f[4];
+    __syncthreads();
+    for (int idx = tid; idx < FJ * (FK + 2); idx += 256) {
+      const int r = idx / (FK + 2), c = idx - r * (FK + 2);
+      const int j = j0 + r, k = k0 - 1 + c;
+      if (j > g.ny + 1 || k > g.nz + 1) continue;
+      if (in_box(g, i, j, k)) {
+        const double A = Yp[j + 1] - Yp[j - 1];  // H_L + H_R
+        const double B = Yp[j + 1] - Yp[j];      // H_R
+        const double C = Yp[j] - Yp[j - 1];      // H_L
+        const double Jm = pf(prof, g.Ym, SET_C, 0, j - 1), J0 = pf(prof, g.Ym, SET_C, 0, j),
+                     Jp = pf(prof, g.Ym, SET_C, 0, j + 1);
 #pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const double a = Ly[v][r][c], y = Ly[v][r][c + 1], cc = Ly[v][r][c + 2];
-        const double t = WIDE ? 0.25 * (a + 2 * y + cc) : 0.125 * (a + 6 * y + cc);
+        for (int v = 0; v < 4; v++) {
+          const double gm = Lx[v][r][c], gx = Lx[v][r + 1][c], gp = Lx[v][r + 2][c];
+          Ly[v][r][c] = WIDE ? 0.5 * (B / A * gm * Jm + gx * J0 + C / A * gp * Jp) / J0
+                             : 0.125 * (2 * B / A * gm * Jm + 6 * gx * J0 + 2 * C / A * gp * Jp) / J0;
+        }
+      } else {
+        // outside the box: U_'s own value, which the x step left in Lx
+#pragma unroll
+        for (int v = 0; v < 4; v++) Ly[v][r][c] = Lx[v][r + 1][c];
+      }
+    }
+    __syncthreads();
+    {
+      const int r = tid / FK, c = tid - r * FK;  // 256 threads = the FJ x FK tile
+      const int j = j0 + r, k = k0 + c;
+      if (j <= g.ny && k <= g.nz) {
+        const long p = at(g, i, j, k);
+        double f[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const double a = Ly[v][r][c], y = Ly[v][r][c + 1], cc = Ly[v][r][c + 2];
+          const double t = WIDE ? 0.25 * (a + 2 * y + cc) : 0.125 * (a + 6 * y + cc);
+          if (LAST) {
+            const long o = v * nf + p;
+            double s = 6 * hist.G[0][o];
+            s = s - 15 * hist.G[1][o];
+            s = s + 20 * hist.G[2][o];
+            s = s - 15 * hist.G[3][o];
+            s = s + 6 * hist.G[4][o];
+            f[v] = s - t;
+          } else {
+            G[v * nf + p] = t;
+          }
+        }
         if (LAST) {
-          const long o = v * g.nf + p;
-          double s = 6 * hist.G[0][o];
-          s = s - 15 * hist.G[1][o];
-          s = s + 20 * hist.G[2][o];
-          s = s - 15 * hist.G[3][o];
-          s = s + 6 * hist.G[4][o];
-          f[v] = s - t;
-        } else {
-          G[v * g.nf + p] = t;
+          const double Ue = U[nf + p] / U[p] - f[1] / f[0];
+          const double Ve = U[2 * nf + p] / U[p] - f[2] / f[0];
+          const double We = U[3 * nf + p] / U[p] - f[3] / f[0];
+          E[p] = Ue * Ue + Ve * Ve + We * We;
         }
       }
-      if (LAST) {
-        const long n = g.nf;
-        const double Ue = U[n + p] / U[p] - f[1] / f[0];
-        const double Ve = U[2 * n + p] / U[p] - f[2] / f[0];
-        const double We = U[3 * n + p] / U[p] - f[3] / f[0];
-        E[p] = Ue * Ue + Ve * Ve + We * We;
-      }
     }
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        w[q][0][v] = w[q][1][v];
+        w[q][1][v] = w[q][2][v];
+        w[q][2][v] = nx[q][v];
+      }
   }
 }
 
@@ -2152,7 +2202,7 @@ void launch_filter(hipStream_t st, const double* U, const double* prof, const do
   for (int q = 0; q < 5; q++) hist.G[q] = Gp[q] = W + 4 * q * n;
   double *Eh = W + 20 * n, *E2h = W + 21 * n;
   dim3 blk(64, 4);
-  const dim3 gp((g.nz - 1 + FK - 1) / FK, (g.ny - 1 + FJ - 1) / FJ, g.gend - 2);
+  const dim3 gp((g.nz - 1 + FK - 1) / FK, (g.ny - 1 + FJ - 1) / FJ, (g.gend - 2 + FI - 1) / FI);
   for (int wide = 0; wide < 2; wide++) {
     double* E = wide ? E2h : Eh;
     for (int pass = 0; pass < 6; pass++) {
