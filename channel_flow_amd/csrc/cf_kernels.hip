@@ -60,8 +60,45 @@ __global__ void k_zper(double* __restrict__ U, Geo g) {
 }
 
 // ---------------------------------------------------------------------------
-// MUSCL5 + Roe fluxes, one thread per face
+// MUSCL5 + Roe fluxes
 // ---------------------------------------------------------------------------
+// Face f along DIR of the column (i, j, k) (the cell f along the direction):
+// u[v][t] holds U_ of cells f-2 .. f+3 (t = cell - f + 2); each recipe reads
+// only its own cells.
+template <int DIR>
+__device__ __forceinline__ void flux_face(const double (&u)[5][6], int f, int i, int j, int k,
+                                          const double* __restrict__ prof, double* __restrict__ F, const Geo& g) {
+  int rl = recipe_left(DIR, f, g.gend, g.ny, g.nz);
+  int rr = recipe_right(DIR, f, g.gend, g.ny, g.nz);
+  // 1/sum(w/J) varies with j only: with the cell's j along x and z, with the
+  // face index along y
+  const int rj = DIR == 1 ? f : j;
+  double L[5], R[5], out[5];
+  reconstruct_t(rl, u, prow(prof, g.Ym, FT_REC + 7 * DIR + rl, rj), L);
+  reconstruct_t(rr, u, prow(prof, g.Ym, FT_REC + 7 * DIR + rr, rj), R);
+  long o;
+  int set, jp;
+  if (DIR == 0) { o = at(g, f, j - 1, k - 1); set = SET_U; jp = j - 1; }
+  else if (DIR == 1) { o = at(g, i - 1, f, k - 1); set = SET_V; jp = f; }
+  else { o = at(g, i - 1, j - 1, f); set = SET_W; jp = j - 1; }
+  double n[3] = {pf(prof, g.Ym, set, 1 + 3 * DIR, jp), pf(prof, g.Ym, set, 2 + 3 * DIR, jp),
+                 pf(prof, g.Ym, set, 3 + 3 * DIR, jp)};
+  double nt[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) nt[c] = prow(prof, g.Ym, FT_NRM + 4 * DIR + c, jp);
+  double Jf = pf(prof, g.Ym, set, 0, jp);
+  bool wall = DIR == 1 && (f == 1 || f == g.ny);
+  const double rJf = prow(prof, g.Ym, FT_RCP + 2 * DIR, jp), rin = prow(prof, g.Ym, FT_RCP + 2 * DIR + 1, jp);
+  // a wave covers one face row, so this vote is one branch per wave
+  if (__all(rJf != 0.0 && rin != 0.0))
+    roe_flux<true>(L, R, n, nt, Jf, rJf, rin, DIR == 1, wall, out);
+  else
+    roe_flux<false>(L, R, n, nt, Jf, rJf, rin, DIR == 1, wall, out);
+#pragma unroll
+  for (int v = 0; v < 5; v++) F[v * g.nf + o] = out[v];
+}
+
+// one thread per face (Flux_Z: the lanes run along k, the direction itself)
 template <int DIR>
 __global__ void __launch_bounds__(256) k_flux(const double* __restrict__ U, const double* __restrict__ prof,
                                               double* __restrict__ F, Geo g) {
@@ -95,32 +132,60 @@ __global__ void __launch_bounds__(256) k_flux(const double* __restrict__ U, cons
 #pragma unroll
     for (int v = 0; v < 5; v++) u[v][t] = U[v * g.nf + p];
   }
-  // 1/sum(w/J) varies with j only: with the cell's j along x and z, with the
-  // face index along y
-  const int rj = DIR == 1 ? f : j;
-  double L[5], R[5], out[5];
-  reconstruct_t(rl, u, prow(prof, g.Ym, FT_REC + 7 * DIR + rl, rj), L);
-  reconstruct_t(rr, u, prow(prof, g.Ym, FT_REC + 7 * DIR + rr, rj), R);
-  long o;
-  int set, jp;
-  if (DIR == 0) { o = at(g, f, j - 1, k - 1); set = SET_U; jp = j - 1; }
-  else if (DIR == 1) { o = at(g, i - 1, f, k - 1); set = SET_V; jp = f; }
-  else { o = at(g, i - 1, j - 1, f); set = SET_W; jp = j - 1; }
-  double n[3] = {pf(prof, g.Ym, set, 1 + 3 * DIR, jp), pf(prof, g.Ym, set, 2 + 3 * DIR, jp),
-                 pf(prof, g.Ym, set, 3 + 3 * DIR, jp)};
-  double nt[4];
+  flux_face<DIR>(u, f, i, j, k, prof, F, g);
+}
+
+// Flux_X and Flux_Y with one thread per column of faces: lanes run along k
+// and each thread walks FC consecutive faces along the direction with cells
+// f-2 .. f+3 in registers, loading one new cell per face (and that one a
+// face ahead) where a thread per face loads up to six.
+#ifndef CF_FLUX_FC
+#define CF_FLUX_FC 16
+#endif
+constexpr int FC = CF_FLUX_FC;
+// at least 3 waves per SIMD: 168 VGPRs (4 would spill)
+#ifndef CF_FLUX_COL_WAVES
+#define CF_FLUX_COL_WAVES 3
+#endif
+
+template <int DIR>
+__global__ void __launch_bounds__(256, CF_FLUX_COL_WAVES) k_flux_col(const double* __restrict__ U, const double* __restrict__ prof,
+                                                  double* __restrict__ F, Geo g) {
+  static_assert(DIR == 0 || DIR == 1, "columns along x or y");
+  // DIR 0: f in [2,gend] (i), a = j in [2,ny]; DIR 1: f in [1,ny] (j), a = i in [3,gend]
+  const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int a = blockIdx.y * blockDim.y + threadIdx.y;
+  const int i = DIR == 0 ? 0 : 3 + a, j = DIR == 0 ? 2 + a : 0;
+  if (k > g.nz || (DIR == 0 ? j > g.ny : i > g.gend)) return;
+  const int fmin = DIR == 0 ? 2 : 1, fmax = DIR == 0 ? g.gend : g.ny;
+  const int f0 = fmin + blockIdx.z * FC, f1 = min(f0 + FC - 1, fmax);
+  // cells outside the array (j = -1 and j = ny + 3 next to the walls) are
+  // clamped onto it; no recipe of those faces reads them
+  const int cmax = DIR == 0 ? g.Xnp - 1 : g.Ym - 1;
+  auto cell = [&](int c) -> long {
+    c = min(max(c, 0), cmax);
+    return DIR == 0 ? at(g, c, j, k) : at(g, i, c, k);
+  };
+  double u[5][6];
 #pragma unroll
-  for (int c = 0; c < 4; c++) nt[c] = prow(prof, g.Ym, FT_NRM + 4 * DIR + c, jp);
-  double Jf = pf(prof, g.Ym, set, 0, jp);
-  bool wall = DIR == 1 && (f == 1 || f == g.ny);
-  const double rJf = prow(prof, g.Ym, FT_RCP + 2 * DIR, jp), rin = prow(prof, g.Ym, FT_RCP + 2 * DIR + 1, jp);
-  // a wave covers one face row, so this vote is one branch per wave
-  if (__all(rJf != 0.0 && rin != 0.0))
-    roe_flux<true>(L, R, n, nt, Jf, rJf, rin, DIR == 1, wall, out);
-  else
-    roe_flux<false>(L, R, n, nt, Jf, rJf, rin, DIR == 1, wall, out);
+  for (int t = 0; t < 6; t++) {
+    const long p = cell(f0 - 2 + t);
 #pragma unroll
-  for (int v = 0; v < 5; v++) F[v * g.nf + o] = out[v];
+    for (int v = 0; v < 5; v++) u[v][t] = U[v * g.nf + p];
+  }
+  for (int f = f0; f <= f1; f++) {
+    double nx[5];
+    const long p = cell(f + 4);
+#pragma unroll
+    for (int v = 0; v < 5; v++) nx[v] = U[v * g.nf + p];
+    flux_face<DIR>(u, f, DIR == 0 ? f : i, DIR == 0 ? j : f, k, prof, F, g);
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+#pragma unroll
+      for (int t = 0; t < 5; t++) u[v][t] = u[v][t + 1];
+      u[v][5] = nx[v];
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2175,9 +2240,11 @@ void launch_flux_tables(hipStream_t st, double* prof, const Geo& g) {
 void launch_flux(hipStream_t st, int dir, const double* U, const double* prof, double* F, const Geo& g) {
   dim3 blk(64, 4);
   if (dir == 0)
-    hipLaunchKernelGGL(k_flux<0>, box_grid(g.nz - 1, g.ny - 1, g.gend - 1, blk), blk, 0, st, U, prof, F, g);
+    hipLaunchKernelGGL(k_flux_col<0>, box_grid(g.nz - 1, g.ny - 1, (g.gend - 1 + FC - 1) / FC, blk), blk, 0, st, U,
+                       prof, F, g);
   else if (dir == 1)
-    hipLaunchKernelGGL(k_flux<1>, box_grid(g.nz - 1, g.ny, g.gend - 2, blk), blk, 0, st, U, prof, F, g);
+    hipLaunchKernelGGL(k_flux_col<1>, box_grid(g.nz - 1, g.gend - 2, (g.ny + FC - 1) / FC, blk), blk, 0, st, U, prof,
+                       F, g);
   else
     hipLaunchKernelGGL(k_flux<2>, box_grid(g.nz, g.ny - 1, g.gend - 2, blk), blk, 0, st, U, prof, F, g);
 }
