@@ -42,7 +42,7 @@ def stage_fields(n_sweeps):
 
 
 # stages that are one kernel per launch, and that kernel's name in rocprofv3
-STAGE_KERNEL = {"flux_x": "cf::k_flux_col<0>", "flux_y": "cf::k_flux_col<1>", "flux_z": "cf::k_flux<2>",
+STAGE_KERNEL = {"flux_x": "cf::k_flux_col<0>", "flux_y": "cf::k_flux_col<1>", "flux_z": "cf::k_flux_zs<2>",
                 "rhs": "cf::k_rhs",
                 "lusgs_fwd": "cf::k_sweep_pipe<true, 3, 2, double>", "lusgs_bwd": "cf::k_sweep_pipe<false, 3, 2, double>",
                 "sweep_prep": "cf::k_sweep_prep_rhs<double>"}

@@ -135,6 +135,38 @@ __global__ void __launch_bounds__(256) k_flux(const double* __restrict__ U, cons
   flux_face<DIR>(u, f, i, j, k, prof, F, g);
 }
 
+// Flux_Z: a block of 64 faces (lanes, along k) by 4 rows j stages each row's
+// cells f0-2 .. f0+66 in LDS once, and each thread reads its six stencil
+// cells from there (cells beyond the row, next to the periodic seam, are
+// clamped onto it; no recipe of those faces reads them)
+template <int DIR>
+__global__ void __launch_bounds__(256) k_flux_zs(const double* __restrict__ U, const double* __restrict__ prof,
+                                                 double* __restrict__ F, Geo g) {
+  static_assert(DIR == 2, "faces along k");
+  __shared__ double S[4][5][72];
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int f0 = 1 + blockIdx.x * 64;
+  const int j = 2 + blockIdx.y * 4 + ty, i = 3 + blockIdx.z;
+  const bool row = j <= g.ny;
+  for (int c = tx; c < 69; c += 64) {
+    const int kc = min(max(f0 - 2 + c, 0), g.Zm - 1);
+    if (row) {
+      const long p = at(g, i, j, kc);
+#pragma unroll
+      for (int v = 0; v < 5; v++) S[ty][v][c] = U[v * g.nf + p];
+    }
+  }
+  __syncthreads();
+  const int f = f0 + tx;
+  if (!row || f > g.nz) return;
+  double u[5][6];
+#pragma unroll
+  for (int t = 0; t < 6; t++)
+#pragma unroll
+    for (int v = 0; v < 5; v++) u[v][t] = S[ty][v][tx + t];
+  flux_face<DIR>(u, f, i, j, f, prof, F, g);
+}
+
 // Flux_X and Flux_Y with one thread per column of faces: lanes run along k
 // and each thread walks FC consecutive faces along the direction with cells
 // f-2 .. f+3 in registers, loading one new cell per face (and that one a
@@ -192,30 +224,12 @@ __global__ void __launch_bounds__(256, CF_FLUX_COL_WAVES) k_flux_col(const doubl
 // limiter fluxes, one thread per face (Flux_X_limiter.cpp:104-547,
 // Flux_Y_limiter.cpp:106-685, Flux_Z_limiter.cpp:113-710)
 // ---------------------------------------------------------------------------
+// the limiter face flux of face f along DIR of the column (i, j, k) from the
+// cells f-2 .. f+3 (u, with zero outside the array) and their J (jj)
 template <int DIR>
-__global__ void __launch_bounds__(256) k_flux_lim(const double* __restrict__ U, const double* __restrict__ Ep,
-                                                  const double* __restrict__ prof, double* __restrict__ F, Geo g) {
-  // faces and transverse cells as k_flux
-  int x = blockIdx.x * blockDim.x + threadIdx.x;
-  int y = blockIdx.y * blockDim.y + threadIdx.y;
-  int z = blockIdx.z;
-  int f, i, j, k;
-  if (DIR == 0) { k = 2 + x; j = 2 + y; f = 2 + z; i = f; if (k > g.nz || j > g.ny) return; }
-  else if (DIR == 1) { k = 2 + x; f = 1 + y; i = 3 + z; j = f; if (k > g.nz || f > g.ny) return; }
-  else { f = 1 + x; j = 2 + y; i = 3 + z; k = f; if (f > g.nz || j > g.ny) return; }
-  const long stride = DIR == 0 ? g.plane : (DIR == 1 ? (long)g.Zp : 1L);
-  const long base = at(g, i, j, k);  // cell f along the direction
-  // cells f-2 .. f+3 (t = 0..5); ghosts the reference never fills are zero
-  double u[5][6], jj[6];
-#pragma unroll
-  for (int t = 0; t < 6; t++) {
-    const int c = f + t - 2;
-    const bool in = DIR == 0 ? (c >= 0 && c < g.Xnp) : (DIR == 1 ? (c >= 0 && c < g.Ym) : (c >= 0 && c < g.Zm));
-    const long p = base + (long)(t - 2) * stride;
-    jj[t] = pf(prof, g.Ym, SET_C, 0, DIR == 1 ? min(max(c, 0), g.Ym - 1) : j);
-#pragma unroll
-    for (int v = 0; v < 5; v++) u[v][t] = in ? U[v * g.nf + p] : 0.0;
-  }
+__device__ __forceinline__ void lim_face(const double (&u)[5][6], const double (&jj)[6], int f, int i, int j, int k,
+                                         const double* __restrict__ Ep, const double* __restrict__ prof,
+                                         double* __restrict__ F, const Geo& g) {
   long o;
   int set, jp;
   if (DIR == 0) { o = at(g, f, j - 1, k - 1); set = SET_U; jp = j - 1; }
@@ -298,6 +312,80 @@ __global__ void __launch_bounds__(256) k_flux_lim(const double* __restrict__ U, 
   }
 #pragma unroll
   for (int v = 0; v < 5; v++) F[v * g.nf + o] = out[v];
+}
+
+
+template <int DIR>
+__global__ void __launch_bounds__(256) k_flux_lim(const double* __restrict__ U, const double* __restrict__ Ep,
+                                                  const double* __restrict__ prof, double* __restrict__ F, Geo g) {
+  // faces and transverse cells as k_flux
+  int x = blockIdx.x * blockDim.x + threadIdx.x;
+  int y = blockIdx.y * blockDim.y + threadIdx.y;
+  int z = blockIdx.z;
+  int f, i, j, k;
+  if (DIR == 0) { k = 2 + x; j = 2 + y; f = 2 + z; i = f; if (k > g.nz || j > g.ny) return; }
+  else if (DIR == 1) { k = 2 + x; f = 1 + y; i = 3 + z; j = f; if (k > g.nz || f > g.ny) return; }
+  else { f = 1 + x; j = 2 + y; i = 3 + z; k = f; if (f > g.nz || j > g.ny) return; }
+  const long stride = DIR == 0 ? g.plane : (DIR == 1 ? (long)g.Zp : 1L);
+  const long base = at(g, i, j, k);  // cell f along the direction
+  // cells f-2 .. f+3 (t = 0..5); ghosts the reference never fills are zero
+  double u[5][6], jj[6];
+#pragma unroll
+  for (int t = 0; t < 6; t++) {
+    const int c = f + t - 2;
+    const bool in = DIR == 0 ? (c >= 0 && c < g.Xnp) : (DIR == 1 ? (c >= 0 && c < g.Ym) : (c >= 0 && c < g.Zm));
+    const long p = base + (long)(t - 2) * stride;
+    jj[t] = pf(prof, g.Ym, SET_C, 0, DIR == 1 ? min(max(c, 0), g.Ym - 1) : j);
+#pragma unroll
+    for (int v = 0; v < 5; v++) u[v][t] = in ? U[v * g.nf + p] : 0.0;
+  }
+  lim_face<DIR>(u, jj, f, i, j, k, Ep, prof, F, g);
+}
+
+// cell c along DIR of the column as the limiter fluxes read it: U_, or zero
+// outside the array (ghosts the reference never fills), and its J
+template <int DIR>
+__device__ __forceinline__ void lim_cell(const double* __restrict__ U, const double* __restrict__ prof, const Geo& g,
+                                         int c, int i, int j, int k, double (&u)[5][6], double (&jj)[6], int t) {
+  const bool in = DIR == 0 ? (c >= 0 && c < g.Xnp) : (DIR == 1 ? (c >= 0 && c < g.Ym) : (c >= 0 && c < g.Zm));
+  const int cc = DIR == 0 ? min(max(c, 0), g.Xnp - 1) : (DIR == 1 ? min(max(c, 0), g.Ym - 1) : min(max(c, 0), g.Zm - 1));
+  const long p = DIR == 0 ? at(g, cc, j, k) : (DIR == 1 ? at(g, i, cc, k) : at(g, i, j, cc));
+  jj[t] = pf(prof, g.Ym, SET_C, 0, DIR == 1 ? cc : j);
+#pragma unroll
+  for (int v = 0; v < 5; v++) u[v][t] = in ? U[v * g.nf + p] : 0.0;
+}
+
+// Flux_X/Y_limiter by columns of FC faces, as k_flux_col (2 waves per SIMD:
+// the limiter face needs more registers than the Roe face)
+template <int DIR>
+__global__ void __launch_bounds__(256, 2) k_flux_lim_col(const double* __restrict__ U,
+                                                                         const double* __restrict__ Ep,
+                                                                         const double* __restrict__ prof,
+                                                                         double* __restrict__ F, Geo g) {
+  static_assert(DIR == 0 || DIR == 1, "columns along x or y");
+  const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int a = blockIdx.y * blockDim.y + threadIdx.y;
+  const int i = DIR == 0 ? 0 : 3 + a, j = DIR == 0 ? 2 + a : 0;
+  if (k > g.nz || (DIR == 0 ? j > g.ny : i > g.gend)) return;
+  const int fmin = DIR == 0 ? 2 : 1, fmax = DIR == 0 ? g.gend : g.ny;
+  const int f0 = fmin + blockIdx.z * FC, f1 = min(f0 + FC - 1, fmax);
+  double u[5][6], jj[6];
+#pragma unroll
+  for (int t = 0; t < 6; t++) lim_cell<DIR>(U, prof, g, f0 - 2 + t, i, j, k, u, jj, t);
+  for (int f = f0; f <= f1; f++) {
+    double nu[5][6], nj[6];
+    lim_cell<DIR>(U, prof, g, f + 4, i, j, k, nu, nj, 0);
+    lim_face<DIR>(u, jj, f, DIR == 0 ? f : i, DIR == 0 ? j : f, k, Ep, prof, F, g);
+#pragma unroll
+    for (int t = 0; t < 5; t++) {
+      jj[t] = jj[t + 1];
+#pragma unroll
+      for (int v = 0; v < 5; v++) u[v][t] = u[v][t + 1];
+    }
+    jj[5] = nj[0];
+#pragma unroll
+    for (int v = 0; v < 5; v++) u[v][5] = nu[v][0];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2246,16 +2334,18 @@ void launch_flux(hipStream_t st, int dir, const double* U, const double* prof, d
     hipLaunchKernelGGL(k_flux_col<1>, box_grid(g.nz - 1, g.gend - 2, (g.ny + FC - 1) / FC, blk), blk, 0, st, U, prof,
                        F, g);
   else
-    hipLaunchKernelGGL(k_flux<2>, box_grid(g.nz, g.ny - 1, g.gend - 2, blk), blk, 0, st, U, prof, F, g);
+    hipLaunchKernelGGL(k_flux_zs<2>, box_grid(g.nz, g.ny - 1, g.gend - 2, blk), blk, 0, st, U, prof, F, g);
 }
 
 void launch_flux_limiter(hipStream_t st, int dir, const double* U, const double* Ep, const double* prof, double* F,
                          const Geo& g) {
   dim3 blk(64, 4);
   if (dir == 0)
-    hipLaunchKernelGGL(k_flux_lim<0>, box_grid(g.nz - 1, g.ny - 1, g.gend - 1, blk), blk, 0, st, U, Ep, prof, F, g);
+    hipLaunchKernelGGL(k_flux_lim_col<0>, box_grid(g.nz - 1, g.ny - 1, (g.gend - 1 + FC - 1) / FC, blk), blk, 0, st, U,
+                       Ep, prof, F, g);
   else if (dir == 1)
-    hipLaunchKernelGGL(k_flux_lim<1>, box_grid(g.nz - 1, g.ny, g.gend - 2, blk), blk, 0, st, U, Ep, prof, F, g);
+    hipLaunchKernelGGL(k_flux_lim_col<1>, box_grid(g.nz - 1, g.gend - 2, (g.ny + FC - 1) / FC, blk), blk, 0, st, U, Ep,
+                       prof, F, g);
   else
     hipLaunchKernelGGL(k_flux_lim<2>, box_grid(g.nz, g.ny - 1, g.gend - 2, blk), blk, 0, st, U, Ep, prof, F, g);
 }
