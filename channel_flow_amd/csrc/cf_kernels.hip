@@ -1682,7 +1682,6 @@ __device__ void pipe_compute(PipeLds<T>& L, T* __restrict__ SRC, T* __restrict__
   const T sv = sentinel_value<T>();
   const float fXI = float(1.0 / g.dXI), fET = float(1.0 / g.dET), fZT = float(1.0 / g.dZT);
   T zN[5] = {0, 0, 0, 0, 0};
-  SplitCoefT<T> zc = {0, 0, 0, 0, 0, 0};
   int cs = 0, ps = 2;
   lds_barrier();  // step 0 inputs are in LDS
   for (int tau = 0; tau < nsteps; tau++) {
@@ -1703,6 +1702,10 @@ __device__ void pipe_compute(PipeLds<T>& L, T* __restrict__ SRC, T* __restrict__
         Ny[v] = L.N[nr][v][yiN];
       }
       const SplitCoefT<T> cx = coef_load(L.xc[ps], xi), cy = coef_load(L.yc[ps], yi);
+      // the z neighbour is this lane's cell of the previous step: its split
+      // Jacobian is still in that step's own slot (the loaders fill another)
+      const SplitCoefT<T> cz = {L.own[ps][PO_Z + 0][lane], L.own[ps][PO_Z + 1][lane], L.own[ps][PO_Z + 2][lane],
+                                L.own[ps][PO_Z + 3][lane], L.own[ps][PO_Z + 4][lane], L.own[ps][PO_Z + 5][lane]};
       T acc[3][5];
       diag_apply<0, FWD>(cx, Nx, acc[0]);
       diag_apply<1, FWD>(cy, Ny, acc[1]);
@@ -1711,12 +1714,9 @@ __device__ void pipe_compute(PipeLds<T>& L, T* __restrict__ SRC, T* __restrict__
         acc[0][v] = x_dom ? acc[0][v] : T(0);
         acc[1][v] = y_dom ? acc[1][v] : T(0);
       }
-      if (kk > 0) {
-        diag_apply<2, FWD>(zc, zN, acc[2]);
-      } else {
+      diag_apply<2, FWD>(cz, zN, acc[2]);
 #pragma unroll
-        for (int v = 0; v < 5; v++) acc[2][v] = 0;
-      }
+      for (int v = 0; v < 5; v++) acc[2][v] = kk > 0 ? acc[2][v] : T(0);
       T Lv[5];
 #pragma unroll
       for (int v = 0; v < 5; v++) {
@@ -1758,8 +1758,6 @@ __device__ void pipe_compute(PipeLds<T>& L, T* __restrict__ SRC, T* __restrict__
         L.N[nw][v][lane] = out[v];
         zN[v] = out[v];
       }
-      zc = SplitCoefT<T>{L.own[cs][PO_Z + 0][lane], L.own[cs][PO_Z + 1][lane], L.own[cs][PO_Z + 2][lane],
-                         L.own[cs][PO_Z + 3][lane], L.own[cs][PO_Z + 4][lane], L.own[cs][PO_Z + 5][lane]};
     }
     ps = cs;
     cs = cs == 2 ? 0 : cs + 1;
