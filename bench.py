@@ -66,6 +66,30 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def host_cores():
+    """Cores this process may run on: its CPU affinity set, capped by the
+    cgroup's CPU quota where one is set (cpu.max), with the CPU model."""
+    affinity = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = max(1, int(int(q) // int(period)))
+    except (OSError, ValueError):
+        pass
+    model = ""
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    cores = min(affinity, quota) if quota else affinity
+    return cores, {"model": model, "online_cpus": os.cpu_count(), "affinity_cpus": affinity,
+                   "cgroup_cpu_quota": quota, "threads_used": cores}
+
+
 def cpu_baseline(cfg, seconds_budget=30.0):
     """The reference solver (oracle/_ref, built from its own sources) timed on
     this host's cores for a few inner iterations of the same grid."""
@@ -73,7 +97,7 @@ def cpu_baseline(cfg, seconds_budget=30.0):
     if not exe.exists():
         return None
     from channel_flow_amd.config import FlowParams, poiseuille_state, write_initial_bin
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    threads, cpu_info = host_cores()
     nit = 3
     with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as td:
         Q = poiseuille_state(cfg, FlowParams(), seed=1, stream=0)
@@ -89,17 +113,11 @@ def cpu_baseline(cfg, seconds_budget=30.0):
         line = [ln for ln in r.stdout.splitlines() if ln.startswith("{")][-1]
         rec = json.loads(line)
     mcell = cfg.cells * rec["inner_iterations"] / rec["seconds"] / 1e6
-    cpu_model = ""
-    try:
-        for ln in open("/proc/cpuinfo"):
-            if ln.startswith("model name"):
-                cpu_model = ln.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
     return {"value": round(mcell, 4), "unit": "Mcell-iter/s", "cores": rec["threads"], "kind": "reference",
+            "host": cpu_info,
             "sample": f"{rec['inner_iterations']} inner iterations of the reference Dual_time_stepping path at "
-                      f"{GRID[0]}x{GRID[1]}x{GRID[2]} (OpenMP, {rec['seconds']:.1f} s, {cpu_model})"}
+                      f"{GRID[0]}x{GRID[1]}x{GRID[2]} (OpenMP, {rec['threads']} threads, OMP_PROC_BIND=close, "
+                      f"{rec['seconds']:.1f} s, {cpu_info['model']})"}
 
 
 def physical_steps(s, Q, steps=2, cap=800):

@@ -54,6 +54,8 @@ struct cf_ctx {
   bool fp32 = false;      // CF_FP32_SWEEPS: LU-SGS sweeps and split Jacobians in float
   int guess_order = 0;    // initial iterate of cf_physical_step: 0 U^n, 1 linear, 2 quadratic extrapolation
   double* Uqq = nullptr;  // U^{n-2} (5 fields), kept while guess_order is 2 (allocated on first use)
+  double* Hs = nullptr;   // 15 fields that stand in for U_, U, Uq while a host-array entry point runs on a
+                          // context that holds a device-resident state (allocated on first such call)
   bool uqq_valid = false; // Uqq holds the level before Uq
   long spin_limit = 1L << 22;
   double* partial = nullptr;
@@ -264,7 +266,7 @@ extern "C" void cf_destroy(cf_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   double* ptrs[] = {c->Ui, c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->MR,  c->NL,     c->NR,    c->Pr,
-                    c->VS, c->Ep, c->Fw, c->ypt, c->Sm, c->partial, c->er_dev, c->prof, c->stat, c->Uqq};
+                    c->VS, c->Ep, c->Fw, c->ypt, c->Sm, c->partial, c->er_dev, c->prof, c->stat, c->Uqq, c->Hs};
   for (double* q : ptrs)
     if (q) (void)hipFree(q);
   if (c->order) (void)hipFree(c->order);
@@ -416,9 +418,48 @@ extern "C" int cf_hyperplane(cf_ctx* c, int* counts, int capacity, int* nplanes)
 // ---------------------------------------------------------------------------
 // per-function entry points
 // ---------------------------------------------------------------------------
+// The per-function entry points upload the caller's arrays into the state
+// fields U_, U, Uq and run the kernels on them.  When the context also holds
+// a device-resident state (cf_load_state, cf_load_initial, a checkpoint),
+// these fields are swapped for the Hs stand-ins for the length of the call,
+// so that the resident iterate and levels come through unchanged.  The work
+// fields (fluxes, vF, MR, NL, NR, sensors) are rewritten by every inner
+// iteration and are shared.
+struct HostArrayScope {
+  cf_ctx* c;
+  double *Ui = nullptr, *Un = nullptr, *Uq = nullptr;
+  explicit HostArrayScope(cf_ctx* ctx) : c(ctx) {}
+  int enter() {
+    if (!c->have_state) return CF_OK;
+    const size_t nf = (size_t)c->g.nf;
+    if (!c->Hs) {
+      if (hipMalloc(&c->Hs, nf * 15 * sizeof(double)) != hipSuccess) {
+        c->Hs = nullptr;
+        return fail(c, CF_ERR_HIP, "out of device memory for the host-array state fields");
+      }
+      HIPCHK(c, hipMemsetAsync(c->Hs, 0, nf * 15 * sizeof(double), c->stream));
+    }
+    Ui = c->Ui;
+    Un = c->Un;
+    Uq = c->Uq;
+    c->Ui = c->Hs;
+    c->Un = c->Hs + 5 * nf;
+    c->Uq = c->Hs + 10 * nf;
+    return CF_OK;
+  }
+  ~HostArrayScope() {
+    if (!Ui) return;
+    c->Ui = Ui;
+    c->Un = Un;
+    c->Uq = Uq;
+  }
+};
+
 extern "C" int cf_boundary_conditions(cf_ctx* c, double* const U_[5], const double* J) {
   if (!c || !U_) return CF_ERR_ARG;
+  HostArrayScope hs(c);
   int r;
+  if ((r = hs.enter())) return r;
   if ((r = ingest(c, J, cf::SET_C, 0, "J")) || (r = upload_prof(c))) return r;
   if ((r = h2d_n(c, c->Ui, (const double* const*)U_, 5))) return r;
   cf::launch_halo_bc(c->stream, c->Ui, c->prof, c->g, c->p.x_periodic);
@@ -429,7 +470,9 @@ extern "C" int cf_boundary_conditions(cf_ctx* c, double* const U_[5], const doub
 static int flux_entry(cf_ctx* c, int dir, const double* const U_[5], const double* J, const double* Jf,
                       const double* const mf[9], double* const inF[5]) {
   if (!c || !U_ || !inF) return CF_ERR_ARG;
+  HostArrayScope hs(c);
   int r;
+  if ((r = hs.enter())) return r;
   const int set = dir == 0 ? cf::SET_U : dir == 1 ? cf::SET_V : cf::SET_W;
   if ((r = ingest(c, J, cf::SET_C, 0, "J")) || (r = ingest_set(c, set, Jf, mf)) || (r = upload_prof(c))) return r;
   if ((r = h2d_n(c, c->Ui, U_, 5))) return r;
@@ -457,7 +500,9 @@ extern "C" int cf_flux_z(cf_ctx* c, const double* const U_[5], const double* J, 
 static int flux_limiter_entry(cf_ctx* c, int dir, const double* const U_[5], const double* J, const double* Jf,
                               const double* const mf[9], const double* Ep, double* const inF[5]) {
   if (!c || !U_ || !inF || !Ep) return CF_ERR_ARG;
+  HostArrayScope hs(c);
   int r;
+  if ((r = hs.enter())) return r;
   const int set = dir == 0 ? cf::SET_U : dir == 1 ? cf::SET_V : cf::SET_W;
   if ((r = ingest(c, J, cf::SET_C, 0, "J")) || (r = ingest_set(c, set, Jf, mf)) || (r = upload_prof(c))) return r;
   if ((r = h2d_n(c, c->Ui, U_, 5))) return r;
@@ -525,7 +570,9 @@ extern "C" int cf_filter(cf_ctx* c, double Roe_criterion, double E_high, double 
                          const double* Y_point, double* EpX, double* EpY, double* EpZ, const double* J) {
   if (!c || !U_ || !Y_point || !EpX || !EpY || !EpZ) return CF_ERR_ARG;
   const Geo& g = c->g;
+  HostArrayScope hs(c);
   int r;
+  if ((r = hs.enter())) return r;
   if ((r = ingest(c, J, cf::SET_C, 0, "J")) || (r = upload_prof(c))) return r;
   // Y_point as read by the filter: j in [1, ny+1] over the interior i, k
   std::vector<double> yp(g.Ym, 0.0);
@@ -594,7 +641,9 @@ extern "C" int cf_viscous_terms(cf_ctx* c, const double* const U_[5], double* co
                                 const double* J_v, const double* const m_v[9], const double* J_w,
                                 const double* const m_w[9]) {
   if (!c || !U_ || !vF) return CF_ERR_ARG;
+  HostArrayScope hs(c);
   int r;
+  if ((r = hs.enter())) return r;
   if ((r = ingest(c, J, cf::SET_C, 0, "J")) || (r = ingest_set(c, cf::SET_U, J_u, m_u)) ||
       (r = ingest_set(c, cf::SET_V, J_v, m_v)) || (r = ingest_set(c, cf::SET_W, J_w, m_w)) || (r = upload_prof(c)))
     return r;
@@ -735,7 +784,9 @@ static int implicit_entry(cf_ctx* c, int solver, double deltaT, double er[5], do
                                      double* const NL[5]) {
   if (!c || !er || !U_ || !U || !Uq || !inFx || !inFy || !inFz || !vF) return fail(c, CF_ERR_ARG, "null argument");
   if (!(deltaT > 0)) return fail(c, CF_ERR_ARG, "deltaT must be positive");
+  HostArrayScope hs(c);
   int r;
+  if ((r = hs.enter())) return r;
   if ((r = ingest_set(c, cf::SET_C, J, m)) || (r = upload_prof(c))) return r;
   if ((r = h2d_n(c, c->Ui, (const double* const*)U_, 5)) || (r = h2d_n(c, c->Un, U, 5)) ||
       (r = h2d_n(c, c->Uq, Uq, 5)) || (r = h2d_n(c, c->Fx, inFx, 5)) || (r = h2d_n(c, c->Fy, inFy, 5)) ||
@@ -769,7 +820,9 @@ extern "C" int cf_rk3nodt(cf_ctx* c, int RK, double deltaT, double er[5], double
   if (RK < 1 || RK > 3) return fail(c, CF_ERR_ARG, "RK stage must be 1, 2 or 3");
   (void)er;  // RK3NODT.cpp leaves er1..er5 untouched
   (void)Uq;
+  HostArrayScope hs(c);
   int r;
+  if ((r = hs.enter())) return r;
   if ((r = ingest_set(c, cf::SET_C, J, m)) || (r = upload_prof(c))) return r;
   if ((r = h2d_n(c, c->Ui, (const double* const*)U_, 5)) || (r = h2d_n(c, c->Un, U, 5)) ||
       (r = h2d_n(c, c->Fx, inFx, 5)) || (r = h2d_n(c, c->Fy, inFy, 5)) || (r = h2d_n(c, c->Fz, inFz, 5)) ||
@@ -895,6 +948,24 @@ extern "C" int cf_advance_levels(cf_ctx* c) {
   return sync(c);
 }
 
+// HIP events created together and destroyed on every exit path
+struct EventSet {
+  std::vector<hipEvent_t> ev;
+  hipError_t create(size_t n) {
+    ev.reserve(n);
+    for (size_t q = 0; q < n; q++) {
+      hipEvent_t e;
+      hipError_t r = hipEventCreate(&e);
+      if (r != hipSuccess) return r;
+      ev.push_back(e);
+    }
+    return hipSuccess;
+  }
+  ~EventSet() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  }
+};
+
 extern "C" int cf_profile_inner_iteration(cf_ctx* c, double er[5], double stage_ms[CF_PROFILE_STAGES],
                                           int launches[CF_PROFILE_STAGES]) {
   if (!c || !er || !stage_ms || !launches) return CF_ERR_ARG;
@@ -904,8 +975,9 @@ extern "C" int cf_profile_inner_iteration(cf_ctx* c, double er[5], double stage_
   const int ns = c->p.n_sweeps;
   // ev[0..6]: stage ends up to the RHS; then the sweep marks; then the end
   const int nm = 1 + 3 * ns;
-  std::vector<hipEvent_t> ev(7 + nm + 1);
-  for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
+  EventSet es;
+  HIPCHK(c, es.create(7 + nm + 1));
+  std::vector<hipEvent_t>& ev = es.ev;
   hipEvent_t* mk = ev.data() + 7;
   HIPCHK(c, hipEventRecord(ev[0], c->stream));
   cf::launch_halo_bc(c->stream, c->Ui, c->prof, g, c->p.x_periodic);
@@ -950,7 +1022,6 @@ extern "C" int cf_profile_inner_iteration(cf_ctx* c, double er[5], double stage_
   const int nl[CF_PROFILE_STAGES] = {3, 1, 1, 1, viscous_fused(c) ? 1 : 3, fuse ? 0 : 1, tiled ? 1 : 0, tiled ? ns : 0, tiled ? ns : nplanes * ns,
                                      tiled ? ns : nplanes * ns, 2};
   for (int q = 0; q < CF_PROFILE_STAGES; q++) launches[q] = nl[q];
-  for (auto& e : ev) (void)hipEventDestroy(e);
   return CF_OK;
 }
 
@@ -962,8 +1033,9 @@ extern "C" int cf_profile_variants(cf_ctx* c, double Roe_criterion, double ms[CF
     if (hipMalloc(&c->Fw, (size_t)g.nf * cf::FILTER_WORK_FIELDS * sizeof(double)) != hipSuccess)
       return fail(c, CF_ERR_HIP, "hipMalloc of the filter workspace failed");
   }
-  hipEvent_t ev[CF_VARIANT_STAGES + 1];
-  for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
+  EventSet es;
+  HIPCHK(c, es.create(CF_VARIANT_STAGES + 1));
+  hipEvent_t* ev = es.ev.data();
   // on the resident iterate with its ghosts set, as in an inner iteration
   cf::launch_halo_bc(c->stream, c->Ui, c->prof, g, c->p.x_periodic);
   HIPCHK(c, hipEventRecord(ev[0], c->stream));
@@ -982,7 +1054,6 @@ extern "C" int cf_profile_variants(cf_ctx* c, double Roe_criterion, double ms[CF
     HIPCHK(c, hipEventElapsedTime(&t, ev[q], ev[q + 1]));
     ms[q] = t;
   }
-  for (auto& e : ev) (void)hipEventDestroy(e);
   return CF_OK;
 }
 
@@ -1125,7 +1196,9 @@ extern "C" int cf_driven_force(cf_ctx* c, double Q0, const double* U2_, const do
                                double* ER_Q) {
   if (!c || !Qall || !ER_Q) return fail(c, CF_ERR_ARG, "null argument");
   const Geo& g = c->g;
+  HostArrayScope hs(c);
   int r;
+  if (U2_ && (r = hs.enter())) return r;
   if (J && ((r = ingest(c, J, cf::SET_C, 0, "J")) || (r = upload_prof(c)))) return r;
   const double* U2 = c->Ui + g.nf;
   if (U2_) {
@@ -1156,8 +1229,10 @@ extern "C" int cf_statistic(cf_ctx* c, int step, int statistic_step, const doubl
                             const char* directory) {
   if (!c || statistic_step < 1) return fail(c, CF_ERR_ARG, "statistic_step must be >= 1");
   const Geo& g = c->g;
+  HostArrayScope hs(c);
   int r;
   if (U_) {
+    if ((r = hs.enter())) return r;
     if ((r = h2d_n(c, c->Ui, U_, 5))) return r;
   } else if (!c->have_state) {
     return fail(c, CF_ERR_STATE, "no state loaded");

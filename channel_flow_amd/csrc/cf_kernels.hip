@@ -170,18 +170,14 @@ __global__ void __launch_bounds__(256) k_flux_zs(const double* __restrict__ U, c
 // Flux_X and Flux_Y with one thread per column of faces: lanes run along k
 // and each thread walks FC consecutive faces along the direction with cells
 // f-2 .. f+3 in registers, loading one new cell per face (and that one a
-// face ahead) where a thread per face loads up to six.
-#ifndef CF_FLUX_FC
-#define CF_FLUX_FC 16
-#endif
-constexpr int FC = CF_FLUX_FC;
+// face ahead) where a thread per face loads up to six.  16 faces per
+// thread measured faster than 8 and equal to 32 at 256x192x256.
+constexpr int FC = 16;
 // at least 3 waves per SIMD: 168 VGPRs (4 would spill)
-#ifndef CF_FLUX_COL_WAVES
-#define CF_FLUX_COL_WAVES 3
-#endif
+constexpr int FLUX_COL_WAVES = 3;
 
 template <int DIR>
-__global__ void __launch_bounds__(256, CF_FLUX_COL_WAVES) k_flux_col(const double* __restrict__ U, const double* __restrict__ prof,
+__global__ void __launch_bounds__(256, FLUX_COL_WAVES) k_flux_col(const double* __restrict__ U, const double* __restrict__ prof,
                                                   double* __restrict__ F, Geo g) {
   static_assert(DIR == 0 || DIR == 1, "columns along x or y");
   // DIR 0: f in [2,gend] (i), a = j in [2,ny]; DIR 1: f in [1,ny] (j), a = i in [3,gend]
@@ -413,10 +409,9 @@ __device__ __forceinline__ bool in_box(const Geo& g, int i, int j, int k) {
 // from the stored ones, accumulated in the reference's order
 // (Filter.cpp:222-775), and from it the energy E = |u - u_f|^2
 // (Filter.cpp:790-800, 1465-1475).
-#ifndef CF_FILT_FI
-#define CF_FILT_FI 8
-#endif
-constexpr int FJ = 8, FK = 32, FI = CF_FILT_FI;  // FI planes per block
+// 8 planes per block measured fastest (1, 4, 16, 32: 5.28, 4.55, 4.78,
+// 5.35 ms against 4.53 ms for the whole Filter at 256x192x256)
+constexpr int FJ = 8, FK = 32, FI = 8;  // FI planes per block
 constexpr int FNP = (FJ + 2) * (FK + 2);  // tile and rim positions of a plane
 static_assert(FNP <= 512 && FJ * FK == 256, "two rim positions per thread, one tile cell per thread");
 
@@ -1613,14 +1608,10 @@ __global__ void __launch_bounds__(64) k_sweep_tiles(const double* __restrict__ S
 // neighbour entries of step s, with the halo in columns 64..71 (x: by b,
 // y: by a) so that every lane reads its neighbours at fixed offsets.
 // ---------------------------------------------------------------------------
-#ifndef CF_PIPE_QS
-#define CF_PIPE_QS 3
-#endif
-#ifndef CF_PIPE_QD
-#define CF_PIPE_QD 2
-#endif
-constexpr int PIPE_QS = CF_PIPE_QS;  // own-field lookahead (steps)
-constexpr int PIPE_QD = CF_PIPE_QD;  // halo lookahead (steps)
+// Lookaheads measured at 256x192x256 with one tile per CU: QD = 1 and 3
+// were slower than 2, QS = 2 and 4 equal to 3.
+constexpr int PIPE_QS = 3;  // own-field lookahead (steps)
+constexpr int PIPE_QD = 2;  // halo lookahead (steps)
 constexpr int PO_D11 = 0, PO_D22 = 1, PO_D51 = 2, PO_M = 3, PO_Z = 8, PO_N = 14;  // rows of PipeLds::own
 
 __device__ __forceinline__ void lds_barrier() {

@@ -122,7 +122,9 @@ def _p(a):
 def _pp(arrs):
     if arrs is None:
         return None
-    return (C.POINTER(C.c_double) * len(arrs))(*[_p(a) for a in arrs])
+    out = (C.POINTER(C.c_double) * len(arrs))(*[_p(a) for a in arrs])
+    out._keep = list(arrs)  # the pointer array alone does not keep the arrays alive
+    return out
 
 
 class Solver:
@@ -294,8 +296,12 @@ class Solver:
         """Statistic.cpp:1043-1154 mean profiles; writes st%05d.dat when
         step % statistic_step == 0.  U_: host arrays, or None for the
         device iterate."""
-        ptr = _pp([np.ascontiguousarray(a, dtype=np.float64) for a in U_]) if U_ is not None else None
+        # the converted arrays must outlive the call: the pointer array does
+        # not keep them alive
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in U_] if U_ is not None else None
+        ptr = _pp(arrs)
         self._chk(self.lib.cf_statistic(self._h, int(step), int(statistic_step), ptr, str(directory).encode()))
+        del arrs
 
     def Driven_force(self, Q0=20.464969232814912, U2_=None, J=None):
         """Driven_force.cpp:38-73: (Qall, ER_Q) of U2_ (host array) or of the
@@ -328,7 +334,11 @@ class Solver:
         self._chk(self.lib.cf_load_state(self._h, _pp(Qn), _pp(Qnm1)))
 
     def load_initial(self, path):
+        """Initial_condition.cpp:64-116: levels n and n-1 from an initial.bin /
+        LES%05d.bin file (Q = U J, ten raw FP64 blocks)."""
         self._chk(self.lib.cf_load_initial(self._h, str(path).encode()))
+
+    Initial_condition = load_initial
 
     def set_iterate(self, Q):
         Q = [np.ascontiguousarray(q, dtype=np.float64) for q in Q]

@@ -20,6 +20,13 @@
  * the context's one HIP stream and every host-array entry point is
  * synchronous.  No C++ exception crosses this boundary.
  *
+ * Host arrays and the device-resident state: an entry point that takes the
+ * state as host arrays (U_, U, Uq, U2_) works on stand-in device fields when
+ * the context also holds a resident state (cf_load_state / cf_load_initial /
+ * cf_load_checkpoint), so that state comes through unchanged.  The metrics
+ * such an entry point receives do replace the context's (they are what the
+ * next resident iteration uses).
+ *
  * Geometry: the reference metrics (Jacobian*.cpp) vary only with j.  Entry
  * points that receive 3-D metric arrays reduce them to j-profiles and return
  * CF_ERR_METRICS if a metric varies along i or k inside the region it is used.

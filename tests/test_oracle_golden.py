@@ -318,3 +318,27 @@ def test_driven_force_closed_form():
     U2[sl] = (prof[None, :, None] / J[sl])
     Q, _, _ = O.driven_force(cfg, U2, J)
     assert abs(Q - (prof * w).sum() / w.sum()) <= 1e-13
+
+
+@pytest.mark.parametrize("grid", [(16, 12, 16), (9, 10, 13)])
+def test_oracle_x_ghosts_closed_form(grid):
+    """main.cpp:205-240: with np >= 2 (x_periodic) ghost plane 2 holds plane
+    gend and gend+1 holds plane 3; with one rank the self-Sendrecv copies the
+    neighbours (plane 3 and plane gend).  The walls and the z ghosts of the
+    ghost planes follow from those planes' values, so whole planes agree."""
+    from channel_flow_amd.config import ChannelConfig
+    rng = np.random.default_rng(3)
+    for periodic in (True, False):
+        cfg = ChannelConfig(*grid, gamma_metric=1.5, x_periodic=periodic)
+        met = O.metrics(cfg)
+        U = [np.ascontiguousarray(1.0 + 0.1 * rng.standard_normal(cfg.shape)) for _ in range(5)]
+        U[4] += 2.5e5
+        O.halo_bc(cfg, U, met["J"])
+        g, ny, nz = cfg.gend, cfg.ny, cfg.nz
+        lo, hi = (g, 3) if periodic else (3, g)
+        reg = (slice(1, ny + 2), slice(1, nz + 2))
+        for v in range(5):
+            np.testing.assert_array_equal(U[v][2][reg], U[v][lo][reg])
+            np.testing.assert_array_equal(U[v][g + 1][reg], U[v][hi][reg])
+        # a planted mistake (source planes swapped) would break it
+        assert not np.array_equal(U[0][2][reg], U[0][hi][reg])
