@@ -11,6 +11,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -22,6 +23,40 @@ using cf::Geo;
 namespace {
 constexpr int NPROF = cf::PROF_METRIC_ROWS;  // 4 sets x (J + 9 metrics), uploaded from the host
 }
+
+// ---------------------------------------------------------------------------
+// Scratch of cf_driven_force_noalloc: per-i-plane partial sums of the bulk
+// flux in scratch[0, CF_NOALLOC_MAX_PLANES) and their in-order sum in
+// scratch[CF_NOALLOC_MAX_PLANES].  Each GPU holds its own copy of the array
+// until the process ends (4.3 KB).  Host code never passes `scratch` itself
+// to a kernel: its device address is looked up once per device (in
+// cf_create, with that device current) and kept in scratch_dev[device].
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[CF_NOALLOC_MAX_PLANES + 32];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+double* scratch_dev[MAX_DEVICES] = {};
+
+// device address of `scratch` on the current device `device`, looked up once
+hipError_t scratch_init(int device) {
+  if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  if (scratch_dev[device]) return hipSuccess;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  scratch_dev[device] = static_cast<double*>(p);
+  return hipSuccess;
+}
+
+double* scratch_of(int device) {
+  if (device < 0 || device >= MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  return scratch_dev[device];
+}
+}  // namespace
 
 struct cf_ctx {
   cf_params p{};
@@ -216,6 +251,10 @@ extern "C" int cf_create(const cf_params* p, cf_ctx** out) {
     return code;
   };
   if (hipGetDevice(&c->device) != hipSuccess) return bail(CF_ERR_HIP);
+  if (hipError_t e = scratch_init(c->device); e != hipSuccess) {
+    fprintf(stderr, "cf_create: hipGetSymbolAddress(scratch) on device %d: %s\n", c->device, hipGetErrorString(e));
+    return bail(CF_ERR_HIP);
+  }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(CF_ERR_HIP);
   size_t nf = (size_t)g.nf;
   struct {
@@ -1219,6 +1258,45 @@ extern "C" int cf_driven_force(cf_ctx* c, double Q0, const double* U2_, const do
   if ((r = sync(c))) return r;
   double Q = 0;
   for (int i = 0; i < X; i++) Q = Q + part[i];
+  Q = Q / c->p.X_out / c->p.Z_out;
+  *Qall = Q;
+  *ER_Q = fabs(Q0 - Q) / Q0;
+  return CF_OK;
+}
+
+// Driven_force.cpp:38-73 like cf_driven_force, with no device allocation:
+// the per-plane partials and their in-order sum live in `scratch`
+// (scratch_of(device)), the plane sum is formed on the device in the order
+// the host forms it in cf_driven_force, so the results are the same bits.
+extern "C" int cf_driven_force_noalloc(cf_ctx* c, double Q0, const double* U2_dev, double* Qall, double* ER_Q) {
+  if (!c || !Qall || !ER_Q) return fail(c, CF_ERR_ARG, "null argument");
+  const Geo& g = c->g;
+  const int X = g.gend - 2;
+  if (X > CF_NOALLOC_MAX_PLANES)
+    return fail(c, CF_ERR_ARG, "cf_driven_force_noalloc: X_out = %d exceeds the %d planes of its scratch", X,
+                CF_NOALLOC_MAX_PLANES);
+  double* scr = scratch_of(c->device);
+  if (!scr) return fail(c, CF_ERR_STATE, "cf_driven_force_noalloc: no scratch address for device %d", c->device);
+  Geo gg = g;
+  const double* U2 = c->Ui + g.nf;
+  if (U2_dev) {
+    // a reference-layout device array: rows of Z_m, no padding
+    gg.Zp = g.Zm;
+    gg.plane = (long)g.Ym * g.Zm;
+    gg.nf = (long)g.Xnp * gg.plane;
+    U2 = U2_dev;
+  } else if (!c->have_state) {
+    return fail(c, CF_ERR_STATE, "no state loaded");
+  }
+  double Jsum = 0;  // Driven_force.cpp:45-47, in the reference's order
+  for (int j = 2; j <= g.ny; j++) Jsum = Jsum + 1. / c->hprof[j];
+  cf::launch_bulk_flux(c->stream, U2, c->prof, Jsum, scr, gg);
+  cf::launch_sum_in_order(c->stream, scr, X, scr + CF_NOALLOC_MAX_PLANES);
+  HIPCHK(c, hipGetLastError());
+  double Q = 0;
+  HIPCHK(c, hipMemcpyAsync(&Q, scr + CF_NOALLOC_MAX_PLANES, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  int r;
+  if ((r = sync(c))) return r;
   Q = Q / c->p.X_out / c->p.Z_out;
   *Qall = Q;
   *ER_Q = fabs(Q0 - Q) / Q0;

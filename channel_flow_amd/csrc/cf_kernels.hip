@@ -2530,6 +2530,18 @@ void launch_bulk_flux(hipStream_t st, const double* U2, const double* prof, doub
   hipLaunchKernelGGL(k_bulk_flux, dim3(g.gend - 2), dim3(256), 0, st, U2, prof, Jsum, part, g);
 }
 
+// out[0] = ((0 + a[0]) + a[1]) + ... + a[n-1], one thread: the order of a host loop
+__global__ void k_sum_in_order(const double* __restrict__ a, int n, double* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s = 0;
+  for (int i = 0; i < n; i++) s = s + a[i];
+  out[0] = s;
+}
+
+void launch_sum_in_order(hipStream_t st, const double* a, int n, double* out) {
+  hipLaunchKernelGGL(k_sum_in_order, dim3(1), dim3(64), 0, st, a, n, out);
+}
+
 void launch_rhs(hipStream_t st, const double* Ui, const double* Un, const double* Uq, const double* Fx,
                 const double* Fy, const double* Fz, const double* vF, const double* prof, double* MR, double* NL,
                 const Geo& g, double deltaT, double f_drive) {

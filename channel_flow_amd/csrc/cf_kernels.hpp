@@ -30,6 +30,8 @@ void launch_stat(hipStream_t st, const double* U, double* sums, const Geo& g);
 // per-i-plane sums of the bulk flux integrand of Driven_force into part[X_out]
 void launch_bulk_flux(hipStream_t st, const double* U2, const double* prof, double Jsum, double* part,
                       const Geo& g);
+// out[0] = a[0] + a[1] + ... + a[n-1], added in index order by one thread
+void launch_sum_in_order(hipStream_t st, const double* a, int n, double* out);
 void launch_rhs(hipStream_t st, const double* Ui, const double* Un, const double* Uq, const double* Fx,
                 const double* Fy, const double* Fz, const double* vF, const double* prof, double* MR, double* NL,
                 const Geo& g, double deltaT, double f_drive);

@@ -35,6 +35,7 @@ EXPORTS = [
     "cf_flux_z_limiter", "cf_grid", "cf_filter", "cf_profile_variants", "cf_dplusgs", "cf_set_implicit_solver",
     "cf_set_viscous_schedule", "cf_rk3nodt", "cf_rk3_step",
     "cf_statistic", "cf_driven_force", "cf_save_checkpoint", "cf_load_checkpoint", "cf_set_initial_guess",
+    "cf_driven_force_noalloc",
 ]
 
 
@@ -103,6 +104,7 @@ def load_library() -> C.CDLL:
     lib.cf_rk3_step.argtypes = [vp, C.c_int]
     lib.cf_statistic.argtypes = [vp, C.c_int, C.c_int, PP, C.c_char_p]
     lib.cf_driven_force.argtypes = [vp, C.c_double, P, P, P, P]
+    lib.cf_driven_force_noalloc.argtypes = [vp, C.c_double, vp, P, P]
     lib.cf_save_checkpoint.argtypes = [vp, C.c_char_p]
     lib.cf_load_checkpoint.argtypes = [vp, C.c_char_p]
     lib.cf_write_output.argtypes = [vp, C.c_int, C.c_char_p]
@@ -310,6 +312,14 @@ class Solver:
         u = _p(np.ascontiguousarray(U2_, dtype=np.float64)) if U2_ is not None else None
         jj = _p(np.ascontiguousarray(J, dtype=np.float64)) if J is not None else None
         self._chk(self.lib.cf_driven_force(self._h, float(Q0), u, jj, C.byref(Q), C.byref(E)))
+        return Q.value, E.value
+
+    def Driven_force_noalloc(self, Q0=20.464969232814912, U2_dev=None):
+        """cf_driven_force_noalloc: Driven_force with no device allocation.
+        U2_dev: device address (int) of a reference-layout rho u field, or
+        None for the resident iterate."""
+        Q, E = C.c_double(), C.c_double()
+        self._chk(self.lib.cf_driven_force_noalloc(self._h, float(Q0), U2_dev, C.byref(Q), C.byref(E)))
         return Q.value, E.value
 
     def save_checkpoint(self, path):

@@ -259,6 +259,19 @@ int cf_rk3_step(cf_ctx* ctx, int viscous);
  * context's.  The sum is formed per i-plane in a fixed tree and the planes
  * are added in order: Qall equals the reference's to rounding. */
 int cf_driven_force(cf_ctx* ctx, double Q0, const double* U2_, const double* J, double* Qall, double* ER_Q);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  The bulk flux
+ * of cf_driven_force with the context's metrics, and no device allocation
+ * of any kind: the per-plane partial sums and their sum live in a
+ * file-scope device array of CF_NOALLOC_MAX_PLANES + 32 doubles that each
+ * GPU holds until the process ends.  U2_dev: a DEVICE pointer to the rho u
+ * field in the reference layout [X_np][Y_m][Z_m], or NULL for the resident
+ * iterate.  The results are the same bits as cf_driven_force's.  X_out
+ * above CF_NOALLOC_MAX_PLANES is refused (CF_ERR_ARG).  All calls on one GPU
+ * share that array: calls that overlap on one GPU (other threads, other
+ * contexts' streams) race. */
+#define CF_NOALLOC_MAX_PLANES 512
+int cf_driven_force_noalloc(cf_ctx* ctx, double Q0, const double* U2_dev, double* Qall, double* ER_Q);
 /* Statistic.cpp:1043-1154 (main.h Statistic), the mean profiles: adds the
  * per-j sums over i and k of U, V, W, UU, VV, WW, UV (U = U2_/U1_ ...) of
  * the state to running sums kept in the context; when step % statistic_step
