@@ -13,7 +13,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libchannel_flow.so"
-SOURCES = ["cf_kernels.hip", "cf_api.hip"]
+SOURCES = ["cf_kernels.hip", "cf_api.hip", "cf_predictor.hip", "cf_pred.hip"]
 
 # -ffp-contract=off: one rounding per FP64 operation, in source order, so the
 # device arithmetic matches the reference's; packed FP32 instructions are

@@ -17,6 +17,7 @@
 
 #include "../../include/channel_flow.h"
 #include "cf_kernels.hpp"
+#include "cf_pred_internal.hpp"
 
 using cf::Geo;
 
@@ -92,10 +93,41 @@ struct cf_ctx {
   double* Hs = nullptr;   // 15 fields that stand in for U_, U, Uq while a host-array entry point runs on a
                           // context that holds a device-resident state (allocated on first such call)
   bool uqq_valid = false; // Uqq holds the level before Uq
+  double* U3 = nullptr;   // U^{n-3} (5 fields), kept while four levels are kept (cf_keep_levels)
+  bool u3_valid = false;  // U3 holds the level before Uqq
+  // learned initial iterate (cf_set_predictor; guess_order 3)
+  cf_pred* pred = nullptr;
+  unsigned short* px = nullptr;  // features of the whole domain (bf16, 16 per cell)
+  float* py = nullptr;           // network outputs (5 per cell)
+  double* Ualt = nullptr;        // the learned guess after its first inner iteration (5 fields)
+  long pred_steps = 0, pred_fallbacks = 0;
+  // LU-SGS sweep pairs executed: one for any n_sweeps >= 1 while sweep_reuse
+  // is on (a single domain's later pairs reproduce the first bit for bit)
+  bool sweep_reuse = true;
+  // hipGraph of one inner iteration's kernels (cf_set_graph), rebuilt when
+  // what it was captured with changes
+  bool use_graph = false;
+  hipGraphExec_t gexec = nullptr;
+  struct GraphKey {
+    const double *Ui, *Un, *Uq;
+    int schedule, visc_schedule, solver, x_periodic, pairs;
+    bool fp32;
+    double deltaT, f_drive;
+    bool operator==(const GraphKey& o) const {
+      return Ui == o.Ui && Un == o.Un && Uq == o.Uq && schedule == o.schedule && visc_schedule == o.visc_schedule &&
+             solver == o.solver && x_periodic == o.x_periodic && pairs == o.pairs && fp32 == o.fp32 &&
+             deltaT == o.deltaT && f_drive == o.f_drive;
+    }
+  } gkey{};
+  long graph_builds = 0;
   long spin_limit = 1L << 22;
   double* partial = nullptr;
   double* er_dev = nullptr;
   double* stat = nullptr;  // running per-j sums of Statistic's mean profiles (7 * Ym)
+  double* shear = nullptr; // wall values of Statistic's y-face viscous flux (2 * X_out * Z_out)
+  // Shear_stress.dat: running sums SXYm[1], SXYm[ny] and the previous running
+  // averages SXY_bottom, SXY_up (Statistic.cpp:92-97, 879-911)
+  double sxy_sum[2] = {0, 0}, sxy_prev[2] = {0, 0};
   double* prof = nullptr;  // metric j-profiles (NPROF * Ym)
   std::vector<double> hprof;
   std::vector<double> staging;  // reference-layout staging for 2D copies
@@ -290,6 +322,7 @@ extern "C" int cf_create(const cf_params* p, cf_ctx** out) {
   if (hipMalloc(&c->partial, (size_t)nb * 5 * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
   if (hipMalloc(&c->er_dev, 8 * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
   if (hipMalloc(&c->stat, (size_t)7 * g.Ym * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
+  if (hipMalloc(&c->shear, (size_t)2 * p->X_out * p->Z_out * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
   if (hipMemsetAsync(c->stat, 0, (size_t)7 * g.Ym * sizeof(double), c->stream) != hipSuccess) return bail(CF_ERR_HIP);
   if (hipMalloc(&c->prof, (size_t)cf::PROF_ROWS * g.Ym * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
   if (hipMalloc(&c->ypt, (size_t)g.Ym * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
@@ -304,11 +337,14 @@ extern "C" int cf_create(const cf_params* p, cf_ctx** out) {
 extern "C" void cf_destroy(cf_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   double* ptrs[] = {c->Ui, c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->MR,  c->NL,     c->NR,    c->Pr,
-                    c->VS, c->Ep, c->Fw, c->ypt, c->Sm, c->partial, c->er_dev, c->prof, c->stat, c->Uqq, c->Hs};
+                    c->VS, c->Ep, c->Fw, c->ypt, c->Sm, c->partial, c->er_dev, c->prof, c->stat, c->Uqq, c->Hs, c->U3, c->Ualt, c->shear};
   for (double* q : ptrs)
     if (q) (void)hipFree(q);
   if (c->order) (void)hipFree(c->order);
+  if (c->px) (void)hipFree(c->px);
+  if (c->py) (void)hipFree(c->py);
   if (c->sync) (void)hipFree(c->sync);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -711,15 +747,24 @@ static int effective_schedule(const cf_ctx* c) { return (c->schedule != 1 && dia
 // LU-SGS forward/backward sweeps (Dual_time_stepping.cpp:514-1277)
 // fuse_rhs: the tiled sweeps' pre-pass forms MR itself (the RHS kernel is
 // not launched and MR is not written in the natural layout)
+// Sweep pairs to execute.  Dual_time_stepping.cpp:514-1277 runs n_sweeps
+// forward/backward pairs with NL <- NR between them; in one domain the
+// forward sweep reads only MR and values its own pass has already formed
+// (upstream neighbours in hyperplane order, zero ghosts), and the backward
+// sweep only that forward result and its own pass's values, so every later
+// pair reproduces the first bit for bit.  Across MPI ranks the pairs differ
+// (lagged neighbour planes, :521-559); this library holds one domain.
+static int sweep_pairs(const cf_ctx* c) { return c->sweep_reuse ? std::min(c->p.n_sweeps, 1) : c->p.n_sweeps; }
+
 static void sweeps(cf_ctx* c, double deltaT, const hipEvent_t* marks = nullptr, bool fuse_rhs = false) {
   const Geo& g = c->g;
   if (effective_schedule(c) == 1) {
-    cf::launch_sweeps_planes(c->stream, c->Ui, c->MR, c->NL, c->NR, c->prof, g, deltaT, c->p.n_sweeps, marks);
+    cf::launch_sweeps_planes(c->stream, c->Ui, c->MR, c->NL, c->NR, c->prof, g, deltaT, sweep_pairs(c), marks);
     return;
   }
   double DN = 1. / ((double)c->p.X_out * c->p.Y_out * c->p.Z_out);
   const cf::RhsIn rin{c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->p.f_drive};
-  cf::launch_sweeps(c->stream, c->Ui, c->MR, c->Sm, c->prof, g, deltaT, c->p.n_sweeps, c->order, c->sync,
+  cf::launch_sweeps(c->stream, c->Ui, c->MR, c->Sm, c->prof, g, deltaT, sweep_pairs(c), c->order, c->sync,
                     c->partial, c->er_dev, DN, c->spin_limit, marks, c->schedule != 2, fuse_rhs ? &rin : nullptr,
                     c->fp32);
 }
@@ -751,7 +796,7 @@ static int finish_iteration(cf_ctx* c, double er[5]) {
   HIPCHK(c, hipMemcpyAsync(&flags[1], c->sync + c->ntiles + 3, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   int r = sync(c);
   if (r) return r;
-  if (effective_schedule(c) == 0 && (flags[0] || flags[1]))
+  if (c->solver == 0 && effective_schedule(c) == 0 && (flags[0] || flags[1]))
     return fail(c, CF_ERR_TIMEOUT, "LU-SGS sweep: a tile waited past its bound for an upstream tile");
   for (int v = 0; v < 5; v++)
     if (!std::isfinite(er[v])) return fail(c, CF_ERR_NONFINITE, "residual norm er%d is not finite", v + 1);
@@ -921,6 +966,7 @@ extern "C" int cf_load_state(cf_ctx* c, const double* const Qn[5], const double*
   for (int v = 0; v < 5; v++) c->E[v] = 0;
   c->have_state = true;
   c->uqq_valid = false;
+  c->u3_valid = false;
   return CF_OK;
 }
 
@@ -961,16 +1007,105 @@ extern "C" int cf_read_state(cf_ctx* c, double* const Qn[5], double* const Qnm1[
   return CF_OK;
 }
 
-extern "C" int cf_inner_iteration(cf_ctx* c, double er[5]) {
-  if (!c || !er) return CF_ERR_ARG;
-  if (!c->have_state) return fail(c, CF_ERR_STATE, "no state loaded");
+// Everything one inner iteration of the device state launches: kernels
+// only, no copies, no memsets and no synchronisation, so that the sequence
+// can be captured into a graph.  The checks and allocations it could need
+// happen first (prepare_iteration).
+static int prepare_iteration(cf_ctx* c) {
+  if (!viscous_fused(c))
+    if (int r = ensure_viscous_staging(c)) return r;
+  if (c->solver == 0)
+    if (int r = check_precision(c)) return r;
+  return CF_OK;
+}
+
+static void enqueue_iteration(cf_ctx* c) {
   const Geo& g = c->g;
   cf::launch_halo_bc(c->stream, c->Ui, c->prof, g, c->p.x_periodic);
   cf::launch_flux(c->stream, 0, c->Ui, c->prof, c->Fx, g);
   cf::launch_flux(c->stream, 1, c->Ui, c->prof, c->Fy, g);
   cf::launch_flux(c->stream, 2, c->Ui, c->prof, c->Fz, g);
-  if (int rv = run_viscous(c)) return rv;
-  return dual_time_device(c, c->p.deltaT, er, false);
+  cf::launch_viscous(c->stream, c->Ui, c->prof, c->Pr, c->VS, c->vF, g, viscous_fused(c));
+  if (c->solver == 1) {
+    // DPLUSGS.cpp:189-1361: the RHS, four Jacobi sweeps, the update and norms
+    cf::launch_rhs(c->stream, c->Ui, c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->prof, c->MR, c->NL, g,
+                   c->p.deltaT, c->p.f_drive);
+    cf::launch_dplusgs(c->stream, c->Ui, c->MR, c->NL, c->NR, c->prof, g, c->p.deltaT);
+    const double DN = 1. / ((double)c->p.X_out * c->p.Y_out * c->p.Z_out);
+    cf::launch_update(c->stream, c->Ui, c->NR, c->prof, c->partial, c->er_dev, g, DN);
+    return;
+  }
+  const bool fuse = rhs_fused(c, false);
+  if (!fuse)
+    cf::launch_rhs(c->stream, c->Ui, c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->prof, c->MR, nullptr, g,
+                   c->p.deltaT, c->p.f_drive);
+  sweeps(c, c->p.deltaT, nullptr, fuse);
+  update(c);
+}
+
+// the iteration's kernels as one graph launch, captured on first use and
+// again whenever the fields, schedules or parameters it was captured with
+// have changed
+static int launch_graph(cf_ctx* c) {
+  const cf_ctx::GraphKey key{c->Ui,         c->Un,        c->Uq,        c->schedule, c->visc_schedule,
+                             c->solver,     c->p.x_periodic, sweep_pairs(c), c->fp32,  c->p.deltaT,
+                             c->p.f_drive};
+  if (!c->gexec || !(key == c->gkey)) {
+    if (c->gexec) {
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      (void)hipGraphExecDestroy(c->gexec);
+      c->gexec = nullptr;
+    }
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_iteration(c);
+    hipGraph_t graph = nullptr;
+    HIPCHK(c, hipStreamEndCapture(c->stream, &graph));
+    hipError_t e = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      c->gexec = nullptr;
+      return fail(c, CF_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    c->gkey = key;
+    c->graph_builds++;
+  }
+  HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+  return CF_OK;
+}
+
+extern "C" int cf_inner_iteration(cf_ctx* c, double er[5]) {
+  if (!c || !er) return CF_ERR_ARG;
+  if (!c->have_state) return fail(c, CF_ERR_STATE, "no state loaded");
+  if (int r = prepare_iteration(c)) return r;
+  if (c->use_graph) {
+    if (int r = launch_graph(c)) return r;
+  } else {
+    enqueue_iteration(c);
+  }
+  return finish_iteration(c, er);
+}
+
+extern "C" int cf_set_graph(cf_ctx* c, int enabled) {
+  if (!c) return CF_ERR_ARG;
+  c->use_graph = enabled != 0;
+  if (!c->use_graph && c->gexec) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+  return CF_OK;
+}
+
+extern "C" int cf_graph_builds(cf_ctx* c, long long* builds) {
+  if (!c || !builds) return CF_ERR_ARG;
+  *builds = c->graph_builds;
+  return CF_OK;
+}
+
+extern "C" int cf_set_sweep_reuse(cf_ctx* c, int enabled) {
+  if (!c) return CF_ERR_ARG;
+  c->sweep_reuse = enabled != 0;
+  return CF_OK;
 }
 
 extern "C" int cf_advance_levels(cf_ctx* c) {
@@ -978,6 +1113,10 @@ extern "C" int cf_advance_levels(cf_ctx* c) {
   if (!c->have_state) return fail(c, CF_ERR_STATE, "no state loaded");
   // U^{n-1} <- U^n, U^n <- U_ on the interior (main.cpp:444-464); U^{n-2}
   // <- U^{n-1} first while the quadratic initial guess keeps it
+  if (c->U3) {
+    cf::launch_copy_interior(c->stream, c->Uqq, c->U3, c->g);
+    c->u3_valid = c->uqq_valid;
+  }
   if (c->Uqq) {
     cf::launch_copy_interior(c->stream, c->Uq, c->Uqq, c->g);
     c->uqq_valid = true;
@@ -1011,7 +1150,7 @@ extern "C" int cf_profile_inner_iteration(cf_ctx* c, double er[5], double stage_
   if (!c->have_state) return fail(c, CF_ERR_STATE, "no state loaded");
   if (int r = check_precision(c)) return r;
   const Geo& g = c->g;
-  const int ns = c->p.n_sweeps;
+  const int ns = sweep_pairs(c);
   // ev[0..6]: stage ends up to the RHS; then the sweep marks; then the end
   const int nm = 1 + 3 * ns;
   EventSet es;
@@ -1124,7 +1263,8 @@ extern "C" int cf_rk3_step(cf_ctx* c, int viscous) {
 // exact checkpoint: the device state as stored (U = Q/J), the running maxima
 // of er and the statistic sums, so that a restarted run continues bit for bit
 // ---------------------------------------------------------------------------
-static const char CKPT_MAGIC[8] = {'C', 'F', 'C', 'K', 'P', 'T', '0', '1'};
+static const char CKPT_MAGIC[8] = {'C', 'F', 'C', 'K', 'P', 'T', '0', '2'};
+static const char CKPT_MAGIC_V1[8] = {'C', 'F', 'C', 'K', 'P', 'T', '0', '1'};  // no Shear_stress.dat sums
 
 static int ckpt_field(cf_ctx* c, FILE* f, double* dev, bool write) {
   const Geo& g = c->g;
@@ -1158,8 +1298,9 @@ extern "C" int cf_save_checkpoint(cf_ctx* c, const char* path) {
   const int32_t dims[4] = {c->p.X_out, c->p.Y_out, c->p.Z_out, nextra};
   std::vector<double> st((size_t)7 * c->g.Ym);
   HIPCHK(c, hipMemcpy(st.data(), c->stat, st.size() * 8, hipMemcpyDeviceToHost));
+  const double sxy[4] = {c->sxy_sum[0], c->sxy_sum[1], c->sxy_prev[0], c->sxy_prev[1]};
   bool ok = fwrite(CKPT_MAGIC, 1, 8, f) == 8 && fwrite(dims, 4, 4, f) == 4 && fwrite(c->E, 8, 5, f) == 5 &&
-            fwrite(st.data(), 8, st.size(), f) == st.size();
+            fwrite(st.data(), 8, st.size(), f) == st.size() && fwrite(sxy, 8, 4, f) == 4;
   double* fields[20];
   for (int v = 0; v < 5; v++) {
     fields[v] = c->Ui + v * c->g.nf;
@@ -1186,13 +1327,16 @@ extern "C" int cf_load_checkpoint(cf_ctx* c, const char* path) {
   int32_t dims[4];
   double E[5];
   std::vector<double> st((size_t)7 * c->g.Ym);
-  bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, CKPT_MAGIC, 8) == 0 && fread(dims, 4, 4, f) == 4;
+  bool ok = fread(magic, 1, 8, f) == 8 && (memcmp(magic, CKPT_MAGIC, 8) == 0 || memcmp(magic, CKPT_MAGIC_V1, 8) == 0) &&
+            fread(dims, 4, 4, f) == 4;
+  const bool v1 = memcmp(magic, CKPT_MAGIC_V1, 8) == 0;
   if (!ok || dims[0] != c->p.X_out || dims[1] != c->p.Y_out || dims[2] != c->p.Z_out ||
       (dims[3] != 0 && dims[3] != 5)) {
     fclose(f);
     return fail(c, CF_ERR_IO, "%s is not a checkpoint of this grid", path);
   }
-  if (fread(E, 8, 5, f) != 5 || fread(st.data(), 8, st.size(), f) != st.size()) {
+  double sxy[4] = {0, 0, 0, 0};
+  if (fread(E, 8, 5, f) != 5 || fread(st.data(), 8, st.size(), f) != st.size() || (!v1 && fread(sxy, 8, 4, f) != 4)) {
     fclose(f);
     return fail(c, CF_ERR_IO, "short checkpoint");
   }
@@ -1226,8 +1370,13 @@ extern "C" int cf_load_checkpoint(cf_ctx* c, const char* path) {
   fclose(f);
   HIPCHK(c, hipMemcpy(c->stat, st.data(), st.size() * 8, hipMemcpyHostToDevice));
   for (int v = 0; v < 5; v++) c->E[v] = E[v];
+  c->sxy_sum[0] = sxy[0];
+  c->sxy_sum[1] = sxy[1];
+  c->sxy_prev[0] = sxy[2];
+  c->sxy_prev[1] = sxy[3];
   c->have_state = true;
   c->uqq_valid = nextra == 5;
+  c->u3_valid = false;
   return CF_OK;
 }
 
@@ -1316,8 +1465,35 @@ extern "C" int cf_statistic(cf_ctx* c, int step, int statistic_step, const doubl
     return fail(c, CF_ERR_STATE, "no state loaded");
   }
   cf::launch_stat(c->stream, c->Ui, c->stat, g);
+  // Statistic.cpp:552-911: the wall values of the y-face viscous flux,
+  // added to the running sums in the reference's order (i, then k), and
+  // the change of their running averages appended to Shear_stress.dat
+  cf::launch_wall_shear(c->stream, c->Ui, c->prof, g, c->shear);
   HIPCHK(c, hipGetLastError());
-  if (step % statistic_step != 0) return sync(c);
+  const size_t XZ = (size_t)c->p.X_out * c->p.Z_out;
+  std::vector<double> wall(2 * XZ);
+  HIPCHK(c, hipMemcpyAsync(wall.data(), c->shear, wall.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  if ((r = sync(c))) return r;
+  {
+    double avg[2];
+    for (int w = 0; w < 2; w++) {
+      double acc = c->sxy_sum[w];
+      for (size_t q = 0; q < XZ; q++) acc = acc + wall[w * XZ + q];
+      c->sxy_sum[w] = acc;
+      avg[w] = acc / c->p.X_out / c->p.Z_out;
+    }
+    char spath[4096];
+    snprintf(spath, sizeof spath, "%s/Shear_stress.dat", directory ? directory : ".");
+    FILE* sf = fopen(spath, "a");
+    if (!sf) return fail(c, CF_ERR_IO, "cannot open %s", spath);
+    fprintf(sf, "%f\t%f\n", avg[0] - c->sxy_prev[0], avg[1] - c->sxy_prev[1]);
+    if (fclose(sf) != 0) return fail(c, CF_ERR_IO, "cannot write %s", spath);
+    c->sxy_prev[0] = avg[0];
+    c->sxy_prev[1] = avg[1];
+  }
+  if (step % statistic_step != 0) return CF_OK;
+  // Statistic.cpp:1008-1018: the statistic step resets the wall sums
+  c->sxy_sum[0] = c->sxy_sum[1] = c->sxy_prev[0] = c->sxy_prev[1] = 0;
   std::vector<double> h((size_t)7 * g.Ym);
   HIPCHK(c, hipMemcpyAsync(h.data(), c->stat, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemsetAsync(c->stat, 0, h.size() * 8, c->stream));
@@ -1348,9 +1524,35 @@ extern "C" int cf_set_sweep_schedule(cf_ctx* c, int schedule) {
   return CF_OK;
 }
 
+static int ensure_level(cf_ctx* c, double** level) {
+  if (*level) return CF_OK;
+  const size_t bytes = (size_t)c->g.nf * 5 * sizeof(double);
+  if (hipMalloc(level, bytes) != hipSuccess) {
+    *level = nullptr;
+    return fail(c, CF_ERR_HIP, "hipMalloc of a time level failed");
+  }
+  HIPCHK(c, hipMemsetAsync(*level, 0, bytes, c->stream));
+  return CF_OK;
+}
+
+extern "C" int cf_keep_levels(cf_ctx* c, int levels) {
+  if (!c || levels < 2 || levels > 4) return fail(c, CF_ERR_ARG, "levels must be 2, 3 or 4");
+  int r;
+  if (levels >= 3 && !c->Uqq) {
+    if ((r = ensure_level(c, &c->Uqq))) return r;
+    c->uqq_valid = false;
+  }
+  if (levels >= 4 && !c->U3) {
+    if ((r = ensure_level(c, &c->U3))) return r;
+    c->u3_valid = false;
+  }
+  return sync(c);
+}
+
 extern "C" int cf_set_initial_guess(cf_ctx* c, int order) {
-  if (!c || order < 0 || order > 2) return fail(c, CF_ERR_ARG, "initial-guess order must be 0, 1 or 2");
-  if (order == 2 && !c->Uqq) {
+  if (!c || order < 0 || order > 3) return fail(c, CF_ERR_ARG, "initial-guess order must be 0, 1, 2 or 3");
+  if (order == 3 && !c->pred) return fail(c, CF_ERR_STATE, "no predictor attached (cf_set_predictor)");
+  if (order >= 2 && !c->Uqq) {
     const size_t bytes = (size_t)c->g.nf * 5 * sizeof(double);
     if (hipMalloc(&c->Uqq, bytes) != hipSuccess) {
       c->Uqq = nullptr;
@@ -1363,17 +1565,56 @@ extern "C" int cf_set_initial_guess(cf_ctx* c, int order) {
   return sync(c);
 }
 
+// The learned initial iterate with its fallback: the network's guess and the
+// quadratic extrapolation each take one inner iteration; the step continues
+// from the one whose first residual is smaller in every er1..er4 ratio taken
+// together (max_k er_k(net) / er_k(extrapolated) <= 1 keeps the network's),
+// and both iterations count.
+static int learned_start(cf_ctx* c, double er[5]) {
+  const Geo& g = c->g;
+  const cfp::Norm& nm = c->pred->norm;
+  cfp::launch_feat(c->stream, c->Un, c->Uq, c->Uqq, c->prof, g, nm, c->px);
+  const cfp::Vol v = cfp::make_vol(1, g.gend - 2, g.ny - 1, g.nz - 1, 0);
+  cfp::launch_fwd(c->stream, v, c->px, true, c->pred->W1b, c->pred->W2b, c->pred->P, c->py, false);
+  cfp::launch_apply(c->stream, c->Ui, c->Un, c->Uq, c->Uqq, c->prof, g, nm, c->py);
+  HIPCHK(c, hipGetLastError());
+  double er_net[5], er_ext[5];
+  int r = cf_inner_iteration(c, er_net);
+  if (r) return r;
+  cf::launch_copy_interior(c->stream, c->Ui, c->Ualt, g);
+  cf::launch_extrapolate(c->stream, c->Ui, c->Un, c->Uq, c->Uqq, g, 2);
+  if ((r = cf_inner_iteration(c, er_ext))) return r;
+  double worst = 0;
+  for (int k = 0; k < 4; k++) worst = std::max(worst, er_net[k] / er_ext[k]);
+  c->pred_steps++;
+  if (worst <= 1.0) {
+    cf::launch_copy_interior(c->stream, c->Ualt, c->Ui, g);
+    for (int k = 0; k < 5; k++) er[k] = er_net[k];
+  } else {
+    c->pred_fallbacks++;
+    for (int k = 0; k < 5; k++) er[k] = er_ext[k];
+  }
+  return sync(c);
+}
+
 extern "C" int cf_physical_step(cf_ctx* c, int* iterations, double er[5]) {
   if (!c || !iterations || !er) return CF_ERR_ARG;
   if (!c->have_state) return fail(c, CF_ERR_STATE, "no state loaded");
-  if (c->guess_order > 0) {
+  const bool learned = c->guess_order == 3 && c->pred && c->uqq_valid;
+  if (c->guess_order > 0 && !learned) {
     // quadratic once three levels are known, linear before
-    const int order = c->guess_order == 2 && c->uqq_valid ? 2 : 1;
+    const int order = c->guess_order >= 2 && c->uqq_valid ? 2 : 1;
     cf::launch_extrapolate(c->stream, c->Ui, c->Un, c->Uq, c->Uqq, c->g, order);
     HIPCHK(c, hipGetLastError());
   }
   for (int it = 1;; it++) {
-    int r = cf_inner_iteration(c, er);
+    int r;
+    if (learned && it == 1) {
+      r = learned_start(c, er);
+      it = 2;  // two inner iterations were run
+    } else {
+      r = cf_inner_iteration(c, er);
+    }
     if (r) return r;
     for (int v = 0; v < 5; v++) {
       c->E[v] = std::max(c->E[v], er[v]);
@@ -1385,6 +1626,81 @@ extern "C" int cf_physical_step(cf_ctx* c, int* iterations, double er[5]) {
       return cf_advance_levels(c);
     }
   }
+}
+
+extern "C" int cf_set_predictor(cf_ctx* c, cf_pred* p) {
+  if (!c) return CF_ERR_ARG;
+  if (!p) {
+    c->pred = nullptr;
+    if (c->guess_order == 3) c->guess_order = 2;
+    return CF_OK;
+  }
+  if (!p->have_norm) return fail(c, CF_ERR_STATE, "the predictor has no normalisation");
+  if (p->device != c->device) return fail(c, CF_ERR_ARG, "the predictor lives on another device");
+  const Geo& g = c->g;
+  const size_t cells = (size_t)(g.gend - 2) * (g.ny - 1) * (g.nz - 1);
+  int r;
+  if ((r = ensure_level(c, &c->Uqq)) || (r = ensure_level(c, &c->Ualt))) return r;
+  if (!c->px && hipMalloc(&c->px, cells * cfp::CIN * 2) != hipSuccess) {
+    c->px = nullptr;
+    return fail(c, CF_ERR_HIP, "hipMalloc of the predictor features failed");
+  }
+  if (!c->py && hipMalloc(&c->py, cells * cfp::COUT * 4) != hipSuccess) {
+    c->py = nullptr;
+    return fail(c, CF_ERR_HIP, "hipMalloc of the predictor outputs failed");
+  }
+  // the predictor's weights are written on its own stream
+  HIPCHK(c, hipStreamSynchronize(p->stream));
+  c->pred = p;
+  return sync(c);
+}
+
+extern "C" int cf_predictor_stats(cf_ctx* c, long long* steps, long long* fallbacks) {
+  if (!c || !steps || !fallbacks) return CF_ERR_ARG;
+  *steps = c->pred_steps;
+  *fallbacks = c->pred_fallbacks;
+  return CF_OK;
+}
+
+// crops of the last physical step: inputs from U^{n-2}, U^{n-1}, U^n and the
+// converged U^{n+1}, which after cf_physical_step's level shift sit in U3,
+// Uqq, Uq and U; the inner (VC-2)^3 cells of a crop lie in the domain, its
+// margin may not (zero features, mask 0)
+extern "C" int cf_pred_capture(cf_pred* p, cf_ctx* c, int ncrops, unsigned long long seed) {
+  if (!p || !c || ncrops < 1) return CF_ERR_ARG;
+  if (!c->have_state || !c->U3 || !c->u3_valid || !c->uqq_valid)
+    return fail(c, CF_ERR_STATE, "capture needs four time levels (cf_keep_levels(ctx, 4) and three physical steps)");
+  if (p->n_crops + ncrops > p->cap) {
+    p->err = "the crop dataset is full";
+    return CF_ERR_ARG;
+  }
+  const Geo& g = c->g;
+  const int X = g.gend - 2, Y = g.ny - 1, Z = g.nz - 1, VC = p->VC;
+  if (X < VC - 2 || Y < VC - 2 || Z < VC - 2) return fail(c, CF_ERR_ARG, "grid smaller than a crop");
+  std::vector<int> org((size_t)3 * ncrops);
+  unsigned long long s = seed;
+  auto next = [&]() {
+    unsigned long long z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  };
+  const int ext[3] = {X, Y, Z};
+  for (int q = 0; q < ncrops; q++)
+    for (int d = 0; d < 3; d++) org[3 * q + d] = -1 + (int)(next() % (unsigned long long)(ext[d] - (VC - 2) + 1));
+  int* dorg = nullptr;
+  HIPCHK(c, hipMalloc(&dorg, org.size() * sizeof(int)));
+  hipError_t e = hipMemcpyAsync(dorg, org.data(), org.size() * sizeof(int), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    float* D = p->D + p->n_crops * (long)VC * VC * VC * cfp::NDATA;
+    cfp::launch_capture(c->stream, c->U3, c->Uqq, c->Uq, c->Un, c->prof, g, dorg, ncrops, VC, D);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(dorg);
+  if (e != hipSuccess) return fail(c, CF_ERR_HIP, "capture: %s", hipGetErrorString(e));
+  p->n_crops += ncrops;
+  return CF_OK;
 }
 
 extern "C" int cf_write_output(cf_ctx* c, int step, const char* dir) {

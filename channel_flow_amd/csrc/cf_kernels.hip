@@ -2530,6 +2530,66 @@ void launch_bulk_flux(hipStream_t st, const double* U2, const double* prof, doub
   hipLaunchKernelGGL(k_bulk_flux, dim3(g.gend - 2), dim3(256), 0, st, U2, prof, Jsum, part, g);
 }
 
+// Statistic.cpp:552-828, the part behind Shear_stress.dat: the y-face
+// viscous flux of x-momentum ML2 (:818-824) at the bottom wall face (j = 2,
+// SXYm[1]) and the top one (j = ny + 1, SXYm[ny]) for i = 3..gend and
+// k = 2..nz, into out[(w * X_out + i - 3) * Z_out + k - 2], with the
+// reference's velocities, 4-point x/z and 2-point y derivatives (:734-786)
+// and y-face metrics at j - 1 (:716-724), in its order of operations
+__global__ void k_wall_shear(const double* __restrict__ U, const double* __restrict__ prof, Geo g,
+                             double* __restrict__ out) {
+  const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = 3 + blockIdx.y, w = blockIdx.z;
+  if (k > g.nz) return;
+  const int j = w == 0 ? 2 : g.ny + 1;
+  const double inv4XI = 1. / (4 * g.dXI), invET = 1. / (g.dET), inv4ZT = 1. / (4 * g.dZT);
+  double u[10], v[10], ww[10];
+  const int off[10][3] = {{0, 0, 0},  {-1, 0, 0}, {1, 0, 0},  {0, -1, 0}, {-1, -1, 0},
+                          {1, -1, 0}, {0, 0, 1},  {0, 0, -1}, {0, -1, 1}, {0, -1, -1}};
+#pragma unroll
+  for (int q = 0; q < 10; q++) {
+    const long c = at(g, i + off[q][0], j + off[q][1], k + off[q][2]);
+    const double rho = U[c];
+    u[q] = U[g.nf + c] / rho;
+    v[q] = U[2 * g.nf + c] / rho;
+    ww[q] = U[3 * g.nf + c] / rho;
+  }
+  // q: 0 (0,0,0) 1 (-1,0,0) 2 (1,0,0) 3 (0,-1,0) 4 (-1,-1,0) 5 (1,-1,0)
+  //    6 (0,0,1) 7 (0,0,-1) 8 (0,-1,1) 9 (0,-1,-1)
+  const double du_dx = (u[2] + u[5] - u[1] - u[4]) * inv4XI;
+  const double dv_dx = (v[2] + v[5] - v[1] - v[4]) * inv4XI;
+  const double dw_dx = (ww[2] + ww[5] - ww[1] - ww[4]) * inv4XI;
+  const double du_dy = (u[0] - u[3]) * invET;
+  const double dv_dy = (v[0] - v[3]) * invET;
+  const double dw_dy = (ww[0] - ww[3]) * invET;
+  const double du_dz = (u[6] + u[8] - u[7] - u[9]) * inv4ZT;
+  const double dv_dz = (v[6] + v[8] - v[7] - v[9]) * inv4ZT;
+  const double dw_dz = (ww[6] + ww[8] - ww[7] - ww[9]) * inv4ZT;
+  const int jf = j - 1;
+  const double xix = pf(prof, g.Ym, SET_V, 1, jf), xiy = pf(prof, g.Ym, SET_V, 2, jf),
+               xiz = pf(prof, g.Ym, SET_V, 3, jf), etx = pf(prof, g.Ym, SET_V, 4, jf),
+               ety = pf(prof, g.Ym, SET_V, 5, jf), etz = pf(prof, g.Ym, SET_V, 6, jf),
+               ztx = pf(prof, g.Ym, SET_V, 7, jf), zty = pf(prof, g.Ym, SET_V, 8, jf),
+               ztz = pf(prof, g.Ym, SET_V, 9, jf);
+  const double b1 = (4. / 3) * etx * etx + ety * ety + etz * etz;
+  const double b5 = (1. / 3) * etx * ety;
+  const double b7 = (1. / 3) * etx * etz;
+  const double d1 = (4. / 3) * xix * etx + xiy * ety + xiz * etz;
+  const double d5 = xix * ety - (2. / 3) * xiy * etx;
+  const double d6 = xix * etz - (2. / 3) * xiz * etx;
+  const double f1 = (4. / 3) * etx * ztx + ety * zty + etz * ztz;
+  const double f7 = ety * ztx - (2. / 3) * etx * zty;
+  const double f9 = etz * ztx - (2. / 3) * etx * ztz;
+  const int X = g.gend - 2, Z = g.nz - 1;
+  out[((long)w * X + i - 3) * Z + k - 2] =
+      MU_L * (d1 * du_dx + d5 * dv_dx + d6 * dw_dx + b1 * du_dy + b5 * dv_dy + b7 * dw_dy + f1 * du_dz + f7 * dv_dz +
+              f9 * dw_dz) / pf(prof, g.Ym, SET_V, 0, jf);
+}
+
+void launch_wall_shear(hipStream_t st, const double* U, const double* prof, const Geo& g, double* out) {
+  hipLaunchKernelGGL(k_wall_shear, dim3((g.nz - 1 + 63) / 64, g.gend - 2, 2), dim3(64), 0, st, U, prof, g, out);
+}
+
 // out[0] = ((0 + a[0]) + a[1]) + ... + a[n-1], one thread: the order of a host loop
 __global__ void k_sum_in_order(const double* __restrict__ a, int n, double* __restrict__ out) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;

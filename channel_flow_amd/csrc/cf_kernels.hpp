@@ -30,6 +30,9 @@ void launch_stat(hipStream_t st, const double* U, double* sums, const Geo& g);
 // per-i-plane sums of the bulk flux integrand of Driven_force into part[X_out]
 void launch_bulk_flux(hipStream_t st, const double* U2, const double* prof, double Jsum, double* part,
                       const Geo& g);
+// the wall values of Statistic's y-face viscous x-momentum flux (the terms
+// of Shear_stress.dat): out[2][X_out][Z_out], bottom wall then top
+void launch_wall_shear(hipStream_t st, const double* U, const double* prof, const Geo& g, double* out);
 // out[0] = a[0] + a[1] + ... + a[n-1], added in index order by one thread
 void launch_sum_in_order(hipStream_t st, const double* a, int n, double* out);
 void launch_rhs(hipStream_t st, const double* Ui, const double* Un, const double* Uq, const double* Fx,

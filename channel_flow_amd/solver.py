@@ -36,7 +36,16 @@ EXPORTS = [
     "cf_set_viscous_schedule", "cf_rk3nodt", "cf_rk3_step",
     "cf_statistic", "cf_driven_force", "cf_save_checkpoint", "cf_load_checkpoint", "cf_set_initial_guess",
     "cf_driven_force_noalloc",
+    "cf_pred_create", "cf_pred_destroy", "cf_pred_last_error", "cf_pred_init_params", "cf_pred_set_params",
+    "cf_pred_get_params", "cf_pred_set_norm", "cf_pred_get_norm", "cf_pred_capture", "cf_pred_dataset_size",
+    "cf_pred_fit_norm", "cf_pred_load_dataset", "cf_pred_train", "cf_pred_eval", "cf_pred_forward", "cf_pred_gradient", "cf_pred_profile",
+    "cf_keep_levels", "cf_set_predictor", "cf_predictor_stats", "cf_set_sweep_reuse", "cf_set_graph",
+    "cf_graph_builds",
 ]
+CF_PRED_NPARAM = 112005
+CF_PRED_NNORM = 22
+CF_PRED_NSTATS = 36
+PRED_PROFILE_STAGES = ["infer", "train_forward", "train_head", "train_bwd_data", "train_bwd_weight", "train_reduce"]
 
 
 class CfParams(C.Structure):
@@ -109,6 +118,33 @@ def load_library() -> C.CDLL:
     lib.cf_load_checkpoint.argtypes = [vp, C.c_char_p]
     lib.cf_write_output.argtypes = [vp, C.c_int, C.c_char_p]
     lib.cf_synchronize.argtypes = [vp]
+    F = C.POINTER(C.c_float)
+    u64 = C.c_ulonglong
+    lib.cf_pred_create.argtypes = [C.c_int, C.c_int, C.POINTER(vp)]
+    lib.cf_pred_destroy.argtypes = [vp]
+    lib.cf_pred_destroy.restype = None
+    lib.cf_pred_last_error.argtypes = [vp]
+    lib.cf_pred_last_error.restype = C.c_char_p
+    lib.cf_pred_init_params.argtypes = [vp, u64]
+    lib.cf_pred_set_params.argtypes = [vp, F, C.c_int]
+    lib.cf_pred_get_params.argtypes = [vp, F, C.c_int]
+    lib.cf_pred_set_norm.argtypes = [vp, F]
+    lib.cf_pred_get_norm.argtypes = [vp, F]
+    lib.cf_pred_capture.argtypes = [vp, vp, C.c_int, u64]
+    lib.cf_pred_dataset_size.argtypes = [vp]
+    lib.cf_pred_fit_norm.argtypes = [vp, C.c_int, P]
+    lib.cf_pred_load_dataset.argtypes = [vp, F, C.c_int]
+    lib.cf_pred_train.argtypes = [vp, C.c_int, C.c_int, C.c_float, u64, F]
+    lib.cf_pred_eval.argtypes = [vp, C.c_int, C.c_int, P, P]
+    lib.cf_pred_forward.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, F, F]
+    lib.cf_pred_gradient.argtypes = [vp, C.c_int, F, F, F, P]
+    lib.cf_pred_profile.argtypes = [vp, C.c_int, C.c_int, C.c_int, P]
+    lib.cf_keep_levels.argtypes = [vp, C.c_int]
+    lib.cf_set_sweep_reuse.argtypes = [vp, C.c_int]
+    lib.cf_set_graph.argtypes = [vp, C.c_int]
+    lib.cf_graph_builds.argtypes = [vp, C.POINTER(C.c_longlong)]
+    lib.cf_set_predictor.argtypes = [vp, vp]
+    lib.cf_predictor_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
     _lib = lib
     return lib
 
@@ -409,3 +445,152 @@ class Solver:
 
     def synchronize(self):
         self._chk(self.lib.cf_synchronize(self._h))
+
+    def set_sweep_reuse(self, enabled: bool = True):
+        """One LU-SGS sweep pair for any n_sweeps (bit-identical in one domain)."""
+        self._chk(self.lib.cf_set_sweep_reuse(self._h, int(bool(enabled))))
+
+    def set_graph(self, enabled: bool = True):
+        """Replay each inner iteration's kernels as a captured hipGraph."""
+        self._chk(self.lib.cf_set_graph(self._h, int(bool(enabled))))
+
+    def graph_builds(self) -> int:
+        n = C.c_longlong()
+        self._chk(self.lib.cf_graph_builds(self._h, C.byref(n)))
+        return n.value
+
+    def keep_levels(self, levels: int):
+        """Time levels kept: 2 (U^n, U^{n-1}), 3 (+U^{n-2}) or 4 (+U^{n-3})."""
+        self._chk(self.lib.cf_keep_levels(self._h, int(levels)))
+
+    def set_predictor(self, pred):
+        """Attaches a Predictor (None detaches); set_initial_guess(3) uses it."""
+        self._chk(self.lib.cf_set_predictor(self._h, pred._h if pred is not None else None))
+        self._pred = pred  # keeps the predictor alive while attached
+
+    def predictor_stats(self):
+        """(physical steps started by the network, fallbacks to the extrapolation)."""
+        a, b = C.c_longlong(), C.c_longlong()
+        self._chk(self.lib.cf_predictor_stats(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+
+def _f(a):
+    if a is None:
+        return None
+    if not (isinstance(a, np.ndarray) and a.dtype == np.float32 and a.flags.c_contiguous):
+        raise CfError("arrays must be C-contiguous float32")
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Predictor:
+    """The learned initial iterate (cf_pred_*): parameters, crop dataset,
+    normalisation and training, all on the GPU."""
+
+    CROP = 16  # crop edge in cells (inner 14)
+
+    def __init__(self, dataset_crops: int = 1024, batch_crops: int = 32):
+        self.lib = load_library()
+        h = C.c_void_p()
+        r = self.lib.cf_pred_create(int(dataset_crops), int(batch_crops), C.byref(h))
+        if r != CF_OK:
+            raise CfError(f"cf_pred_create failed: {ERRORS.get(r, r)}")
+        self._h = h
+        self.batch_crops = batch_crops
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.cf_pred_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, r):
+        if r != CF_OK:
+            raise CfError(f"{ERRORS.get(r, r)}: {self.lib.cf_pred_last_error(self._h).decode()}")
+
+    def init_params(self, seed: int = 1):
+        self._chk(self.lib.cf_pred_init_params(self._h, int(seed)))
+
+    def set_params(self, params):
+        a = np.ascontiguousarray(params, dtype=np.float32)
+        self._chk(self.lib.cf_pred_set_params(self._h, _f(a), a.size))
+
+    def get_params(self):
+        a = np.zeros(CF_PRED_NPARAM, np.float32)
+        self._chk(self.lib.cf_pred_get_params(self._h, _f(a), a.size))
+        return a
+
+    def set_norm(self, norm):
+        a = np.ascontiguousarray(norm, dtype=np.float32)
+        assert a.size == CF_PRED_NNORM
+        self._chk(self.lib.cf_pred_set_norm(self._h, _f(a)))
+
+    def get_norm(self):
+        a = np.zeros(CF_PRED_NNORM, np.float32)
+        self._chk(self.lib.cf_pred_get_norm(self._h, _f(a)))
+        return a
+
+    def capture(self, solver, ncrops: int, seed: int):
+        """Crops of the solver's last physical step (it must keep 4 levels)."""
+        r = self.lib.cf_pred_capture(self._h, solver._h, int(ncrops), int(seed))
+        if r != CF_OK:
+            raise CfError(f"{ERRORS.get(r, r)}: {self.lib.cf_pred_last_error(self._h).decode()} / "
+                          f"{self.lib.cf_last_error(solver._h).decode()}")
+
+    def _load_dataset(self, crops):
+        """Appends crops [n][16][16][16][24] (the layout capture writes)."""
+        a = np.ascontiguousarray(crops, dtype=np.float32)
+        self._chk(self.lib.cf_pred_load_dataset(self._h, _f(a), a.shape[0]))
+
+    def dataset_size(self) -> int:
+        return int(self.lib.cf_pred_dataset_size(self._h))
+
+    def fit_norm(self, n=None):
+        n = self.dataset_size() if n is None else n
+        st = np.zeros(CF_PRED_NSTATS)
+        self._chk(self.lib.cf_pred_fit_norm(self._h, int(n), _p(st)))
+        return st
+
+    def train(self, n_train: int, steps: int, lr: float = 1e-3, seed: int = 1):
+        losses = np.zeros(max(steps, 1), np.float32)
+        self._chk(self.lib.cf_pred_train(self._h, int(n_train), int(steps), float(lr), int(seed), _f(losses)))
+        return losses[:steps]
+
+    def eval(self, first: int, n: int):
+        """(mean squared error of the network, of the base guess) on crops [first, first + n)."""
+        a, b = C.c_double(), C.c_double()
+        self._chk(self.lib.cf_pred_eval(self._h, int(first), int(n), C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def forward(self, x, margin: int = 1):
+        """y on the inner region of volumes x [n][VI][VJ][VK][16] (float32)."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        n, VI, VJ, VK, _ = x.shape
+        y = np.zeros((n, VI - 2 * margin, VJ - 2 * margin, VK - 2 * margin, 5), np.float32)
+        self._chk(self.lib.cf_pred_forward(self._h, n, VI, VJ, VK, int(margin), _f(x), _f(y)))
+        return y
+
+    def gradient(self, x, target):
+        """(gradient, loss) of crops x [n][16][16][16][16], targets [n][14][14][14][5]."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        g = np.zeros(CF_PRED_NPARAM, np.float32)
+        loss = C.c_double()
+        self._chk(self.lib.cf_pred_gradient(self._h, x.shape[0], _f(x), _f(t), _f(g), C.byref(loss)))
+        return g, loss.value
+
+    def profile(self, X: int, Y: int, Z: int):
+        ms = np.zeros(len(PRED_PROFILE_STAGES))
+        self._chk(self.lib.cf_pred_profile(self._h, int(X), int(Y), int(Z), _p(ms)))
+        return dict(zip(PRED_PROFILE_STAGES, ms.tolist()))

@@ -191,7 +191,8 @@ int cf_advance_levels(cf_ctx* ctx);
  * (which starts every step at U^n, the level shift of main.cpp:444-464):
  * 0 (default) U^n, as the reference; 1 the linear extrapolation
  * 2 U^n - U^{n-1}; 2 the quadratic 3 U^n - 3 U^{n-1} + U^{n-2}, linear until
- * three levels are known (it keeps a U^{n-2} level, 5 more fields).  The
+ * three levels are known (it keeps a U^{n-2} level, 5 more fields); 3 the
+ * learned predictor attached with cf_set_predictor (below).  The
  * converged state is the same to the convergence tolerance; the number of
  * inner iterations under the running-max test (main.cpp:387-406) drops. */
 int cf_set_initial_guess(cf_ctx* ctx, int order);
@@ -201,9 +202,11 @@ int cf_write_output(cf_ctx* ctx, int step, const char* dir);
  * LES%05d.bin, Q = U J, and starts E_k and the statistics afresh): the
  * interior of the iterate and of levels n and n-1 as stored (U = Q/J), the
  * running maxima E_k of cf_physical_step and the statistic sums.  Format:
- * "CFCKPT01", int32 X_out Y_out Z_out M, double E[5], double sums[7][Y_m],
+ * "CFCKPT02", int32 X_out Y_out Z_out M, double E[5], double sums[7][Y_m],
+ * double wall[4] (the Shear_stress.dat running sums and previous averages),
  * then 15 fields (U_, U, Uq) of X_out*Y_out*Z_out doubles, [x][y][z], then M
- * more: 5 (level n-2) when the quadratic initial guess holds one, else 0. */
+ * more: 5 (level n-2) when the quadratic initial guess holds one, else 0.
+ * cf_load_checkpoint also reads "CFCKPT01" files, which have no wall[4]. */
 int cf_save_checkpoint(cf_ctx* ctx, const char* path);
 int cf_load_checkpoint(cf_ctx* ctx, const char* path);
 /* Runs one inner iteration like cf_inner_iteration and times its stages
@@ -272,16 +275,22 @@ int cf_driven_force(cf_ctx* ctx, double Q0, const double* U2_, const double* J, 
  * contexts' streams) race. */
 #define CF_NOALLOC_MAX_PLANES 512
 int cf_driven_force_noalloc(cf_ctx* ctx, double Q0, const double* U2_dev, double* Qall, double* ER_Q);
-/* Statistic.cpp:1043-1154 (main.h Statistic), the mean profiles: adds the
- * per-j sums over i and k of U, V, W, UU, VV, WW, UV (U = U2_/U1_ ...) of
- * the state to running sums kept in the context; when step % statistic_step
+/* Statistic.cpp:552-1154 (main.h Statistic): the mean profiles and the
+ * wall shear.  Adds the per-j sums over i and k of U, V, W, UU, VV, WW, UV
+ * (U = U2_/U1_ ...) of the state to running sums kept in the context; adds
+ * the bottom and top wall values of the y-face viscous x-momentum flux
+ * (Statistic.cpp:552-828) to running sums in the reference's order and
+ * appends to <directory>/Shear_stress.dat the change of their running
+ * averages, "%f	%f
+" (Statistic.cpp:893-911); when step % statistic_step
  * == 0 writes <directory>/st%05d.dat (the sums / statistic_step / X_out /
  * Z_out, seven blocks of one "%.16f" line per j = 2..ny, the reference's
- * format) and resets the sums.  U_: reference-layout host arrays, or NULL
- * for the device iterate.  The per-j sums are combined in a fixed tree: the
- * means equal the reference's to rounding (not bit for bit).  The
- * reference's other Statistic outputs (ND%05d.dat, Shear_stress.dat) are
- * not produced. */
+ * format) and resets all the sums.  U_: reference-layout host arrays (ghost
+ * cells included), or NULL for the device iterate (with the ghost cells of
+ * its last inner iteration, as in the reference).  The per-j profile sums are
+ * combined in a fixed tree: the means equal the reference's to rounding; the
+ * wall sums are added in the reference's order, so Shear_stress.dat is the
+ * reference's byte for byte.  ND%05d.dat is not produced. */
 int cf_statistic(cf_ctx* ctx, int step, int statistic_step, const double* const U_[5], const char* directory);
 /* Kernel schedule of Viscous_terms (Viscous_terms.cpp:212-883); both give
  * identical bits.  0 (default): one kernel that keeps the primitives and
@@ -291,6 +300,90 @@ int cf_statistic(cf_ctx* ctx, int step, int statistic_step, const double* const 
 int cf_set_viscous_schedule(cf_ctx* ctx, int schedule);
 /* Synchronises the context's stream. */
 int cf_synchronize(cf_ctx* ctx);
+/* LU-SGS sweep pairs of Dual_time_stepping.cpp:514-1277, no counterpart in
+ * the reference's parameters.  With enabled != 0 (the default) one
+ * forward/backward pair runs for any n_sweeps >= 1: in one domain each later
+ * pair reads only MR and values its own pass has formed, so it reproduces
+ * the first bit for bit (MR, NL, NR, U_ and er are the same; across MPI
+ * ranks they would not be, :521-559).  0 runs all n_sweeps pairs. */
+int cf_set_sweep_reuse(cf_ctx* ctx, int enabled);
+/* With enabled != 0, cf_inner_iteration (and so cf_physical_step) replays
+ * the kernels of one inner iteration as a hipGraph on the context's stream,
+ * captured on first use and again when the fields, schedules, precision or
+ * parameters it was captured with change; the results are the same bits as
+ * the launches issued one by one.  No counterpart in the reference. */
+int cf_set_graph(cf_ctx* ctx, int enabled);
+/* Number of graph captures made so far. */
+int cf_graph_builds(cf_ctx* ctx, long long* builds);
+
+/* --- learned initial iterate ------------------------------------------------
+ * No counterpart in the reference, whose physical step starts its inner
+ * iterations at U^n (main.cpp:444-464).  A small network predicts
+ * Q^{n+1} - Q^n from the last three states:
+ *   x'  = ((Q^n - mu) / sig, (Q^n - Q^{n-1}) / s, (Q^{n-1} - Q^{n-2}) / s, 1)
+ *   h1  = W1 x'                          pointwise 16 -> 64
+ *   z2  = b2 + 27-point conv(W2, h1)      3x3x3, 64 -> 64, zeros outside the domain
+ *   y   = b3 + W3 gelu(z2)               pointwise 64 -> 5
+ *   Q^{n+1} = Q^n + c1 d1 + c2 d2 + sr y  (d1, d2 the increments above)
+ * W1, W2 and the activations are bfloat16 MFMA operands (gfx950), every sum
+ * accumulates in FP32; training (forward, backward, Adam) runs on the GPU.
+ * Parameters are one FP32 vector in the order W1[64][16], W2[27][64][64]
+ * (tap 9 (di+1) + 3 (dj+1) + (dk+1), then out, in), b2[64], W3[5][64], b3[5].
+ * The normalisation record is mu[5], sig[5], s[5], sr[5], c1, c2. */
+typedef struct cf_pred cf_pred;
+#define CF_PRED_NPARAM 112005
+#define CF_PRED_NNORM 22
+#define CF_PRED_NSTATS 36
+#define CF_PRED_PROFILE_STAGES 6
+/* A predictor on the current device with room for dataset_crops training
+ * crops of 16^3 cells (inner 14^3) and training batches of batch_crops. */
+int cf_pred_create(int dataset_crops, int batch_crops, cf_pred** out);
+void cf_pred_destroy(cf_pred* p);
+const char* cf_pred_last_error(const cf_pred* p);
+int cf_pred_init_params(cf_pred* p, unsigned long long seed);
+int cf_pred_set_params(cf_pred* p, const float* params, int n);
+int cf_pred_get_params(cf_pred* p, float* params, int n);
+int cf_pred_set_norm(cf_pred* p, const float norm[CF_PRED_NNORM]);
+int cf_pred_get_norm(cf_pred* p, float norm[CF_PRED_NNORM]);
+/* Appends ncrops random crops of the context's last physical step (inputs
+ * from U^{n-2}, U^{n-1}, U^n, target the converged U^{n+1}); the context
+ * must keep four levels (cf_keep_levels) and have run three steps. */
+int cf_pred_capture(cf_pred* p, cf_ctx* ctx, int ncrops, unsigned long long seed);
+int cf_pred_dataset_size(const cf_pred* p);
+/* Appends ncrops crops given as host floats [ncrops][16^3][24]: per cell
+ * Q^n, Q^n - Q^{n-1}, Q^{n-1} - Q^{n-2}, Q^{n+1} - Q^n (5 each), the
+ * in-domain mask, 3 zeros (the layout cf_pred_capture writes). */
+int cf_pred_load_dataset(cf_pred* p, const float* crops, int ncrops);
+/* Feature scales and base guess (c1, c2) from crops [0, n); stats (may be
+ * NULL) receives the sums they come from. */
+int cf_pred_fit_norm(cf_pred* p, int n, double stats[CF_PRED_NSTATS]);
+/* steps Adam steps (cosine-decayed step size lr) on batches drawn from crops
+ * [0, n_train); losses (may be NULL) receives each step's mean squared error
+ * in normalised units. */
+int cf_pred_train(cf_pred* p, int n_train, int steps, float lr, unsigned long long seed, float* losses);
+/* Mean squared error of the network and of the base guess alone on crops
+ * [first, first + n). */
+int cf_pred_eval(cf_pred* p, int first, int n, double* loss, double* base_loss);
+/* Test entry points on host arrays: y on the inner region (margin cells
+ * dropped) of n volumes of features [n][VI][VJ][VK][16]; the loss and
+ * gradient of n 16^3 crops with targets [n][14^3][5]. */
+int cf_pred_forward(cf_pred* p, int n, int VI, int VJ, int VK, int margin, const float* x, float* y);
+int cf_pred_gradient(cf_pred* p, int n, const float* x, const float* target, float* grad, double* loss);
+/* HIP-event times (ms): inference over an X x Y x Z domain, then one
+ * training batch: pointwise + forward conv, loss head, backward data,
+ * backward weight, reduction. */
+int cf_pred_profile(cf_pred* p, int X, int Y, int Z, double ms[CF_PRED_PROFILE_STAGES]);
+/* Time levels kept besides U^n and U^{n-1}: 3 adds U^{n-2}, 4 adds U^{n-3}. */
+int cf_keep_levels(cf_ctx* ctx, int levels);
+/* Attaches a predictor (NULL detaches); cf_set_initial_guess(ctx, 3) then
+ * starts each cf_physical_step with it once three levels are known.  The
+ * quadratic extrapolation is the fallback: both guesses take one inner
+ * iteration and the step continues from the network's unless some er_k of
+ * its first iteration exceeds the extrapolation's (both iterations count).
+ * The converged state is the same; the iteration count changes. */
+int cf_set_predictor(cf_ctx* ctx, cf_pred* p);
+/* Physical steps started by the network, and how many fell back. */
+int cf_predictor_stats(cf_ctx* ctx, long long* steps, long long* fallbacks);
 
 #ifdef __cplusplus
 }

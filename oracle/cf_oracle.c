@@ -1150,6 +1150,62 @@ void cfo_statistic_accum(const cfo_grid* g, double* const U_[5], double* sums) {
       }
 }
 
+/* Statistic.cpp:552-828, the part behind Shear_stress.dat: the y-face
+ * viscous flux of x-momentum ML2 (Statistic.cpp:818-824) at the bottom wall
+ * face (j = 2, between the wall ghost and the first cell: SXYm[1]) and the
+ * top one (j = ny + 1: SXYm[ny]), for i = 3..gend and k = 2..nz, into
+ * out[(w * X_out + i - 3) * Z_out + k - 2].  Velocities are U_2/U_1 ...
+ * (Statistic.cpp:603-712), derivatives the 4-point averages along x and z
+ * and the 2-point difference along y (:734-786), metrics those of the y
+ * faces at [i-1][j-1][k-1] (:716-724). */
+void cfo_wall_shear(const cfo_grid* g, double* const U_[5], const double* Jv, double* const mv[9], double* out) {
+  dims_t d = dims(g);
+  const double inv4XI = 1. / (4 * g->deltaXI), invET = 1. / (g->deltaET), inv4ZT = 1. / (4 * g->deltaZT);
+  const int X = d.gend - 2, Z = d.nz - 1;
+  for (int w = 0; w < 2; w++) {
+    const int j = w == 0 ? 2 : d.ny + 1;
+    for (int i = 3; i <= d.gend; i++)
+      for (int k = 2; k <= d.nz; k++) {
+        double u[3][3][3], v[3][3][3], wv[3][3][3]; /* [di+1][dj+1][dk+1], dj in {-1, 0} */
+        const int off[10][3] = {{0, 0, 0},  {-1, 0, 0}, {1, 0, 0},  {0, -1, 0}, {-1, -1, 0},
+                                {1, -1, 0}, {0, 0, 1},  {0, 0, -1}, {0, -1, 1}, {0, -1, -1}};
+        for (int q = 0; q < 10; q++) {
+          const size_t c = IX(d, i + off[q][0], j + off[q][1], k + off[q][2]);
+          const double rho = U_[0][c];
+          u[off[q][0] + 1][off[q][1] + 1][off[q][2] + 1] = U_[1][c] / rho;
+          v[off[q][0] + 1][off[q][1] + 1][off[q][2] + 1] = U_[2][c] / rho;
+          wv[off[q][0] + 1][off[q][1] + 1][off[q][2] + 1] = U_[3][c] / rho;
+        }
+#define AT(a, di, dj, dk) a[(di) + 1][(dj) + 1][(dk) + 1]
+        const double du_dx = (AT(u, 1, 0, 0) + AT(u, 1, -1, 0) - AT(u, -1, 0, 0) - AT(u, -1, -1, 0)) * inv4XI;
+        const double dv_dx = (AT(v, 1, 0, 0) + AT(v, 1, -1, 0) - AT(v, -1, 0, 0) - AT(v, -1, -1, 0)) * inv4XI;
+        const double dw_dx = (AT(wv, 1, 0, 0) + AT(wv, 1, -1, 0) - AT(wv, -1, 0, 0) - AT(wv, -1, -1, 0)) * inv4XI;
+        const double du_dy = (AT(u, 0, 0, 0) - AT(u, 0, -1, 0)) * invET;
+        const double dv_dy = (AT(v, 0, 0, 0) - AT(v, 0, -1, 0)) * invET;
+        const double dw_dy = (AT(wv, 0, 0, 0) - AT(wv, 0, -1, 0)) * invET;
+        const double du_dz = (AT(u, 0, 0, 1) + AT(u, 0, -1, 1) - AT(u, 0, 0, -1) - AT(u, 0, -1, -1)) * inv4ZT;
+        const double dv_dz = (AT(v, 0, 0, 1) + AT(v, 0, -1, 1) - AT(v, 0, 0, -1) - AT(v, 0, -1, -1)) * inv4ZT;
+        const double dw_dz = (AT(wv, 0, 0, 1) + AT(wv, 0, -1, 1) - AT(wv, 0, 0, -1) - AT(wv, 0, -1, -1)) * inv4ZT;
+#undef AT
+        const size_t f = IX(d, i - 1, j - 1, k - 1);
+        const double xix = mv[0][f], xiy = mv[1][f], xiz = mv[2][f], etx = mv[3][f], ety = mv[4][f],
+                     etz = mv[5][f], ztx = mv[6][f], zty = mv[7][f], ztz = mv[8][f];
+        const double b1 = (4. / 3) * etx * etx + ety * ety + etz * etz;
+        const double b5 = (1. / 3) * etx * ety;
+        const double b7 = (1. / 3) * etx * etz;
+        const double d1 = (4. / 3) * xix * etx + xiy * ety + xiz * etz;
+        const double d5 = xix * ety - (2. / 3) * xiy * etx;
+        const double d6 = xix * etz - (2. / 3) * xiz * etx;
+        const double f1 = (4. / 3) * etx * ztx + ety * zty + etz * ztz;
+        const double f7 = ety * ztx - (2. / 3) * etx * zty;
+        const double f9 = etz * ztx - (2. / 3) * etx * ztz;
+        out[((size_t)w * X + i - 3) * Z + k - 2] =
+            MU_L * (d1 * du_dx + d5 * dv_dx + d6 * dw_dx + b1 * du_dy + b5 * dv_dy + b7 * dw_dy + f1 * du_dz +
+                    f7 * dv_dz + f9 * dw_dz) / Jv[f];
+      }
+  }
+}
+
 void cfo_sweeps_apply(const cfo_grid* g, double deltaT, int mode, double* const U_[5], const double* J,
                       double* const m[9], double* const MR[5], double* const NL[5], double* const NR[5]) {
   dims_t d = dims(g);

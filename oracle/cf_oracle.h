@@ -106,6 +106,7 @@ void cfo_driven_force(const cfo_grid* g, const double* U2_, const double* J, dou
 /* Statistic.cpp:1043-1071: adds the state's sums over i, k of U, V, W, UU,
  * VV, WW, UV to sums[q * Y_m + j] (q = 0..6), in the reference's order. */
 void cfo_statistic_accum(const cfo_grid* g, double* const U_[5], double* sums);
+void cfo_wall_shear(const cfo_grid* g, double* const U_[5], const double* Jv, double* const mv[9], double* out);
 
 /* The implicit sweeps alone, on a given MR: ghosts of NL/NR zeroed,
  * NL = D^-1 MR, then mode 0 (LU-SGS, lexicographic), 1 (LU-SGS, plane by

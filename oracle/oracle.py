@@ -58,6 +58,7 @@ def lib():
         _lib.cfo_dplusgs.argtypes = dt_args
         _lib.cfo_sweeps_apply.argtypes = [C.POINTER(Grid), C.c_double, C.c_int, PP, P, PP, PP, PP, PP]
         _lib.cfo_statistic_accum.argtypes = [C.POINTER(Grid), PP, P]
+        _lib.cfo_wall_shear.argtypes = [C.POINTER(Grid), PP, P, PP, P]
         _lib.cfo_driven_force.argtypes = [C.POINTER(Grid), P, P, C.c_double, P]
         _lib.cfo_rk3nodt.argtypes = [C.POINTER(Grid), C.c_int, C.c_double, PP, PP, P, PP, PP, PP, PP, PP]
     return _lib
@@ -201,6 +202,43 @@ class Statistic:
                 lines.append("%.16f\n" % (self.sums[q * Ym + j] * inv))
         self.sums[:] = 0
         return "".join(lines)
+
+
+def wall_shear(cfg, U_, met):
+    """Statistic.cpp:552-828: ML2 at the bottom and top wall faces, shape
+    (2, X_out, Z_out) (i = 3..gend, k = 2..nz)."""
+    out = np.zeros((2, cfg.X_out, cfg.Z_out))
+    lib().cfo_wall_shear(C.byref(grid_of(cfg)), _pp(U_), _p(met["J_v"]), _pp(met["m_v"]), _p(out))
+    return out
+
+
+class ShearStress:
+    """Statistic.cpp:879-911 and :1008-1018: the wall values are added to
+    running sums (j, then i, then k, as the reference's SXYm[j]); each call
+    writes "%f\t%f\n" of the change of the running averages SXYm[1] /
+    X_out / Z_out and SXYm[ny] / X_out / Z_out, and the statistic step
+    resets the sums and the previous averages."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        self.cum = [0.0, 0.0]
+        self.prev = [0.0, 0.0]
+
+    def __call__(self, step, statistic_step, U_, met):
+        vals = wall_shear(self.cfg, U_, met)
+        for w in range(2):
+            s = self.cum[w]
+            for v in vals[w].ravel().tolist():
+                s = s + v
+            self.cum[w] = s
+        X, Z = self.cfg.X_out, self.cfg.Z_out
+        avg = [self.cum[0] / X / Z, self.cum[1] / X / Z]
+        line = "%f\t%f\n" % (avg[0] - self.prev[0], avg[1] - self.prev[1])
+        self.prev = avg
+        if step % statistic_step == 0:
+            self.cum = [0.0, 0.0]
+            self.prev = [0.0, 0.0]
+        return line
 
 
 def rk3nodt(cfg, RK, U_, U, J, inFx, inFy, inFz, vF, deltaT=None):

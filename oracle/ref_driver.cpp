@@ -20,6 +20,10 @@
 //       Driven_force's printed line for the iterate and for level n.
 //   ref_driver stat <dir> <deltaT>
 //       Statistic's mean profiles over two steps and over one (st*.dat).
+//   ref_driver shear <dir> <deltaT>
+//       Statistic on the post-BC iterates iterate.bin, iterate2.bin,
+//       iterate.bin, iterate2.bin (steps 1-4, statistic_step 3): the
+//       Shear_stress.dat lines, including the reset after step 3.
 //   ref_driver time <dir> <deltaT> <inner_iterations>
 //       times the inner iteration (halo+BC, fluxes, viscous, dual time).
 #include <stdlib.h>
@@ -385,6 +389,15 @@ int main(int argc, char** argv) {
     statistic(s, myid, s.U_, 1, 2);
     statistic(s, myid, s.U, 2, 2);
     statistic(s, myid, s.U_, 3, 1);
+  } else if (mode == "shear") {
+    // Statistic.cpp:552-911 (Shear_stress.dat) and its reset at the
+    // statistic step (Statistic.cpp:1008-1018)
+    const char* files[4] = {"iterate.bin", "iterate2.bin", "iterate.bin", "iterate2.bin"};
+    for (int step = 1; step <= 4; step++) {
+      load5(files[step - 1], s.U_, s.J);
+      halo_bc(s, myid);
+      statistic(s, myid, s.U_, step, 3);
+    }
   } else if (mode == "force") {
     // Driven_force.cpp:38-73 on the post-BC iterate, then on level n: each
     // call prints its Q0 / Qall / ER_Q line

@@ -245,7 +245,40 @@ def force_fixture(grid, seed):
     return name, meta
 
 
+def shear_fixture(grid, seed):
+    """Statistic's Shear_stress.dat lines (Statistic.cpp:552-911: the mean
+    y-face viscous x-momentum flux on each wall, written per step as the
+    change of its running average) over four steps on post-BC iterates
+    (iterate, iterate2, iterate, iterate2) with statistic_step 3, so that the
+    fourth line follows the reset of Statistic.cpp:1008-1018."""
+    cfg = ChannelConfig(X_out=grid[0], Y_out=grid[1], Z_out=grid[2])
+    exe = ref_driver(grid)
+    with tempfile.TemporaryDirectory() as td:
+        d = Path(td)
+        Qn, Qm = perturbed_state(cfg, seed, 0), perturbed_state(cfg, seed, 1)
+        write_initial_bin(d / "initial.bin", Qn, Qm)
+        np.ascontiguousarray(perturbed_state(cfg, seed, 2)).tofile(d / "iterate.bin")
+        np.ascontiguousarray(perturbed_state(cfg, seed, 3)).tofile(d / "iterate2.bin")
+        subprocess.run([str(exe), "shear", str(d), repr(DT_UNIT)], check=True, stdout=subprocess.DEVNULL,
+                       env=dict(os.environ, OMP_NUM_THREADS="1"))
+        text = (d / "Shear_stress.dat").read_bytes()
+    assert text.count(b"\n") == 4, text
+    meta = dict(grid=list(grid), seed=seed,
+                inputs="perturbed_state(seed, 2) and (seed, 3) as iterates, post-BC, steps 1-4",
+                reference_calls=["Statistic(step 1..4, statistic_step 3) -> Shear_stress.dat"])
+    name = f"shear_g{grid[0]}x{grid[1]}x{grid[2]}"
+    np.savez_compressed(OUT / f"{name}.npz", lines=np.frombuffer(text, dtype=np.uint8))
+    return name, meta
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "shear":
+        manifest = json.loads((OUT / "manifest.json").read_text())
+        for i, g in enumerate(VAR_GRIDS):
+            n, m = shear_fixture(g, seed=21 + i)
+            manifest["fixtures"][n] = m
+        (OUT / "manifest.json").write_text(json.dumps(manifest, indent=1))
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "variants":
         # regenerate only the variant fixtures and merge them into the manifest
         manifest = json.loads((OUT / "manifest.json").read_text())
@@ -291,6 +324,8 @@ def main():
         n, m = stat_fixture(g, seed=21 + i)
         manifest["fixtures"][n] = m
         n, m = force_fixture(g, seed=21 + i)
+        manifest["fixtures"][n] = m
+        n, m = shear_fixture(g, seed=21 + i)
         manifest["fixtures"][n] = m
     n, m = traj_fixture(TRAJ, keep_state=True)
     manifest["fixtures"][n] = m

@@ -39,7 +39,8 @@ def test_gpu_benchmark_grid_matches_reference(tmp_path):
     """256x192x256, stretching 1.5 (the reference's Jacobian*.cpp), dt 1e-8:
     one physical step capped at two inner iterations, on the GPU and in the
     reference (main.cpp:143-498 with Viscous_terms and Dual_time_stepping),
-    from the same seeded Poiseuille state."""
+    from the same seeded Poiseuille state.  The GPU executes one LU-SGS sweep
+    pair per inner iteration (its default), the reference its two."""
     from channel_flow_amd.config import ChannelConfig, FlowParams, poiseuille_state, read_les_bin, write_initial_bin
     from channel_flow_amd.solver import Solver
     exe = ROOT / "oracle" / "_ref" / "g{}x{}x{}".format(*BENCH_GRID) / "ref_driver"
@@ -164,21 +165,42 @@ def test_gpu_benchmark_grid_limiter_fluxes_match_reference(tmp_path):
             np.testing.assert_array_equal(out[a][v][m], ref[m], err_msg=f"inF{name}{v + 1}")
 
 
-def _schedule_runs(cfg, Qn, Qm, runs, iterations):
+def _schedule_runs(cfg, Qn, Qm, runs, iterations, precision=0):
     """State and raw norms after `iterations` inner iterations for each
-    (sweep schedule, viscous schedule) in `runs`, in one context."""
+    (sweep schedule, viscous schedule[, sweep reuse, graph]) in `runs`, in
+    one context."""
     from channel_flow_amd.solver import Solver
     out = []
-    with Solver(cfg) as s:
+    with Solver(cfg, precision=precision) as s:
         s.Jacobian_all(cfg.gamma_metric, want_arrays=False)
-        for sweep, visc in runs:
-            _log(f"{cfg.X_out}x{cfg.Y_out}x{cfg.Z_out}: sweep schedule {sweep}, viscous schedule {visc}")
+        for run in runs:
+            sweep, visc, reuse, graph = (tuple(run) + (True, False))[:4]
+            _log(f"{cfg.X_out}x{cfg.Y_out}x{cfg.Z_out}: sweep schedule {sweep}, viscous schedule {visc}, "
+                 f"one sweep pair {reuse}, graph {graph}, precision {precision}")
             s.set_sweep_schedule(sweep)
             s.set_viscous_schedule(visc)
+            s.set_sweep_reuse(reuse)
+            s.set_graph(graph)
             s.load_state(Qn, Qm)
             ers = np.array([s.inner_iteration() for _ in range(iterations)])
             out.append((ers, s.read_state()[2]))
     return out
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_gpu_benchmark_grid_one_sweep_pair_and_graph(precision):
+    """At 256x192x256 (stretching 2.0, dt 1e-8): one executed sweep pair
+    against the reference's two, and graph replays against eager launches,
+    over three inner iterations: the same bits."""
+    from channel_flow_amd.config import ChannelConfig, FlowParams, poiseuille_state
+    cfg = ChannelConfig(*BENCH_GRID, gamma_metric=2.0, deltaT=1e-8)
+    Qn = poiseuille_state(cfg, FlowParams(), seed=3, stream=0)
+    (e1, q1), (e2, q2), (e3, q3) = _schedule_runs(cfg, Qn, Qn, [(0, 0, True, False), (0, 0, False, False),
+                                                               (0, 0, True, True)], 3, precision)
+    np.testing.assert_array_equal(e1, e2)
+    np.testing.assert_array_equal(q1, q2)
+    np.testing.assert_array_equal(e1, e3)
+    np.testing.assert_array_equal(q1, q3)
 
 
 def test_gpu_benchmark_grid_schedules_identical():
@@ -201,13 +223,16 @@ def test_gpu_benchmark_grid_schedules_identical():
 
 
 def test_gpu_largest_grid_512x384x512():
-    """512x384x512 (stretching 2.2), 100.7 M cells: one inner iteration with
-    the pipelined and with the single-wave tiled sweeps gives the same bits,
-    and the norms are finite and positive."""
+    """512x384x512 (stretching 2.2), 100.7 M cells: two inner iterations with
+    the pipelined sweeps, with the single-wave tiled sweeps and as hipGraph
+    replays give the same bits, and the norms are finite and positive."""
     from channel_flow_amd.config import ChannelConfig, FlowParams, poiseuille_state
     cfg = ChannelConfig(512, 384, 512, gamma_metric=2.2, deltaT=1e-8)
     Qn = poiseuille_state(cfg, FlowParams(), seed=5, stream=0)
-    (e0, q0), (e2, q2) = _schedule_runs(cfg, Qn, Qn, [(0, 0), (2, 0)], 1)
+    (e0, q0), (e2, q2), (eg, qg) = _schedule_runs(cfg, Qn, Qn, [(0, 0), (2, 0), (0, 0, True, True)], 2)
     assert np.isfinite(e0).all() and (e0 > 0).all()
     np.testing.assert_array_equal(e0, e2)
     np.testing.assert_array_equal(q0, q2)
+    # the graph-captured inner iteration (config 5's loop) gives the same bits
+    np.testing.assert_array_equal(e0, eg)
+    np.testing.assert_array_equal(q0, qg)

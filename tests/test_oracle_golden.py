@@ -342,3 +342,58 @@ def test_oracle_x_ghosts_closed_form(grid):
             np.testing.assert_array_equal(U[v][g + 1][reg], U[v][hi][reg])
         # a planted mistake (source planes swapped) would break it
         assert not np.array_equal(U[0][2][reg], U[0][hi][reg])
+
+
+@pytest.mark.parametrize("grid", VAR_GRIDS)
+def test_shear_stress_matches_reference(grid, golden):
+    """Shear_stress.dat (Statistic.cpp:552-911) over four steps on post-BC
+    iterates with statistic_step 3, byte for byte against the reference's
+    file (the fourth line follows the reset of Statistic.cpp:1008-1018)."""
+    import sys
+    from conftest import GOLDEN
+    sys.path.insert(0, str(GOLDEN))
+    from make_golden import perturbed_state
+    cfg = unit_cfg(grid)
+    met = O.metrics(cfg)
+    seed = 21 + VAR_GRIDS.index(grid)
+    iterates = [perturbed_state(cfg, seed, 2), perturbed_state(cfg, seed, 3)]
+    sh = O.ShearStress(cfg)
+    text = ""
+    for step in range(1, 5):
+        U_ = O.state_from_Q(cfg, iterates[(step - 1) % 2], met["J"])
+        O.halo_bc(cfg, U_, met["J"])
+        text += sh(step, 3, U_, met)
+    want = golden(f"shear_g{grid[0]}x{grid[1]}x{grid[2]}")["lines"].tobytes().decode()
+    assert text == want
+
+
+def test_shear_stress_laminar_closed_form():
+    """For the noise-free laminar Poiseuille state u = 1.5 U_b (1 - (y/h - 1)^2)
+    (uniform density and temperature, v = w = 0) the wall values are
+    mu (u_2 - u_ghost) / (deltaET y_eta) with the no-slip ghost u_ghost = -u_2
+    and y_eta the stretching's analytic derivative at the first cell centre
+    (the wall-face metrics are that cell's, Jacobian_v.cpp:124-141), the same
+    on every face and opposite on the two walls; and as the grid is refined
+    they tend to the exact wall stress 3 mu U_b / h of the parabola."""
+    import math
+    from channel_flow_amd.config import MU_L, ChannelConfig, FlowParams, poiseuille_state
+    flow = FlowParams(noise=0.0)
+    errs = []
+    for Y in (16, 32, 64):
+        cfg = ChannelConfig(8, Y, 8, gamma_metric=2.0)
+        met = O.metrics(cfg)
+        U_ = O.state_from_Q(cfg, poiseuille_state(cfg, flow, seed=1), met["J"])
+        O.halo_bc(cfg, U_, met["J"])
+        vals = O.wall_shear(cfg, U_, met)
+        # analytic: y(eta) = h (1 - tanh(g (1 - 2 eta)) / tanh g), eta_wall = 0,
+        # first cell centre at eta = deltaET / 2
+        h, g = cfg.high / 2, 2.0
+        Ub, _ = flow.bulk_state(cfg.high)
+        y2 = h * (1 - math.tanh(g * (1 - cfg.deltaET)) / math.tanh(g))
+        u2 = 1.5 * Ub * (1 - (y2 / h - 1) ** 2)
+        y_eta = 2 * h * g / math.tanh(g) / math.cosh(g * (1 - cfg.deltaET)) ** 2
+        want = MU_L * 2 * u2 / (cfg.deltaET * y_eta)
+        np.testing.assert_allclose(vals[0], want, rtol=1e-9)
+        np.testing.assert_allclose(vals[1], -want, rtol=1e-9)
+        errs.append(abs(vals[0].mean() - 3 * MU_L * Ub / h) / (3 * MU_L * Ub / h))
+    assert errs[2] < errs[1] < errs[0] and errs[2] < 0.05
