@@ -47,6 +47,9 @@ STAGE_KERNEL = {"flux_x": "cf::k_flux_col<0>", "flux_y": "cf::k_flux_col<1>", "f
                 "lusgs_fwd": "cf::k_sweep_pipe<true, 3, 2, double>", "lusgs_bwd": "cf::k_sweep_pipe<false, 3, 2, double>",
                 "sweep_prep": "cf::k_sweep_prep_rhs<double>"}
 TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_256x192x256.md"
+WEIGHTS = ROOT / "channel_flow_amd" / "weights" / "predictor_g256x192x256.npz"
+MFMA_PEAK_TFLOPS = 2500.0  # dense bf16
+PHYS_SEED = 9  # seed of the physical-step legs: held out of the predictor's training (seeds 1-8)
 
 
 def measured_traffic(kernel):
@@ -120,57 +123,18 @@ def cpu_baseline(cfg, seconds_budget=30.0):
                       f"{rec['seconds']:.1f} s, {cpu_info['model']})"}
 
 
-def physical_steps(s, Q, steps=2, cap=800):
-    """From the seeded state (Q at both levels): physical steps, each with
-    inner iterations until max_k er_k/er_k(1) <= 1e-8 (k = 1..4), then the
-    level shift.  The second step, the first with a BDF2 history, is the one
-    reported; the iterate after it is returned for comparisons."""
+def timed(s, steps, warmup):
+    """ms per inner iteration over `steps` timed iterations after `warmup`."""
     import torch
-    s.load_state(Q, Q)
-    recs = []
-    for step in range(steps):
-        if step:
-            s.advance_levels()
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        first, its, ratio = None, 0, 1.0
-        while its < cap:
-            e = s.inner_iteration()
-            its += 1
-            if first is None:
-                first = e.copy()
-            ratio = float(np.max(e[:4] / first[:4]))
-            if ratio <= 1e-8:
-                break
-        torch.cuda.synchronize()
-        recs.append({"wall_s": round(time.perf_counter() - t1, 4), "inner_iterations": its, "residual_drop": ratio,
-                     "reached_1e-8": ratio <= 1e-8})
-    conv = dict(recs[-1], cap=cap, step=steps, earlier_steps=recs[:-1])
-    return conv, s.read_state()[2]
-
-
-def reference_test_steps(cfg, Q, precision, order, steps=6):
-    """Physical steps under the reference's own convergence test
-    (cf_physical_step: er1..er4 over their running maxima below 1e-3, at most
-    200 inner iterations) from the seeded state, with the given sweep
-    precision and initial-guess order; per-step iterations and wall times."""
-    import torch
-    from channel_flow_amd.solver import Solver
-    s = Solver(cfg, precision=precision)
-    s.Jacobian_all(cfg.gamma_metric, want_arrays=False)
-    s.set_initial_guess(order)
-    s.load_state(Q, Q)
-    its, wall = [], []
+    for _ in range(warmup):
+        s.inner_iteration()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    er = None
     for _ in range(steps):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        it, _ = s.physical_step()
-        torch.cuda.synchronize()
-        its.append(it)
-        wall.append(time.perf_counter() - t0)
-    qn = s.read_state()[0]
-    s.close()
-    return its, wall, qn
+        er = s.inner_iteration()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps * 1e3, er
 
 
 def mean_profiles(q):
@@ -183,38 +147,197 @@ def mean_profiles(q):
             "uv": m(u * v) - U * V}
 
 
-def fp32_sweeps_leg(cfg, Q, args, q64):
-    """The same measurements with CF_FP32_SWEEPS (LU-SGS sweeps and split
-    Jacobians in float inside the FP64 loop), and the converged state and
-    mean profiles against the FP64 run's."""
+def profile_errors(q, q64):
+    p, p64 = mean_profiles(q), mean_profiles(q64)
+    return {k: float(np.abs(p[k] - p64[k]).max() / np.abs(p64[k]).max()) for k in p64}
+
+
+def stage_profile(s, cfg, pairs):
+    """Per-stage HIP-event times of the eager inner iteration and the
+    roofline of the dominant single-kernel stage."""
+    fields = stage_fields(pairs)
+    stages = {}
+    for _ in range(3):
+        _, st = s.profile_inner_iteration()
+        for k, (ms, nl) in st.items():
+            stages.setdefault(k, []).append((ms, nl))
+    stage_ms = {k: float(np.median([m for m, _ in v])) for k, v in stages.items()}
+    launches = {k: v[0][1] for k, v in stages.items()}
+    if not launches.get("rhs"):
+        # the RHS runs inside the sweep pre-pass (k_sweep_prep_rhs)
+        fields["sweep_prep"], fields["rhs"] = fields["rhs"], 0
+    dom = max((k for k in STAGE_KERNEL if launches.get(k)), key=stage_ms.get)
+    per_launch_ms = stage_ms[dom] / launches[dom]
+    bytes_per_launch = fields[dom] * 8 * cfg.cells / launches[dom]
+    achieved = bytes_per_launch / (per_launch_ms * 1e-3) / 1e9
+    traffic = measured_traffic(STAGE_KERNEL[dom])
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": STAGE_KERNEL[dom],
+                "stage": dom, "launches_per_iteration": launches[dom], "avg_launch_us": round(per_launch_ms * 1e3, 3),
+                "algorithmic_bytes_per_launch": int(bytes_per_launch),
+                "algorithmic_fields_per_cell": fields[dom] // launches[dom],
+                "traffic_source": str(TRAFFIC_FILE.relative_to(ROOT)) if traffic else None}
+    per_stage = {k: {"ms": round(v, 4), "launches": launches[k],
+                     "GBps_algorithmic": (round(fields[k] * 8 * cfg.cells / (v * 1e-3) / 1e9, 1)
+                                          if fields[k] and v > 0 else None)} for k, v in stage_ms.items()}
+    return per_stage, roofline
+
+
+def physical_steps(cfg, flow, precision, order, pred, steps):
+    """Physical steps from the seeded state (seed PHYS_SEED, which the
+    predictor's training did not see) with the reference's convergence test
+    at tol 1e-8: er_k / E_k < 1e-8 for k = 1..4, E_k the running maxima of
+    the run (main.cpp:387-406); inner iterations run as graph replays."""
+    from channel_flow_amd.learned import run_steps
+    r, qn = run_steps(cfg, flow, PHYS_SEED, steps, order, pred, precision, graph=True)
+    return r, qn
+
+
+def predictor_leg(cfg, pred, meta):
+    """The predictor's kernels timed with HIP events: inference over the whole
+    256x192x256 domain and one training batch, with their MFMA rates."""
+    prof = pred.profile(*GRID)
+    cells = cfg.cells
+    # algorithmic FLOPs per cell: 27-point conv 2*27*64*64, pointwise 2*16*64
+    # and 2*64*5; training: forward (pointwise + conv) on the inner cells,
+    # backward data on every crop cell, backward weight on the inner cells
+    conv = 2 * 27 * 64 * 64
+    infer_flop = cells * (conv + 2 * 16 * 64 + 2 * 64 * 5)
+    B, inner, vol = pred.batch_crops, 14 ** 3, 16 ** 3
+    t = {"forward": B * inner * conv + B * vol * 2 * 16 * 64, "bwd_data": B * vol * conv, "bwd_weight": B * inner * conv}
+    mfma = {"infer": {"ms": round(prof["infer"], 4), "TFLOPs": round(infer_flop / prof["infer"] * 1e-9, 1),
+                      "frac_of_2.5_PFLOPs_bf16": round(infer_flop / prof["infer"] * 1e-9 / MFMA_PEAK_TFLOPS, 4),
+                      "kernel": "cfp::k_fwd<true, false>"},
+            "train_forward": {"ms": round(prof["train_forward"], 4),
+                              "TFLOPs": round(t["forward"] / prof["train_forward"] * 1e-9, 1)},
+            "train_bwd_data": {"ms": round(prof["train_bwd_data"], 4),
+                               "TFLOPs": round(t["bwd_data"] / prof["train_bwd_data"] * 1e-9, 1)},
+            "train_bwd_weight": {"ms": round(prof["train_bwd_weight"], 4),
+                                 "TFLOPs": round(t["bwd_weight"] / prof["train_bwd_weight"] * 1e-9, 1)},
+            "train_head_ms": round(prof["train_head"], 4), "train_reduce_ms": round(prof["train_reduce"], 4)}
+    for k in ("train_forward", "train_bwd_data", "train_bwd_weight"):
+        mfma[k]["frac_of_2.5_PFLOPs_bf16"] = round(mfma[k]["TFLOPs"] / MFMA_PEAK_TFLOPS, 4)
+    keep = ("grid", "seeds", "snapshots", "steps_per_seed", "crops_per_step", "n_crops", "train_steps",
+            "held_out_loss", "held_out_base_loss", "seconds")
+    return {"weights": str(WEIGHTS.relative_to(ROOT)), "training": {k: meta.get(k) for k in keep},
+            "kernels": mfma}
+
+
+def config5_leg(args):
+    """512x384x512, Re_b 10000, Mach 0.2, beta 2.2: FP32 sweeps with the
+    graph-captured inner iteration against plain FP64, throughput, device
+    memory in use, and the converged state / mean profiles against FP64."""
     import torch
-    from channel_flow_amd.solver import CF_FP32_SWEEPS, Solver
-    s = Solver(cfg, precision=CF_FP32_SWEEPS)
-    s.Jacobian_all(cfg.gamma_metric, want_arrays=False)
-    s.load_state(Q, Q)
-    for _ in range(args.warmup):
-        s.inner_iteration()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        s.inner_iteration()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    sts = [s.profile_inner_iteration()[1] for _ in range(3)]
-    stage_ms = {k: round(float(np.median([x[k][0] for x in sts])), 4) for k in sts[0]}
-    out = {"value": round(cfg.cells * args.steps / dt / 1e6, 3), "unit": "Mcell-iter/s",
-           "ms_per_step": round(dt / args.steps * 1e3, 4), "stages_ms": stage_ms}
-    if q64 is not None:
-        conv, q32 = physical_steps(s, Q)
-        out["phys_step_to_1e-8"] = conv
-        change = float(np.linalg.norm(q64 - Q))
-        diff = float(np.linalg.norm(q32 - q64))
-        p64, p32 = mean_profiles(q64), mean_profiles(q32)
-        out["vs_fp64"] = {
-            "state_rel_l2": diff / float(np.linalg.norm(q64)),
-            "state_diff_over_change": diff / change if change > 0 else None,
-            "profile_max_rel_err": {k: float(np.abs(p32[k] - p64[k]).max() / np.abs(p64[k]).max()) for k in p64}}
-    s.close()
+    from channel_flow_amd.learned import channel_cfg, run_steps
+    from channel_flow_amd.solver import CF_FP32_SWEEPS, CF_FP64, Predictor, Solver
+    from channel_flow_amd.config import poiseuille_state
+    cfg, flow = channel_cfg((512, 384, 512), beta=2.2, Re=10000.0, deltaT=1e-8, tol=1e-8)
+    Q = poiseuille_state(cfg, flow, seed=1, stream=0)
+    out = {"grid": "512x384x512", "cells": cfg.cells, "Re_bulk": 10000.0, "beta": 2.2, "deltaT": 1e-8}
+    for name, prec, graph in (("fp64_eager", CF_FP64, False), ("fp32_sweeps_graph", CF_FP32_SWEEPS, True)):
+        free0, total = torch.cuda.mem_get_info()
+        with Solver(cfg, precision=prec) as s:
+            s.Jacobian_all(cfg.gamma_metric, want_arrays=False)
+            s.set_graph(graph)
+            s.load_state(Q, Q)
+            free1, _ = torch.cuda.mem_get_info()
+            ms, _ = timed(s, 5, 2)
+        out[name] = {"ms_per_iteration": round(ms, 3), "Mcell_iter_s": round(cfg.cells / ms * 1e-3, 1),
+                     "device_memory_GB": round((free0 - free1) / 1e9, 2), "device_total_GB": round(total / 1e9, 1)}
+        log(f"config 5 {name}: {out[name]}")
+    steps = args.config5_steps
+    pred = None
+    if WEIGHTS.exists():
+        from channel_flow_amd.learned import load
+        pred = Predictor(dataset_crops=1, batch_crops=8)
+        load(pred, WEIGHTS)
+    r64, q64 = run_steps(cfg, flow, 1, steps, 0, None, CF_FP64, graph=True)
+    r32, q32 = run_steps(cfg, flow, 1, steps, 3 if pred else 2, pred, CF_FP32_SWEEPS, graph=True)
+    out["phys_steps_fp64_start_Un"] = {k: r64[k] for k in ("iterations", "wall_s")}
+    out["phys_steps_fp32_sweeps_" + ("predictor" if pred else "extrapolated")] = {
+        k: r32[k] for k in ("iterations", "wall_s", "predictor_steps", "fallbacks")}
+    out["predictor_weights"] = "trained at 256x192x256, Re_b 5600 (applied here with its fallback)" if pred else None
+    out["state_rel_l2_vs_fp64"] = float(np.linalg.norm(q32 - q64) / np.linalg.norm(q64))
+    out["profile_max_rel_err_vs_fp64"] = profile_errors(q32, q64)
+    if pred:
+        pred.close()
+    return out
+
+
+def read_st(path, ny):
+    """st%05d.dat (Statistic.cpp:1117-1146): seven blocks U, V, W, UU, VV,
+    WW, UV of ny - 1 lines (j = 2..ny)."""
+    v = np.array([float(x) for x in Path(path).read_text().split()])
+    return v.reshape(7, ny - 1)
+
+
+def long_runs(args):
+    """Config 2: 100 FP64 physical steps to a 1e-8 drop of er_k / E_k at
+    256x192x256 (iterations and wall time per step).  Config 3: 1000
+    physical steps under the reference's own test (er_k / E_k < 1e-3, at most
+    200 inner iterations, main.cpp:387-406) in FP64 and with FP32 sweeps,
+    with cf_statistic after every step (statistic_step 1000, the reference's
+    st file), and the L2 error of mean u(y), u'v'(y), u'u'(y) of the FP32 run
+    against the FP64 run."""
+    import torch
+    from channel_flow_amd.learned import channel_cfg
+    from channel_flow_amd.config import poiseuille_state
+    from channel_flow_amd.solver import CF_FP32_SWEEPS, CF_FP64, Solver
+    out = {}
+    cfg, flow = channel_cfg(GRID, beta=2.0, deltaT=1e-8, tol=1e-8, cap=2000)
+    Q = poiseuille_state(cfg, flow, seed=1, stream=0)
+    with Solver(cfg) as s:
+        s.Jacobian_all(cfg.gamma_metric, want_arrays=False)
+        s.set_graph(True)
+        s.load_state(Q, Q)
+        its, wall = [], []
+        for step in range(args.config2_steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            it, _ = s.physical_step()
+            torch.cuda.synchronize()
+            its.append(it)
+            wall.append(round(time.perf_counter() - t0, 4))
+            if step % 10 == 9:
+                log(f"config 2: {step + 1} steps, last {it} inner iterations {wall[-1]} s")
+    out["config2_fp64_tol_1e-8"] = {"steps": len(its), "iterations": its, "wall_s": wall,
+                                    "total_inner_iterations": int(sum(its)), "total_wall_s": round(sum(wall), 2),
+                                    "mean_wall_s_per_step": round(float(np.mean(wall)), 4)}
+    del Q
+    cfg3, _ = channel_cfg(GRID, beta=2.0, deltaT=1e-8, tol=1e-3, cap=200)
+    Q = poiseuille_state(cfg3, flow, seed=1, stream=0)
+    n = args.config3_steps
+    prof = {}
+    for name, prec in (("fp64", CF_FP64), ("fp32_sweeps", CF_FP32_SWEEPS)):
+        with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as td, \
+                Solver(cfg3, precision=prec) as s:
+            s.Jacobian_all(cfg3.gamma_metric, want_arrays=False)
+            s.set_graph(True)
+            s.load_state(Q, Q)
+            its = []
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for step in range(1, n + 1):
+                it, _ = s.physical_step()
+                its.append(it)
+                s.Statistic(step, n, None, td)
+                if step % 100 == 0:
+                    log(f"config 3 {name}: {step} steps")
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            prof[name] = read_st(Path(td) / f"st{n:05d}.dat", cfg3.ny)
+            shear = (Path(td) / "Shear_stress.dat").read_text().splitlines()
+        out[f"config3_{name}"] = {"steps": n, "total_inner_iterations": int(sum(its)), "wall_s": round(dt, 2),
+                                  "mean_inner_iterations": round(float(np.mean(its)), 2),
+                                  "wall_s_per_step": round(dt / n, 4),
+                                  "wall_shear_last_step": shear[-1].split("\t")}
+        log(f"config 3 {name}: {out[f'config3_{name}']}")
+    a, b = prof["fp32_sweeps"], prof["fp64"]
+    l2 = lambda x, y: float(np.linalg.norm(x - y) / np.linalg.norm(y))  # noqa: E731
+    U = (a[0], b[0])
+    uv = (a[6] - a[0] * a[1], b[6] - b[0] * b[1])
+    uu = (a[3] - a[0] * a[0], b[3] - b[0] * b[0])
+    out["config3_fp32_vs_fp64_profile_rel_l2"] = {"U": l2(*U), "u'v'": l2(*uv), "u'u'": l2(*uu)}
     return out
 
 
@@ -225,7 +348,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-converge", action="store_true", help="skip the physical steps to 1e-8")
-    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-sweep leg")
+    ap.add_argument("--phys-steps", type=int, default=8, help="physical steps per run of the 1e-8 legs")
+    ap.add_argument("--config5", action="store_true", help="also run the 512x384x512 leg (config 5)")
+    ap.add_argument("--config5-steps", type=int, default=3)
+    ap.add_argument("--long-runs", action="store_true", help="also run configs 2 and 3 (100 and 1000 steps)")
+    ap.add_argument("--config2-steps", type=int, default=100)
+    ap.add_argument("--config3-steps", type=int, default=1000)
+    ap.add_argument("--only", choices=["long-runs", "config5"], default=None,
+                    help="run only that leg and print its JSON")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain 1 GiB device copy (the reference kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -236,9 +366,12 @@ def main():
 
     import torch
     from channel_flow_amd.config import ChannelConfig, FlowParams, poiseuille_state
-    from channel_flow_amd.solver import Solver
+    from channel_flow_amd.solver import CF_FP32_SWEEPS, CF_FP64, Predictor, Solver
 
     torch.cuda.set_device(0)
+    if args.only:
+        print(json.dumps(long_runs(args) if args.only == "long-runs" else config5_leg(args)), flush=True)
+        return
     if args.copy_calibration:
         a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
         b = torch.empty_like(a)
@@ -250,51 +383,30 @@ def main():
         del a, b
     flow = FlowParams(Re_bulk=5600.0, Mach=0.2)
     cfg = ChannelConfig(*GRID, gamma_metric=2.0, gamma_grid=2.0, deltaT=1e-8,
-                        f_drive=flow.driving_force(0.005), x_periodic=False)
+                        f_drive=flow.driving_force(0.005), x_periodic=False, tol=1e-8, iteration_end_step=2000)
     Q = poiseuille_state(cfg, flow, seed=1, stream=0)
     log(f"grid {GRID}, cells {cfg.cells}")
+
+    # the inner iteration: graph replays, one executed sweep pair (the
+    # reference's second pair reproduces the first bit for bit in one domain)
     s = Solver(cfg)
     s.Jacobian_all(cfg.gamma_metric, want_arrays=False)
+    s.set_graph(True)
     s.load_state(Q, Q)
-    for _ in range(args.warmup):
-        s.inner_iteration()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    er = None
-    for _ in range(args.steps):
-        er = s.inner_iteration()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    ms_per_step = dt / args.steps * 1e3
-    value = cfg.cells * args.steps / dt / 1e6
-    log(f"timed {args.steps} inner iterations: {ms_per_step:.3f} ms/iter, {value:.1f} Mcell-iter/s, er={er}")
-
-    # per-stage HIP-event times (events on the context's stream)
-    fields = stage_fields(cfg.n_sweeps)
-    stages = {}
-    for _ in range(3):
-        _, st = s.profile_inner_iteration()
-        for k, (ms, nl) in st.items():
-            stages.setdefault(k, []).append((ms, nl))
-    stage_ms = {k: float(np.median([m for m, _ in v])) for k, v in stages.items()}
-    launches = {k: v[0][1] for k, v in stages.items()}
-    if not launches.get("rhs"):
-        # the RHS runs inside the sweep pre-pass (k_sweep_prep_rhs)
-        fields["sweep_prep"], fields["rhs"] = fields["rhs"], 0
-    # dominant kernel: the single-kernel stage with the largest time per iteration
-    dom = max((k for k in STAGE_KERNEL if launches.get(k)), key=stage_ms.get)
-    per_launch_ms = stage_ms[dom] / launches[dom]
-    bytes_per_launch = fields[dom] * 8 * cfg.cells / launches[dom]
-    achieved = bytes_per_launch / (per_launch_ms * 1e-3) / 1e9
-    traffic = measured_traffic(STAGE_KERNEL[dom])
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": STAGE_KERNEL[dom],
-                "stage": dom, "launches_per_iteration": launches[dom], "avg_launch_us": round(per_launch_ms * 1e3, 3),
-                "algorithmic_bytes_per_launch": int(bytes_per_launch),
-                "traffic_source": str(TRAFFIC_FILE.relative_to(ROOT)) if traffic else None}
-    per_stage = {k: {"ms": round(v, 4), "launches": launches[k],
-                     "GBps_algorithmic": (round(fields[k] * 8 * cfg.cells / (v * 1e-3) / 1e9, 1)
-                                          if fields[k] and v > 0 else None)} for k, v in stage_ms.items()}
+    ms_per_step, er = timed(s, args.steps, args.warmup)
+    value = cfg.cells / ms_per_step * 1e-3
+    log(f"timed {args.steps} inner iterations (graph, one sweep pair): {ms_per_step:.3f} ms/iter, "
+        f"{value:.1f} Mcell-iter/s, er={er}")
+    s.set_graph(False)
+    s.load_state(Q, Q)
+    ms_eager, _ = timed(s, args.steps, args.warmup)
+    s.set_sweep_reuse(False)
+    s.load_state(Q, Q)
+    ms_two, _ = timed(s, args.steps, args.warmup)
+    s.set_sweep_reuse(True)
+    log(f"eager launches: {ms_eager:.3f} ms/iter; both sweep pairs executed: {ms_two:.3f} ms/iter")
+    s.load_state(Q, Q)
+    per_stage, roofline = stage_profile(s, cfg, 1)
     log(f"stages: {json.dumps(per_stage)}")
 
     # variant kernels off the dual-time path (Filter sensor, limiter fluxes) on
@@ -311,36 +423,66 @@ def main():
                     "GBps_algorithmic": round(vfields[k] * 8 * cfg.cells / (float(np.median(v)) * 1e-3) / 1e9, 1)}
                 for k, v in vt.items()}
     log(f"variants: {json.dumps(variants)}")
-
-    conv = q64 = None
-    if not args.no_converge:
-        conv, q64 = physical_steps(s, Q)
-        log(f"physical steps to 1e-8 (FP64): {conv}")
     s.close()
 
-    fp32 = None
-    if not args.no_fp32:
-        fp32 = fp32_sweeps_leg(cfg, Q, args, q64)
-        log(f"FP32 sweeps: {json.dumps(fp32)}")
+    # FP32 sweeps: the same inner iteration
+    s = Solver(cfg, precision=CF_FP32_SWEEPS)
+    s.Jacobian_all(cfg.gamma_metric, want_arrays=False)
+    s.set_graph(True)
+    s.load_state(Q, Q)
+    ms32, _ = timed(s, args.steps, args.warmup)
+    s.load_state(Q, Q)
+    stages32, _ = stage_profile(s, cfg, 1)
+    s.close()
+    fp32 = {"value": round(cfg.cells / ms32 * 1e-3, 3), "unit": "Mcell-iter/s", "ms_per_step": round(ms32, 4),
+            "stages_ms": {k: v["ms"] for k, v in stages32.items()}}
+    log(f"FP32 sweeps: {json.dumps(fp32)}")
 
-    ref_test = None
+    pred, meta, predictor = None, None, None
+    if WEIGHTS.exists():
+        from channel_flow_amd.learned import load
+        pred = Predictor(dataset_crops=1, batch_crops=32)
+        meta = load(pred, WEIGHTS)
+        predictor = predictor_leg(cfg, pred, meta)
+        log(f"predictor: {json.dumps(predictor)}")
+
+    phys = None
     if not args.no_converge:
-        # the reference's convergence test over six physical steps: its own
-        # configuration (FP64, steps start at U^n), the FP32 sweeps, and the
-        # linearly extrapolated start
-        from channel_flow_amd.solver import CF_FP32_SWEEPS, CF_FP64
-        ref_test, base = {}, None
-        for name, prec, order in (("fp64_start_Un", CF_FP64, 0), ("fp32_sweeps_start_Un", CF_FP32_SWEEPS, 0),
-                                  ("fp64_start_linear", CF_FP64, 1)):
-            its, wall, qn = reference_test_steps(cfg, Q, prec, order)
-            rec = {"inner_iterations": its, "wall_s": [round(w, 4) for w in wall],
-                   "wall_s_per_step_3_to_6": round(float(np.mean(wall[2:])), 4)}
-            if base is None:
-                base = qn
-            else:
-                rec["state_rel_l2_vs_fp64_start_Un"] = float(np.linalg.norm(qn - base) / np.linalg.norm(base))
-            ref_test[name] = rec
-            log(f"reference convergence test, {name}: {rec}")
+        # wall time per physical step to a 1e-8 drop of er_k / E_k: the plain
+        # FP64 path (start U^n, the reference's) against FP32 sweeps with the
+        # learned start; mean over steps 4.. (the first ones are the seeded
+        # transient, and the predictor needs three levels)
+        n = args.phys_steps
+        runs = [("fp64_start_Un", CF_FP64, 0), ("fp64_extrapolated", CF_FP64, 2),
+                ("fp32_sweeps_start_Un", CF_FP32_SWEEPS, 0)]
+        if pred is not None:
+            runs += [("fp64_predictor", CF_FP64, 3), ("fp32_sweeps_predictor", CF_FP32_SWEEPS, 3)]
+        phys, states = {}, {}
+        for name, prec, order in runs:
+            r, qn = physical_steps(cfg, flow, prec, order, pred, n)
+            r["mean_iterations_steps_4_on"] = round(float(np.mean(r["iterations"][3:])), 2)
+            r["mean_wall_s_steps_4_on"] = round(float(np.mean(r["wall_s"][3:])), 4)
+            r["wall_s"] = [round(w, 4) for w in r["wall_s"]]
+            states[name] = qn
+            phys[name] = r
+            log(f"physical steps {name}: {r}")
+        base = states["fp64_start_Un"]
+        for name in phys:
+            if name != "fp64_start_Un":
+                phys[name]["state_rel_l2_vs_fp64_start_Un"] = float(np.linalg.norm(states[name] - base) /
+                                                                    np.linalg.norm(base))
+                phys[name]["profile_max_rel_err_vs_fp64_start_Un"] = profile_errors(states[name], base)
+        best = "fp32_sweeps_predictor" if "fp32_sweeps_predictor" in phys else "fp32_sweeps_start_Un"
+        phys["speedup_" + best + "_vs_fp64_start_Un"] = round(
+            phys["fp64_start_Un"]["mean_wall_s_steps_4_on"] / phys[best]["mean_wall_s_steps_4_on"], 3)
+        if pred is not None:
+            p = phys["fp64_predictor"]
+            phys["predictor_fallback_rate"] = p["fallbacks"] / p["predictor_steps"] if p["predictor_steps"] else None
+        del states
+    if pred is not None:
+        pred.close()
+    config5 = config5_leg(args) if args.config5 else None
+    longr = long_runs(args) if args.long_runs else None
     del Q
 
     cpu = None if args.no_cpu else cpu_baseline(cfg)
@@ -353,10 +495,21 @@ def main():
         "config": {"workload": "plain FP64 dual-time inner iteration, 256x192x256 channel "
                                "(2pi x 2 x pi, tanh beta=2.0, Re_b=5600, Mach 0.2, dt=1e-8)",
                    "grid": "x".join(map(str, GRID)), "cells": cfg.cells, "n_sweeps": cfg.n_sweeps,
+                   "sweeps_executed": "1 of 2 (the second pair reproduces the first bit for bit in one domain: "
+                                      "tests/test_gpu_graph_sweeps.py::test_gpu_one_sweep_pair_is_two, "
+                                      "tests/test_gpu_sizes.py::test_gpu_benchmark_grid_one_sweep_pair_and_graph, "
+                                      "and bit for bit against the reference's two pairs in "
+                                      "tests/test_gpu_sizes.py::test_gpu_benchmark_grid_matches_reference)",
+                   "launch": "hipGraph replay of each inner iteration (same bits as eager launches)",
                    "parallelism": "single GPU"},
-        "phys_step_to_1e-8": conv,
+        "ms_per_step_eager_launches": round(ms_eager, 4),
+        "ms_per_step_both_sweep_pairs_executed": round(ms_two, 4),
+        "Mcell_iter_s_both_sweep_pairs_executed": round(cfg.cells / ms_two * 1e-3, 3),
+        "phys_step_to_1e-8": phys,
         "fp32_sweeps": fp32,
-        "phys_steps_reference_test": ref_test,
+        "predictor": predictor,
+        "config5": config5,
+        "long_runs": longr,
         "stages_ms": per_stage,
         "variants_ms": variants,
         "roofline": roofline,

@@ -111,15 +111,22 @@ struct cf_ctx {
   struct GraphKey {
     const double *Ui, *Un, *Uq;
     int schedule, visc_schedule, solver, x_periodic, pairs;
-    bool fp32;
+    bool fp32, fork;
     double deltaT, f_drive;
     bool operator==(const GraphKey& o) const {
       return Ui == o.Ui && Un == o.Un && Uq == o.Uq && schedule == o.schedule && visc_schedule == o.visc_schedule &&
              solver == o.solver && x_periodic == o.x_periodic && pairs == o.pairs && fp32 == o.fp32 &&
+             fork == o.fork &&
              deltaT == o.deltaT && f_drive == o.f_drive;
     }
   } gkey{};
   long graph_builds = 0;
+  // the flux and viscous kernels of an inner iteration are independent: with
+  // `fork` on they run on the context's stream and three helper streams
+  // (default priority), joined by events, eagerly and inside the graph
+  bool fork = true;
+  hipStream_t aux[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t fork_ev = nullptr, join_ev[3] = {nullptr, nullptr, nullptr};
   long spin_limit = 1L << 22;
   double* partial = nullptr;
   double* er_dev = nullptr;
@@ -338,6 +345,12 @@ extern "C" void cf_destroy(cf_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  for (int q = 0; q < 3; q++) {
+    if (c->aux[q]) (void)hipStreamSynchronize(c->aux[q]);
+    if (c->aux[q]) (void)hipStreamDestroy(c->aux[q]);
+    if (c->join_ev[q]) (void)hipEventDestroy(c->join_ev[q]);
+  }
+  if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
   double* ptrs[] = {c->Ui, c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->MR,  c->NL,     c->NR,    c->Pr,
                     c->VS, c->Ep, c->Fw, c->ypt, c->Sm, c->partial, c->er_dev, c->prof, c->stat, c->Uqq, c->Hs, c->U3, c->Ualt, c->shear};
   for (double* q : ptrs)
@@ -1014,6 +1027,13 @@ extern "C" int cf_read_state(cf_ctx* c, double* const Qn[5], double* const Qnm1[
 static int prepare_iteration(cf_ctx* c) {
   if (!viscous_fused(c))
     if (int r = ensure_viscous_staging(c)) return r;
+  if (c->fork && !c->fork_ev) {
+    for (int q = 0; q < 3; q++) {
+      HIPCHK(c, hipStreamCreateWithFlags(&c->aux[q], hipStreamNonBlocking));
+      HIPCHK(c, hipEventCreateWithFlags(&c->join_ev[q], hipEventDisableTiming));
+    }
+    HIPCHK(c, hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
+  }
   if (c->solver == 0)
     if (int r = check_precision(c)) return r;
   return CF_OK;
@@ -1022,10 +1042,25 @@ static int prepare_iteration(cf_ctx* c) {
 static void enqueue_iteration(cf_ctx* c) {
   const Geo& g = c->g;
   cf::launch_halo_bc(c->stream, c->Ui, c->prof, g, c->p.x_periodic);
-  cf::launch_flux(c->stream, 0, c->Ui, c->prof, c->Fx, g);
-  cf::launch_flux(c->stream, 1, c->Ui, c->prof, c->Fy, g);
-  cf::launch_flux(c->stream, 2, c->Ui, c->prof, c->Fz, g);
-  cf::launch_viscous(c->stream, c->Ui, c->prof, c->Pr, c->VS, c->vF, g, viscous_fused(c));
+  if (c->fork && c->fork_ev) {
+    // Flux_X on the context's stream, Flux_Y, Flux_Z and Viscous_terms on
+    // the helper streams, all after the ghost cells, all before the RHS
+    (void)hipEventRecord(c->fork_ev, c->stream);
+    for (int q = 0; q < 3; q++) (void)hipStreamWaitEvent(c->aux[q], c->fork_ev, 0);
+    cf::launch_flux(c->stream, 0, c->Ui, c->prof, c->Fx, g);
+    cf::launch_flux(c->aux[0], 1, c->Ui, c->prof, c->Fy, g);
+    cf::launch_flux(c->aux[1], 2, c->Ui, c->prof, c->Fz, g);
+    cf::launch_viscous(c->aux[2], c->Ui, c->prof, c->Pr, c->VS, c->vF, g, viscous_fused(c));
+    for (int q = 0; q < 3; q++) {
+      (void)hipEventRecord(c->join_ev[q], c->aux[q]);
+      (void)hipStreamWaitEvent(c->stream, c->join_ev[q], 0);
+    }
+  } else {
+    cf::launch_flux(c->stream, 0, c->Ui, c->prof, c->Fx, g);
+    cf::launch_flux(c->stream, 1, c->Ui, c->prof, c->Fy, g);
+    cf::launch_flux(c->stream, 2, c->Ui, c->prof, c->Fz, g);
+    cf::launch_viscous(c->stream, c->Ui, c->prof, c->Pr, c->VS, c->vF, g, viscous_fused(c));
+  }
   if (c->solver == 1) {
     // DPLUSGS.cpp:189-1361: the RHS, four Jacobi sweeps, the update and norms
     cf::launch_rhs(c->stream, c->Ui, c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->prof, c->MR, c->NL, g,
@@ -1047,9 +1082,9 @@ static void enqueue_iteration(cf_ctx* c) {
 // again whenever the fields, schedules or parameters it was captured with
 // have changed
 static int launch_graph(cf_ctx* c) {
-  const cf_ctx::GraphKey key{c->Ui,         c->Un,        c->Uq,        c->schedule, c->visc_schedule,
-                             c->solver,     c->p.x_periodic, sweep_pairs(c), c->fp32,  c->p.deltaT,
-                             c->p.f_drive};
+  const cf_ctx::GraphKey key{c->Ui,     c->Un,          c->Uq,          c->schedule, c->visc_schedule,
+                             c->solver, c->p.x_periodic, sweep_pairs(c), c->fp32,     c->fork,
+                             c->p.deltaT, c->p.f_drive};
   if (!c->gexec || !(key == c->gkey)) {
     if (c->gexec) {
       HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1099,6 +1134,14 @@ extern "C" int cf_set_graph(cf_ctx* c, int enabled) {
 extern "C" int cf_graph_builds(cf_ctx* c, long long* builds) {
   if (!c || !builds) return CF_ERR_ARG;
   *builds = c->graph_builds;
+  return CF_OK;
+}
+
+extern "C" int cf_set_concurrent_stencils(cf_ctx* c, int enabled) {
+  if (!c) return CF_ERR_ARG;
+  for (int q = 0; q < 3; q++)
+    if (c->aux[q]) HIPCHK(c, hipStreamSynchronize(c->aux[q]));
+  c->fork = enabled != 0;
   return CF_OK;
 }
 

@@ -40,7 +40,7 @@ EXPORTS = [
     "cf_pred_get_params", "cf_pred_set_norm", "cf_pred_get_norm", "cf_pred_capture", "cf_pred_dataset_size",
     "cf_pred_fit_norm", "cf_pred_load_dataset", "cf_pred_train", "cf_pred_eval", "cf_pred_forward", "cf_pred_gradient", "cf_pred_profile",
     "cf_keep_levels", "cf_set_predictor", "cf_predictor_stats", "cf_set_sweep_reuse", "cf_set_graph",
-    "cf_graph_builds",
+    "cf_graph_builds", "cf_set_concurrent_stencils",
 ]
 CF_PRED_NPARAM = 112005
 CF_PRED_NNORM = 22
@@ -143,6 +143,7 @@ def load_library() -> C.CDLL:
     lib.cf_set_sweep_reuse.argtypes = [vp, C.c_int]
     lib.cf_set_graph.argtypes = [vp, C.c_int]
     lib.cf_graph_builds.argtypes = [vp, C.POINTER(C.c_longlong)]
+    lib.cf_set_concurrent_stencils.argtypes = [vp, C.c_int]
     lib.cf_set_predictor.argtypes = [vp, vp]
     lib.cf_predictor_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
     _lib = lib
@@ -453,6 +454,10 @@ class Solver:
     def set_graph(self, enabled: bool = True):
         """Replay each inner iteration's kernels as a captured hipGraph."""
         self._chk(self.lib.cf_set_graph(self._h, int(bool(enabled))))
+
+    def set_concurrent_stencils(self, enabled: bool = True):
+        """Fluxes and viscous terms on four streams (same bits)."""
+        self._chk(self.lib.cf_set_concurrent_stencils(self._h, int(bool(enabled))))
 
     def graph_builds(self) -> int:
         n = C.c_longlong()

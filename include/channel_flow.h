@@ -315,6 +315,12 @@ int cf_set_sweep_reuse(cf_ctx* ctx, int enabled);
 int cf_set_graph(cf_ctx* ctx, int enabled);
 /* Number of graph captures made so far. */
 int cf_graph_builds(cf_ctx* ctx, long long* builds);
+/* With enabled != 0 (the default) the inner iteration runs Flux_X, Flux_Y,
+ * Flux_Z and Viscous_terms, which all read the state after its ghost cells
+ * and write disjoint outputs, concurrently on the context's stream and three
+ * helper streams of default priority (fork and join by events, also inside
+ * the graph); 0 runs them one after another.  Same bits either way. */
+int cf_set_concurrent_stencils(cf_ctx* ctx, int enabled);
 
 /* --- learned initial iterate ------------------------------------------------
  * No counterpart in the reference, whose physical step starts its inner

@@ -24,13 +24,14 @@ def _need_gpu():
         pytest.fail("no GPU visible: the gpu tests must run on an MI355X")
 
 
-def _run(cfg, Q, iters, precision=0, reuse=True, graph=False, schedule=0, between=None):
+def _run(cfg, Q, iters, precision=0, reuse=True, graph=False, schedule=0, between=None, concurrent=True):
     from channel_flow_amd.solver import Solver
     with Solver(cfg, precision=precision) as s:
         s.Jacobian_all(cfg.gamma_metric, want_arrays=False)
         s.set_sweep_schedule(schedule)
         s.set_sweep_reuse(reuse)
         s.set_graph(graph)
+        s.set_concurrent_stencils(concurrent)
         s.load_state(Q, Q)
         ers = []
         for it in range(iters):
@@ -129,3 +130,17 @@ def test_gpu_graph_recaptures_on_change():
     np.testing.assert_array_equal(runs[0][0], runs[1][0])
     np.testing.assert_array_equal(runs[0][1], runs[1][1])
     assert runs[1][2] == 3
+
+
+@pytest.mark.parametrize("grid", [(21, 13, 18), (70, 45, 37)])
+@pytest.mark.parametrize("graph", [False, True])
+def test_gpu_concurrent_stencils_same_bits(grid, graph):
+    """Flux_X/Y/Z and Viscous_terms on four streams (the default) against one
+    after another, eagerly and inside the graph."""
+    from channel_flow_amd.config import ChannelConfig, FlowParams, poiseuille_state
+    cfg = ChannelConfig(*grid, gamma_metric=2.0, deltaT=5e-8)
+    Q = poiseuille_state(cfg, FlowParams(), seed=10, stream=0)
+    e1, s1, _ = _run(cfg, Q, 4, graph=graph, concurrent=True)
+    e2, s2, _ = _run(cfg, Q, 4, graph=graph, concurrent=False)
+    np.testing.assert_array_equal(e1, e2)
+    np.testing.assert_array_equal(s1, s2)

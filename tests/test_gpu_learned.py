@@ -1,0 +1,80 @@
+"""The learned initial iterate inside the solver (cf_set_predictor,
+cf_set_initial_guess(ctx, 3)).
+
+- The whole pipeline at a small grid: crops captured from the solver's own
+  converged steps, feature scales, training on the GPU, the predictor
+  attached; the converged state of physical steps started by it equals the
+  one started at U^n (the reference's start) to 1e-10 of the change over the
+  steps: the network changes the iteration count, never the answer.
+- What the trained network gains: with the committed weights (trained on the
+  GPU by channel_flow_amd.learned on 8 seeds at 64^3), on held-out seed 9,
+  the inner iterations per physical step to er_k / E_k < 1e-8 (the
+  reference's running-max normalisation, main.cpp:387-397) against the U^n
+  start and the quadratic extrapolation (the fallback), with the fallback
+  rate reported.
+"""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, gpu_available
+
+pytestmark = pytest.mark.gpu
+
+WEIGHTS64 = ROOT / "channel_flow_amd" / "weights" / "predictor_g64x64x64_dt5e-8.npz"
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if not gpu_available():
+        pytest.fail("no GPU visible: the gpu tests must run on an MI355X")
+
+
+def test_gpu_learned_pipeline_same_answer():
+    from channel_flow_amd import learned as L
+    from channel_flow_amd.config import poiseuille_state
+    from channel_flow_amd.solver import Predictor, Solver
+    cfg, flow = L.channel_cfg((32, 24, 40), beta=2.0, deltaT=5e-8, tol=1e-8, cap=3000)
+    with Predictor(dataset_crops=2 * 4 * 8, batch_crops=8) as pred:
+        its = L.generate(pred, cfg, flow, seeds=[1, 2], steps=6, crops_per_step=8)
+        assert pred.dataset_size() == 2 * 4 * 8
+        info = L.train(pred, steps=150, lr=2e-3)
+        assert np.all(np.isfinite(info["losses"])) and info["losses"][-1] < info["losses"][0]
+        # the same steps, converged tightly, from U^n and from the predictor
+        tight, _ = L.channel_cfg((32, 24, 40), beta=2.0, deltaT=5e-8, tol=1e-20, cap=6000)
+        Q = poiseuille_state(tight, flow, seed=3, stream=0)
+        finals = {}
+        for order in (0, 3):
+            with Solver(tight) as s:
+                s.Jacobian_all(2.0, want_arrays=False)
+                s.set_predictor(pred)
+                s.set_initial_guess(order)
+                s.load_state(Q, Q)
+                n_its = [s.physical_step()[0] for _ in range(5)]
+                steps, falls = s.predictor_stats()
+                finals[order] = s.read_state()[0]
+            print(f"order {order}: iterations {n_its}, predictor steps {steps}, fallbacks {falls}", flush=True)
+            # the first step has no U^{n-2} yet and starts from the extrapolation
+            assert steps == (4 if order == 3 else 0)
+    change = np.linalg.norm(finals[0] - np.asarray(Q))
+    diff = np.linalg.norm(finals[3] - finals[0])
+    print(f"state difference {diff / change:.2e} of the change over 5 steps", flush=True)
+    assert diff <= 1e-10 * change
+    del its
+
+
+def test_gpu_learned_gain_64cubed():
+    from channel_flow_amd import learned as L
+    from channel_flow_amd.solver import Predictor
+    assert WEIGHTS64.exists()
+    cfg, flow = L.channel_cfg((64, 64, 64), beta=2.0, deltaT=5e-8, tol=1e-8)
+    with Predictor(dataset_crops=1, batch_crops=8) as pred:
+        meta = L.load(pred, WEIGHTS64)
+        assert 9 not in meta["seeds"]
+        res = L.compare(cfg, flow, pred, seed=9, steps=8, skip=3)
+    its = {k: res[k]["mean_iterations"] for k in ("start_Un", "extrapolated", "predictor")}
+    print(f"held-out seed 9, steps 4-8, inner iterations per physical step: {its}; fallback rate "
+          f"{res['fallback_rate']}; wall gain vs U^n {res['gain_wall_vs_start_Un']:.2f}", flush=True)
+    assert its["predictor"] < 0.85 * min(its["start_Un"], its["extrapolated"])
+    assert res["fallback_rate"] is not None and res["fallback_rate"] <= 0.5
