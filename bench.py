@@ -43,11 +43,13 @@ def stage_fields(n_sweeps):
 
 # stages that are one kernel per launch, and that kernel's name in rocprofv3
 STAGE_KERNEL = {"flux_x": "cf::k_flux_col<0>", "flux_y": "cf::k_flux_col<1>", "flux_z": "cf::k_flux_zs<2>",
-                "rhs": "cf::k_rhs",
+                "rhs": "cf::k_rhs", "viscous": "cf::k_visc_fused",
                 "lusgs_fwd": "cf::k_sweep_pipe<true, 3, 2, double>", "lusgs_bwd": "cf::k_sweep_pipe<false, 3, 2, double>",
                 "sweep_prep": "cf::k_sweep_prep_rhs<double>"}
 TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_256x192x256.md"
+VALU_FILE = ROOT / "profiles" / "valu_inner_iteration_256x192x256.md"
 WEIGHTS = ROOT / "channel_flow_amd" / "weights" / "predictor_g256x192x256.npz"
+CONFIG5_DT = 2.5e-9
 MFMA_PEAK_TFLOPS = 2500.0  # dense bf16
 PHYS_SEED = 9  # seed of the physical-step legs: held out of the predictor's training (seeds 1-8)
 
@@ -61,6 +63,24 @@ def measured_traffic(kernel):
             if len(cols) >= 2 and cols[0] == kernel:
                 return int(float(cols[-1]))
     except (OSError, ValueError):
+        return None
+    return None
+
+
+def valu_floor_us(kernel):
+    """VALU-issue floor of one launch of `kernel` in microseconds (its VALU
+    instructions / (256 CUs x 2.4 GHz)) from the committed rocprofv3 SQ
+    counter summary of this workload, or None."""
+    try:
+        lines = VALU_FILE.read_text().splitlines()
+        head = next(ln for ln in lines if ln.startswith("| kernel"))
+        names = [c.strip() for c in head.strip().strip("|").split("|")]
+        col = names.index("VALU-bound us")
+        for ln in lines:
+            cols = [c.strip().strip("`") for c in ln.strip().strip("|").split("|")]
+            if len(cols) == len(names) and cols[0] == kernel:
+                return float(cols[col])
+    except (OSError, ValueError, StopIteration):
         return None
     return None
 
@@ -177,9 +197,17 @@ def stage_profile(s, cfg, pairs):
                 "algorithmic_bytes_per_launch": int(bytes_per_launch),
                 "algorithmic_fields_per_cell": fields[dom] // launches[dom],
                 "traffic_source": str(TRAFFIC_FILE.relative_to(ROOT)) if traffic else None}
-    per_stage = {k: {"ms": round(v, 4), "launches": launches[k],
-                     "GBps_algorithmic": (round(fields[k] * 8 * cfg.cells / (v * 1e-3) / 1e9, 1)
-                                          if fields[k] and v > 0 else None)} for k, v in stage_ms.items()}
+    per_stage = {}
+    for k, v in stage_ms.items():
+        gbs = fields[k] * 8 * cfg.cells / (v * 1e-3) / 1e9 if fields[k] and v > 0 else None
+        # the whole stage's VALU floor over its measured time (stages of one kernel)
+        floor = valu_floor_us(STAGE_KERNEL[k]) if k in STAGE_KERNEL and cfg.cells == GRID[0] * GRID[1] * GRID[2] \
+            else None
+        per_stage[k] = {"ms": round(v, 4), "launches": launches[k],
+                        "GBps_algorithmic": round(gbs, 1) if gbs else None,
+                        "hbm_frac": round(gbs / HBM_PEAK_GBS, 4) if gbs else None,
+                        "valu_floor_frac": (round(floor * launches[k] * 1e-3 / v, 4)
+                                            if floor is not None and v > 0 else None)}
     return per_stage, roofline
 
 
@@ -231,9 +259,12 @@ def config5_leg(args):
     from channel_flow_amd.learned import channel_cfg, run_steps
     from channel_flow_amd.solver import CF_FP32_SWEEPS, CF_FP64, Predictor, Solver
     from channel_flow_amd.config import poiseuille_state
-    cfg, flow = channel_cfg((512, 384, 512), beta=2.2, Re=10000.0, deltaT=1e-8, tol=1e-8)
+    # the wall cells are 2.7x thinner than at 256x192x256, beta 2.0: 1e-8 s
+    # diverges in the first step, 5e-9 s converges without a downward trend
+    # in the iteration count, 2.5e-9 s converges steadily
+    cfg, flow = channel_cfg((512, 384, 512), beta=2.2, Re=10000.0, deltaT=CONFIG5_DT, tol=1e-8)
     Q = poiseuille_state(cfg, flow, seed=1, stream=0)
-    out = {"grid": "512x384x512", "cells": cfg.cells, "Re_bulk": 10000.0, "beta": 2.2, "deltaT": 1e-8}
+    out = {"grid": "512x384x512", "cells": cfg.cells, "Re_bulk": 10000.0, "beta": 2.2, "deltaT": CONFIG5_DT}
     for name, prec, graph in (("fp64_eager", CF_FP64, False), ("fp32_sweeps_graph", CF_FP32_SWEEPS, True)):
         free0, total = torch.cuda.mem_get_info()
         with Solver(cfg, precision=prec) as s:
@@ -348,6 +379,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-converge", action="store_true", help="skip the physical steps to 1e-8")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-sweep leg")
+    ap.add_argument("--no-predictor", action="store_true", help="skip the learned-predictor legs")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every kernel eagerly (for counter collection); the timed value then has no graph")
+    ap.add_argument("--serial-stencils", action="store_true",
+                    help="run the fluxes and the viscous terms one after another on one stream (per-kernel traces)")
     ap.add_argument("--phys-steps", type=int, default=8, help="physical steps per run of the 1e-8 legs")
     ap.add_argument("--config5", action="store_true", help="also run the 512x384x512 leg (config 5)")
     ap.add_argument("--config5-steps", type=int, default=3)
@@ -391,7 +428,8 @@ def main():
     # reference's second pair reproduces the first bit for bit in one domain)
     s = Solver(cfg)
     s.Jacobian_all(cfg.gamma_metric, want_arrays=False)
-    s.set_graph(True)
+    s.set_graph(not args.no_graph)
+    s.set_concurrent_stencils(not args.serial_stencils)
     s.load_state(Q, Q)
     ms_per_step, er = timed(s, args.steps, args.warmup)
     value = cfg.cells / ms_per_step * 1e-3
@@ -425,21 +463,24 @@ def main():
     log(f"variants: {json.dumps(variants)}")
     s.close()
 
-    # FP32 sweeps: the same inner iteration
-    s = Solver(cfg, precision=CF_FP32_SWEEPS)
-    s.Jacobian_all(cfg.gamma_metric, want_arrays=False)
-    s.set_graph(True)
-    s.load_state(Q, Q)
-    ms32, _ = timed(s, args.steps, args.warmup)
-    s.load_state(Q, Q)
-    stages32, _ = stage_profile(s, cfg, 1)
-    s.close()
-    fp32 = {"value": round(cfg.cells / ms32 * 1e-3, 3), "unit": "Mcell-iter/s", "ms_per_step": round(ms32, 4),
-            "stages_ms": {k: v["ms"] for k, v in stages32.items()}}
-    log(f"FP32 sweeps: {json.dumps(fp32)}")
+    fp32 = None
+    if not args.no_fp32:
+        # FP32 sweeps: the same inner iteration
+        s = Solver(cfg, precision=CF_FP32_SWEEPS)
+        s.Jacobian_all(cfg.gamma_metric, want_arrays=False)
+        s.set_graph(not args.no_graph)
+        s.set_concurrent_stencils(not args.serial_stencils)
+        s.load_state(Q, Q)
+        ms32, _ = timed(s, args.steps, args.warmup)
+        s.load_state(Q, Q)
+        stages32, _ = stage_profile(s, cfg, 1)
+        s.close()
+        fp32 = {"value": round(cfg.cells / ms32 * 1e-3, 3), "unit": "Mcell-iter/s", "ms_per_step": round(ms32, 4),
+                "stages_ms": {k: v["ms"] for k, v in stages32.items()}}
+        log(f"FP32 sweeps: {json.dumps(fp32)}")
 
     pred, meta, predictor = None, None, None
-    if WEIGHTS.exists():
+    if WEIGHTS.exists() and not args.no_predictor:
         from channel_flow_amd.learned import load
         pred = Predictor(dataset_crops=1, batch_crops=32)
         meta = load(pred, WEIGHTS)
@@ -453,10 +494,13 @@ def main():
         # learned start; mean over steps 4.. (the first ones are the seeded
         # transient, and the predictor needs three levels)
         n = args.phys_steps
-        runs = [("fp64_start_Un", CF_FP64, 0), ("fp64_extrapolated", CF_FP64, 2),
-                ("fp32_sweeps_start_Un", CF_FP32_SWEEPS, 0)]
+        runs = [("fp64_start_Un", CF_FP64, 0), ("fp64_extrapolated", CF_FP64, 2)]
+        if not args.no_fp32:
+            runs.append(("fp32_sweeps_start_Un", CF_FP32_SWEEPS, 0))
         if pred is not None:
-            runs += [("fp64_predictor", CF_FP64, 3), ("fp32_sweeps_predictor", CF_FP32_SWEEPS, 3)]
+            runs.append(("fp64_predictor", CF_FP64, 3))
+            if not args.no_fp32:
+                runs.append(("fp32_sweeps_predictor", CF_FP32_SWEEPS, 3))
         phys, states = {}, {}
         for name, prec, order in runs:
             r, qn = physical_steps(cfg, flow, prec, order, pred, n)
@@ -472,9 +516,10 @@ def main():
                 phys[name]["state_rel_l2_vs_fp64_start_Un"] = float(np.linalg.norm(states[name] - base) /
                                                                     np.linalg.norm(base))
                 phys[name]["profile_max_rel_err_vs_fp64_start_Un"] = profile_errors(states[name], base)
-        best = "fp32_sweeps_predictor" if "fp32_sweeps_predictor" in phys else "fp32_sweeps_start_Un"
-        phys["speedup_" + best + "_vs_fp64_start_Un"] = round(
-            phys["fp64_start_Un"]["mean_wall_s_steps_4_on"] / phys[best]["mean_wall_s_steps_4_on"], 3)
+        best = [k for k in ("fp32_sweeps_predictor", "fp64_predictor", "fp32_sweeps_start_Un") if k in phys]
+        if best:
+            phys["speedup_" + best[0] + "_vs_fp64_start_Un"] = round(
+                phys["fp64_start_Un"]["mean_wall_s_steps_4_on"] / phys[best[0]]["mean_wall_s_steps_4_on"], 3)
         if pred is not None:
             p = phys["fp64_predictor"]
             phys["predictor_fallback_rate"] = p["fallbacks"] / p["predictor_steps"] if p["predictor_steps"] else None
@@ -500,7 +545,10 @@ def main():
                                       "tests/test_gpu_sizes.py::test_gpu_benchmark_grid_one_sweep_pair_and_graph, "
                                       "and bit for bit against the reference's two pairs in "
                                       "tests/test_gpu_sizes.py::test_gpu_benchmark_grid_matches_reference)",
-                   "launch": "hipGraph replay of each inner iteration (same bits as eager launches)",
+                   "launch": ("eager launches" if args.no_graph else
+                              "hipGraph replay of each inner iteration (same bits as eager launches)") +
+                             ("; fluxes and viscous terms one after another" if args.serial_stencils else
+                              "; fluxes and viscous terms on four streams"),
                    "parallelism": "single GPU"},
         "ms_per_step_eager_launches": round(ms_eager, 4),
         "ms_per_step_both_sweep_pairs_executed": round(ms_two, 4),

@@ -25,3 +25,4 @@ def test_bench_kernel_names_are_in_the_committed_profiles():
         assert f"`{kernel}`" in stats, kernel
         assert bench.measured_traffic(kernel), kernel
         assert f"`{kernel}`" in traffic, kernel
+        assert bench.valu_floor_us(kernel), kernel
