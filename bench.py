@@ -50,6 +50,9 @@ TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_256x192x256.md"
 VALU_FILE = ROOT / "profiles" / "valu_inner_iteration_256x192x256.md"
 WEIGHTS = ROOT / "channel_flow_amd" / "weights" / "predictor_g256x192x256.npz"
 CONFIG5_DT = 2.5e-9
+# trained on a 64x384x64 periodic patch with the 512x384x512 cell sizes,
+# Re_b 10000, beta 2.2 and the time step above
+CONFIG5_WEIGHTS = ROOT / "channel_flow_amd" / "weights" / "predictor_g512x384x512_re10000.npz"
 MFMA_PEAK_TFLOPS = 2500.0  # dense bf16
 PHYS_SEED = 9  # seed of the physical-step legs: held out of the predictor's training (seeds 1-8)
 
@@ -277,17 +280,27 @@ def config5_leg(args):
                      "device_memory_GB": round((free0 - free1) / 1e9, 2), "device_total_GB": round(total / 1e9, 1)}
         log(f"config 5 {name}: {out[name]}")
     steps = args.config5_steps
-    pred = None
-    if WEIGHTS.exists():
+    pred, meta = None, None
+    wpath = CONFIG5_WEIGHTS if CONFIG5_WEIGHTS.exists() else WEIGHTS
+    if wpath.exists():
         from channel_flow_amd.learned import load
         pred = Predictor(dataset_crops=1, batch_crops=8)
-        load(pred, WEIGHTS)
+        meta = load(pred, wpath)
     r64, q64 = run_steps(cfg, flow, 1, steps, 0, None, CF_FP64, graph=True)
     r32, q32 = run_steps(cfg, flow, 1, steps, 3 if pred else 2, pred, CF_FP32_SWEEPS, graph=True)
     out["phys_steps_fp64_start_Un"] = {k: r64[k] for k in ("iterations", "wall_s")}
     out["phys_steps_fp32_sweeps_" + ("predictor" if pred else "extrapolated")] = {
         k: r32[k] for k in ("iterations", "wall_s", "predictor_steps", "fallbacks")}
-    out["predictor_weights"] = "trained at 256x192x256, Re_b 5600 (applied here with its fallback)" if pred else None
+    out["predictor_weights"] = ({"file": str(wpath.relative_to(ROOT)), "grid": meta.get("grid"),
+                                 "cells_of": meta.get("cells_of"), "Re_bulk": meta.get("Re_bulk"),
+                                 "deltaT": meta.get("deltaT"), "seeds": meta.get("seeds")} if pred else None)
+    skip = min(2, steps - 1)  # the first steps have no third time level yet
+    m64 = float(np.mean(r64["wall_s"][skip:]))
+    m32 = float(np.mean(r32["wall_s"][skip:]))
+    out["mean_over_steps"] = f"{skip + 1}-{steps}"
+    out["mean_iterations_fp64"] = float(np.mean(r64["iterations"][skip:]))
+    out["mean_iterations_fp32_sweeps_predictor"] = float(np.mean(r32["iterations"][skip:]))
+    out["speedup_wall_vs_fp64"] = round(m64 / m32, 3)
     out["state_rel_l2_vs_fp64"] = float(np.linalg.norm(q32 - q64) / np.linalg.norm(q64))
     out["profile_max_rel_err_vs_fp64"] = profile_errors(q32, q64)
     if pred:
@@ -387,7 +400,7 @@ def main():
                     help="run the fluxes and the viscous terms one after another on one stream (per-kernel traces)")
     ap.add_argument("--phys-steps", type=int, default=8, help="physical steps per run of the 1e-8 legs")
     ap.add_argument("--config5", action="store_true", help="also run the 512x384x512 leg (config 5)")
-    ap.add_argument("--config5-steps", type=int, default=3)
+    ap.add_argument("--config5-steps", type=int, default=6)
     ap.add_argument("--long-runs", action="store_true", help="also run configs 2 and 3 (100 and 1000 steps)")
     ap.add_argument("--config2-steps", type=int, default=100)
     ap.add_argument("--config3-steps", type=int, default=1000)

@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import time
 from pathlib import Path
 
@@ -30,12 +31,20 @@ from .solver import CF_FP32_SWEEPS, CF_FP64, Predictor, Solver
 HIGH = 0.005
 
 
-def channel_cfg(grid, beta=2.0, Re=5600.0, deltaT=1e-8, tol=1e-8, cap=2000):
+def channel_cfg(grid, beta=2.0, Re=5600.0, deltaT=1e-8, tol=1e-8, cap=2000, cells_of=None):
     """The channel of the benchmark (2pi x 2 x pi h, tanh beta, Re_b, Mach 0.2,
-    laminar driving force) on `grid`, with the physical-step test tol."""
+    laminar driving force) on `grid`, with the physical-step test tol.
+
+    cells_of = (X, Z): a periodic patch of the X x Z channel instead, with
+    its cell sizes (Lx = 2 pi h grid[0] / X, Lz = pi h grid[2] / Z).  The
+    network sees 3x3x3 neighbourhoods, so weights trained on such a patch
+    serve the full grid of the same spacings, Re_b, beta and time step."""
     flow = FlowParams(Re_bulk=Re, Mach=0.2)
+    extra = {}
+    if cells_of is not None:
+        extra = dict(Lx=2 * math.pi * HIGH * grid[0] / cells_of[0], Lz=math.pi * HIGH * grid[2] / cells_of[1])
     cfg = ChannelConfig(*grid, gamma_metric=beta, gamma_grid=beta, deltaT=deltaT,
-                        f_drive=flow.driving_force(HIGH), iteration_end_step=cap, tol=tol)
+                        f_drive=flow.driving_force(HIGH), iteration_end_step=cap, tol=tol, **extra)
     return cfg, flow
 
 
@@ -181,10 +190,12 @@ def main():
     ap.add_argument("--out", default=None)
     ap.add_argument("--seed", type=int, default=9, help="held-out seed of the evaluation")
     ap.add_argument("--eval-steps", type=int, default=12)
+    ap.add_argument("--cells-of", type=int, nargs=2, default=None, metavar=("X", "Z"),
+                    help="train on a periodic patch with the cell sizes of the X x Z channel")
     a = ap.parse_args()
     import torch
     torch.cuda.set_device(0)
-    cfg, flow = channel_cfg(tuple(a.grid), a.beta, a.re, a.dt, a.tol)
+    cfg, flow = channel_cfg(tuple(a.grid), a.beta, a.re, a.dt, a.tol, cells_of=a.cells_of)
     precision = CF_FP32_SWEEPS if a.fp32_sweeps else CF_FP64
     seeds = _seeds(a.seeds)
     if a.mode == "train":
@@ -192,7 +203,7 @@ def main():
         with Predictor(dataset_crops=max(n_data, 1), batch_crops=a.batch) as pred:
             its = generate(pred, cfg, flow, seeds, a.steps, a.crops, precision)
             info = train(pred, a.train_steps, a.lr)
-            meta = {"grid": a.grid, "beta": a.beta, "Re_bulk": a.re, "deltaT": a.dt, "tol": a.tol, "seeds": seeds,
+            meta = {"grid": a.grid, "cells_of": a.cells_of, "beta": a.beta, "Re_bulk": a.re, "deltaT": a.dt, "tol": a.tol, "seeds": seeds,
                     "steps_per_seed": a.steps, "crops_per_step": a.crops, "snapshots": len(seeds) * (a.steps - 2),
                     "train_steps": a.train_steps, "lr": a.lr, "batch_crops": a.batch,
                     "data_iterations": {str(k): v for k, v in its.items()},

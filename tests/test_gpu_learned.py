@@ -12,6 +12,9 @@ cf_set_initial_guess(ctx, 3)).
   reference's running-max normalisation, main.cpp:387-397) against the U^n
   start and the quadratic extrapolation (the fallback), with the fallback
   rate reported.
+- The same for the 512x384x512 weights (Re_b 10000, beta 2.2, dt 2.5e-9 s),
+  trained on a 64x384x64 periodic patch with that grid's cell sizes, on the
+  patch and held-out seed 1.
 """
 from pathlib import Path
 
@@ -23,6 +26,7 @@ from conftest import ROOT, gpu_available
 pytestmark = pytest.mark.gpu
 
 WEIGHTS64 = ROOT / "channel_flow_amd" / "weights" / "predictor_g64x64x64_dt5e-8.npz"
+WEIGHTS512 = ROOT / "channel_flow_amd" / "weights" / "predictor_g512x384x512_re10000.npz"
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -78,5 +82,21 @@ def test_gpu_learned_gain_64cubed():
     its = {k: res[k]["mean_iterations"] for k in ("start_Un", "extrapolated", "predictor")}
     print(f"held-out seed 9, steps 4-8, inner iterations per physical step: {its}; fallback rate "
           f"{res['fallback_rate']}; wall gain vs U^n {res['gain_wall_vs_start_Un']:.2f}", flush=True)
+    assert its["predictor"] < 0.85 * min(its["start_Un"], its["extrapolated"])
+    assert res["fallback_rate"] is not None and res["fallback_rate"] <= 0.5
+
+
+def test_gpu_learned_gain_512_patch():
+    from channel_flow_amd import learned as L
+    from channel_flow_amd.solver import Predictor
+    assert WEIGHTS512.exists()
+    cfg, flow = L.channel_cfg((64, 384, 64), beta=2.2, Re=10000.0, deltaT=2.5e-9, tol=1e-8, cells_of=(512, 512))
+    with Predictor(dataset_crops=1, batch_crops=8) as pred:
+        meta = L.load(pred, WEIGHTS512)
+        assert 1 not in meta["seeds"] and meta["cells_of"] == [512, 512]
+        res = L.compare(cfg, flow, pred, seed=1, steps=8, skip=3)
+    its = {k: res[k]["mean_iterations"] for k in ("start_Un", "extrapolated", "predictor")}
+    print(f"patch of 512x384x512, held-out seed 1, steps 4-8: {its}; fallback rate {res['fallback_rate']}",
+          flush=True)
     assert its["predictor"] < 0.85 * min(its["start_Un"], its["extrapolated"])
     assert res["fallback_rate"] is not None and res["fallback_rate"] <= 0.5
