@@ -250,7 +250,7 @@ def predictor_leg(cfg, pred, meta):
         mfma[k]["frac_of_2.5_PFLOPs_bf16"] = round(mfma[k]["TFLOPs"] / MFMA_PEAK_TFLOPS, 4)
     keep = ("grid", "seeds", "snapshots", "steps_per_seed", "crops_per_step", "n_crops", "train_steps",
             "held_out_loss", "held_out_base_loss", "seconds")
-    return {"weights": str(WEIGHTS.relative_to(ROOT)), "training": {k: meta.get(k) for k in keep},
+    return {"weights": str(pred.path), "training": {k: meta.get(k) for k in keep},
             "kernels": mfma}
 
 
@@ -396,6 +396,8 @@ def main():
     ap.add_argument("--no-predictor", action="store_true", help="skip the learned-predictor legs")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every kernel eagerly (for counter collection); the timed value then has no graph")
+    ap.add_argument("--weights", default=None,
+                    help="predictor weights for the 256x192x256 legs (default: the committed ones)")
     ap.add_argument("--serial-stencils", action="store_true",
                     help="run the fluxes and the viscous terms one after another on one stream (per-kernel traces)")
     ap.add_argument("--phys-steps", type=int, default=8, help="physical steps per run of the 1e-8 legs")
@@ -493,10 +495,12 @@ def main():
         log(f"FP32 sweeps: {json.dumps(fp32)}")
 
     pred, meta, predictor = None, None, None
-    if WEIGHTS.exists() and not args.no_predictor:
+    wpath = Path(args.weights).resolve() if args.weights else WEIGHTS
+    if wpath.exists() and not args.no_predictor:
         from channel_flow_amd.learned import load
         pred = Predictor(dataset_crops=1, batch_crops=32)
-        meta = load(pred, WEIGHTS)
+        meta = load(pred, wpath)
+        pred.path = wpath.relative_to(ROOT) if wpath.is_relative_to(ROOT) else wpath
         predictor = predictor_leg(cfg, pred, meta)
         log(f"predictor: {json.dumps(predictor)}")
 

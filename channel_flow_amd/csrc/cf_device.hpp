@@ -74,14 +74,16 @@ __device__ __forceinline__ void xcd_block3(int* bx, int* by, int* bz) {
 }
 
 // x/d for a divisor d fixed by the grid, with r = RN(1/d):
-//   q0 = x r,  q = q0 + fma(-q0, d, x) r  (both steps single roundings).
-// When |r d - 1| <= 2^-54, q0 is within one ulp of x/d for every x, and
-// Markstein's theorem (IBM J. Res. Dev. 34(1), 1990) makes q the correctly
-// rounded x/d: the same bits as the division, except that -0/d gives +0.
+//   q0 = x r,  e = fma(q0, d, -x) = q0 d - x,  q = fma(-e, r, q0)
+// (each step one rounding).  When |r d - 1| <= 2^-54, q0 is within one ulp of
+// x/d for every x, the remainder e is exact, and Markstein's theorem (IBM J.
+// Res. Dev. 34(1), 1990) makes q the correctly rounded x/d: the same bits as
+// the division.  Forming e as q0 d - x keeps the sign of a zero quotient
+// (x = -0 gives e = +0 and q = -0 + -0).
 __device__ __forceinline__ double div_const(double x, double d, double r) {
   const double q0 = x * r;
-  const double e = fma(-q0, d, x);
-  return fma(e, r, q0);
+  const double e = fma(q0, d, -x);
+  return fma(-e, r, q0);
 }
 
 // RN(1/d) when it meets the bound above (r d - 1 is exact in one fma), else 0

@@ -2,15 +2,17 @@
 K-1, the face J and 1/|n| in the Roe flux) with a residual-corrected
 reciprocal (Markstein, IBM J. Res. Dev. 34(1), 1990):
 
-    q0 = x * r,  r = RN(1/d);   q = fma(fma(-q0, d, x), r, q0)
+    q0 = x * r,  r = RN(1/d);   q = fma(-fma(q0, d, -x), r, q0)
 
-which must equal the IEEE quotient x/d bit for bit.  When |r d - 1| <= 2^-54,
+which must equal the IEEE quotient x/d bit for bit, the sign of a zero
+quotient included.  When |r d - 1| <= 2^-54,
 q0 = RN(x r) is within one ulp of x/d for every x, and Markstein's theorem
 makes q the correctly rounded quotient.  The library checks that bound for
 each divisor (rcp_checked) and divides plainly where it fails.  Here the bound
 is checked exactly, with rationals, for K-1 and the spacings of every grid
 used, and the identity itself with C's correctly rounded fma() over operands
-spanning 2^-40 .. 2^23 of both signs.
+spanning 2^-40 .. 2^23 of both signs and for +0 and -0; a planted variant,
+the residual formed as x - q0 d, must fail on -0 (it returns +0).
 """
 from fractions import Fraction
 import math
@@ -23,30 +25,34 @@ SRC = textwrap.dedent(r"""
     #include <math.h>
     #include <stdint.h>
     #include <stdio.h>
+    #include <string.h>
+    static int same(double a, double b) { return memcmp(&a, &b, sizeof a) == 0; }
     static uint64_t s = 88172645463325252ULL;
     static uint64_t xr(void) { s ^= s << 13; s ^= s >> 7; s ^= s << 17; return s; }
     int main(void) {
       const double hi = 0.005, pi = 3.14159265358979323846;
       const int grids[][3] = {{256,192,256},{64,64,64},{12,10,16},{8,12,9},{16,12,16},{512,384,512},{21,13,18}};
-      long bad = 0, n = 0;
+      long bad = 0, n = 0, planted = 0;
       for (unsigned g = 0; g <= sizeof grids / sizeof grids[0]; g++) {
         const int last = g == sizeof grids / sizeof grids[0];
         double ds[3] = {0.3999999999999999, 0, 0};  /* K - 1 = 1.4 - 1 in FP64 */
         if (!last) { ds[0] = 2 * pi * hi / grids[g][0]; ds[1] = 1.0 / grids[g][1]; ds[2] = pi * hi / grids[g][2]; }
         for (int t = 0; t < (last ? 1 : 3); t++) {
           double d = ds[t], r = 1.0 / d;
-          for (long i = 0; i < 1000000; i++) {
+          for (long i = 0; i < 1000002; i++) {
             uint64_t u = xr();
             double m = (double)(u >> 11) * (1.0 / 9007199254740992.0) + 0.5;
             int ex = (int)((xr() >> 58) & 63) - 40;
-            double x = ldexp(m, ex) * ((u & 1) ? -1 : 1);
-            double q0 = x * r, e = fma(-q0, d, x), q = fma(e, r, q0);
+            double x = i == 1000000 ? 0.0 : i == 1000001 ? -0.0 : ldexp(m, ex) * ((u & 1) ? -1 : 1);
+            double q0 = x * r, e = fma(q0, d, -x), q = fma(-e, r, q0);
+            double pe = fma(-q0, d, x), pq = fma(pe, r, q0);
             n++;
-            if (q != x / d) bad++;
+            if (!same(q, x / d)) bad++;
+            if (!same(pq, x / d)) planted++;
           }
         }
       }
-      printf("%ld %ld\n", n, bad);
+      printf("%ld %ld %ld\n", n, bad, planted);
       return bad != 0;
     }
 """)
@@ -58,8 +64,10 @@ def test_residual_corrected_division_is_exact(tmp_path):
     exe = tmp_path / "kat"
     subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-o", str(exe), str(c), "-lm"], check=True)
     r = subprocess.run([str(exe)], capture_output=True, text=True)
-    n, bad = map(int, r.stdout.split())
-    assert n == 22_000_000 and bad == 0, r.stdout
+    n, bad, planted = map(int, r.stdout.split())
+    assert n == 22 * 1_000_002 and bad == 0, r.stdout
+    # the old residual sign loses the sign of -0/d for every divisor, and only there
+    assert planted == 22, r.stdout
 
 
 def _bound_ok(d):
