@@ -214,13 +214,15 @@ def stage_profile(s, cfg, pairs):
     return per_stage, roofline
 
 
-def physical_steps(cfg, flow, precision, order, pred, steps):
+def physical_steps(cfg, flow, precision, order, pred, steps, extrapolation=3):
     """Physical steps from the seeded state (seed PHYS_SEED, which the
     predictor's training did not see) with the reference's convergence test
     at tol 1e-8: er_k / E_k < 1e-8 for k = 1..4, E_k the running maxima of
-    the run (main.cpp:387-406); inner iterations run as graph replays."""
+    the run (main.cpp:387-406); inner iterations run as graph replays.  The
+    extrapolated start and the predictor's fallback are cubic once four
+    levels are known."""
     from channel_flow_amd.learned import run_steps
-    r, qn = run_steps(cfg, flow, PHYS_SEED, steps, order, pred, precision, graph=True)
+    r, qn = run_steps(cfg, flow, PHYS_SEED, steps, order, pred, precision, graph=True, extrapolation=extrapolation)
     return r, qn
 
 
@@ -287,7 +289,7 @@ def config5_leg(args):
         pred = Predictor(dataset_crops=1, batch_crops=8)
         meta = load(pred, wpath)
     r64, q64 = run_steps(cfg, flow, 1, steps, 0, None, CF_FP64, graph=True)
-    r32, q32 = run_steps(cfg, flow, 1, steps, 3 if pred else 2, pred, CF_FP32_SWEEPS, graph=True)
+    r32, q32 = run_steps(cfg, flow, 1, steps, 3 if pred else 2, pred, CF_FP32_SWEEPS, graph=True, extrapolation=3)
     out["phys_steps_fp64_start_Un"] = {k: r64[k] for k in ("iterations", "wall_s")}
     out["phys_steps_fp32_sweeps_" + ("predictor" if pred else "extrapolated")] = {
         k: r32[k] for k in ("iterations", "wall_s", "predictor_steps", "fallbacks")}
@@ -305,6 +307,45 @@ def config5_leg(args):
     out["profile_max_rel_err_vs_fp64"] = profile_errors(q32, q64)
     if pred:
         pred.close()
+    return out
+
+
+def phys_long_leg(args):
+    """Physical steps to er_k / E_k < 1e-8 at 256x192x256 past the seeded
+    transient: FP64 from U^n (the reference's start), FP64 and FP32 sweeps
+    from the cubic extrapolation, and FP32 sweeps with the learned start
+    (kept where its first residuals beat the quadratic extrapolation's,
+    otherwise the step restarts from the cubic one), each `--phys-long-steps`
+    steps from seed PHYS_SEED, with iterations and wall time per window of
+    steps and the speedup over FP64 from U^n."""
+    from channel_flow_amd.learned import channel_cfg, load, run_steps
+    from channel_flow_amd.solver import CF_FP32_SWEEPS, CF_FP64, Predictor
+    n = args.phys_long_steps
+    cfg, flow = channel_cfg(GRID, beta=2.0, deltaT=1e-8, tol=1e-8, cap=2000)
+    pred = Predictor(dataset_crops=1, batch_crops=8)
+    load(pred, WEIGHTS)
+    edges = [e for e in (3, 20, 40, 60, 100, 200) if e < n] + [n]
+    windows = list(zip(edges[:-1], edges[1:]))
+    out = {"steps": n, "seed": PHYS_SEED, "weights": str(WEIGHTS.relative_to(ROOT)),
+           "windows": [f"{a + 1}-{b}" for a, b in windows]}
+    runs = (("fp64_start_Un", CF_FP64, 0), ("fp64_cubic", CF_FP64, 2), ("fp32_sweeps_cubic", CF_FP32_SWEEPS, 2),
+            ("fp32_sweeps_predictor", CF_FP32_SWEEPS, 3))
+    for name, prec, order in runs:
+        r, _ = run_steps(cfg, flow, PHYS_SEED, n, order, pred if order == 3 else None, prec, graph=True,
+                         extrapolation=3)
+        its, wall = np.array(r["iterations"], float), np.array(r["wall_s"])
+        out[name] = {"iterations": r["iterations"], "fallbacks": r["fallbacks"], "predictor_steps": r["predictor_steps"],
+                     "mean_iterations": [round(float(its[a:b].mean()), 1) for a, b in windows],
+                     "mean_wall_s": [round(float(wall[a:b].mean()), 4) for a, b in windows],
+                     "total_wall_s_after_step_3": round(float(wall[3:].sum()), 2)}
+        log(f"phys-long {name}: {out[name]['mean_iterations']} its, {out[name]['mean_wall_s']} s per window")
+    base = out["fp64_start_Un"]
+    for name, _, _ in runs[1:]:
+        out[name]["speedup_wall_vs_fp64_start_Un"] = [round(x / y, 3) for x, y in
+                                                      zip(base["mean_wall_s"], out[name]["mean_wall_s"])]
+        out[name]["speedup_total_after_step_3"] = round(base["total_wall_s_after_step_3"] /
+                                                        out[name]["total_wall_s_after_step_3"], 3)
+    pred.close()
     return out
 
 
@@ -406,7 +447,8 @@ def main():
     ap.add_argument("--long-runs", action="store_true", help="also run configs 2 and 3 (100 and 1000 steps)")
     ap.add_argument("--config2-steps", type=int, default=100)
     ap.add_argument("--config3-steps", type=int, default=1000)
-    ap.add_argument("--only", choices=["long-runs", "config5"], default=None,
+    ap.add_argument("--phys-long-steps", type=int, default=60, help="physical steps per run of --only phys-long")
+    ap.add_argument("--only", choices=["long-runs", "config5", "phys-long"], default=None,
                     help="run only that leg and print its JSON")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain 1 GiB device copy (the reference kernel for HBM counter scaling)")
@@ -422,7 +464,8 @@ def main():
 
     torch.cuda.set_device(0)
     if args.only:
-        print(json.dumps(long_runs(args) if args.only == "long-runs" else config5_leg(args)), flush=True)
+        leg = {"long-runs": long_runs, "config5": config5_leg, "phys-long": phys_long_leg}[args.only]
+        print(json.dumps(leg(args)), flush=True)
         return
     if args.copy_calibration:
         a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
@@ -511,7 +554,7 @@ def main():
         # learned start; mean over steps 4.. (the first ones are the seeded
         # transient, and the predictor needs three levels)
         n = args.phys_steps
-        runs = [("fp64_start_Un", CF_FP64, 0), ("fp64_extrapolated", CF_FP64, 2)]
+        runs = [("fp64_start_Un", CF_FP64, 0), ("fp64_cubic", CF_FP64, 2)]
         if not args.no_fp32:
             runs.append(("fp32_sweeps_start_Un", CF_FP32_SWEEPS, 0))
         if pred is not None:

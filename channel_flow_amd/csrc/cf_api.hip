@@ -99,6 +99,9 @@ struct cf_ctx {
   bool uqq_valid = false; // Uqq holds the level before Uq
   double* U3 = nullptr;   // U^{n-3} (5 fields), kept while four levels are kept (cf_keep_levels)
   bool u3_valid = false;  // U3 holds the level before Uqq
+  // highest order of the extrapolated initial iterate (guess order 2 and the
+  // predictor's fallback): 2 quadratic, 3 cubic (keeps U3)
+  int extrap_order = 2;
   // learned initial iterate (cf_set_predictor; guess_order 3)
   cf_pred* pred = nullptr;
   unsigned short* px = nullptr;  // features of the whole domain (bf16, 16 per cell)
@@ -1605,6 +1608,14 @@ extern "C" int cf_keep_levels(cf_ctx* c, int levels) {
   return sync(c);
 }
 
+extern "C" int cf_set_extrapolation_order(cf_ctx* c, int order) {
+  if (!c || order < 2 || order > 3) return fail(c, CF_ERR_ARG, "extrapolation order must be 2 or 3");
+  if (order == 3)
+    if (int r = cf_keep_levels(c, 4)) return r;
+  c->extrap_order = order;
+  return CF_OK;
+}
+
 extern "C" int cf_set_initial_guess(cf_ctx* c, int order) {
   if (!c || order < 0 || order > 3) return fail(c, CF_ERR_ARG, "initial-guess order must be 0, 1, 2 or 3");
   if (order == 3 && !c->pred) return fail(c, CF_ERR_STATE, "no predictor attached (cf_set_predictor)");
@@ -1621,12 +1632,19 @@ extern "C" int cf_set_initial_guess(cf_ctx* c, int order) {
   return sync(c);
 }
 
-// The learned initial iterate with its fallback: the network's guess and the
-// quadratic extrapolation each take one inner iteration; the step continues
-// from the one whose first residual is smaller in every er1..er4 ratio taken
-// together (max_k er_k(net) / er_k(extrapolated) <= 1 keeps the network's),
-// and both iterations count.
-static int learned_start(cf_ctx* c, double er[5]) {
+// the extrapolation order once three levels are known
+static int extrapolation_order(const cf_ctx* c) { return c->extrap_order == 3 && c->U3 && c->u3_valid ? 3 : 2; }
+
+// The learned initial iterate with its fallback.  The network refines the
+// quadratic extrapolation, so its guess and the quadratic extrapolation each
+// take one inner iteration, and the step continues from the network's unless
+// the quadratic's first residual is smaller in every er1..er4 ratio taken
+// together (max_k er_k(net) / er_k(quadratic) <= 1 keeps the network's).  On
+// a fallback with the cubic extrapolation selected (and four levels known),
+// the step restarts from the cubic one, which is the better start once the
+// flow changes smoothly.  *its receives the inner iterations run (2 or 3);
+// all of them count.
+static int learned_start(cf_ctx* c, double er[5], int* its) {
   const Geo& g = c->g;
   const cfp::Norm& nm = c->pred->norm;
   cfp::launch_feat(c->stream, c->Un, c->Uq, c->Uqq, c->prof, g, nm, c->px);
@@ -1638,16 +1656,22 @@ static int learned_start(cf_ctx* c, double er[5]) {
   int r = cf_inner_iteration(c, er_net);
   if (r) return r;
   cf::launch_copy_interior(c->stream, c->Ui, c->Ualt, g);
-  cf::launch_extrapolate(c->stream, c->Ui, c->Un, c->Uq, c->Uqq, g, 2);
+  cf::launch_extrapolate(c->stream, c->Ui, c->Un, c->Uq, c->Uqq, c->U3, g, 2);
   if ((r = cf_inner_iteration(c, er_ext))) return r;
   double worst = 0;
   for (int k = 0; k < 4; k++) worst = std::max(worst, er_net[k] / er_ext[k]);
   c->pred_steps++;
+  *its = 2;
   if (worst <= 1.0) {
     cf::launch_copy_interior(c->stream, c->Ualt, c->Ui, g);
     for (int k = 0; k < 5; k++) er[k] = er_net[k];
   } else {
     c->pred_fallbacks++;
+    if (extrapolation_order(c) == 3) {
+      cf::launch_extrapolate(c->stream, c->Ui, c->Un, c->Uq, c->Uqq, c->U3, g, 3);
+      if ((r = cf_inner_iteration(c, er_ext))) return r;
+      *its = 3;
+    }
     for (int k = 0; k < 5; k++) er[k] = er_ext[k];
   }
   return sync(c);
@@ -1658,16 +1682,16 @@ extern "C" int cf_physical_step(cf_ctx* c, int* iterations, double er[5]) {
   if (!c->have_state) return fail(c, CF_ERR_STATE, "no state loaded");
   const bool learned = c->guess_order == 3 && c->pred && c->uqq_valid;
   if (c->guess_order > 0 && !learned) {
-    // quadratic once three levels are known, linear before
-    const int order = c->guess_order >= 2 && c->uqq_valid ? 2 : 1;
-    cf::launch_extrapolate(c->stream, c->Ui, c->Un, c->Uq, c->Uqq, c->g, order);
+    // linear with two known levels, then quadratic (cubic with four levels
+    // when asked for)
+    const int order = c->guess_order >= 2 && c->uqq_valid ? extrapolation_order(c) : 1;
+    cf::launch_extrapolate(c->stream, c->Ui, c->Un, c->Uq, c->Uqq, c->U3, c->g, order);
     HIPCHK(c, hipGetLastError());
   }
   for (int it = 1;; it++) {
     int r;
     if (learned && it == 1) {
-      r = learned_start(c, er);
-      it = 2;  // two inner iterations were run
+      r = learned_start(c, er, &it);  // two or three inner iterations were run
     } else {
       r = cf_inner_iteration(c, er);
     }
@@ -1677,7 +1701,7 @@ extern "C" int cf_physical_step(cf_ctx* c, int* iterations, double er[5]) {
       er[v] = er[v] / c->E[v];
     }
     bool conv = er[0] < c->p.tol && er[1] < c->p.tol && er[2] < c->p.tol && er[3] < c->p.tol;
-    if (conv || it == c->p.iteration_end_step) {
+    if (conv || it >= c->p.iteration_end_step) {
       *iterations = it;
       return cf_advance_levels(c);
     }

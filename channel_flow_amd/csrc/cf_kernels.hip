@@ -2117,10 +2117,11 @@ __global__ void __launch_bounds__(256) k_jacobi(const double* __restrict__ Ui, c
 }
 
 // Initial iterate of a physical step from the time levels, on the interior:
-// order 1: 2 U^n - U^{n-1}; order 2: 3 U^n - 3 U^{n-1} + U^{n-2}.  The
-// reference starts every physical step at U^n (main.cpp:444-464).
+// order 1: 2 U^n - U^{n-1}; order 2: 3 U^n - 3 U^{n-1} + U^{n-2}; order 3:
+// 4 U^n - 6 U^{n-1} + 4 U^{n-2} - U^{n-3}.  The reference starts every
+// physical step at U^n (main.cpp:444-464).
 __global__ void k_extrapolate(double* __restrict__ Ui, const double* __restrict__ Un, const double* __restrict__ Uq,
-                              const double* __restrict__ Uqq, Geo g, int order) {
+                              const double* __restrict__ Uqq, const double* __restrict__ U3, Geo g, int order) {
   const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
   const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
   const int i = 3 + blockIdx.z;
@@ -2129,7 +2130,9 @@ __global__ void k_extrapolate(double* __restrict__ Ui, const double* __restrict_
 #pragma unroll
   for (int v = 0; v < 5; v++) {
     const long o = v * g.nf + c;
-    Ui[o] = order == 2 ? 3 * Un[o] - 3 * Uq[o] + Uqq[o] : 2 * Un[o] - Uq[o];
+    Ui[o] = order == 3   ? 4 * Un[o] - 6 * Uq[o] + 4 * Uqq[o] - U3[o]
+            : order == 2 ? 3 * Un[o] - 3 * Uq[o] + Uqq[o]
+                         : 2 * Un[o] - Uq[o];
   }
 }
 
@@ -2742,10 +2745,10 @@ void launch_sweeps_planes(hipStream_t st, const double* Ui, const double* MR, do
   }
 }
 
-void launch_extrapolate(hipStream_t st, double* Ui, const double* Un, const double* Uq, const double* Uqq, const Geo& g,
-                        int order) {
+void launch_extrapolate(hipStream_t st, double* Ui, const double* Un, const double* Uq, const double* Uqq,
+                        const double* U3, const Geo& g, int order) {
   dim3 blk(64, 4);
-  hipLaunchKernelGGL(k_extrapolate, box_grid(g.nz - 1, g.ny - 1, g.gend - 2, blk), blk, 0, st, Ui, Un, Uq, Uqq, g,
+  hipLaunchKernelGGL(k_extrapolate, box_grid(g.nz - 1, g.ny - 1, g.gend - 2, blk), blk, 0, st, Ui, Un, Uq, Uqq, U3, g,
                      order);
 }
 

@@ -71,9 +71,10 @@ void launch_sweeps_planes(hipStream_t st, const double* Ui, const double* MR, do
 void launch_dplusgs(hipStream_t st, const double* Ui, const double* MR, double* NL, double* NR, const double* prof,
                     const Geo& g, double deltaT);
 void launch_copy_interior(hipStream_t st, const double* src, double* dst, const Geo& g);
-// U_ <- 2 U^n - U^{n-1} (order 1) or 3 U^n - 3 U^{n-1} + U^{n-2} (order 2), interior
-void launch_extrapolate(hipStream_t st, double* Ui, const double* Un, const double* Uq, const double* Uqq, const Geo& g,
-                        int order);
+// U_ <- 2 U^n - U^{n-1} (order 1), 3 U^n - 3 U^{n-1} + U^{n-2} (order 2) or
+// 4 U^n - 6 U^{n-1} + 4 U^{n-2} - U^{n-3} (order 3, U3 = U^{n-3}), interior
+void launch_extrapolate(hipStream_t st, double* Ui, const double* Un, const double* Uq, const double* Uqq,
+                        const double* U3, const Geo& g, int order);
 long update_blocks(const Geo& g);
 void launch_update(hipStream_t st, double* Ui, const double* NR, const double* prof, double* partial, double* er_dev,
                    const Geo& g, double DN);

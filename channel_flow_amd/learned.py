@@ -111,9 +111,10 @@ def load(pred: Predictor, path):
     return json.loads(z["meta"].tobytes().decode())
 
 
-def run_steps(cfg, flow, seed, steps, order, pred=None, precision=CF_FP64, graph=False):
+def run_steps(cfg, flow, seed, steps, order, pred=None, precision=CF_FP64, graph=False, extrapolation=2):
     """Physical steps from the seed's state with initial guess `order` (0 U^n,
-    2 quadratic extrapolation, 3 the attached predictor): per-step inner
+    2 extrapolation, 3 the attached predictor; `extrapolation` 2 quadratic or
+    3 cubic, for order 2 and the predictor's fallback): per-step inner
     iterations and wall seconds, the predictor's fallback count, the final
     level-n state."""
     Q = poiseuille_state(cfg, flow, seed=seed, stream=0)
@@ -122,6 +123,7 @@ def run_steps(cfg, flow, seed, steps, order, pred=None, precision=CF_FP64, graph
         s.set_graph(graph)
         if pred is not None:
             s.set_predictor(pred)
+        s.set_extrapolation_order(extrapolation)
         s.set_initial_guess(order)
         s.load_state(Q, Q)
         its, wall = [], []
@@ -190,6 +192,7 @@ def main():
     ap.add_argument("--out", default=None)
     ap.add_argument("--seed", type=int, default=9, help="held-out seed of the evaluation")
     ap.add_argument("--eval-steps", type=int, default=12)
+    ap.add_argument("--skip", type=int, default=3, help="steps left out of the evaluation means")
     ap.add_argument("--cells-of", type=int, nargs=2, default=None, metavar=("X", "Z"),
                     help="train on a periodic patch with the cell sizes of the X x Z channel")
     a = ap.parse_args()
@@ -212,12 +215,12 @@ def main():
             if a.out:
                 save(pred, a.out, meta)
             if a.eval_steps:
-                res = compare(cfg, flow, pred, a.seed, a.eval_steps, precision=precision)
+                res = compare(cfg, flow, pred, a.seed, a.eval_steps, skip=a.skip, precision=precision)
                 print(json.dumps({"train": meta, "evaluate": res}, default=float))
     else:
         with Predictor(dataset_crops=1, batch_crops=a.batch) as pred:
             meta = load(pred, a.weights)
-            res = compare(cfg, flow, pred, a.seed, a.eval_steps, precision=precision)
+            res = compare(cfg, flow, pred, a.seed, a.eval_steps, skip=a.skip, precision=precision)
             print(json.dumps({"weights": meta, "evaluate": res}, default=float))
 
 

@@ -40,7 +40,7 @@ EXPORTS = [
     "cf_pred_get_params", "cf_pred_set_norm", "cf_pred_get_norm", "cf_pred_capture", "cf_pred_dataset_size",
     "cf_pred_fit_norm", "cf_pred_load_dataset", "cf_pred_train", "cf_pred_eval", "cf_pred_forward", "cf_pred_gradient", "cf_pred_profile",
     "cf_keep_levels", "cf_set_predictor", "cf_predictor_stats", "cf_set_sweep_reuse", "cf_set_graph",
-    "cf_graph_builds", "cf_set_concurrent_stencils",
+    "cf_graph_builds", "cf_set_concurrent_stencils", "cf_set_extrapolation_order",
 ]
 CF_PRED_NPARAM = 112005
 CF_PRED_NNORM = 22
@@ -109,6 +109,7 @@ def load_library() -> C.CDLL:
     lib.cf_set_sweep_schedule.argtypes = [vp, C.c_int]
     lib.cf_set_viscous_schedule.argtypes = [vp, C.c_int]
     lib.cf_set_initial_guess.argtypes = [vp, C.c_int]
+    lib.cf_set_extrapolation_order.argtypes = [vp, C.c_int]
     lib.cf_rk3nodt.argtypes = [vp, C.c_int, C.c_double, P, PP, PP, PP, P, PP, PP, PP, PP, PP, PP]
     lib.cf_rk3_step.argtypes = [vp, C.c_int]
     lib.cf_statistic.argtypes = [vp, C.c_int, C.c_int, PP, C.c_char_p]
@@ -443,6 +444,11 @@ class Solver:
         """Initial iterate of each physical_step(): 0 U^n (default, the
         reference's); 1 2U^n - U^{n-1}; 2 3U^n - 3U^{n-1} + U^{n-2}."""
         self._chk(self.lib.cf_set_initial_guess(self._h, int(order)))
+
+    def set_extrapolation_order(self, order: int = 2):
+        """Order of the extrapolated start (initial guess 2 and the predictor's
+        fallback): 2 quadratic (default), 3 cubic once four levels are known."""
+        self._chk(self.lib.cf_set_extrapolation_order(self._h, int(order)))
 
     def synchronize(self):
         self._chk(self.lib.cf_synchronize(self._h))

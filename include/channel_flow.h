@@ -196,6 +196,12 @@ int cf_advance_levels(cf_ctx* ctx);
  * converged state is the same to the convergence tolerance; the number of
  * inner iterations under the running-max test (main.cpp:387-406) drops. */
 int cf_set_initial_guess(cf_ctx* ctx, int order);
+/* Highest order of the extrapolated initial iterate, for initial guess 2 and
+ * for the learned predictor's fallback; no counterpart in the reference:
+ * 2 (default) quadratic; 3 cubic 4 U^n - 6 U^{n-1} + 4 U^{n-2} - U^{n-3}
+ * once four levels are known (it keeps a U^{n-3} level, cf_keep_levels(ctx,
+ * 4)), quadratic before. */
+int cf_set_extrapolation_order(cf_ctx* ctx, int order);
 /* Output (main.h:872-896, Output.cpp:56-190): writes <dir>/LES%05d.bin. */
 int cf_write_output(cf_ctx* ctx, int step, const char* dir);
 /* Exact restart, no counterpart in the reference (whose restart reads
