@@ -87,10 +87,7 @@ struct cf_ctx {
   int schedule = 0;       // 0: persistent tiles, 1: one hyperplane per launch
   int solver = 0;         // implicit part: 0 LU-SGS (Dual_time_stepping), 1 DPLUSGS
   int visc_schedule = 0;  // 0: fused viscous kernel, 1: three kernels through Pr/VS (allocated on first use)
-  // every face J the fused viscous kernel divides by has a checked reciprocal
-  // (set with the metrics); prof_gen counts metric uploads
-  bool visc_face_rcp = false;
-  long prof_gen = 0;
+  long prof_gen = 0;  // metric uploads so far
   bool fp32 = false;      // CF_FP32_SWEEPS: LU-SGS sweeps and split Jacobians in float
   int guess_order = 0;    // initial iterate of cf_physical_step: 0 U^n, 1 linear, 2 quadratic extrapolation
   double* Uqq = nullptr;  // U^{n-2} (5 fields), kept while guess_order is 2 (allocated on first use)
@@ -120,7 +117,7 @@ struct cf_ctx {
     int schedule, visc_schedule, solver, x_periodic, pairs;
     bool fp32, fork;
     double deltaT, f_drive;
-    long prof_gen;  // launch choices that follow the metrics (fused viscous kernel, its divisions)
+    long prof_gen;  // launch choices that follow the metrics (the fused viscous kernel)
     bool operator==(const GraphKey& o) const {
       return Ui == o.Ui && Un == o.Un && Uq == o.Uq && schedule == o.schedule && visc_schedule == o.visc_schedule &&
              solver == o.solver && x_periodic == o.x_periodic && pairs == o.pairs && fp32 == o.fp32 &&
@@ -242,14 +239,6 @@ static int ingest(cf_ctx* c, const double* A, int set, int comp, const char* nam
 }
 
 static int upload_prof(cf_ctx* c) {
-  // the fused viscous kernel divides by the face J of the three face sets at
-  // rows 1..ny; through their reciprocals when every one passes the check
-  // (the device forms the same rcp_checked values in the profile rows)
-  const Geo& g = c->g;
-  bool rcp = true;
-  for (int set = cf::SET_U; set <= cf::SET_W; set++)
-    for (int j = 1; j <= g.ny; j++) rcp = rcp && cf::rcp_checked(c->hprof[(size_t)(set * 10) * g.Ym + j]) != 0.0;
-  c->visc_face_rcp = rcp;
   c->prof_gen++;
   HIPCHK(c, hipMemcpyAsync(c->prof, c->hprof.data(), c->hprof.size() * 8, hipMemcpyHostToDevice, c->stream));
   cf::launch_flux_tables(c->stream, c->prof, c->g);
@@ -736,7 +725,7 @@ static int run_viscous(cf_ctx* c) {
     int r = ensure_viscous_staging(c);
     if (r) return r;
   }
-  cf::launch_viscous(c->stream, c->Ui, c->prof, c->Pr, c->VS, c->vF, c->g, fused, c->visc_face_rcp);
+  cf::launch_viscous(c->stream, c->Ui, c->prof, c->Pr, c->VS, c->vF, c->g, fused);
   return CF_OK;
 }
 
@@ -1066,7 +1055,7 @@ static void enqueue_iteration(cf_ctx* c) {
     cf::launch_flux(c->stream, 0, c->Ui, c->prof, c->Fx, g);
     cf::launch_flux(c->aux[0], 1, c->Ui, c->prof, c->Fy, g);
     cf::launch_flux(c->aux[1], 2, c->Ui, c->prof, c->Fz, g);
-    cf::launch_viscous(c->aux[2], c->Ui, c->prof, c->Pr, c->VS, c->vF, g, viscous_fused(c), c->visc_face_rcp);
+    cf::launch_viscous(c->aux[2], c->Ui, c->prof, c->Pr, c->VS, c->vF, g, viscous_fused(c));
     for (int q = 0; q < 3; q++) {
       (void)hipEventRecord(c->join_ev[q], c->aux[q]);
       (void)hipStreamWaitEvent(c->stream, c->join_ev[q], 0);
@@ -1075,7 +1064,7 @@ static void enqueue_iteration(cf_ctx* c) {
     cf::launch_flux(c->stream, 0, c->Ui, c->prof, c->Fx, g);
     cf::launch_flux(c->stream, 1, c->Ui, c->prof, c->Fy, g);
     cf::launch_flux(c->stream, 2, c->Ui, c->prof, c->Fz, g);
-    cf::launch_viscous(c->stream, c->Ui, c->prof, c->Pr, c->VS, c->vF, g, viscous_fused(c), c->visc_face_rcp);
+    cf::launch_viscous(c->stream, c->Ui, c->prof, c->Pr, c->VS, c->vF, g, viscous_fused(c));
   }
   if (c->solver == 1) {
     // DPLUSGS.cpp:189-1361: the RHS, four Jacobi sweeps, the update and norms

@@ -753,9 +753,7 @@ __device__ __forceinline__ DiagCoef diag_face(const double* __restrict__ prof, i
 
 __device__ __forceinline__ double lp(const VLds& L, int s, int f, int r, int c) { return L.P[s][f][r][c]; }
 
-// x face between planes (slot sm) and (slot s0) at LDS cell (r, c), row j;
-// FAST: the division by the face J through its checked reciprocal
-template <bool FAST>
+// x face between planes (slot sm) and (slot s0) at LDS cell (r, c), row j
 __device__ __forceinline__ void vface_x(const VLds& L, const double* __restrict__ prof, const Geo& g, int s0, int sm,
                                         int r, int c, int j, double invXI, double inv4ET, double inv4ZT,
                                         double o[4]) {
@@ -766,21 +764,20 @@ __device__ __forceinline__ void vface_x(const VLds& L, const double* __restrict_
   const double dzu = (lp(L, s0, 0, r, c + 1) + lp(L, sm, 0, r, c + 1) - lp(L, s0, 0, r, c - 1) - lp(L, sm, 0, r, c - 1)) * inv4ZT;
   const double dzw = (lp(L, s0, 2, r, c + 1) + lp(L, sm, 2, r, c + 1) - lp(L, s0, 2, r, c - 1) - lp(L, sm, 2, r, c - 1)) * inv4ZT;
   const DiagCoef m = diag_face(prof, g.Ym, SET_U, j - 1);
-  const double Jf = pf(prof, g.Ym, SET_U, 0, j - 1), rJf = prow(prof, g.Ym, FT_RCP, j - 1);
+  const double Jf = pf(prof, g.Ym, SET_U, 0, j - 1);
   const double a1 = (4. / 3) * m.X * m.X, a2 = m.X * m.X, a3 = m.X * m.X, a4 = m.X * m.X;
   const double d5 = m.X * m.E, d7 = -((2. / 3) * m.X * m.E);
   const double e6 = m.X * m.Z, e9 = -((2. / 3) * m.X * m.Z);
   const double Ux = 0.5 * (P0.U + iP.U), Vx = 0.5 * (P0.V + iP.V), Wx = 0.5 * (P0.W + iP.W);
-  o[0] = div_face<FAST>(MU_L * (a1 * dx.u + d7 * dyv + e9 * dzw), Jf, rJf);
-  o[1] = div_face<FAST>(MU_L * (a2 * dx.v + d5 * dyu), Jf, rJf);
-  o[2] = div_face<FAST>(MU_L * (a3 * dx.w + e6 * dzu), Jf, rJf);
-  o[3] = div_face<FAST>(MU_L * (0.5 * a1 * dx.uu + 0.5 * a2 * dx.vv + 0.5 * a3 * dx.ww + CF_CVK_PR(a4, dx.T) +
-                                d5 * Vx * dyu + d7 * Ux * dyv + e6 * Wx * dzu + e9 * Ux * dzw),
-                        Jf, rJf);
+  o[0] = MU_L * (a1 * dx.u + d7 * dyv + e9 * dzw) / Jf;
+  o[1] = MU_L * (a2 * dx.v + d5 * dyu) / Jf;
+  o[2] = MU_L * (a3 * dx.w + e6 * dzu) / Jf;
+  o[3] = MU_L * (0.5 * a1 * dx.uu + 0.5 * a2 * dx.vv + 0.5 * a3 * dx.ww + CF_CVK_PR(a4, dx.T) + d5 * Vx * dyu +
+                 d7 * Ux * dyv + e6 * Wx * dzu + e9 * Ux * dzw) /
+         Jf;
 }
 
 // y face between rows r-1 and r of plane slot s0
-template <bool FAST>
 __device__ __forceinline__ void vface_y(const VLds& L, const double* __restrict__ prof, const Geo& g, int sp, int s0,
                                         int sm, int r, int c, int j, double invET, double inv4XI, double inv4ZT,
                                         double o[4]) {
@@ -791,21 +788,20 @@ __device__ __forceinline__ void vface_y(const VLds& L, const double* __restrict_
   const double dzv = (lp(L, s0, 1, r, c + 1) + lp(L, s0, 1, r - 1, c + 1) - lp(L, s0, 1, r, c - 1) - lp(L, s0, 1, r - 1, c - 1)) * inv4ZT;
   const double dzw = (lp(L, s0, 2, r, c + 1) + lp(L, s0, 2, r - 1, c + 1) - lp(L, s0, 2, r, c - 1) - lp(L, s0, 2, r - 1, c - 1)) * inv4ZT;
   const DiagCoef m = diag_face(prof, g.Ym, SET_V, j - 1);
-  const double Jf = pf(prof, g.Ym, SET_V, 0, j - 1), rJf = prow(prof, g.Ym, FT_RCP + 2, j - 1);
+  const double Jf = pf(prof, g.Ym, SET_V, 0, j - 1);
   const double b1 = m.E * m.E, b2 = (4. / 3) * m.E * m.E, b3 = m.E * m.E, b4 = m.E * m.E;
   const double d5 = m.X * m.E, d7 = -((2. / 3) * m.X * m.E);
   const double f8 = m.E * m.Z, f10 = -((2. / 3) * m.E * m.Z);
   const double Uy = 0.5 * (P0.U + jP.U), Vy = 0.5 * (P0.V + jP.V), Wy = 0.5 * (P0.W + jP.W);
-  o[0] = div_face<FAST>(MU_L * (d5 * dxv + b1 * dy.u), Jf, rJf);
-  o[1] = div_face<FAST>(MU_L * (d7 * dxu + b2 * dy.v + f10 * dzw), Jf, rJf);
-  o[2] = div_face<FAST>(MU_L * (b3 * dy.w + f8 * dzv), Jf, rJf);
-  o[3] = div_face<FAST>(MU_L * (0.5 * b1 * dy.uu + 0.5 * b2 * dy.vv + 0.5 * b3 * dy.ww + CF_CVK_PR(b4, dy.T) +
-                                d5 * Uy * dxv + d7 * Vy * dxu + f8 * Wy * dzv + f10 * Vy * dzw),
-                        Jf, rJf);
+  o[0] = MU_L * (d5 * dxv + b1 * dy.u) / Jf;
+  o[1] = MU_L * (d7 * dxu + b2 * dy.v + f10 * dzw) / Jf;
+  o[2] = MU_L * (b3 * dy.w + f8 * dzv) / Jf;
+  o[3] = MU_L * (0.5 * b1 * dy.uu + 0.5 * b2 * dy.vv + 0.5 * b3 * dy.ww + CF_CVK_PR(b4, dy.T) + d5 * Uy * dxv +
+                 d7 * Vy * dxu + f8 * Wy * dzv + f10 * Vy * dzw) /
+         Jf;
 }
 
 // z face between columns c-1 and c of plane slot s0
-template <bool FAST>
 __device__ __forceinline__ void vface_z(const VLds& L, const double* __restrict__ prof, const Geo& g, int sp, int s0,
                                         int sm, int r, int c, int j, double invZT, double inv4XI, double inv4ET,
                                         double o[4]) {
@@ -816,20 +812,19 @@ __device__ __forceinline__ void vface_z(const VLds& L, const double* __restrict_
   const double dyv = (lp(L, s0, 1, r + 1, c) + lp(L, s0, 1, r + 1, c - 1) - lp(L, s0, 1, r - 1, c) - lp(L, s0, 1, r - 1, c - 1)) * inv4ET;
   const double dyw = (lp(L, s0, 2, r + 1, c) + lp(L, s0, 2, r + 1, c - 1) - lp(L, s0, 2, r - 1, c) - lp(L, s0, 2, r - 1, c - 1)) * inv4ET;
   const DiagCoef m = diag_face(prof, g.Ym, SET_W, j - 1);
-  const double Jf = pf(prof, g.Ym, SET_W, 0, j - 1), rJf = prow(prof, g.Ym, FT_RCP + 4, j - 1);
+  const double Jf = pf(prof, g.Ym, SET_W, 0, j - 1);
   const double c1 = m.Z * m.Z, c2 = m.Z * m.Z, c3 = (4. / 3) * m.Z * m.Z, c4 = m.Z * m.Z;
   const double e6 = m.X * m.Z, e9 = -((2. / 3) * m.X * m.Z);
   const double f8 = m.E * m.Z, f10 = -((2. / 3) * m.E * m.Z);
   const double Uz = 0.5 * (P0.U + kP.U), Vz = 0.5 * (P0.V + kP.V), Wz = 0.5 * (P0.W + kP.W);
-  o[0] = div_face<FAST>(MU_L * (e6 * dxw + c1 * dz.u), Jf, rJf);
-  o[1] = div_face<FAST>(MU_L * (f8 * dyw + c2 * dz.v), Jf, rJf);
-  o[2] = div_face<FAST>(MU_L * (e9 * dxu + f10 * dyv + c3 * dz.w), Jf, rJf);
-  o[3] = div_face<FAST>(MU_L * (0.5 * c1 * dz.uu + 0.5 * c2 * dz.vv + 0.5 * c3 * dz.ww + CF_CVK_PR(c4, dz.T) +
-                                e6 * Uz * dxw + e9 * Wz * dxu + f8 * Vz * dyw + f10 * Wz * dyv),
-                        Jf, rJf);
+  o[0] = MU_L * (e6 * dxw + c1 * dz.u) / Jf;
+  o[1] = MU_L * (f8 * dyw + c2 * dz.v) / Jf;
+  o[2] = MU_L * (e9 * dxu + f10 * dyv + c3 * dz.w) / Jf;
+  o[3] = MU_L * (0.5 * c1 * dz.uu + 0.5 * c2 * dz.vv + 0.5 * c3 * dz.ww + CF_CVK_PR(c4, dz.T) + e6 * Uz * dxw +
+                 e9 * Wz * dxu + f8 * Vz * dyw + f10 * Wz * dyv) /
+         Jf;
 }
 
-template <bool FAST>
 __global__ void __launch_bounds__(VTHREADS) k_visc_fused(const double* __restrict__ U,
                                                          const double* __restrict__ prof, double* __restrict__ vF,
                                                          Geo g, double invXI, double invET, double invZT,
@@ -855,31 +850,31 @@ __global__ void __launch_bounds__(VTHREADS) k_visc_fused(const double* __restric
   vload_plane(L, U, prof, g, ib, ib % 3, j0, k0, tid);
   __syncthreads();
   double LLc[4] = {0., 0., 0., 0.};
-  if (cell) vface_x<FAST>(L, prof, g, ib % 3, (ib - 1) % 3, ty + 1, tx + 1, j, invXI, inv4ET, inv4ZT, LLc);
+  if (cell) vface_x(L, prof, g, ib % 3, (ib - 1) % 3, ty + 1, tx + 1, j, invXI, inv4ET, inv4ZT, LLc);
   for (int i = ib; i <= ie; i++) {
     const int sm = (i - 1) % 3, s0 = i % 3, sp = (i + 1) % 3;
     vload_plane(L, U, prof, g, i + 1, sp, j0, k0, tid);
     __syncthreads();
     double LLn[4] = {0., 0., 0., 0.}, o[4];
-    if (cell) vface_x<FAST>(L, prof, g, sp, s0, ty + 1, tx + 1, j, invXI, inv4ET, inv4ZT, LLn);
+    if (cell) vface_x(L, prof, g, sp, s0, ty + 1, tx + 1, j, invXI, inv4ET, inv4ZT, LLn);
     if (doML) {
-      vface_y<FAST>(L, prof, g, sp, s0, sm, ty + 1, tx + 1, j, invET, inv4XI, inv4ZT, o);
+      vface_y(L, prof, g, sp, s0, sm, ty + 1, tx + 1, j, invET, inv4XI, inv4ZT, o);
 #pragma unroll
       for (int v = 0; v < 4; v++) L.ML[v][ty][tx] = o[v];
     }
     if (doNL) {
-      vface_z<FAST>(L, prof, g, sp, s0, sm, ty + 1, tx + 1, j, invZT, inv4XI, inv4ET, o);
+      vface_z(L, prof, g, sp, s0, sm, ty + 1, tx + 1, j, invZT, inv4XI, inv4ET, o);
 #pragma unroll
       for (int v = 0; v < 4; v++) L.NL[v][ty][tx] = o[v];
     }
     if (exML) {
-      vface_y<FAST>(L, prof, g, sp, s0, sm, VJ + 1, e + 1, j0 + VJ, invET, inv4XI, inv4ZT, o);
+      vface_y(L, prof, g, sp, s0, sm, VJ + 1, e + 1, j0 + VJ, invET, inv4XI, inv4ZT, o);
 #pragma unroll
       for (int v = 0; v < 4; v++) L.ML[v][VJ][e] = o[v];
     }
     if (exNL) {
       const int r = e - VK;
-      vface_z<FAST>(L, prof, g, sp, s0, sm, r + 1, VK + 1, j0 + r, invZT, inv4XI, inv4ET, o);
+      vface_z(L, prof, g, sp, s0, sm, r + 1, VK + 1, j0 + r, invZT, inv4XI, inv4ET, o);
 #pragma unroll
       for (int v = 0; v < 4; v++) L.NL[v][r][VK] = o[v];
     }
@@ -2402,18 +2397,14 @@ void launch_filter(hipStream_t st, const double* U, const double* prof, const do
 }
 
 void launch_viscous(hipStream_t st, const double* U, const double* prof, double* Pr, double* VS, double* vF,
-                    const Geo& g, bool fused_viscous, bool face_rcp) {
+                    const Geo& g, bool fused_viscous) {
   dim3 blk(64, 4);
   double invXI = 1. / (g.dXI), invET = 1. / (g.dET), invZT = 1. / (g.dZT);
   double inv4XI = 1. / (4 * g.dXI), inv4ET = 1. / (4 * g.dET), inv4ZT = 1. / (4 * g.dZT);
   if (fused_viscous) {
     dim3 gr((g.nz - 1 + VK - 1) / VK, (g.ny - 1 + VJ - 1) / VJ, (g.gend - 2 + VC - 1) / VC);
-    if (face_rcp)
-      hipLaunchKernelGGL(k_visc_fused<true>, gr, dim3(VTHREADS), 0, st, U, prof, vF, g, invXI, invET, invZT, inv4XI,
-                         inv4ET, inv4ZT);
-    else
-      hipLaunchKernelGGL(k_visc_fused<false>, gr, dim3(VTHREADS), 0, st, U, prof, vF, g, invXI, invET, invZT, inv4XI,
-                         inv4ET, inv4ZT);
+    hipLaunchKernelGGL(k_visc_fused, gr, dim3(VTHREADS), 0, st, U, prof, vF, g, invXI, invET, invZT, inv4XI, inv4ET,
+                       inv4ZT);
     return;
   }
   hipLaunchKernelGGL(k_visc_prim, box_grid(g.nz + 1, g.ny + 1, g.gend, blk), blk, 0, st, U, prof, Pr, g);

@@ -18,11 +18,9 @@ void launch_filter(hipStream_t st, const double* U, const double* prof, const do
                    double E_high, double E_low, double* EpX, double* EpY, double* EpZ, const Geo& g);
 // fused: one kernel with the primitives and face fluxes in LDS (default);
 // otherwise primitives, face fluxes and divergence as three kernels through
-// the staging fields Pr (4) and VS (12).  face_rcp (fused kernel only): every
-// face J at rows j = 1..ny of the three face sets has a checked reciprocal in
-// the profile rows, so the face fluxes divide by J through it (same bits)
+// the staging fields Pr (4) and VS (12)
 void launch_viscous(hipStream_t st, const double* U, const double* prof, double* Pr, double* VS, double* vF,
-                    const Geo& g, bool fused = true, bool face_rcp = false);
+                    const Geo& g, bool fused = true);
 // RK3NODT stage RK (1..3) on the interior; vF and MR may be null
 void launch_rk3(hipStream_t st, int RK, double* Ui, const double* Un, const double* Fx, const double* Fy,
                 const double* Fz, const double* vF, const double* prof, double* MR, const Geo& g, double deltaT,
