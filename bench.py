@@ -288,8 +288,9 @@ def config5_leg(args):
         from channel_flow_amd.learned import load
         pred = Predictor(dataset_crops=1, batch_crops=8)
         meta = load(pred, wpath)
-    r64, q64 = run_steps(cfg, flow, 1, steps, 0, None, CF_FP64, graph=True)
-    r32, q32 = run_steps(cfg, flow, 1, steps, 3 if pred else 2, pred, CF_FP32_SWEEPS, graph=True, extrapolation=3)
+    r64, q64 = run_steps(cfg, flow, 1, steps, 0, None, CF_FP64, graph=True, log=lambda m: log("config 5 fp64 " + m))
+    r32, q32 = run_steps(cfg, flow, 1, steps, 3 if pred else 2, pred, CF_FP32_SWEEPS, graph=True, extrapolation=3,
+                         log=lambda m: log("config 5 fp32 sweeps + predictor " + m))
     out["phys_steps_fp64_start_Un"] = {k: r64[k] for k in ("iterations", "wall_s")}
     out["phys_steps_fp32_sweeps_" + ("predictor" if pred else "extrapolated")] = {
         k: r32[k] for k in ("iterations", "wall_s", "predictor_steps", "fallbacks")}
@@ -332,7 +333,7 @@ def phys_long_leg(args):
             ("fp32_sweeps_predictor", CF_FP32_SWEEPS, 3))
     for name, prec, order in runs:
         r, _ = run_steps(cfg, flow, PHYS_SEED, n, order, pred if order == 3 else None, prec, graph=True,
-                         extrapolation=3)
+                         extrapolation=3, log=(lambda m, nm=name: log(f"phys-long {nm} {m}")))
         its, wall = np.array(r["iterations"], float), np.array(r["wall_s"])
         out[name] = {"iterations": r["iterations"], "fallbacks": r["fallbacks"], "predictor_steps": r["predictor_steps"],
                      "mean_iterations": [round(float(its[a:b].mean()), 1) for a, b in windows],

@@ -111,12 +111,12 @@ def load(pred: Predictor, path):
     return json.loads(z["meta"].tobytes().decode())
 
 
-def run_steps(cfg, flow, seed, steps, order, pred=None, precision=CF_FP64, graph=False, extrapolation=2):
+def run_steps(cfg, flow, seed, steps, order, pred=None, precision=CF_FP64, graph=False, extrapolation=2, log=None):
     """Physical steps from the seed's state with initial guess `order` (0 U^n,
     2 extrapolation, 3 the attached predictor; `extrapolation` 2 quadratic or
     3 cubic, for order 2 and the predictor's fallback): per-step inner
     iterations and wall seconds, the predictor's fallback count, the final
-    level-n state."""
+    level-n state.  `log`, if given, receives one line per step."""
     Q = poiseuille_state(cfg, flow, seed=seed, stream=0)
     with Solver(cfg, precision=precision) as s:
         s.Jacobian_all(cfg.gamma_metric, want_arrays=False)
@@ -134,6 +134,8 @@ def run_steps(cfg, flow, seed, steps, order, pred=None, precision=CF_FP64, graph
             _sync()
             its.append(it)
             wall.append(time.perf_counter() - t0)
+            if log is not None:
+                log(f"step {len(its)}: {it} inner iterations, {wall[-1]:.3f} s")
         stats = s.predictor_stats()
         qn = s.read_state()[0]
     return {"iterations": its, "wall_s": wall, "predictor_steps": stats[0], "fallbacks": stats[1]}, qn
