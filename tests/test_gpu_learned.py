@@ -51,22 +51,27 @@ def test_gpu_learned_pipeline_same_answer():
         tight, _ = L.channel_cfg((32, 24, 40), beta=2.0, deltaT=5e-8, tol=1e-24, cap=6000)
         Q = poiseuille_state(tight, flow, seed=3, stream=0)
         finals = {}
-        for order in (0, 3):
+        # U^n; the predictor with the quadratic fallback; the predictor with
+        # the cubic restart (a loss to the quadratic check restarts the step
+        # from the cubic extrapolation once four levels are known)
+        for name, order, extrapolation in (("Un", 0, 2), ("pred", 3, 2), ("pred_cubic", 3, 3)):
             with Solver(tight) as s:
                 s.Jacobian_all(2.0, want_arrays=False)
                 s.set_predictor(pred)
+                s.set_extrapolation_order(extrapolation)
                 s.set_initial_guess(order)
                 s.load_state(Q, Q)
                 n_its = [s.physical_step()[0] for _ in range(5)]
                 steps, falls = s.predictor_stats()
-                finals[order] = s.read_state()[0]
-            print(f"order {order}: iterations {n_its}, predictor steps {steps}, fallbacks {falls}", flush=True)
+                finals[name] = s.read_state()[0]
+            print(f"{name}: iterations {n_its}, predictor steps {steps}, fallbacks {falls}", flush=True)
             # the first step has no U^{n-2} yet and starts from the extrapolation
             assert steps == (4 if order == 3 else 0)
-    change = np.linalg.norm(finals[0] - np.asarray(Q))
-    diff = np.linalg.norm(finals[3] - finals[0])
-    print(f"state difference {diff / change:.2e} of the change over 5 steps", flush=True)
-    assert diff <= 1e-10 * change
+    change = np.linalg.norm(finals["Un"] - np.asarray(Q))
+    for name in ("pred", "pred_cubic"):
+        diff = np.linalg.norm(finals[name] - finals["Un"])
+        print(f"{name}: state difference {diff / change:.2e} of the change over 5 steps", flush=True)
+        assert diff <= 1e-10 * change
     del its
 
 
