@@ -1176,7 +1176,11 @@ __device__ __forceinline__ void prep_summary(const double u5[5], const double* _
 // cells k-2 = s - a - b at steps s0 .. s0+PC-1: one run of PC consecutive k,
 // which the block stages in LDS with coalesced loads; slot (s, lane) then
 // reads its cell at LDS index s - s0 whatever the column.
-constexpr int PC = 24, PCP = PC + 1;  // steps per block; LDS column stride
+// 16 steps (53 KB of LDS with the update's reduction buffer, three blocks
+// per CU) measured fastest for the
+// update pass at 256x192x256: 0.327 ms against 0.342 / 0.358 / 0.361 ms for
+// 12 / 24 / 8 steps (FP64 NR), 0.286 against 0.310 / 0.334 / 0.325 ms (float)
+constexpr int PC = 16, PCP = PC + 1;  // steps per block; LDS column stride
 
 __device__ __forceinline__ void prep_stage(double (*L)[64][PCP], const double* __restrict__ src, const Geo& g,
                                            const Sk& q, int tile, int s0) {

@@ -389,10 +389,13 @@ int cf_pred_profile(cf_pred* p, int X, int Y, int Z, double ms[CF_PRED_PROFILE_S
 int cf_keep_levels(cf_ctx* ctx, int levels);
 /* Attaches a predictor (NULL detaches); cf_set_initial_guess(ctx, 3) then
  * starts each cf_physical_step with it once three levels are known.  The
- * quadratic extrapolation is the fallback: both guesses take one inner
- * iteration and the step continues from the network's unless some er_k of
- * its first iteration exceeds the extrapolation's (both iterations count).
- * The converged state is the same; the iteration count changes. */
+ * network refines the quadratic extrapolation, which is the check: both
+ * guesses take one inner iteration and the step continues from the network's
+ * unless some er_k of its first iteration exceeds the quadratic's.  On such a
+ * fallback the step continues from the quadratic one, or restarts from the
+ * cubic one with cf_set_extrapolation_order(ctx, 3) and four known levels
+ * (all iterations count).  The converged state is the same; the iteration
+ * count changes. */
 int cf_set_predictor(cf_ctx* ctx, cf_pred* p);
 /* Physical steps started by the network, and how many fell back. */
 int cf_predictor_stats(cf_ctx* ctx, long long* steps, long long* fallbacks);
