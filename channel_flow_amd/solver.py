@@ -11,6 +11,7 @@ the library is missing, or no GPU is visible, construction raises.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from pathlib import Path
 
 import numpy as np
@@ -477,6 +478,11 @@ class Solver:
     def set_predictor(self, pred):
         """Attaches a Predictor (None detaches); set_initial_guess(3) uses it."""
         self._chk(self.lib.cf_set_predictor(self._h, pred._h if pred is not None else None))
+        old = getattr(self, "_pred", None)
+        if old is not None and old is not pred:
+            old._attached.discard(self)
+        if pred is not None:
+            pred._attached.add(self)
         self._pred = pred  # keeps the predictor alive while attached
 
     def predictor_stats(self):
@@ -508,9 +514,16 @@ class Predictor:
             raise CfError(f"cf_pred_create failed: {ERRORS.get(r, r)}")
         self._h = h
         self.batch_crops = batch_crops
+        self._attached = weakref.WeakSet()  # Solvers holding this predictor
 
     def close(self):
+        """Destroys the predictor, after detaching it from every open Solver
+        it is attached to (a context keeps the pointer)."""
         if getattr(self, "_h", None):
+            for s in list(getattr(self, "_attached", ())):
+                if getattr(s, "_h", None):
+                    s.lib.cf_set_predictor(s._h, None)
+                s._pred = None
             self.lib.cf_pred_destroy(self._h)
             self._h = None
 

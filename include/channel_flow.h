@@ -395,7 +395,8 @@ int cf_keep_levels(cf_ctx* ctx, int levels);
  * fallback the step continues from the quadratic one, or restarts from the
  * cubic one with cf_set_extrapolation_order(ctx, 3) and four known levels
  * (all iterations count).  The converged state is the same; the iteration
- * count changes. */
+ * count changes.  The context keeps the pointer: detach (NULL) or destroy
+ * the context before cf_pred_destroy. */
 int cf_set_predictor(cf_ctx* ctx, cf_pred* p);
 /* Physical steps started by the network, and how many fell back. */
 int cf_predictor_stats(cf_ctx* ctx, long long* steps, long long* fallbacks);

@@ -105,3 +105,29 @@ def test_gpu_learned_gain_512_patch():
           flush=True)
     assert its["predictor"] < 0.85 * min(its["start_Un"], its["extrapolated"])
     assert res["fallback_rate"] is not None and res["fallback_rate"] <= 0.5
+
+
+def test_gpu_predictor_close_detaches():
+    """Closing a predictor that a context still holds detaches it first: the
+    next physical steps start from the extrapolation (the context keeps no
+    pointer to freed memory)."""
+    from channel_flow_amd import learned as L
+    from channel_flow_amd.config import poiseuille_state
+    from channel_flow_amd.solver import Predictor, Solver
+    cfg, flow = L.channel_cfg((24, 20, 24), beta=2.0, deltaT=5e-8, tol=1e-6, cap=400)
+    Q = poiseuille_state(cfg, flow, seed=5, stream=0)
+    pred = Predictor(dataset_crops=1, batch_crops=8)
+    L.load(pred, WEIGHTS64)
+    with Solver(cfg) as s:
+        s.Jacobian_all(2.0, want_arrays=False)
+        s.set_predictor(pred)
+        s.set_initial_guess(3)
+        s.load_state(Q, Q)
+        for _ in range(3):
+            s.physical_step()
+        assert s.predictor_stats()[0] == 2
+        pred.close()
+        assert s._pred is None
+        its = [s.physical_step()[0] for _ in range(2)]
+        assert all(0 < i <= 400 for i in its)
+        assert np.isfinite(np.concatenate(s.read_state())).all()
