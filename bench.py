@@ -323,11 +323,13 @@ def phys_long_leg(args):
     from channel_flow_amd.solver import CF_FP32_SWEEPS, CF_FP64, Predictor
     n = args.phys_long_steps
     cfg, flow = channel_cfg(GRID, beta=2.0, deltaT=1e-8, tol=1e-8, cap=2000)
+    wpath = Path(args.weights).resolve() if args.weights else WEIGHTS
     pred = Predictor(dataset_crops=1, batch_crops=8)
-    load(pred, WEIGHTS)
+    load(pred, wpath)
     edges = [e for e in (3, 20, 40, 60, 100, 200) if e < n] + [n]
     windows = list(zip(edges[:-1], edges[1:]))
-    out = {"steps": n, "seed": PHYS_SEED, "weights": str(WEIGHTS.relative_to(ROOT)),
+    out = {"steps": n, "seed": PHYS_SEED,
+           "weights": str(wpath.relative_to(ROOT) if wpath.is_relative_to(ROOT) else wpath),
            "windows": [f"{a + 1}-{b}" for a, b in windows]}
     runs = (("fp64_start_Un", CF_FP64, 0), ("fp64_cubic", CF_FP64, 2), ("fp32_sweeps_cubic", CF_FP32_SWEEPS, 2),
             ("fp32_sweeps_predictor", CF_FP32_SWEEPS, 3))
