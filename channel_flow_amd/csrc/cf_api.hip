@@ -1341,20 +1341,22 @@ extern "C" int cf_save_checkpoint(cf_ctx* c, const char* path) {
   if ((r = sync(c))) return r;
   FILE* f = fopen(path, "wb");
   if (!f) return fail(c, CF_ERR_IO, "cannot open %s", path);
-  // dims[3]: fields after the 15 levels: 5 when U^{n-2} is kept and valid
-  const int nextra = c->Uqq && c->uqq_valid ? 5 : 0;
+  // dims[3]: fields after the 15 levels: 5 when U^{n-2} is kept and valid,
+  // 10 when U^{n-3} is as well (the cubic initial iterate restarts exactly)
+  const int nextra = c->Uqq && c->uqq_valid ? (c->U3 && c->u3_valid ? 10 : 5) : 0;
   const int32_t dims[4] = {c->p.X_out, c->p.Y_out, c->p.Z_out, nextra};
   std::vector<double> st((size_t)7 * c->g.Ym);
   HIPCHK(c, hipMemcpy(st.data(), c->stat, st.size() * 8, hipMemcpyDeviceToHost));
   const double sxy[4] = {c->sxy_sum[0], c->sxy_sum[1], c->sxy_prev[0], c->sxy_prev[1]};
   bool ok = fwrite(CKPT_MAGIC, 1, 8, f) == 8 && fwrite(dims, 4, 4, f) == 4 && fwrite(c->E, 8, 5, f) == 5 &&
             fwrite(st.data(), 8, st.size(), f) == st.size() && fwrite(sxy, 8, 4, f) == 4;
-  double* fields[20];
+  double* fields[25];
   for (int v = 0; v < 5; v++) {
     fields[v] = c->Ui + v * c->g.nf;
     fields[5 + v] = c->Un + v * c->g.nf;
     fields[10 + v] = c->Uq + v * c->g.nf;
     fields[15 + v] = c->Uqq ? c->Uqq + v * c->g.nf : nullptr;
+    fields[20 + v] = c->U3 ? c->U3 + v * c->g.nf : nullptr;
   }
   for (int q = 0; ok && q < 15 + nextra; q++)
     if ((r = ckpt_field(c, f, fields[q], true))) {
@@ -1379,7 +1381,7 @@ extern "C" int cf_load_checkpoint(cf_ctx* c, const char* path) {
             fread(dims, 4, 4, f) == 4;
   const bool v1 = memcmp(magic, CKPT_MAGIC_V1, 8) == 0;
   if (!ok || dims[0] != c->p.X_out || dims[1] != c->p.Y_out || dims[2] != c->p.Z_out ||
-      (dims[3] != 0 && dims[3] != 5)) {
+      (dims[3] != 0 && dims[3] != 5 && dims[3] != 10)) {
     fclose(f);
     return fail(c, CF_ERR_IO, "%s is not a checkpoint of this grid", path);
   }
@@ -1390,25 +1392,29 @@ extern "C" int cf_load_checkpoint(cf_ctx* c, const char* path) {
   }
   int r;
   const int nextra = dims[3];
-  if (nextra && !c->Uqq) {
-    // the file keeps U^{n-2}: hold it even if the quadratic guess is not selected yet
+  // the file keeps U^{n-2} (and U^{n-3}): hold them even if the initial
+  // guess that uses them is not selected yet
+  auto hold = [&](double** level, const char* name) -> int {
+    if (*level) return CF_OK;
     const size_t bytes = (size_t)c->g.nf * 5 * sizeof(double);
-    if (hipMalloc(&c->Uqq, bytes) != hipSuccess) {
-      c->Uqq = nullptr;
-      fclose(f);
-      return fail(c, CF_ERR_HIP, "hipMalloc of the U^{n-2} level failed");
+    if (hipMalloc(level, bytes) != hipSuccess) {
+      *level = nullptr;
+      return fail(c, CF_ERR_HIP, "hipMalloc of the %s level failed", name);
     }
-    if (hipMemset(c->Uqq, 0, bytes) != hipSuccess) {
-      fclose(f);
-      return fail(c, CF_ERR_HIP, "hipMemset of the U^{n-2} level failed");
-    }
+    if (hipMemset(*level, 0, bytes) != hipSuccess) return fail(c, CF_ERR_HIP, "hipMemset of the %s level failed", name);
+    return CF_OK;
+  };
+  if ((nextra >= 5 && (r = hold(&c->Uqq, "U^{n-2}"))) || (nextra >= 10 && (r = hold(&c->U3, "U^{n-3}")))) {
+    fclose(f);
+    return r;
   }
-  double* fields[20];
+  double* fields[25];
   for (int v = 0; v < 5; v++) {
     fields[v] = c->Ui + v * c->g.nf;
     fields[5 + v] = c->Un + v * c->g.nf;
     fields[10 + v] = c->Uq + v * c->g.nf;
     fields[15 + v] = c->Uqq ? c->Uqq + v * c->g.nf : nullptr;
+    fields[20 + v] = c->U3 ? c->U3 + v * c->g.nf : nullptr;
   }
   for (int q = 0; q < 15 + nextra; q++)
     if ((r = ckpt_field(c, f, fields[q], false))) {
@@ -1423,8 +1429,8 @@ extern "C" int cf_load_checkpoint(cf_ctx* c, const char* path) {
   c->sxy_prev[0] = sxy[2];
   c->sxy_prev[1] = sxy[3];
   c->have_state = true;
-  c->uqq_valid = nextra == 5;
-  c->u3_valid = false;
+  c->uqq_valid = nextra >= 5;
+  c->u3_valid = nextra == 10;
   return CF_OK;
 }
 

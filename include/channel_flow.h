@@ -211,7 +211,8 @@ int cf_write_output(cf_ctx* ctx, int step, const char* dir);
  * "CFCKPT02", int32 X_out Y_out Z_out M, double E[5], double sums[7][Y_m],
  * double wall[4] (the Shear_stress.dat running sums and previous averages),
  * then 15 fields (U_, U, Uq) of X_out*Y_out*Z_out doubles, [x][y][z], then M
- * more: 5 (level n-2) when the quadratic initial guess holds one, else 0.
+ * more: 5 (level n-2) when the quadratic initial guess holds one, 10 (levels
+ * n-2 and n-3) when the cubic one holds both, else 0.
  * cf_load_checkpoint also reads "CFCKPT01" files, which have no wall[4]. */
 int cf_save_checkpoint(cf_ctx* ctx, const char* path);
 int cf_load_checkpoint(cf_ctx* ctx, const char* path);

@@ -85,14 +85,16 @@ def test_gpu_initial_guess_fewer_iterations_closer_state():
         assert err[order] <= err[0]
 
 
-def test_gpu_initial_guess_checkpoint_restart_is_exact(tmp_path):
-    """The checkpoint keeps U^{n-2}: a run restarted after step 3 continues
-    the quadratic guess bit for bit."""
+@pytest.mark.parametrize("extrapolation", [2, 3])
+def test_gpu_initial_guess_checkpoint_restart_is_exact(tmp_path, extrapolation):
+    """The checkpoint keeps U^{n-2} (and U^{n-3} for the cubic start): a run
+    restarted after step 3 continues the quadratic or cubic guess bit for
+    bit."""
     from channel_flow_amd.config import poiseuille_state
     cfg, fl = _cfg()
     Q = poiseuille_state(cfg, fl, seed=2)
     ck = tmp_path / "guess.ckpt"
-    its_full, q_full = _run(2, 6, cfg, Q, Q, checkpoint=(2, ck))
-    its_rest, q_rest = _run(2, 3, cfg, None, None, restart=ck)
+    its_full, q_full = _run(2, 6, cfg, Q, Q, checkpoint=(2, ck), extrapolation=extrapolation)
+    its_rest, q_rest = _run(2, 3, cfg, None, None, restart=ck, extrapolation=extrapolation)
     np.testing.assert_array_equal(its_rest, its_full[3:])
     np.testing.assert_array_equal(q_rest, q_full)
