@@ -98,6 +98,11 @@ __device__ __forceinline__ void flux_face(const double (&u)[5][6], int f, int i,
   for (int v = 0; v < 5; v++) F[v * g.nf + o] = out[v];
 }
 
+// threadIdx.y of a block 64 threads wide: one wave per row, so the value is
+// uniform across the wave, and what depends on it alone (j, the grid
+// profiles at j) is held in scalar registers and read with scalar loads
+__device__ __forceinline__ int wave_row() { return __builtin_amdgcn_readfirstlane(threadIdx.y); }
+
 // one thread per face (Flux_Z: the lanes run along k, the direction itself)
 template <int DIR>
 __global__ void __launch_bounds__(256) k_flux(const double* __restrict__ U, const double* __restrict__ prof,
@@ -144,7 +149,7 @@ __global__ void __launch_bounds__(256) k_flux_zs(const double* __restrict__ U, c
                                                  double* __restrict__ F, Geo g) {
   static_assert(DIR == 2, "faces along k");
   __shared__ double S[4][5][72];
-  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int tx = threadIdx.x, ty = wave_row();  // j's profiles in scalar registers: 84 -> 76 VGPRs
   const int f0 = 1 + blockIdx.x * 64;
   const int j = 2 + blockIdx.y * 4 + ty, i = 3 + blockIdx.z;
   const bool row = j <= g.ny;
@@ -173,7 +178,8 @@ __global__ void __launch_bounds__(256) k_flux_zs(const double* __restrict__ U, c
 // face ahead) where a thread per face loads up to six.  16 faces per
 // thread measured faster than 8 and equal to 32 at 256x192x256.
 constexpr int FC = 16;
-// at least 3 waves per SIMD: 168 VGPRs (4 would spill)
+// at least 3 waves per SIMD: x takes 126 VGPRs (4 waves), y 144 (forcing
+// 4 waves spills y and took 3% longer)
 constexpr int FLUX_COL_WAVES = 3;
 
 template <int DIR>
@@ -182,7 +188,10 @@ __global__ void __launch_bounds__(256, FLUX_COL_WAVES) k_flux_col(const double* 
   static_assert(DIR == 0 || DIR == 1, "columns along x or y");
   // DIR 0: f in [2,gend] (i), a = j in [2,ny]; DIR 1: f in [1,ny] (j), a = i in [3,gend]
   const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int a = blockIdx.y * blockDim.y + threadIdx.y;
+  // x: the row is j, whose profiles every face reads (a wave-uniform row
+  // took the kernel from 168 to 126 VGPRs and 0.398 to 0.318 ms); y: the
+  // row is i, and a uniform i measured 2% slower
+  const int a = blockIdx.y * blockDim.y + (DIR == 0 ? wave_row() : (int)threadIdx.y);
   const int i = DIR == 0 ? 0 : 3 + a, j = DIR == 0 ? 2 + a : 0;
   if (k > g.nz || (DIR == 0 ? j > g.ny : i > g.gend)) return;
   const int fmin = DIR == 0 ? 2 : 1, fmax = DIR == 0 ? g.gend : g.ny;
@@ -360,7 +369,7 @@ __global__ void __launch_bounds__(256, 2) k_flux_lim_col(const double* __restric
                                                                          double* __restrict__ F, Geo g) {
   static_assert(DIR == 0 || DIR == 1, "columns along x or y");
   const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int a = blockIdx.y * blockDim.y + threadIdx.y;
+  const int a = blockIdx.y * blockDim.y + (DIR == 0 ? wave_row() : (int)threadIdx.y);
   const int i = DIR == 0 ? 0 : 3 + a, j = DIR == 0 ? 2 + a : 0;
   if (k > g.nz || (DIR == 0 ? j > g.ny : i > g.gend)) return;
   const int fmin = DIR == 0 ? 2 : 1, fmax = DIR == 0 ? g.gend : g.ny;
@@ -694,13 +703,17 @@ __global__ void __launch_bounds__(256) k_visc_div(const double* __restrict__ VS,
 // (k0+VK).  The x face below a cell is the one computed at the previous
 // plane.  The face arithmetic and the divergence are those of k_visc_faces
 // and k_visc_div, operand for operand, so vF is unchanged bit for bit while
-// the primitives and the twelve face-flux fields never reach HBM.
+// the primitives and the twelve face-flux fields never reach HBM.  Each
+// thread holds the raw U_ of plane i+1 in registers while plane i's faces are
+// formed (0.638 -> 0.55 ms at 256x192x256; at 166 VGPRs before the face
+// metrics moved to LDS it cost a block per CU and gained nothing).
 constexpr int VJ = 8, VK = 32, VC = 16, VR = VJ + 2, VKK = VK + 2, VTHREADS = VJ * VK + 64;
 
 struct VLds {
   double P[3][4][VR][VKK];
   double ML[4][VJ + 1][VK];
   double NL[4][VJ][VK + 1];
+  double PJ[3][4][VJ + 1];  // face sets U, V, W: xidx, etdy, ztdz and J at j - 1, row j - j0
 };
 
 __device__ __forceinline__ Prim vp(const VLds& L, int s, int r, int c) {
@@ -712,17 +725,41 @@ __device__ __forceinline__ Prim vp(const VLds& L, int s, int r, int c) {
   return q;
 }
 
-__device__ __forceinline__ void vload_plane(VLds& L, const double* __restrict__ U, const double* __restrict__ prof,
-                                            const Geo& g, int ip, int s, int j0, int k0, int tid) {
-  for (int idx = tid; idx < VR * VKK; idx += VTHREADS) {
+// raw U_ and J of the rim tile's cells tid and tid + VTHREADS of one plane
+// (VR * VKK <= 2 * VTHREADS), fetched a plane ahead so that the loads are in
+// flight while the faces of the current plane are formed
+static_assert(VR * VKK <= 2 * VTHREADS, "two rim cells per thread");
+struct VRaw {
+  double u[2][5];
+  double J[2];
+};
+
+__device__ __forceinline__ void vfetch(VRaw& R, const double* __restrict__ U, const double* __restrict__ prof,
+                                       const Geo& g, int ip, int j0, int k0, int tid) {
+#pragma unroll
+  for (int m = 0; m < 2; m++) {
+    const int idx = tid + m * VTHREADS;
+    if (idx >= VR * VKK) continue;
+    const int r = idx / VKK, c = idx - r * VKK;
+    const int jj = j0 - 1 + r, kk = k0 - 1 + c;
+    if (jj <= g.ny + 1 && kk <= g.nz + 1) {
+      const long p = at(g, ip, jj, kk);
+#pragma unroll
+      for (int v = 0; v < 5; v++) R.u[m][v] = U[v * g.nf + p];
+      R.J[m] = pf(prof, g.Ym, SET_C, 0, jj);
+    }
+  }
+}
+
+__device__ __forceinline__ void vstore(VLds& L, const VRaw& R, const Geo& g, int s, int j0, int k0, int tid) {
+#pragma unroll
+  for (int m = 0; m < 2; m++) {
+    const int idx = tid + m * VTHREADS;
+    if (idx >= VR * VKK) continue;
     const int r = idx / VKK, c = idx - r * VKK;
     const int jj = j0 - 1 + r, kk = k0 - 1 + c;
     Prim q{0., 0., 0., 0.};
-    if (jj <= g.ny + 1 && kk <= g.nz + 1) {
-      const long p = at(g, ip, jj, kk);
-      q = prim_visc(U[p], U[g.nf + p], U[2 * g.nf + p], U[3 * g.nf + p], U[4 * g.nf + p],
-                    pf(prof, g.Ym, SET_C, 0, jj));
-    }
+    if (jj <= g.ny + 1 && kk <= g.nz + 1) q = prim_visc(R.u[m][0], R.u[m][1], R.u[m][2], R.u[m][3], R.u[m][4], R.J[m]);
     L.P[s][0][r][c] = q.U;
     L.P[s][1][r][c] = q.V;
     L.P[s][2][r][c] = q.W;
@@ -743,28 +780,20 @@ struct DiagCoef {
   double X, E, Z;  // xidx, etdy, ztdz of the face set
 };
 
-__device__ __forceinline__ DiagCoef diag_face(const double* __restrict__ prof, int Ym, int set, int j) {
-  DiagCoef q;
-  q.X = pf(prof, Ym, set, 1, j);
-  q.E = pf(prof, Ym, set, 5, j);
-  q.Z = pf(prof, Ym, set, 9, j);
-  return q;
-}
-
 __device__ __forceinline__ double lp(const VLds& L, int s, int f, int r, int c) { return L.P[s][f][r][c]; }
 
-// x face between planes (slot sm) and (slot s0) at LDS cell (r, c), row j
-__device__ __forceinline__ void vface_x(const VLds& L, const double* __restrict__ prof, const Geo& g, int s0, int sm,
-                                        int r, int c, int j, double invXI, double inv4ET, double inv4ZT,
-                                        double o[4]) {
+// x face between planes (slot sm) and (slot s0) at LDS cell (r, c); the
+// face metrics of LDS row r are L.PJ[.][.][r - 1]
+__device__ __forceinline__ void vface_x(const VLds& L, int s0, int sm, int r, int c, double invXI, double inv4ET,
+                                        double inv4ZT, double o[4]) {
   const Prim P0 = vp(L, s0, r, c), iP = vp(L, sm, r, c);
   const DSet dx = d_two(P0, iP, invXI);
   const double dyu = (lp(L, s0, 0, r + 1, c) + lp(L, sm, 0, r + 1, c) - lp(L, s0, 0, r - 1, c) - lp(L, sm, 0, r - 1, c)) * inv4ET;
   const double dyv = (lp(L, s0, 1, r + 1, c) + lp(L, sm, 1, r + 1, c) - lp(L, s0, 1, r - 1, c) - lp(L, sm, 1, r - 1, c)) * inv4ET;
   const double dzu = (lp(L, s0, 0, r, c + 1) + lp(L, sm, 0, r, c + 1) - lp(L, s0, 0, r, c - 1) - lp(L, sm, 0, r, c - 1)) * inv4ZT;
   const double dzw = (lp(L, s0, 2, r, c + 1) + lp(L, sm, 2, r, c + 1) - lp(L, s0, 2, r, c - 1) - lp(L, sm, 2, r, c - 1)) * inv4ZT;
-  const DiagCoef m = diag_face(prof, g.Ym, SET_U, j - 1);
-  const double Jf = pf(prof, g.Ym, SET_U, 0, j - 1);
+  const DiagCoef m{L.PJ[0][0][r - 1], L.PJ[0][1][r - 1], L.PJ[0][2][r - 1]};
+  const double Jf = L.PJ[0][3][r - 1];
   const double a1 = (4. / 3) * m.X * m.X, a2 = m.X * m.X, a3 = m.X * m.X, a4 = m.X * m.X;
   const double d5 = m.X * m.E, d7 = -((2. / 3) * m.X * m.E);
   const double e6 = m.X * m.Z, e9 = -((2. / 3) * m.X * m.Z);
@@ -778,17 +807,16 @@ __device__ __forceinline__ void vface_x(const VLds& L, const double* __restrict_
 }
 
 // y face between rows r-1 and r of plane slot s0
-__device__ __forceinline__ void vface_y(const VLds& L, const double* __restrict__ prof, const Geo& g, int sp, int s0,
-                                        int sm, int r, int c, int j, double invET, double inv4XI, double inv4ZT,
-                                        double o[4]) {
+__device__ __forceinline__ void vface_y(const VLds& L, int sp, int s0, int sm, int r, int c, double invET,
+                                        double inv4XI, double inv4ZT, double o[4]) {
   const Prim P0 = vp(L, s0, r, c), jP = vp(L, s0, r - 1, c);
   const DSet dy = d_two(P0, jP, invET);
   const double dxu = (lp(L, sp, 0, r, c) + lp(L, sp, 0, r - 1, c) - lp(L, sm, 0, r, c) - lp(L, sm, 0, r - 1, c)) * inv4XI;
   const double dxv = (lp(L, sp, 1, r, c) + lp(L, sp, 1, r - 1, c) - lp(L, sm, 1, r, c) - lp(L, sm, 1, r - 1, c)) * inv4XI;
   const double dzv = (lp(L, s0, 1, r, c + 1) + lp(L, s0, 1, r - 1, c + 1) - lp(L, s0, 1, r, c - 1) - lp(L, s0, 1, r - 1, c - 1)) * inv4ZT;
   const double dzw = (lp(L, s0, 2, r, c + 1) + lp(L, s0, 2, r - 1, c + 1) - lp(L, s0, 2, r, c - 1) - lp(L, s0, 2, r - 1, c - 1)) * inv4ZT;
-  const DiagCoef m = diag_face(prof, g.Ym, SET_V, j - 1);
-  const double Jf = pf(prof, g.Ym, SET_V, 0, j - 1);
+  const DiagCoef m{L.PJ[1][0][r - 1], L.PJ[1][1][r - 1], L.PJ[1][2][r - 1]};
+  const double Jf = L.PJ[1][3][r - 1];
   const double b1 = m.E * m.E, b2 = (4. / 3) * m.E * m.E, b3 = m.E * m.E, b4 = m.E * m.E;
   const double d5 = m.X * m.E, d7 = -((2. / 3) * m.X * m.E);
   const double f8 = m.E * m.Z, f10 = -((2. / 3) * m.E * m.Z);
@@ -802,17 +830,16 @@ __device__ __forceinline__ void vface_y(const VLds& L, const double* __restrict_
 }
 
 // z face between columns c-1 and c of plane slot s0
-__device__ __forceinline__ void vface_z(const VLds& L, const double* __restrict__ prof, const Geo& g, int sp, int s0,
-                                        int sm, int r, int c, int j, double invZT, double inv4XI, double inv4ET,
-                                        double o[4]) {
+__device__ __forceinline__ void vface_z(const VLds& L, int sp, int s0, int sm, int r, int c, double invZT,
+                                        double inv4XI, double inv4ET, double o[4]) {
   const Prim P0 = vp(L, s0, r, c), kP = vp(L, s0, r, c - 1);
   const DSet dz = d_two(P0, kP, invZT);
   const double dxu = (lp(L, sp, 0, r, c) + lp(L, sp, 0, r, c - 1) - lp(L, sm, 0, r, c) - lp(L, sm, 0, r, c - 1)) * inv4XI;
   const double dxw = (lp(L, sp, 2, r, c) + lp(L, sp, 2, r, c - 1) - lp(L, sm, 2, r, c) - lp(L, sm, 2, r, c - 1)) * inv4XI;
   const double dyv = (lp(L, s0, 1, r + 1, c) + lp(L, s0, 1, r + 1, c - 1) - lp(L, s0, 1, r - 1, c) - lp(L, s0, 1, r - 1, c - 1)) * inv4ET;
   const double dyw = (lp(L, s0, 2, r + 1, c) + lp(L, s0, 2, r + 1, c - 1) - lp(L, s0, 2, r - 1, c) - lp(L, s0, 2, r - 1, c - 1)) * inv4ET;
-  const DiagCoef m = diag_face(prof, g.Ym, SET_W, j - 1);
-  const double Jf = pf(prof, g.Ym, SET_W, 0, j - 1);
+  const DiagCoef m{L.PJ[2][0][r - 1], L.PJ[2][1][r - 1], L.PJ[2][2][r - 1]};
+  const double Jf = L.PJ[2][3][r - 1];
   const double c1 = m.Z * m.Z, c2 = m.Z * m.Z, c3 = (4. / 3) * m.Z * m.Z, c4 = m.Z * m.Z;
   const double e6 = m.X * m.Z, e9 = -((2. / 3) * m.X * m.Z);
   const double f8 = m.E * m.Z, f10 = -((2. / 3) * m.E * m.Z);
@@ -846,35 +873,52 @@ __global__ void __launch_bounds__(VTHREADS) k_visc_fused(const double* __restric
   const bool exNL = !main_t && e >= VK && e < VK + VJ && j0 + (e - VK) <= g.ny && k0 + VK <= g.nz + 1;
   const long nf = g.nf;
 
-  vload_plane(L, U, prof, g, ib - 1, (ib - 1) % 3, j0, k0, tid);
-  vload_plane(L, U, prof, g, ib, ib % 3, j0, k0, tid);
+  // the face metrics of the tile's rows, read once: a wave spans two rows j,
+  // so reading them per face from global memory takes vector loads and
+  // registers (128 -> 98 VGPRs, 0.69 -> 0.654 ms at 256x192x256)
+  for (int idx = tid; idx < 3 * (VJ + 1); idx += VTHREADS) {
+    const int set = idx / (VJ + 1), r = idx - set * (VJ + 1), jp = j0 + r - 1;
+    const int sid = set == 0 ? SET_U : set == 1 ? SET_V : SET_W;
+    const bool in = jp < g.Ym;
+    L.PJ[set][0][r] = in ? pf(prof, g.Ym, sid, 1, jp) : 0.;
+    L.PJ[set][1][r] = in ? pf(prof, g.Ym, sid, 5, jp) : 0.;
+    L.PJ[set][2][r] = in ? pf(prof, g.Ym, sid, 9, jp) : 0.;
+    L.PJ[set][3][r] = in ? pf(prof, g.Ym, sid, 0, jp) : 0.;
+  }
+  VRaw R;
+  vfetch(R, U, prof, g, ib - 1, j0, k0, tid);
+  vstore(L, R, g, (ib - 1) % 3, j0, k0, tid);
+  vfetch(R, U, prof, g, ib, j0, k0, tid);
+  vstore(L, R, g, ib % 3, j0, k0, tid);
+  vfetch(R, U, prof, g, ib + 1, j0, k0, tid);
   __syncthreads();
   double LLc[4] = {0., 0., 0., 0.};
-  if (cell) vface_x(L, prof, g, ib % 3, (ib - 1) % 3, ty + 1, tx + 1, j, invXI, inv4ET, inv4ZT, LLc);
+  if (cell) vface_x(L, ib % 3, (ib - 1) % 3, ty + 1, tx + 1, invXI, inv4ET, inv4ZT, LLc);
   for (int i = ib; i <= ie; i++) {
     const int sm = (i - 1) % 3, s0 = i % 3, sp = (i + 1) % 3;
-    vload_plane(L, U, prof, g, i + 1, sp, j0, k0, tid);
+    vstore(L, R, g, sp, j0, k0, tid);
+    if (i + 1 <= ie) vfetch(R, U, prof, g, i + 2, j0, k0, tid);
     __syncthreads();
     double LLn[4] = {0., 0., 0., 0.}, o[4];
-    if (cell) vface_x(L, prof, g, sp, s0, ty + 1, tx + 1, j, invXI, inv4ET, inv4ZT, LLn);
+    if (cell) vface_x(L, sp, s0, ty + 1, tx + 1, invXI, inv4ET, inv4ZT, LLn);
     if (doML) {
-      vface_y(L, prof, g, sp, s0, sm, ty + 1, tx + 1, j, invET, inv4XI, inv4ZT, o);
+      vface_y(L, sp, s0, sm, ty + 1, tx + 1, invET, inv4XI, inv4ZT, o);
 #pragma unroll
       for (int v = 0; v < 4; v++) L.ML[v][ty][tx] = o[v];
     }
     if (doNL) {
-      vface_z(L, prof, g, sp, s0, sm, ty + 1, tx + 1, j, invZT, inv4XI, inv4ET, o);
+      vface_z(L, sp, s0, sm, ty + 1, tx + 1, invZT, inv4XI, inv4ET, o);
 #pragma unroll
       for (int v = 0; v < 4; v++) L.NL[v][ty][tx] = o[v];
     }
     if (exML) {
-      vface_y(L, prof, g, sp, s0, sm, VJ + 1, e + 1, j0 + VJ, invET, inv4XI, inv4ZT, o);
+      vface_y(L, sp, s0, sm, VJ + 1, e + 1, invET, inv4XI, inv4ZT, o);
 #pragma unroll
       for (int v = 0; v < 4; v++) L.ML[v][VJ][e] = o[v];
     }
     if (exNL) {
       const int r = e - VK;
-      vface_z(L, prof, g, sp, s0, sm, r + 1, VK + 1, j0 + r, invZT, inv4XI, inv4ET, o);
+      vface_z(L, sp, s0, sm, r + 1, VK + 1, invZT, inv4XI, inv4ET, o);
 #pragma unroll
       for (int v = 0; v < 4; v++) L.NL[v][r][VK] = o[v];
     }
@@ -900,20 +944,19 @@ __device__ __forceinline__ void cell_metric(const double* __restrict__ prof, int
 }
 
 // MR = Gamma^-1 Rk of interior cell (i, j, k) (Dual_time_stepping.cpp:
-// 338-436); s receives the cell's primitive state
+// 338-436); s receives the cell's primitive state and u5 its U_
 __device__ __forceinline__ void rhs_cell(const double* __restrict__ Ui, const double* __restrict__ Un,
                                          const double* __restrict__ Uq, const double* __restrict__ Fx,
                                          const double* __restrict__ Fy, const double* __restrict__ Fz,
                                          const double* __restrict__ vF, const double* __restrict__ prof,
                                          const Geo& g, double deltaT, double f_drive, int i, int j, int k,
-                                         double M[5], PState& s) {
+                                         double M[5], PState& s, double u5[5]) {
   const long nf = g.nf;
   long c = at(g, i, j, k);
   long fx = at(g, i, j - 1, k - 1), fxm = fx - g.plane;
   long fy = at(g, i - 1, j, k - 1), fym = fy - g.Zp;
   long fz = at(g, i - 1, j - 1, k), fzm = fz - 1;
   double J = pf(prof, g.Ym, SET_C, 0, j);
-  double u5[5];
 #pragma unroll
   for (int v = 0; v < 5; v++) u5[v] = Ui[v * nf + c];
   s = prim_dt(u5[0], u5[1], u5[2], u5[3], u5[4], J);
@@ -959,9 +1002,9 @@ __global__ void __launch_bounds__(256) k_rhs(const double* __restrict__ Ui, cons
   if (k > g.nz || j > g.ny) return;
   const long nf = g.nf;
   const long c = at(g, i, j, k);
-  double M[5];
+  double M[5], u5[5];
   PState s;
-  rhs_cell(Ui, Un, Uq, Fx, Fy, Fz, vF, prof, g, deltaT, f_drive, i, j, k, M, s);
+  rhs_cell(Ui, Un, Uq, Fx, Fy, Fz, vF, prof, g, deltaT, f_drive, i, j, k, M, s, u5);
 #pragma unroll
   for (int q = 0; q < 5; q++) MR[q * nf + c] = M[q];
   if (NL) {
@@ -1244,31 +1287,38 @@ __global__ void __launch_bounds__(256) k_sweep_prep(const double* __restrict__ U
 // (as k_sweep_prep does) in two rounds through one 5-field LDS tile: it walks
 // each column's run of PR consecutive k in the natural layout (coalesced),
 // forms MR of each cell with rhs_cell and writes whole 512-byte sweep-order
-// rows of MR; then it stages the runs of U_ (just read, so from cache) and
-// writes the rows of the summary.  MR never goes to HBM in the natural
-// layout.  The single 5-field tile (43 KB) lets three blocks share a CU;
-// ten fields staged at once (PR = 12, 66 KB, two blocks) took 3.4% longer
-// in FP64 and 5.5% in float.
-constexpr int PR = 16, PRP = PR + 1;
+// rows of MR; then it stages the runs of U_ and writes the rows of the
+// summary.  MR never goes to HBM in the natural layout.  The single 5-field
+// tile (43 KB) lets three blocks share a CU; ten fields staged at once
+// (PR = 12, 66 KB, two blocks) took 3.4% longer in FP64 and 5.5% in float.
+// Each thread keeps the U_ of its PR / 4 cells in registers from the first
+// round to the second: a block's first round outlasts the L2's turnover, so
+// reading U_ again would fetch it from HBM a second time.
+constexpr int PR = 16, PRP = PR + 1, PR_THREADS = 256, PR_CELLS = 64 * PR / PR_THREADS;
+static_assert(64 * PR % PR_THREADS == 0, "whole cells per thread");
 
 template <class T>
-__global__ void __launch_bounds__(256) k_sweep_prep_rhs(const double* __restrict__ Ui, RhsIn rin,
-                                                        const double* __restrict__ prof, T* __restrict__ S,
-                                                        T* __restrict__ MRs, T* __restrict__ NLs, Geo g,
-                                                        Sk q, double deltaT) {
+__global__ void __launch_bounds__(PR_THREADS) k_sweep_prep_rhs(const double* __restrict__ Ui, RhsIn rin,
+                                                               const double* __restrict__ prof, T* __restrict__ S,
+                                                               T* __restrict__ MRs, T* __restrict__ NLs, Geo g,
+                                                               Sk q, double deltaT) {
   __shared__ double L[5][64][PRP];
   int bx, by, bz;
   xcd_block3(&bx, &by, &bz);
   const int tile = by, s0 = bx * PR;
   const int TX = tile % q.nTI, TY = tile / q.nTI;
-  for (int idx = threadIdx.x; idx < 64 * PR; idx += blockDim.x) {
+  double keep[PR_CELLS][5];
+#pragma unroll
+  for (int m = 0; m < PR_CELLS; m++) {
+    const int idx = threadIdx.x + m * PR_THREADS;
     const int col = idx / PR, t = idx - col * PR;
     const int a = col & 7, b = col >> 3;
     const int i = 3 + TX * TI + a, j = 2 + TY * TJ + b, kk = s0 - a - b + t;
     if (i > g.gend || j > g.ny || kk < 0 || kk >= q.Nz) continue;
     double M[5];
     PState st;
-    rhs_cell(Ui, rin.Un, rin.Uq, rin.Fx, rin.Fy, rin.Fz, rin.vF, prof, g, deltaT, rin.f_drive, i, j, 2 + kk, M, st);
+    rhs_cell(Ui, rin.Un, rin.Uq, rin.Fx, rin.Fy, rin.Fz, rin.vF, prof, g, deltaT, rin.f_drive, i, j, 2 + kk, M, st,
+             keep[m]);
 #pragma unroll
     for (int f = 0; f < 5; f++) L[f][col][t] = M[f];
   }
@@ -1288,14 +1338,15 @@ __global__ void __launch_bounds__(256) k_sweep_prep_rhs(const double* __restrict
     for (int r = 0; r < 5; r++) MRs[r * q.nsk + o] = T(L[r][lane][t]);
   }
   __syncthreads();
-  for (int idx = threadIdx.x; idx < 64 * PR; idx += blockDim.x) {
+#pragma unroll
+  for (int m = 0; m < PR_CELLS; m++) {
+    const int idx = threadIdx.x + m * PR_THREADS;
     const int col = idx / PR, t = idx - col * PR;
     const int a = col & 7, b = col >> 3;
     const int i = 3 + TX * TI + a, j = 2 + TY * TJ + b, kk = s0 - a - b + t;
     if (i > g.gend || j > g.ny || kk < 0 || kk >= q.Nz) continue;
-    const long c = at(g, i, j, 2 + kk);
 #pragma unroll
-    for (int f = 0; f < 5; f++) L[f][col][t] = Ui[f * g.nf + c];
+    for (int f = 0; f < 5; f++) L[f][col][t] = keep[m][f];
   }
   __syncthreads();
   for (int idx = threadIdx.x; idx < PR * 64; idx += blockDim.x) {
@@ -2654,7 +2705,7 @@ static void sweeps_t(hipStream_t st, double* Ui, const double* MR, double* Sbuf,
   T* NLs = MRs + 5 * q.nsk;
   T* NRs = NLs + 5 * q.nsk;
   if (rin)
-    hipLaunchKernelGGL(k_sweep_prep_rhs<T>, dim3((q.nsteps + PR - 1) / PR, q.ntiles), dim3(256), 0, st, Ui, *rin,
+    hipLaunchKernelGGL(k_sweep_prep_rhs<T>, dim3((q.nsteps + PR - 1) / PR, q.ntiles), dim3(PR_THREADS), 0, st, Ui, *rin,
                        prof, S, MRs, NLs, g, q, deltaT);
   else
     hipLaunchKernelGGL(k_sweep_prep<T>, dim3((q.nsteps + PC - 1) / PC, q.ntiles), dim3(256), 0, st, Ui, MR, prof, S,
