@@ -132,6 +132,9 @@ struct cf_ctx {
   hipStream_t aux[3] = {nullptr, nullptr, nullptr};
   hipEvent_t fork_ev = nullptr, join_ev[3] = {nullptr, nullptr, nullptr};
   long spin_limit = 1L << 22;
+  // what the sweep-order NL slots that the forward sweep hands across tiles
+  // hold between iterations: 1 the double sentinel, 2 the float one, 0 unknown
+  int hand_off = 0;
   double* partial = nullptr;
   double* er_dev = nullptr;
   double* stat = nullptr;  // running per-j sums of Statistic's mean profiles (7 * Ym)
@@ -774,6 +777,20 @@ static int effective_schedule(const cf_ctx* c) { return (c->schedule != 1 && dia
 // (lagged neighbour planes, :521-559); this library holds one domain.
 static int sweep_pairs(const cf_ctx* c) { return c->sweep_reuse ? std::min(c->p.n_sweeps, 1) : c->p.n_sweeps; }
 
+// The forward tiled sweep needs its cross-tile NL slots at the sentinel, and
+// every backward tiled sweep leaves them so: they are seeded here only when
+// that is not known (a new context, a sweep that gave up), eagerly and never
+// inside a graph capture (cf_inner_iteration seeds before it captures)
+static void seed_hand_off(cf_ctx* c) {
+  if (c->solver != 0 || effective_schedule(c) != 0) return;
+  const int want = c->fp32 ? 2 : 1;
+  if (c->hand_off == want) return;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(c->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
+  cf::launch_seed_hand_off(c->stream, c->Sm, c->g, c->fp32);
+  c->hand_off = want;
+}
+
 static void sweeps(cf_ctx* c, double deltaT, const hipEvent_t* marks = nullptr, bool fuse_rhs = false) {
   const Geo& g = c->g;
   if (effective_schedule(c) == 1) {
@@ -782,6 +799,7 @@ static void sweeps(cf_ctx* c, double deltaT, const hipEvent_t* marks = nullptr, 
   }
   double DN = 1. / ((double)c->p.X_out * c->p.Y_out * c->p.Z_out);
   const cf::RhsIn rin{c->Un, c->Uq, c->Fx, c->Fy, c->Fz, c->vF, c->p.f_drive};
+  seed_hand_off(c);
   cf::launch_sweeps(c->stream, c->Ui, c->MR, c->Sm, c->prof, g, deltaT, sweep_pairs(c), c->order, c->sync,
                     c->partial, c->er_dev, DN, c->spin_limit, marks, c->schedule != 2, fuse_rhs ? &rin : nullptr,
                     c->fp32);
@@ -813,9 +831,14 @@ static int finish_iteration(cf_ctx* c, double er[5]) {
   HIPCHK(c, hipMemcpyAsync(&flags[0], c->sync + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(&flags[1], c->sync + c->ntiles + 3, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   int r = sync(c);
-  if (r) return r;
-  if (c->solver == 0 && effective_schedule(c) == 0 && (flags[0] || flags[1]))
+  if (r) {
+    c->hand_off = 0;
+    return r;
+  }
+  if (c->solver == 0 && effective_schedule(c) == 0 && (flags[0] || flags[1])) {
+    c->hand_off = 0;  // a sweep that gave up may leave slots without the sentinel
     return fail(c, CF_ERR_TIMEOUT, "LU-SGS sweep: a tile waited past its bound for an upstream tile");
+  }
   for (int v = 0; v < 5; v++)
     if (!std::isfinite(er[v])) return fail(c, CF_ERR_NONFINITE, "residual norm er%d is not finite", v + 1);
   return CF_OK;
@@ -1041,6 +1064,7 @@ static int prepare_iteration(cf_ctx* c) {
   }
   if (c->solver == 0)
     if (int r = check_precision(c)) return r;
+  seed_hand_off(c);
   return CF_OK;
 }
 
@@ -1197,6 +1221,7 @@ extern "C" int cf_profile_inner_iteration(cf_ctx* c, double er[5], double stage_
   if (!c || !er || !stage_ms || !launches) return CF_ERR_ARG;
   if (!c->have_state) return fail(c, CF_ERR_STATE, "no state loaded");
   if (int r = check_precision(c)) return r;
+  seed_hand_off(c);  // outside the timed stages
   const Geo& g = c->g;
   const int ns = sweep_pairs(c);
   // ev[0..6]: stage ends up to the RHS; then the sweep marks; then the end

@@ -1239,30 +1239,22 @@ __device__ __forceinline__ void prep_stage(double (*L)[64][PCP], const double* _
   }
 }
 
-// the summary and MR in sweep order, and the NL hand-off slots seeded with the
-// sentinel; each wave writes whole 512-byte rows
+// the summary and MR in sweep order; each wave writes whole 512-byte rows
 template <class T>
 __global__ void __launch_bounds__(256) k_sweep_prep(const double* __restrict__ Ui, const double* __restrict__ MR,
                                                     const double* __restrict__ prof, T* __restrict__ S,
-                                                    T* __restrict__ MRs, T* __restrict__ NLs, Geo g, Sk q) {
+                                                    T* __restrict__ MRs, Geo g, Sk q) {
   __shared__ double L[5][64][PCP];
   int bx, by, bz;
   xcd_block3(&bx, &by, &bz);
   const int tile = by, s0 = bx * PC;
   prep_stage(L, Ui, g, q, tile, s0);
   __syncthreads();
-  const T sv = sentinel_value<T>();
   for (int idx = threadIdx.x; idx < PC * 64; idx += blockDim.x) {
     const int t = idx >> 6, lane = idx & 63, s = s0 + t;
-    if (s >= q.nsteps) continue;
-    const long o = sk_at(q, tile, s, lane);
-    if ((lane & 7) == TI - 1 || (lane >> 3) == TJ - 1) {
-      // slots the first forward sweep hands across tiles start as the sentinel
-#pragma unroll
-      for (int v = 0; v < 5; v++) NLs[v * q.nsk + o] = sv;
-    }
     int i, j, k;
     if (!sk_cell(q, g, tile, s, lane, &i, &j, &k)) continue;
+    const long o = sk_at(q, tile, s, lane);
     double u5[5], v[G_N];
 #pragma unroll
     for (int f = 0; f < 5; f++) u5[f] = L[f][lane][t];
@@ -1300,8 +1292,7 @@ static_assert(64 * PR % PR_THREADS == 0, "whole cells per thread");
 template <class T>
 __global__ void __launch_bounds__(PR_THREADS) k_sweep_prep_rhs(const double* __restrict__ Ui, RhsIn rin,
                                                                const double* __restrict__ prof, T* __restrict__ S,
-                                                               T* __restrict__ MRs, T* __restrict__ NLs, Geo g,
-                                                               Sk q, double deltaT) {
+                                                               T* __restrict__ MRs, Geo g, Sk q, double deltaT) {
   __shared__ double L[5][64][PRP];
   int bx, by, bz;
   xcd_block3(&bx, &by, &bz);
@@ -1323,17 +1314,11 @@ __global__ void __launch_bounds__(PR_THREADS) k_sweep_prep_rhs(const double* __r
     for (int f = 0; f < 5; f++) L[f][col][t] = M[f];
   }
   __syncthreads();
-  const T sv = sentinel_value<T>();
   for (int idx = threadIdx.x; idx < PR * 64; idx += blockDim.x) {
     const int t = idx >> 6, lane = idx & 63, s = s0 + t;
-    if (s >= q.nsteps) continue;
-    const long o = sk_at(q, tile, s, lane);
-    if ((lane & 7) == TI - 1 || (lane >> 3) == TJ - 1) {
-#pragma unroll
-      for (int v = 0; v < 5; v++) NLs[v * q.nsk + o] = sv;
-    }
     int i, j, k;
     if (!sk_cell(q, g, tile, s, lane, &i, &j, &k)) continue;
+    const long o = sk_at(q, tile, s, lane);
 #pragma unroll
     for (int r = 0; r < 5; r++) MRs[r * q.nsk + o] = T(L[r][lane][t]);
   }
@@ -1374,6 +1359,20 @@ __global__ void __launch_bounds__(1024) k_sk_to_natural(const T* __restrict__ sr
   if (!sk_cell(q, g, tile, s, lane, &i, &j, &k)) return;
   const long c = at(g, i, j, k), o = sk_at(q, tile, s, lane);
   for (int v = 0; v < nfields; v++) dst[v * g.nf + c] = double(src[v * q.nsk + o]);
+}
+
+// The NL slots a forward sweep hands across tiles (lanes on a tile's high x
+// or y edge, every step) set to the sentinel.  Each backward sweep sets every
+// such slot it has read back to the sentinel, so this runs only when the
+// slots may hold anything else: a new buffer, or after a sweep that gave up.
+template <class T>
+__global__ void __launch_bounds__(256) k_seed_hand_off(T* __restrict__ NLs, Sk q) {
+  const int lane = threadIdx.x & 63, s = blockIdx.x * 4 + (threadIdx.x >> 6), tile = blockIdx.y;
+  if (s >= q.nsteps || ((lane & 7) != TI - 1 && (lane >> 3) != TJ - 1)) return;
+  const long o = sk_at(q, tile, s, lane);
+  const T sv = sentinel_value<T>();
+#pragma unroll
+  for (int v = 0; v < 5; v++) NLs[v * q.nsk + o] = sv;
 }
 
 __global__ void k_sweep_reset(int* __restrict__ sync, int n) {
@@ -2706,10 +2705,10 @@ static void sweeps_t(hipStream_t st, double* Ui, const double* MR, double* Sbuf,
   T* NRs = NLs + 5 * q.nsk;
   if (rin)
     hipLaunchKernelGGL(k_sweep_prep_rhs<T>, dim3((q.nsteps + PR - 1) / PR, q.ntiles), dim3(PR_THREADS), 0, st, Ui, *rin,
-                       prof, S, MRs, NLs, g, q, deltaT);
+                       prof, S, MRs, g, q, deltaT);
   else
     hipLaunchKernelGGL(k_sweep_prep<T>, dim3((q.nsteps + PC - 1) / PC, q.ntiles), dim3(256), 0, st, Ui, MR, prof, S,
-                       MRs, NLs, g, q);
+                       MRs, g, q);
   if (marks) (void)hipEventRecord(marks[0], st);
   const int nt = q.ntiles;
   for (int sweep = 0; sweep < n_sweeps; sweep++) {
@@ -2738,6 +2737,20 @@ static void sweeps_t(hipStream_t st, double* Ui, const double* MR, double* Sbuf,
   dim3 gr((q.nsteps + PC - 1) / PC, q.ntiles);
   hipLaunchKernelGGL(k_update_sk<T>, gr, dim3(256), 0, st, Ui, NRs, prof, partial, g, q);
   hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, st, partial, (long)gr.x * gr.y, er_dev, DN);
+}
+
+template <class T>
+static void seed_hand_off_t(hipStream_t st, double* Sbuf, const Geo& g) {
+  Sk q = sweep_layout(g);
+  T* NLs = reinterpret_cast<T*>(Sbuf) + (long)(G_N + 5) * q.nsk;
+  hipLaunchKernelGGL(k_seed_hand_off<T>, dim3((q.nsteps + 3) / 4, q.ntiles), dim3(256), 0, st, NLs, q);
+}
+
+void launch_seed_hand_off(hipStream_t st, double* Sbuf, const Geo& g, bool fp32) {
+  if (fp32)
+    seed_hand_off_t<float>(st, Sbuf, g);
+  else
+    seed_hand_off_t<double>(st, Sbuf, g);
 }
 
 void launch_sweeps(hipStream_t st, double* Ui, const double* MR, double* Sbuf, const double* prof, const Geo& g,

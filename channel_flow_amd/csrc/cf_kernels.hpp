@@ -60,6 +60,8 @@ void launch_sweeps(hipStream_t st, double* Ui, const double* MR, double* Sbuf, c
                    const RhsIn* rin = nullptr, bool fp32 = false);
 // NL and NR of the last sweep back in the natural layout
 void launch_sweep_outputs(hipStream_t st, const double* Sbuf, double* NL, double* NR, const Geo& g, bool fp32 = false);
+// the forward sweep's cross-tile NL slots in Sbuf set to the sentinel
+void launch_seed_hand_off(hipStream_t st, double* Sbuf, const Geo& g, bool fp32);
 
 void launch_sweeps_planes(hipStream_t st, const double* Ui, const double* MR, double* NL, double* NR,
                           const double* prof, const Geo& g, double deltaT, int n_sweeps,
