@@ -136,7 +136,8 @@ struct cf_ctx {
   // hold between iterations: 1 the double sentinel, 2 the float one, 0 unknown
   int hand_off = 0;
   double* partial = nullptr;
-  double* er_dev = nullptr;
+  double* er_dev = nullptr;   // the norms, then the two sweep error flags
+  double* er_host = nullptr;  // pinned copy of er_dev, read after each iteration
   double* stat = nullptr;  // running per-j sums of Statistic's mean profiles (7 * Ym)
   double* shear = nullptr; // wall values of Statistic's y-face viscous flux (2 * X_out * Z_out)
   // Shear_stress.dat: running sums SXYm[1], SXYm[ny] and the previous running
@@ -336,6 +337,7 @@ extern "C" int cf_create(const cf_params* p, cf_ctx** out) {
   long nb = std::max(cf::update_blocks(g), cf::sweep_update_blocks(g));
   if (hipMalloc(&c->partial, (size_t)nb * 5 * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
   if (hipMalloc(&c->er_dev, 8 * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
+  if (hipHostMalloc(&c->er_host, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) return bail(CF_ERR_HIP);
   if (hipMalloc(&c->stat, (size_t)7 * g.Ym * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
   if (hipMalloc(&c->shear, (size_t)2 * p->X_out * p->Z_out * sizeof(double)) != hipSuccess) return bail(CF_ERR_HIP);
   if (hipMemsetAsync(c->stat, 0, (size_t)7 * g.Ym * sizeof(double), c->stream) != hipSuccess) return bail(CF_ERR_HIP);
@@ -367,6 +369,7 @@ extern "C" void cf_destroy(cf_ctx* c) {
   if (c->px) (void)hipFree(c->px);
   if (c->py) (void)hipFree(c->py);
   if (c->sync) (void)hipFree(c->sync);
+  if (c->er_host) (void)hipHostFree(c->er_host);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -825,17 +828,17 @@ static void update(cf_ctx* c) {
 }
 
 // er to the host and the sweep error flags checked
+// (one copy into pinned memory: the norms and the flags, which the last
+// reduction kernel placed after them)
 static int finish_iteration(cf_ctx* c, double er[5]) {
-  int flags[2] = {0, 0};
-  HIPCHK(c, hipMemcpyAsync(er, c->er_dev, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipMemcpyAsync(&flags[0], c->sync + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipMemcpyAsync(&flags[1], c->sync + c->ntiles + 3, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->er_host, c->er_dev, 7 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   int r = sync(c);
   if (r) {
     c->hand_off = 0;
     return r;
   }
-  if (c->solver == 0 && effective_schedule(c) == 0 && (flags[0] || flags[1])) {
+  for (int v = 0; v < 5; v++) er[v] = c->er_host[v];
+  if (c->solver == 0 && effective_schedule(c) == 0 && (c->er_host[5] != 0. || c->er_host[6] != 0.)) {
     c->hand_off = 0;  // a sweep that gave up may leave slots without the sentinel
     return fail(c, CF_ERR_TIMEOUT, "LU-SGS sweep: a tile waited past its bound for an upstream tile");
   }
@@ -854,9 +857,10 @@ static int dplusgs_device(cf_ctx* c, double deltaT, double er[5]) {
   cf::launch_dplusgs(c->stream, c->Ui, c->MR, c->NL, c->NR, c->prof, g, deltaT);
   double DN = 1. / ((double)c->p.X_out * c->p.Y_out * c->p.Z_out);
   cf::launch_update(c->stream, c->Ui, c->NR, c->prof, c->partial, c->er_dev, g, DN);
-  HIPCHK(c, hipMemcpyAsync(er, c->er_dev, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->er_host, c->er_dev, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   int r = sync(c);
   if (r) return r;
+  for (int v = 0; v < 5; v++) er[v] = c->er_host[v];
   for (int v = 0; v < 5; v++)
     if (!std::isfinite(er[v])) return fail(c, CF_ERR_NONFINITE, "residual norm er%d is not finite", v + 1);
   return CF_OK;

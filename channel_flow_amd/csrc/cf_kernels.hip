@@ -2321,8 +2321,11 @@ __global__ void __launch_bounds__(256) k_update_sk(double* __restrict__ Ui, cons
   }
 }
 
+// out[0..4]: the norms; out[5], out[6]: the forward and backward tiled
+// sweeps' error flags (0 without them), so that one copy brings back both
 __global__ void __launch_bounds__(1024) k_reduce_partials(const double* __restrict__ partial, long nblocks,
-                                                          double* __restrict__ out, double DN) {
+                                                          double* __restrict__ out, double DN,
+                                                          const int* __restrict__ fa, const int* __restrict__ fb) {
   __shared__ double red[5][1024];
   int tid = threadIdx.x;
   double s[5] = {0, 0, 0, 0, 0};
@@ -2339,6 +2342,8 @@ __global__ void __launch_bounds__(1024) k_reduce_partials(const double* __restri
     __syncthreads();
   }
   if (tid < 5) out[tid] = sqrt(red[tid][0]) * DN;
+  else if (tid == 5) out[5] = fa ? double(*fa) : 0.;
+  else if (tid == 6) out[6] = fb ? double(*fb) : 0.;
 }
 
 // ---------------------------------------------------------------------------
@@ -2736,7 +2741,8 @@ static void sweeps_t(hipStream_t st, double* Ui, const double* MR, double* Sbuf,
   // primitive update reading NR from the sweep order
   dim3 gr((q.nsteps + PC - 1) / PC, q.ntiles);
   hipLaunchKernelGGL(k_update_sk<T>, gr, dim3(256), 0, st, Ui, NRs, prof, partial, g, q);
-  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, st, partial, (long)gr.x * gr.y, er_dev, DN);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, st, partial, (long)gr.x * gr.y, er_dev, DN, sync + 1,
+                     sync + nt + 3);
 }
 
 template <class T>
@@ -2840,7 +2846,8 @@ void launch_update(hipStream_t st, double* Ui, const double* NR, const double* p
   dim3 blk(64, 4);
   dim3 gr = box_grid(g.nz - 1, g.ny - 1, g.gend - 2, blk);
   hipLaunchKernelGGL(k_update, gr, blk, 0, st, Ui, NR, prof, partial, g);
-  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, st, partial, update_blocks(g), er_dev, DN);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, st, partial, update_blocks(g), er_dev, DN, nullptr,
+                     nullptr);
 }
 
 }  // namespace cf
