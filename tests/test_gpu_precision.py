@@ -5,8 +5,10 @@ The right-hand side (fluxes, viscous terms, the BDF2 term), D^-1's inputs
 and the state update stay FP64, so the converged state is the FP64 one: the
 sweeps only shape the path to it.  The tests check that the float path is
 the one that runs (its increments differ from FP64's at float precision),
-that physical steps converged to a 1e-8 drop of er land on the FP64 state
-and mean profiles, and that the unsupported schedules are refused.
+that one iteration's increment equals the CPU oracle's FP64 one to float
+precision, that physical steps converged to a 1e-8 drop of er land on the
+FP64 state and mean profiles, and that the unsupported schedules are
+refused.
 """
 import numpy as np
 import pytest
@@ -83,6 +85,42 @@ def test_gpu_fp32_sweeps_run_in_float():
     assert 0 < rel <= 1e-5, rel
     np.testing.assert_allclose(e32, e64, rtol=1e-4)
     assert not np.array_equal(e32, e64)
+
+
+def test_gpu_fp32_sweeps_against_the_oracle():
+    """One inner iteration with the FP32 sweeps on the GPU against the CPU
+    oracle's FP64 inner iteration (Dual_time_stepping.cpp) from the same
+    state: the state increments agree to 1e-6 relative (measured 7e-8: the
+    float rounding of the sweeps' inputs and arithmetic).  The bound separates
+    sweeps: on the CPU, the oracle's DPLUSGS Jacobi sweeps in place of the
+    LU-SGS ones (a plausible mix-up of the two implicit solvers) move the
+    increment by 2.8e-2 relative."""
+    from oracle import oracle as O
+    from channel_flow_amd.config import poiseuille_state
+    from channel_flow_amd.solver import Solver
+    cfg, fl = _cfg()
+    Qn = poiseuille_state(cfg, fl, seed=3, stream=0)
+    Qm = poiseuille_state(cfg, fl, seed=3, stream=1)
+    met = O.metrics(cfg)
+    U, Uq = O.state_from_Q(cfg, Qn, met["J"]), O.state_from_Q(cfg, Qm, met["J"])
+    sl = (slice(3, cfg.gend + 1), slice(2, cfg.ny + 1), slice(2, cfg.nz + 1))
+
+    def oracle_increment(dplusgs):
+        Ui = [a.copy() for a in U]
+        O.inner_iteration(cfg, Ui, U, Uq, met, dplusgs=dplusgs)
+        return np.stack([Ui[v][sl] * met["J"][sl] for v in range(5)]) - Qn
+
+    d_o = oracle_increment(False)
+    with Solver(cfg, precision=FP32_SWEEPS) as s:
+        s.Jacobian_all(cfg.gamma_metric, want_arrays=False)
+        s.load_state(Qn, Qm)
+        s.inner_iteration()
+        d32 = s.read_state()[2] - Qn
+    rel = np.linalg.norm(d32 - d_o) / np.linalg.norm(d_o)
+    print(f"FP32 sweeps against the oracle: increment relative difference {rel:.2e}", flush=True)
+    assert 0 < rel <= 1e-6, rel
+    planted = np.linalg.norm(oracle_increment(True) - d_o) / np.linalg.norm(d_o)
+    assert planted > 1e-2, planted
 
 
 def test_gpu_fp32_sweeps_converge_to_the_fp64_state():
