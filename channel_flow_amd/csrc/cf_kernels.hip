@@ -103,43 +103,6 @@ __device__ __forceinline__ void flux_face(const double (&u)[5][6], int f, int i,
 // profiles at j) is held in scalar registers and read with scalar loads
 __device__ __forceinline__ int wave_row() { return __builtin_amdgcn_readfirstlane(threadIdx.y); }
 
-// one thread per face (Flux_Z: the lanes run along k, the direction itself)
-template <int DIR>
-__global__ void __launch_bounds__(256) k_flux(const double* __restrict__ U, const double* __restrict__ prof,
-                                              double* __restrict__ F, Geo g) {
-  // DIR 0: f in [2,gend] (i), a = j in [2,ny], b = k in [2,nz]
-  // DIR 1: f in [1,ny]   (j), a = i in [3,gend], b = k in [2,nz]
-  // DIR 2: f in [1,nz]   (k), a = i in [3,gend], b = j in [2,ny]
-  int x = blockIdx.x * blockDim.x + threadIdx.x;
-  int y = blockIdx.y * blockDim.y + threadIdx.y;
-  int z = blockIdx.z;
-  int f, i, j, k;
-  if (DIR == 0) { k = 2 + x; j = 2 + y; f = 2 + z; i = f; if (k > g.nz || j > g.ny) return; }
-  else if (DIR == 1) { k = 2 + x; f = 1 + y; i = 3 + z; j = f; if (k > g.nz || f > g.ny) return; }
-  else { f = 1 + x; j = 2 + y; i = 3 + z; k = f; if (f > g.nz || j > g.ny) return; }
-  long stride = DIR == 0 ? g.plane : (DIR == 1 ? (long)g.Zp : 1L);
-  long base = at(g, i, j, k);  // cell f along the direction
-  int rl = recipe_left(DIR, f, g.gend, g.ny, g.nz);
-  int rr = recipe_right(DIR, f, g.gend, g.ny, g.nz);
-  // cells f-2 .. f+3 actually used by the two recipes
-  auto lo_of = [](int r) { return r == O5L ? -2 : (r == O5R || r == O3L || r == AVGM) ? -1 : (r == AVG1 ? 1 : 0); };
-  auto hi_of = [](int r) { return r == O5L ? 2 : r == O5R ? 3 : r == O3L ? 1 : r == O3R ? 2 : r == AVGM ? 0 : r == AVG1 ? 2 : 1; };
-  int tlo = min(lo_of(rl), lo_of(rr)) + 2, thi = max(hi_of(rl), hi_of(rr)) + 2;
-  double u[5][6];
-#pragma unroll
-  for (int t = 0; t < 6; t++) {
-    if (t < tlo || t > thi) {
-#pragma unroll
-      for (int v = 0; v < 5; v++) u[v][t] = 0.0;
-      continue;
-    }
-    long p = base + (long)(t - 2) * stride;
-#pragma unroll
-    for (int v = 0; v < 5; v++) u[v][t] = U[v * g.nf + p];
-  }
-  flux_face<DIR>(u, f, i, j, k, prof, F, g);
-}
-
 // Flux_Z: a block of 64 faces (lanes, along k) by 4 rows j stages each row's
 // cells f0-2 .. f0+66 in LDS once, and each thread reads its six stencil
 // cells from there (cells beyond the row, next to the periodic seam, are
@@ -323,7 +286,10 @@ __device__ __forceinline__ void lim_face(const double (&u)[5][6], const double (
 template <int DIR>
 __global__ void __launch_bounds__(256) k_flux_lim(const double* __restrict__ U, const double* __restrict__ Ep,
                                                   const double* __restrict__ prof, double* __restrict__ F, Geo g) {
-  // faces and transverse cells as k_flux
+  // one thread per face.  DIR 0: f in [2,gend] (i), j in [2,ny], k in
+  // [2,nz]; DIR 1: f in [1,ny] (j), i in [3,gend], k in [2,nz]; DIR 2: f in
+  // [1,nz] (k), i in [3,gend], j in [2,ny] (the lanes run along k, for
+  // Flux_Z the direction itself)
   int x = blockIdx.x * blockDim.x + threadIdx.x;
   int y = blockIdx.y * blockDim.y + threadIdx.y;
   int z = blockIdx.z;
