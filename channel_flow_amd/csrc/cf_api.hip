@@ -118,10 +118,12 @@ struct cf_ctx {
     bool fp32, fork;
     double deltaT, f_drive;
     long prof_gen;  // launch choices that follow the metrics (the fused viscous kernel)
+    long spin_limit;
     bool operator==(const GraphKey& o) const {
       return Ui == o.Ui && Un == o.Un && Uq == o.Uq && schedule == o.schedule && visc_schedule == o.visc_schedule &&
              solver == o.solver && x_periodic == o.x_periodic && pairs == o.pairs && fp32 == o.fp32 &&
-             fork == o.fork && deltaT == o.deltaT && f_drive == o.f_drive && prof_gen == o.prof_gen;
+             fork == o.fork && deltaT == o.deltaT && f_drive == o.f_drive && prof_gen == o.prof_gen &&
+             spin_limit == o.spin_limit;
     }
   } gkey{};
   long graph_builds = 0;
@@ -1117,7 +1119,7 @@ static void enqueue_iteration(cf_ctx* c) {
 static int launch_graph(cf_ctx* c) {
   const cf_ctx::GraphKey key{c->Ui,     c->Un,          c->Uq,          c->schedule, c->visc_schedule,
                              c->solver, c->p.x_periodic, sweep_pairs(c), c->fp32,     c->fork,
-                             c->p.deltaT, c->p.f_drive,  c->prof_gen};
+                             c->p.deltaT, c->p.f_drive,  c->prof_gen, c->spin_limit};
   if (!c->gexec || !(key == c->gkey)) {
     if (c->gexec) {
       HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1181,6 +1183,12 @@ extern "C" int cf_set_concurrent_stencils(cf_ctx* c, int enabled) {
 extern "C" int cf_set_sweep_reuse(cf_ctx* c, int enabled) {
   if (!c) return CF_ERR_ARG;
   c->sweep_reuse = enabled != 0;
+  return CF_OK;
+}
+
+extern "C" int cf_set_spin_limit(cf_ctx* c, long long polls) {
+  if (!c || polls < 1) return CF_ERR_ARG;
+  c->spin_limit = (long)polls;
   return CF_OK;
 }
 

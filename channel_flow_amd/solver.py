@@ -41,7 +41,7 @@ EXPORTS = [
     "cf_pred_get_params", "cf_pred_set_norm", "cf_pred_get_norm", "cf_pred_capture", "cf_pred_dataset_size",
     "cf_pred_fit_norm", "cf_pred_load_dataset", "cf_pred_train", "cf_pred_eval", "cf_pred_forward", "cf_pred_gradient", "cf_pred_profile",
     "cf_keep_levels", "cf_set_predictor", "cf_predictor_stats", "cf_set_sweep_reuse", "cf_set_graph",
-    "cf_graph_builds", "cf_set_concurrent_stencils", "cf_set_extrapolation_order",
+    "cf_graph_builds", "cf_set_concurrent_stencils", "cf_set_extrapolation_order", "cf_set_spin_limit",
 ]
 CF_PRED_NPARAM = 112005
 CF_PRED_NNORM = 22
@@ -143,6 +143,7 @@ def load_library() -> C.CDLL:
     lib.cf_pred_profile.argtypes = [vp, C.c_int, C.c_int, C.c_int, P]
     lib.cf_keep_levels.argtypes = [vp, C.c_int]
     lib.cf_set_sweep_reuse.argtypes = [vp, C.c_int]
+    lib.cf_set_spin_limit.argtypes = [vp, C.c_longlong]
     lib.cf_set_graph.argtypes = [vp, C.c_int]
     lib.cf_graph_builds.argtypes = [vp, C.POINTER(C.c_longlong)]
     lib.cf_set_concurrent_stencils.argtypes = [vp, C.c_int]
@@ -457,6 +458,11 @@ class Solver:
     def set_sweep_reuse(self, enabled: bool = True):
         """One LU-SGS sweep pair for any n_sweeps (bit-identical in one domain)."""
         self._chk(self.lib.cf_set_sweep_reuse(self._h, int(bool(enabled))))
+
+    def set_spin_limit(self, polls: int):
+        """Polls a tiled sweep waits for one upstream tile's value before it
+        gives up (CF_ERR_TIMEOUT; the iterate must then be reloaded)."""
+        self._chk(self.lib.cf_set_spin_limit(self._h, int(polls)))
 
     def set_graph(self, enabled: bool = True):
         """Replay each inner iteration's kernels as a captured hipGraph."""

@@ -314,6 +314,13 @@ int cf_synchronize(cf_ctx* ctx);
  * the first bit for bit (MR, NL, NR, U_ and er are the same; across MPI
  * ranks they would not be, :521-559).  0 runs all n_sweeps pairs. */
 int cf_set_sweep_reuse(cf_ctx* ctx, int enabled);
+/* Polls a tiled sweep's halo wave makes for one value from an upstream tile
+ * before the sweep gives up (default 4194304; polls >= 1).  A sweep that gives
+ * up runs to its end without that value; the iteration then returns
+ * CF_ERR_TIMEOUT and its iterate is not usable: reload the state.  No
+ * counterpart in the reference, whose ranks wait in MPI receives without a
+ * bound. */
+int cf_set_spin_limit(cf_ctx* ctx, long long polls);
 /* With enabled != 0, cf_inner_iteration (and so cf_physical_step) replays
  * the kernels of one inner iteration as a hipGraph on the context's stream,
  * captured on first use and again when the fields, schedules, precision or

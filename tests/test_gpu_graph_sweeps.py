@@ -144,3 +144,40 @@ def test_gpu_concurrent_stencils_same_bits(grid, graph):
     e2, s2, _ = _run(cfg, Q, 4, graph=graph, concurrent=False)
     np.testing.assert_array_equal(e1, e2)
     np.testing.assert_array_equal(s1, s2)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("graph", [False, True])
+def test_gpu_sweep_gives_up_then_recovers(precision, graph):
+    """A tiled sweep that gives up, forced from the host by a one-poll bound
+    on its halo wave's wait (every kernel still runs to its end), reports
+    CF_ERR_TIMEOUT.  With the state reloaded and the bound restored, the next
+    iterations give a fresh context's bits: the cross-tile hand-off slots,
+    which the abandoned sweeps may have left without the sentinel, are seeded
+    again before the forward sweep reads them."""
+    from channel_flow_amd.config import ChannelConfig, FlowParams, poiseuille_state
+    from channel_flow_amd.solver import CfError, Solver
+    print(f"sweep gives up: precision {precision} graph {graph}", flush=True)
+    cfg = ChannelConfig(64, 48, 32, gamma_metric=2.0, deltaT=1e-8)
+    Qn = poiseuille_state(cfg, FlowParams(), seed=6, stream=0)
+    Qm = poiseuille_state(cfg, FlowParams(), seed=6, stream=1)
+
+    def start(s):
+        s.Jacobian_all(2.0, want_arrays=False)
+        s.set_graph(graph)
+        s.load_state(Qn, Qm)
+
+    with Solver(cfg, precision=precision) as s:
+        start(s)
+        e_ref = np.array([s.inner_iteration() for _ in range(3)])
+        q_ref = s.read_state()[2]
+    with Solver(cfg, precision=precision) as s:
+        start(s)
+        s.set_spin_limit(1)
+        with pytest.raises(CfError, match="TIMEOUT"):
+            s.inner_iteration()
+        s.set_spin_limit(1 << 22)
+        s.load_state(Qn, Qm)
+        e = np.array([s.inner_iteration() for _ in range(3)])
+        np.testing.assert_array_equal(e, e_ref)
+        np.testing.assert_array_equal(s.read_state()[2], q_ref)
